@@ -1,0 +1,151 @@
+/*
+ * gplum_gpu.h -- C ABI of the MI355X (gfx950) implementation of GPLUM's P3T hot path:
+ * FP32 soft force (EP-EP changeover + neighbour counting, EP-SP monopole/quadrupole),
+ * FP64 neighbour decision / changeover correction / cluster labelling, and the FP64
+ * block-timestep Hermite integration of the resulting hard clusters.
+ *
+ * Every entry point returns an int status: 0 on success, < 0 on error
+ * (GPLUM_E_*); gplum_last_error() returns a message for the calling thread.
+ * No entry point retains a caller pointer after it returns.
+ *
+ * Struct layouts are byte-compatible with the reference's `individual_cutoff=yes`
+ * build (src/particle.h:16-170, FDPS SPJMonopole/SPJQuadrupole), so FDPS arrays can
+ * be passed unchanged.
+ */
+#ifndef GPLUM_GPU_H
+#define GPLUM_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define GPLUM_OK            0
+#define GPLUM_E_HIP        -1   /* HIP runtime error                                  */
+#define GPLUM_E_ARG        -2   /* invalid argument (null pointer, bad size)          */
+#define GPLUM_E_OVERFLOW   -3   /* a device buffer was too small; caller may retry    */
+#define GPLUM_E_NODEVICE   -4   /* no usable gfx950 device                             */
+#define GPLUM_E_STATE      -5   /* call order violated (e.g. retrieve before dispatch) */
+
+/* ------------------------------------------------------- reference-layout PODs */
+
+/* NeighborInfo, src/particle.h:16-68 (16 B). Clear value: {0, 0, -1, 2^31-1}. */
+typedef struct gplum_neighbor_info {
+    int32_t number;
+    int32_t rank;
+    int32_t id_max;
+    int32_t id_min;
+} gplum_neighbor_info;
+
+/* ForceGrav, src/particle.h:70-86 (32 B). */
+typedef struct gplum_force {
+    float acc[3];
+    float phi;
+    gplum_neighbor_info neighbor;
+} gplum_force;
+
+/* EPIGrav with USE_INDIVIDUAL_CUTOFF, src/particle.h:93-141 (48 B). */
+typedef struct gplum_epi {
+    int32_t id_local;
+    int32_t myrank;
+    double  pos[3];
+    double  r_out;
+    double  r_search;
+} gplum_epi;
+
+/* EPJGrav, src/particle.h:149-170 (112 B). */
+typedef struct gplum_epj {
+    int32_t id_local;
+    int32_t myrank;
+    double  pos[3];
+    double  r_out;
+    double  r_search;
+    int64_t id;
+    double  mass;
+    double  vel[3];
+    double  acc_d[3];
+} gplum_epj;
+
+/* FDPS SPJMonopole (MySPJMonopole, src/particle.h:1628-1647) (32 B). */
+typedef struct gplum_spj_mono {
+    double mass;
+    double pos[3];
+} gplum_spj_mono;
+
+/* FDPS SPJQuadrupole (MySPJQuadrupole, src/particle.h:1717-1737) (80 B).
+ * quad = sum m (x-com)(x-com)^T, stored {xx, yy, zz, xy, xz, yz}. */
+typedef struct gplum_spj_quad {
+    double mass;
+    double pos[3];
+    double quad[6];
+} gplum_spj_quad;
+
+/* --------------------------------------------------------------- run parameters */
+
+/* The static FPGrav parameters (src/particle.h:421-479, defaults :917-961) plus the
+ * tree parameters of main_p3t.cpp:345.  gplum_params_default() fills the reference
+ * defaults; gplum_params_set_gamma() mirrors FPGrav::setGamma (particle.h:619-633). */
+typedef struct gplum_params {
+    double m_sun, dens, eps2, eps2_sun;
+    double R_cut0, R_cut1, R_search0, R_search1, R_search2, R_search3;
+    double gamma, g_1_inv, g_1_inv7, w_y, f1;
+    double r_cut_min, r_cut_max, p_cut, increase_factor;
+    double dt_tree, dt_min, eta, eta_0, eta_sun, eta_sun0, alpha2;
+    double theta;
+    int32_t n_leaf_limit;
+    int32_t n_group_limit;
+    int32_t quadrupole;      /* 1: EP-SP quadrupole (USE_QUAD), 0: monopole          */
+    int32_t collision;       /* 1: perfect-merging collisions (COLLISION)            */
+} gplum_params;
+
+void gplum_params_default(gplum_params *p);
+void gplum_params_set_gamma(gplum_params *p, double g);
+
+/* ------------------------------------------------------------------- utilities */
+const char *gplum_last_error(void);
+int gplum_device_info(char *name, int name_len, int *n_cu, double *clock_ghz);
+
+/* ---------------------------------------------- PIKG-shaped soft-force kernels */
+
+/* Replaces the PIKG functor CalcForceLongEPEP(eps2)(epi, ni, epj, nj, force)
+ * called from calcForceEPEPWithSearch::operator() (src/gravity_kernel.hpp:12-23;
+ * DSL src/gravity_kernel_epep.pikg:1-97).  Accumulates into force[0..ni) exactly as
+ * the PIKG kernel does: acc += sum m r/r^3, phi -= sum m/r with r^2 clamped at
+ * max(r_out_i, r_out_j)^2, and neighbour number/rank/id_max/id_min for
+ * |r|^2+eps2 < 1.0201 max(r_search)^2.  Host pointers; synchronous. */
+int gplum_calc_force_epep(const gplum_epi *epi, int ni,
+                          const gplum_epj *epj, int nj,
+                          gplum_force *force, float eps2);
+
+/* Replaces CalcForceLongEPSP(eps2)(epi, ni, spj, nj, force) (src/gravity_kernel.hpp:
+ * 128-136; DSL src/gravity_kernel_epsp.pikg:1-100), quadrupole variant (USE_QUAD). */
+int gplum_calc_force_epsp_quad(const gplum_epi *epi, int ni,
+                               const gplum_spj_quad *spj, int nj,
+                               gplum_force *force, float eps2);
+
+/* Monopole variant of the same functor (USE_QUAD undefined). */
+int gplum_calc_force_epsp_mono(const gplum_epi *epi, int ni,
+                               const gplum_spj_mono *spj, int nj,
+                               gplum_force *force, float eps2);
+
+/* FP64 twins of the three kernels above (same arithmetic in double, results
+ * rounded to the FP32 Force fields only at the end; the acc/phi sums are also
+ * returned unrounded in acc64[3*ni] / phi64[ni] when those are non-null).
+ * They are the "vs FP64" reference for the FP32 kernels. */
+int gplum_calc_force_epep_f64(const gplum_epi *epi, int ni,
+                              const gplum_epj *epj, int nj,
+                              gplum_force *force, float eps2,
+                              double *acc64, double *phi64);
+int gplum_calc_force_epsp_quad_f64(const gplum_epi *epi, int ni,
+                                   const gplum_spj_quad *spj, int nj,
+                                   gplum_force *force, float eps2,
+                                   double *acc64, double *phi64);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPLUM_GPU_H */

@@ -1,0 +1,118 @@
+/*
+ * orc_step.c -- oracle: initialisation and one soft step of the P3T loop
+ * (main_p3t.cpp:289-364, 423-699) on one rank.  Test infrastructure only.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+/* tree soft force + correctForceLong[Initial] */
+static void soft_and_correct(const gplum_params *p, gplum_particle *pp, int n,
+                             int initial, orc_nlist *nl)
+{
+    orc_tree t;
+    orc_tree_build(p, pp, n, &t);
+    orc_soft_force(p, pp, n, &t, 0, NULL);
+    orc_tree_free(&t);
+    orc_nlist_free(nl);
+    orc_correct_force_long(p, pp, n, initial, nl, NULL, NULL);
+}
+
+/* main_p3t.cpp:289-364 */
+void orc_p3t_init(const gplum_params *p, gplum_particle *pp, int n, orc_nlist *nl, double time_sys)
+{
+    for (int i = 0; i < n; i++) {
+        pp[i].time = time_sys;
+        pp[i].neighbor.number = 0;
+        pp[i].n_cluster = 0;
+        pp[i].flags = 0;
+        if (pp[i].r_planet <= 0.) pp[i].r_planet = pow(0.75 * pp[i].mass / (3.14159265358979323846 * p->dens), 1. / 3.);
+        if (pp[i].f == 0.) pp[i].f = p->increase_factor;
+    }
+    orc_set_cutoff_radii(p, pp, n);
+    for (int round = 0; round < 2; round++) soft_and_correct(p, pp, n, 1, nl);
+}
+
+/* MergeParticle (func.h:160-207): fold dead twins into their target, remove them. */
+static int merge_particles(gplum_particle *pp, int n, double *edisp)
+{
+    char *rm = (char *)calloc((size_t)n + 1, 1);
+    double ed = 0.;
+    for (int i = 0; i < n; i++) {
+        if (!((pp[i].flags & GPLUM_FLAG_MERGED) && !(pp[i].flags & GPLUM_FLAG_DEAD))) continue;
+        for (int j = 0; j < n; j++) {
+            if (pp[j].id == pp[i].id && i != j) {
+                double mi = pp[i].mass, mj = pp[j].mass;
+                double vr[3] = {pp[j].vel[0] - pp[i].vel[0], pp[j].vel[1] - pp[i].vel[1], pp[j].vel[2] - pp[i].vel[2]};
+                pp[i].mass += mj;
+                for (int k = 0; k < 3; k++) pp[i].vel[k] = (mi * pp[i].vel[k] + mj * pp[j].vel[k]) / (mi + mj);
+                pp[i].phi = (mi * pp[i].phi + mj * pp[j].phi) / (mi + mj);
+                pp[i].phi_d = (mi * pp[i].phi_d + mj * pp[j].phi_d) / (mi + mj);
+                double s = 0.5 * mi * mj / (mi + mj);
+                ed -= (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
+                rm[j] = 1;
+            }
+        }
+        pp[i].flags &= ~GPLUM_FLAG_MERGED;
+    }
+    int m = 0;
+    for (int i = 0; i < n; i++) if (!rm[i]) pp[m++] = pp[i];
+    free(rm);
+    *edisp += ed;
+    return m;
+}
+
+/* One soft step, main_p3t.cpp:423-699. */
+int orc_p3t_step(const gplum_params *p, gplum_particle *pp, int *n_io, orc_nlist *nl,
+                 double *edisp, double time_sys)
+{
+    int n = *n_io;
+    const double hdt = 0.5 * p->dt_tree;
+    /* 1st kick (gravity_soft.h:5-12, particle.h:878-884) */
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) pp[i].vel[k] += hdt * pp[i].acc[k];
+    /* clusters */
+    orc_neighbor_cluster(pp, n, nl);
+    int32_t *cl_off = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 2));
+    int32_t *cl_idx = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int nc = orc_make_cluster_list(pp, n, cl_off, cl_idx);
+    int n_col = 0;
+    double ed = 0.;
+#pragma omp parallel for schedule(dynamic, 2) reduction(+ : n_col, ed)
+    for (int c = 0; c < nc; c++) {
+        int m = cl_off[c + 1] - cl_off[c];
+        gplum_particle *buf = (gplum_particle *)malloc(sizeof(gplum_particle) * m);
+        for (int a = 0; a < m; a++) buf[a] = pp[cl_idx[cl_off[c] + a]];
+        double e = 0.;
+        n_col += orc_hermite_cluster(p, buf, m, &e, NULL);
+        ed += e;
+        for (int a = 0; a < m; a++) pp[cl_idx[cl_off[c] + a]] = buf[a];
+        free(buf);
+    }
+    free(cl_off);
+    free(cl_idx);
+    /* isolated particles (hard.h:793-817) */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        if (pp[i].neighbor.number) continue;
+        if (orc_eccentricity(p, &pp[i], NULL) < 0.8 && p->eps2_sun == 0.)
+            orc_kepler_isolated(p, &pp[i], time_sys, time_sys + p->dt_tree);
+        else
+            orc_hermite_isolated(p, &pp[i]);
+        pp[i].n_cluster = 1;
+    }
+    *edisp += ed;
+    /* soft force + correction */
+    soft_and_correct(p, pp, n, 0, nl);
+    /* 2nd kick */
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) pp[i].vel[k] += hdt * pp[i].acc[k];
+    if (n_col) {
+        n = merge_particles(pp, n, edisp);
+        orc_set_cutoff_radii(p, pp, n);
+        soft_and_correct(p, pp, n, 1, nl);
+    }
+    *n_io = n;
+    return n_col;
+}
