@@ -1,0 +1,103 @@
+// common.hpp -- shared host/device helpers of libgplum_gpu (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/gplum_gpu.h"
+#include "../../include/gplum_system.h"
+
+static_assert(sizeof(gplum_neighbor_info) == 16, "NeighborInfo layout");
+static_assert(sizeof(gplum_force) == 32, "ForceGrav layout");
+static_assert(sizeof(gplum_epi) == 48, "EPIGrav layout");
+static_assert(sizeof(gplum_epj) == 112, "EPJGrav layout");
+static_assert(sizeof(gplum_spj_mono) == 32, "SPJMonopole layout");
+static_assert(sizeof(gplum_spj_quad) == 80, "SPJQuadrupole layout");
+
+namespace gplum {
+
+// ---------------------------------------------------------------- error state
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define GPLUM_HIP(call)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return ::gplum::fail(GPLUM_E_HIP, std::string(#call " -> ") +            \
+                                                  hipGetErrorString(e_));            \
+    } while (0)
+
+// Bounds on data-dependent loops in device code.  The reference's step-halving
+// loops (particle.h:902-912, 1304-1311) terminate only for power-of-two times; the
+// caps keep a non-dyadic or NaN input from spinning a wave forever (the result is
+// then dt_min, and the block-step cap raises an error on the host).
+constexpr int HALVE_CAP = 160;
+constexpr long long BLOCK_STEP_CAP = 1ll << 22;
+
+// Clear values of NeighborInfo: S32_MAX-1 with S32_MAX = 2^31 (main_p3t.cpp:32).
+constexpr int32_t ID_MIN_CLEAR = 2147483647;
+constexpr int32_t ID_MAX_CLEAR = -1;
+
+// ------------------------------------------------------------ device helpers
+// std::min / std::max semantics: min(a,b) = (b<a)?b:a, max(a,b) = (a<b)?b:a.
+__host__ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+// Exact-order FP64 helpers (the library is compiled with -ffp-contract=off, so
+// these are plain IEEE mul/add in the written order).
+__host__ __device__ inline double dot3(double ax, double ay, double az,
+                                       double bx, double by, double bz) {
+    return ax * bx + ay * by + az * bz;
+}
+
+// Changeover functions, src/cutfunc.h:4-66 and FPGrav::setGamma constants.
+struct Cutoff {
+    double gamma, g_1_inv, g_1_inv7, w_y, f1;
+};
+
+__host__ __device__ inline double cutoff_f(const Cutoff &c, double y) {
+    const double g = c.gamma;
+    const double g2 = g * g;
+    return (((((((-10. / 3. * y + 14. * (g + 1.)) * y - 21. * ((g + 3.) * g + 1.)) * y
+                + 35. / 3. * (((g + 9.) * g + 9.) * g + 1.)) * y
+               - 70. * ((g + 3.) * g + 1.) * g) * y
+              + 210. * (g + 1.) * g2) * y - 140. * g2 * g * log(y)) * y
+            + (((g - 7.) * g + 21.) * g - 35.) * g2 * g2) * c.g_1_inv7;
+}
+__host__ __device__ inline double cutoff_W(const Cutoff &c, double rij, double r_out_inv) {
+    const double y = rij * r_out_inv;
+    if (1.0 <= y) return 1.0;
+    if (y <= c.gamma) return y * c.w_y;
+    return cutoff_f(c, y) + y * (1. - c.f1);
+}
+__host__ __device__ inline double cutoff_Kx(double x) {
+    if (x < 0.) return 0.;
+    if (x >= 1.) return 1.;
+    const double x2 = x * x;
+    return (((-20. * x + 70.) * x - 84.) * x + 35.) * x2 * x2;
+}
+__host__ __device__ inline double cutoff_K(const Cutoff &c, double rij, double r_out_inv) {
+    return cutoff_Kx((c.gamma - rij * r_out_inv) * c.g_1_inv);
+}
+__host__ __device__ inline double cutoff_dKdt(const Cutoff &c, double rij, double r_out_inv,
+                                              double alpha) {
+    const double x = (c.gamma - rij * r_out_inv) * c.g_1_inv;
+    const double x_1 = x - 1.;
+    const double dKdr = (x < 0. || x >= 1.) ? 0.
+                        : (140. * x * x * x * x_1 * x_1 * x_1 * r_out_inv * c.g_1_inv);
+    return alpha * rij * dKdr;
+}
+
+inline Cutoff cutoff_of(const gplum_params &p) {
+    return Cutoff{p.gamma, p.g_1_inv, p.g_1_inv7, p.w_y, p.f1};
+}
+
+// Wave-level helpers (wave64).
+__device__ inline int lane_id() { return __lane_id(); }
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace gplum

@@ -1,0 +1,810 @@
+// hermite.hip -- batched FP64 4th-order Hermite with block time steps (hard part),
+// Kepler drift and Sun-only Hermite for isolated particles.
+//
+// One workgroup integrates one cluster over local time [0, dt_tree] exactly as
+// timeIntegrate_multi does (src/hermite.h:224-479): a block step picks the minimum
+// time+dt, predicts every particle, checks collisions (perfect merging), corrects
+// the active particles with two P(EC) iterations and updates their block steps.
+// The active particles of one block step are independent (each reads only the
+// predicted positions of the others), so they are corrected in parallel with one
+// barrier per block step.  The j-data of the cluster (predicted position/velocity,
+// mass, 1/r_out, id) lives in LDS for clusters of up to HERMITE_LDS particles.
+// The kernel is persistent: workgroups pull clusters (largest first) from an atomic
+// queue until it is empty.
+#include <cfloat>
+
+#include "hermite.hpp"
+
+namespace gplum {
+
+#define SECONDORDER 5.e-1
+#define THIRDORDER 1.6666666666666667e-1
+#define ALPHA_Q1 1.1666666666666667e-3   // quirk Q1, src/particle.h:339
+#define ONE_TWELFTH 8.3333333333333333e-2
+#define ONE_SIXTIETH 1.6666666666666667e-2
+
+struct HP {                                  // physics constants for the kernels
+    double m_sun, eps2, eps2_sun, dt_tree, dt_min, eta, eta_0, eta_sun, eta_sun0, alpha2;
+    Cutoff cut;
+    int collision;
+};
+
+__device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
+
+// ---------------------------------------------------------- Sun gravity
+
+__device__ inline void star_acc_jerk(const HP &P, const double x[3], const double v[3],
+                                     double acc[3], double jerk[3], double *phi) {
+    const double dr[3] = {-x[0], -x[1], -x[2]};
+    const double dv[3] = {-v[0], -v[1], -v[2]};
+    const double r2inv = 1. / (dot3(dr[0], dr[1], dr[2], dr[0], dr[1], dr[2]) + P.eps2_sun);
+    const double rinv = sqrt(r2inv);
+    const double r3inv = rinv * r2inv;
+    const double mj = P.m_sun * r3inv;
+    const double alpha = dot3(dr[0], dr[1], dr[2], dv[0], dv[1], dv[2]) * r2inv;
+    if (phi) *phi = -P.m_sun * rinv;
+    for (int k = 0; k < 3; k++) {
+        acc[k] = mj * dr[k];
+        jerk[k] = mj * (dv[k] - 3. * alpha * dr[k]);
+    }
+}
+
+// -------------------------------------------------------------- time steps
+
+__device__ inline double calc_dt2(double eta, double alpha2, double acc0, const double a[3],
+                                  const double j[3]) {
+    const double Acc2 = dot3(a[0], a[1], a[2], a[0], a[1], a[2]) + alpha2 * acc0 * acc0;
+    const double Jerk2 = dot3(j[0], j[1], j[2], j[0], j[1], j[2]);
+    return (Jerk2 > 0.) ? eta * sqrt(Acc2 / Jerk2) : DBL_MAX;
+}
+__device__ inline double calc_dt4(double eta, double alpha2, double acc0, const double a[3],
+                                  const double j[3], const double s[3], const double c[3]) {
+    const double Acc = sqrt(dot3(a[0], a[1], a[2], a[0], a[1], a[2]) + alpha2 * acc0 * acc0);
+    const double Jerk2 = dot3(j[0], j[1], j[2], j[0], j[1], j[2]);
+    const double Jerk = sqrt(Jerk2);
+    const double Snap2 = dot3(s[0], s[1], s[2], s[0], s[1], s[2]);
+    const double Snap = sqrt(Snap2);
+    const double Crac = sqrt(dot3(c[0], c[1], c[2], c[0], c[1], c[2]));
+    return (Jerk > 0.) ? eta * sqrt((Acc * Snap + Jerk2) / (Jerk * Crac + Snap2)) : DBL_MAX;
+}
+
+// FPGrav::calcDeltatInitial, src/particle.h:886-914.
+__device__ inline double dt_initial(const HP &P, double time, double dt, double acc0,
+                                    const double acc_d[3], const double jerk_d[3],
+                                    const double acc_s[3], const double jerk_s[3]) {
+    double dt_next = 0.5 * P.dt_tree;
+    const double dt_1 = dmin(calc_dt2(P.eta_0, P.alpha2, acc0, acc_d, jerk_d),
+                             calc_dt2(P.eta_sun0, P.alpha2, 0., acc_s, jerk_s));
+    double rem = fmod(time, dt_next);
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(time, dt_next); }
+    if (rem != 0.0) return P.dt_min;
+    if (dt > 0.)
+        for (int it = 0; 2. * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
+    for (int it = 0; dt_1 < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
+    if (dt_next < 2. * P.dt_min) dt_next = P.dt_min;
+    return dt_next;
+}
+
+// FPHard::calcDeltat, src/particle.h:1276-1314.
+__device__ inline double dt_block(const HP &P, double time, double dt, double acc0,
+                                  const double acc_d[3], const double jerk_d[3], const double a2_d[3],
+                                  const double a3_d[3], const double acc_s[3], const double jerk_s[3],
+                                  const double a2_s[3], const double a3_s[3]) {
+    double dt_next = dmin(2. * dt, 0.5 * P.dt_tree);
+    const double dt_1 = dmin(calc_dt4(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
+                             calc_dt4(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
+    double rem = fmod(time, dt_next);
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(time, dt_next); }
+    if (rem != 0.0) return P.dt_min;
+    for (int it = 0; dt_1 < dt_next && 0.5 * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
+    if (dt_next < 2. * P.dt_min) dt_next = P.dt_min;
+    return dt_next;
+}
+
+// ------------------------------------------------------------- hard pair
+
+// gravity_hard.h:219-255: acc_d, jerk_d (and phi_d) of i from j.
+__device__ inline void hard_pair(const HP &P, const double xi[3], const double vi[3], double rinv_i,
+                                 double xj0, double xj1, double xj2, double vj0, double vj1, double vj2,
+                                 double mj, double rinv_j, double acc[3], double jerk[3], double *phi) {
+    const double dr[3] = {xj0 - xi[0], xj1 - xi[1], xj2 - xi[2]};
+    double dr2 = dot3(dr[0], dr[1], dr[2], dr[0], dr[1], dr[2]);
+    dr2 += P.eps2;
+    const double dv[3] = {vj0 - vi[0], vj1 - vi[1], vj2 - vi[2]};
+    const double rij = sqrt(dr2);
+    const double rinv = 1. / rij;
+    const double r2inv = rinv * rinv;
+    const double r3inv = r2inv * rinv;
+    const double r_out_inv = dmin(rinv_i, rinv_j);
+    const double mj_rij3 = mj * r3inv;
+    const double alpha = dot3(dr[0], dr[1], dr[2], dv[0], dv[1], dv[2]) * r2inv;
+    const double _K = 1. - cutoff_K(P.cut, rij, r_out_inv);
+    const double dKdt = cutoff_dKdt(P.cut, rij, r_out_inv, alpha);
+    const double alpha_c = alpha * _K;
+    if (phi) {
+        const double _W = 1. - cutoff_W(P.cut, rij, r_out_inv);
+        *phi -= mj * rinv * _W;
+    }
+    const double c2 = 3. * alpha_c + dKdt;
+    for (int k = 0; k < 3; k++) {
+        acc[k] += mj_rij3 * _K * dr[k];
+        jerk[k] += mj_rij3 * (_K * dv[k] - c2 * dr[k]);
+    }
+}
+
+// ---------------------------------------------------------- cluster state
+
+struct HardView {          // SoA arrays of the gathered cluster particles
+    double *pos, *vel, *acc_s, *jerk_s, *acc_d, *jerk_d, *xp, *vp, *a0_s, *j0_s, *a0_d, *j0_d;
+    double *a2_s, *a3_s, *a2_d, *a3_d;                       // 3 per particle
+    double *mass, *rinv, *time, *dt, *acc0, *r_planet, *f, *phi_d, *phi_s;
+    int64_t *id;
+    int32_t *flags, *twin_of;
+};
+
+__device__ inline void ld3(const double *a, int i, double v[3]) {
+    v[0] = a[3 * i]; v[1] = a[3 * i + 1]; v[2] = a[3 * i + 2];
+}
+__device__ inline void st3(double *a, int i, const double v[3]) {
+    a[3 * i] = v[0]; a[3 * i + 1] = v[1]; a[3 * i + 2] = v[2];
+}
+
+// j-data accessor: LDS when the cluster fits, else global memory.
+struct JData {
+    const double *x, *v, *m, *ri;
+    const int64_t *id;
+    int stride;            // 3 for interleaved global arrays
+};
+
+__device__ inline void gravity_loop(const HP &P, const JData &J, int n, int64_t idi,
+                                    const double xi[3], const double vi[3], double rinv_i,
+                                    double acc[3], double jerk[3], double *phi) {
+    acc[0] = acc[1] = acc[2] = 0.;
+    jerk[0] = jerk[1] = jerk[2] = 0.;
+    for (int j = 0; j < n; j++) {
+        if (J.id[j] == idi) continue;
+        hard_pair(P, xi, vi, rinv_i, J.x[3 * j], J.x[3 * j + 1], J.x[3 * j + 2], J.v[3 * j],
+                  J.v[3 * j + 1], J.v[3 * j + 2], J.m[j], J.ri[j], acc, jerk, phi);
+    }
+}
+
+// calcGravity (full, with phi_d and Sun phi), gravity_hard.h:203-256, current pos/vel.
+__device__ void gravity_full(const HP &P, HardView &H, int b, int n, int i) {
+    double x[3], v[3], as[3], js[3], ad[3], jd[3], phs, phd = 0.;
+    ld3(H.pos, b + i, x);
+    ld3(H.vel, b + i, v);
+    star_acc_jerk(P, x, v, as, js, &phs);
+    ad[0] = ad[1] = ad[2] = jd[0] = jd[1] = jd[2] = 0.;
+    const int64_t idi = H.id[b + i];
+    const double ri = H.rinv[b + i];
+    for (int j = 0; j < n; j++) {
+        if (H.id[b + j] == idi) continue;
+        hard_pair(P, x, v, ri, H.pos[3 * (b + j)], H.pos[3 * (b + j) + 1], H.pos[3 * (b + j) + 2],
+                  H.vel[3 * (b + j)], H.vel[3 * (b + j) + 1], H.vel[3 * (b + j) + 2], H.mass[b + j],
+                  H.rinv[b + j], ad, jd, &phd);
+    }
+    st3(H.acc_s, b + i, as);
+    st3(H.jerk_s, b + i, js);
+    st3(H.acc_d, b + i, ad);
+    st3(H.jerk_d, b + i, jd);
+    H.phi_s[b + i] = phs;
+    H.phi_d[b + i] = phd;
+}
+
+// FPHard::predict, src/particle.h:1183-1207.
+__device__ inline void predict(HardView &H, int g, double Dt) {
+    for (int k = 0; k < 3; k++) {
+        const double pos = H.pos[3 * g + k], vel = H.vel[3 * g + k];
+        const double as = H.acc_s[3 * g + k], ad = H.acc_d[3 * g + k];
+        const double js = H.jerk_s[3 * g + k], jd = H.jerk_d[3 * g + k];
+        H.a0_s[3 * g + k] = as;
+        H.j0_s[3 * g + k] = js;
+        H.j0_d[3 * g + k] = jd;
+        H.a0_d[3 * g + k] = ad;
+        const double acc_sd = as + ad;
+        const double jerk_sd = js + jd;
+        H.xp[3 * g + k] = pos + Dt * (vel + Dt * (SECONDORDER * acc_sd + THIRDORDER * Dt * jerk_sd));
+        H.vp[3 * g + k] = vel + Dt * (acc_sd + Dt * SECONDORDER * jerk_sd);
+    }
+}
+
+// FPHard::correct, src/particle.h:1208-1274 (quirk Q1).
+__device__ inline void correct(HardView &H, int g, double Dt) {
+    const double Dt2 = Dt * Dt;
+    const double Dt_inv = 1. / Dt;
+    const double Dt_inv2 = Dt_inv * Dt_inv;
+    const double Dt_inv3 = Dt_inv2 * Dt_inv;
+    for (int k = 0; k < 3; k++) {
+        const int q = 3 * g + k;
+        const double Am_s = H.acc_s[q] - H.a0_s[q];
+        const double J0_s = Dt * H.j0_s[q];
+        const double J1_s = Dt * H.jerk_s[q];
+        const double A2_s = 3. * Am_s - (J1_s + 2. * J0_s);
+        const double A3_s = -2. * Am_s + (J1_s + J0_s);
+        H.a2_s[q] = 2. * Dt_inv2 * (A2_s + 3. * A3_s);
+        H.a3_s[q] = 6. * Dt_inv3 * A3_s;
+        const double Am_d = H.acc_d[q] - H.a0_d[q];
+        const double J0_d = H.j0_d[q] * Dt;
+        const double J1_d = H.jerk_d[q] * Dt;
+        const double A2_d = 3. * Am_d - (J1_d + 2. * J0_d);
+        const double A3_d = -2. * Am_d + (J1_d + J0_d);
+        H.a2_d[q] = 2. * (A2_d + 3. * A3_d) * Dt_inv2;
+        H.a3_d[q] = 6. * A3_d * Dt_inv3;
+        const double A2_sd = A2_s + A2_d;
+        const double A3_sd = A3_s + A3_d;
+        H.pos[q] = H.xp[q] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+        H.vel[q] = H.vp[q] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
+    }
+}
+
+// gravity_p / gravity_c of particle g (cluster-local i) against the predicted j-data.
+__device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int n, int g,
+                                  bool corrected) {
+    double x[3], v[3], as[3], js[3], ad[3], jd[3];
+    if (corrected) { ld3(H.pos, g, x); ld3(H.vel, g, v); }
+    else { ld3(H.xp, g, x); ld3(H.vp, g, v); }
+    star_acc_jerk(P, x, v, as, js, nullptr);
+    gravity_loop(P, J, n, H.id[g], x, v, H.rinv[g], ad, jd, nullptr);
+    st3(H.acc_s, g, as);
+    st3(H.jerk_s, g, js);
+    st3(H.acc_d, g, ad);
+    st3(H.jerk_d, g, jd);
+}
+
+// mergeAccJerk (hermite.h:36-74) for target t.  A twin sits at the target's
+// position with the target's id, so its own freshly computed acc_d/jerk_d equal the
+// target's bit for bit; the mass-weighted mean is formed from the target's raw
+// values in twin index order.
+__device__ inline void merge_acc_jerk(HardView &H, int b, int n, int t) {
+    double acc[3], jerk[3], at[3], jt[3], mass = H.mass[b + t];
+    ld3(H.acc_d, b + t, at);
+    ld3(H.jerk_d, b + t, jt);
+    for (int k = 0; k < 3; k++) {
+        acc[k] = H.mass[b + t] * at[k];
+        jerk[k] = H.mass[b + t] * jt[k];
+    }
+    for (int j = 0; j < n; j++) {
+        if (H.twin_of[b + j] != t) continue;
+        for (int k = 0; k < 3; k++) {
+            acc[k] += H.mass[b + j] * at[k];
+            jerk[k] += H.mass[b + j] * jt[k];
+        }
+        mass += H.mass[b + j];
+    }
+    const double minv = 1. / mass;
+    for (int k = 0; k < 3; k++) { acc[k] *= minv; jerk[k] *= minv; }
+    st3(H.acc_d, b + t, acc);
+    st3(H.jerk_d, b + t, jerk);
+    for (int j = 0; j < n; j++) {
+        if (H.twin_of[b + j] != t) continue;
+        st3(H.acc_d, b + j, acc);
+        st3(H.jerk_d, b + j, jerk);
+    }
+}
+
+// copy the integrated state of target t onto its twins (identical inputs give
+// identical outputs, so this equals integrating them separately)
+__device__ inline void sync_twins(HardView &H, int b, int n, int t) {
+    for (int j = 0; j < n; j++) {
+        if (H.twin_of[b + j] != t) continue;
+        const int s = b + t, d = b + j;
+        for (int k = 0; k < 3; k++) {
+            H.pos[3 * d + k] = H.pos[3 * s + k];
+            H.vel[3 * d + k] = H.vel[3 * s + k];
+            H.acc_s[3 * d + k] = H.acc_s[3 * s + k];
+            H.jerk_s[3 * d + k] = H.jerk_s[3 * s + k];
+            H.acc_d[3 * d + k] = H.acc_d[3 * s + k];
+            H.jerk_d[3 * d + k] = H.jerk_d[3 * s + k];
+            H.a2_s[3 * d + k] = H.a2_s[3 * s + k];
+            H.a3_s[3 * d + k] = H.a3_s[3 * s + k];
+            H.a2_d[3 * d + k] = H.a2_d[3 * s + k];
+            H.a3_d[3 * d + k] = H.a3_d[3 * s + k];
+        }
+        H.time[d] = H.time[s];
+    }
+}
+
+// Perfect merging (collisionA.h:91-103, 155-356, 358-669; hermite.h:413-423), run
+// by one thread.  Returns the dissipated energy.
+__device__ double perfect_merge(const HP &P, HardView &H, int b, int n, int ic, int jc) {
+    const int I = b + ic, T = b + jc;
+    double pos_imp[3], pos_tar[3], vel_imp[3], vel_tar[3];
+    ld3(H.pos, I, pos_imp); ld3(H.pos, T, pos_tar);
+    ld3(H.vel, I, vel_imp); ld3(H.vel, T, vel_tar);
+    double mass_imp = H.mass[I], mass_tar = H.mass[T];
+    if (H.flags[I] & GPLUM_FLAG_MERGED)
+        for (int a = 0; a < n; a++) if (H.twin_of[b + a] == ic) mass_imp += H.mass[b + a];
+    if (H.flags[T] & GPLUM_FLAG_MERGED)
+        for (int a = 0; a < n; a++) if (H.twin_of[b + a] == jc) mass_tar += H.mass[b + a];
+    double vel_g[3], pos_new[3], vel_new[3];
+    for (int k = 0; k < 3; k++) {
+        vel_g[k] = (mass_imp * vel_imp[k] + mass_tar * vel_tar[k]) / (mass_imp + mass_tar);
+        pos_new[k] = (mass_imp * pos_imp[k] + mass_tar * pos_tar[k]) / (mass_imp + mass_tar);
+        vel_new[k] = vel_g[k];
+    }
+    const double mass_rem = mass_imp + mass_tar;
+    H.r_planet[I] = 0.;
+    H.r_planet[T] *= pow(mass_rem / mass_tar, 1. / 3.);
+    const int64_t tar_id = H.id[T];
+    H.id[I] = tar_id;
+    H.flags[I] |= GPLUM_FLAG_DEAD;
+    for (int a = 0; a < n; a++)
+        if (H.twin_of[b + a] == ic) { H.id[b + a] = tar_id; H.flags[b + a] |= GPLUM_FLAG_DEAD; }
+    H.flags[T] |= GPLUM_FLAG_MERGED;
+    for (int a = 0; a < n; a++) {
+        if (a == ic || a == jc || H.twin_of[b + a] == ic || H.twin_of[b + a] == jc) {
+            st3(H.pos, b + a, pos_new);
+            st3(H.vel, b + a, vel_new);
+        }
+    }
+    // calcEnergyDissipation (n_frag = 0)
+    double e_kin = 0., s, vr[3];
+    for (int k = 0; k < 3; k++) vr[k] = vel_imp[k] - vel_tar[k];
+    s = 0.5 * mass_imp * mass_tar / (mass_imp + mass_tar);
+    e_kin -= (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
+    for (int k = 0; k < 3; k++) vr[k] = vel_new[k] - vel_g[k];
+    s = 0.5 * (mass_imp - 0.);
+    e_kin += (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
+    s = 0.5 * mass_tar;
+    e_kin += (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
+    double e_int = 0.;
+    const double dr[3] = {pos_imp[0] - pos_tar[0], pos_imp[1] - pos_tar[1], pos_imp[2] - pos_tar[2]};
+    const double dr2 = dot3(dr[0], dr[1], dr[2], dr[0], dr[1], dr[2]) + P.eps2;
+    const double rinv = sqrt(1. / dr2);
+    double dphi = H.mass[I] * rinv;
+    e_int += H.mass[T] * dphi;
+    for (int a = 0; a < n; a++) if (H.twin_of[b + a] == jc) e_int += H.mass[b + a] * dphi;
+    for (int i2 = 0; i2 < n; i2++) {
+        if (H.twin_of[b + i2] != ic) continue;
+        dphi = H.mass[b + i2] * rinv;
+        e_int += H.mass[T] * dphi;
+        for (int a = 0; a < n; a++) if (H.twin_of[b + a] == jc) e_int += H.mass[b + a] * dphi;
+    }
+    double e_sun = 0.;
+    e_sun += mass_imp / sqrt(dot3(pos_imp[0], pos_imp[1], pos_imp[2], pos_imp[0], pos_imp[1], pos_imp[2]) + P.eps2) +
+             mass_tar / sqrt(dot3(pos_tar[0], pos_tar[1], pos_tar[2], pos_tar[0], pos_tar[1], pos_tar[2]) + P.eps2);
+    double d2 = dot3(H.pos[3 * T], H.pos[3 * T + 1], H.pos[3 * T + 2], H.pos[3 * T], H.pos[3 * T + 1],
+                     H.pos[3 * T + 2]) + P.eps2;
+    e_sun -= mass_tar * sqrt(1. / d2);
+    d2 = dot3(H.pos[3 * I], H.pos[3 * I + 1], H.pos[3 * I + 2], H.pos[3 * I], H.pos[3 * I + 1],
+              H.pos[3 * I + 2]) + P.eps2;
+    e_sun -= (mass_imp - 0.) * sqrt(1. / d2);
+    e_sun *= P.m_sun;
+    // merge_list update
+    H.twin_of[I] = jc;
+    if (H.flags[I] & GPLUM_FLAG_MERGED) {
+        for (int a = 0; a < n; a++) if (H.twin_of[b + a] == ic) H.twin_of[b + a] = jc;
+        H.flags[I] &= ~GPLUM_FLAG_MERGED;
+    }
+    return e_kin + e_int + e_sun;
+}
+
+// --------------------------------------------------------------- kernel
+
+struct Reduce {              // block-level reductions through LDS
+    double d[HERMITE_BLOCK];
+    int i[HERMITE_BLOCK];
+    int j[HERMITE_BLOCK];
+};
+
+__device__ inline double block_min(Reduce &R, double v) {
+    const int t = threadIdx.x;
+    __syncthreads();
+    R.d[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w && R.d[t + w] < R.d[t]) R.d[t] = R.d[t + w];
+        __syncthreads();
+    }
+    const double r = R.d[0];
+    __syncthreads();
+    return r;
+}
+
+// argmin of (R, i) lexicographic, j carried along; returns R, sets *bi, *bj
+__device__ inline double block_argmin(Reduce &R, double v, int i, int j, int *bi, int *bj) {
+    const int t = threadIdx.x;
+    __syncthreads();
+    R.d[t] = v; R.i[t] = i; R.j[t] = j;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            const double a = R.d[t], c = R.d[t + w];
+            if (c < a || (c == a && R.i[t + w] < R.i[t])) {
+                R.d[t] = c; R.i[t] = R.i[t + w]; R.j[t] = R.j[t + w];
+            }
+        }
+        __syncthreads();
+    }
+    const double r = R.d[0];
+    *bi = R.i[0];
+    *bj = R.j[0];
+    __syncthreads();
+    return r;
+}
+
+template <bool LDSJ>
+__global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
+                                                           const int32_t *__restrict__ cl_off,
+                                                           const int32_t *__restrict__ order, int ncl,
+                                                           int *__restrict__ next, double *__restrict__ edisp,
+                                                           int *__restrict__ ncol, long long *__restrict__ nblock,
+                                                           int persistent) {
+    __shared__ double s_x[3 * HERMITE_LDS], s_v[3 * HERMITE_LDS], s_m[HERMITE_LDS], s_ri[HERMITE_LDS];
+    __shared__ int64_t s_id[HERMITE_LDS];
+    __shared__ Reduce R;
+    __shared__ int s_cl;
+    const int t = threadIdx.x;
+    const int B = blockDim.x;
+    for (int iter = 0;; iter++) {
+        if (persistent) {
+            if (t == 0) s_cl = atomicAdd(next, 1);
+            __syncthreads();
+        } else {
+            if (t == 0) s_cl = (iter == 0) ? (int)blockIdx.x : ncl;
+            __syncthreads();
+        }
+        const int q = s_cl;
+        __syncthreads();
+        if (q >= ncl) break;
+        const int c = order ? order[q] : q;
+        const int b = cl_off[c];
+        const int n = cl_off[c + 1] - b;
+        if (LDSJ && n > HERMITE_LDS) continue;   // handled by the global-memory variant
+        if (!LDSJ && n <= HERMITE_LDS) continue;
+        JData J;
+        if (LDSJ) {
+            J = JData{s_x, s_v, s_m, s_ri, s_id, 3};
+            for (int i = t; i < n; i += B) { s_m[i] = H.mass[b + i]; s_ri[i] = H.rinv[b + i]; }
+        } else {
+            J = JData{H.xp + 3 * b, H.vp + 3 * b, H.mass + b, H.rinv + b, H.id + b, 3};
+        }
+        // initial calcGravity for all, flags cleared (hermite.h:259-270)
+        for (int i = t; i < n; i += B) {
+            gravity_full(P, H, b, n, i);
+            H.flags[b + i] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+            H.twin_of[b + i] = -1;
+        }
+        __syncthreads();
+        double ed = 0.;
+        int n_col = 0;
+        long long loop = 0;
+        double sys_time = 0.;
+        const double time_end = P.dt_tree;
+        while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
+            // makeActiveList: t_s = min(time + dt) over live particles
+            double tl = 1.e10;
+            for (int i = t; i < n; i += B)
+                if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tl = dmin(tl, H.time[b + i] + H.dt[b + i]);
+            const double t_s = block_min(R, tl);
+            // predict all (hermite.h:277-283)
+            for (int i = t; i < n; i += B) {
+                predict(H, b + i, t_s - H.time[b + i]);
+                if (LDSJ) {
+                    for (int k = 0; k < 3; k++) {
+                        s_x[3 * i + k] = H.xp[3 * (b + i) + k];
+                        s_v[3 * i + k] = H.vp[3 * (b + i) + k];
+                    }
+                    s_id[i] = H.id[b + i];
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            // collisionDetermination (hermite.h:98-196)
+            int flag_col = 0, ic = -1, jc = -1;
+            if (P.collision) {
+                double rmin = 1.;
+                int bi = 0x7fffffff, bj = -1;
+                for (int i = t; i < n; i += B) {
+                    if (H.flags[b + i] & GPLUM_FLAG_DEAD) continue;
+                    for (int j = 0; j < n; j++) {
+                        if (i == j || (H.flags[b + j] & GPLUM_FLAG_DEAD)) continue;
+                        const double dx = H.xp[3 * (b + i)] - H.xp[3 * (b + j)];
+                        const double dy = H.xp[3 * (b + i) + 1] - H.xp[3 * (b + j) + 1];
+                        const double dz = H.xp[3 * (b + i) + 2] - H.xp[3 * (b + j) + 2];
+                        const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                        const double r2 = H.f[b + i] * H.r_planet[b + i] + H.f[b + j] * H.r_planet[b + j];
+                        const double Rr = r1 / r2;
+                        if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
+                    }
+                }
+                int gi, gj;
+                const double rm = block_argmin(R, rmin, bi, bj, &gi, &gj);
+                if (rm < 1.) {
+                    flag_col = 1;
+                    if (H.mass[b + gi] < H.mass[b + gj]) { ic = gi; jc = gj; } else { ic = gj; jc = gi; }
+                }
+                if (flag_col)
+                    for (int i = t; i < n; i += B) H.dt[b + i] = t_s - H.time[b + i];
+                __threadfence_block();
+                __syncthreads();
+            }
+            // correct the active particles (hermite.h:300-375)
+            for (int i = t; i < n; i += B) {
+                const int g = b + i;
+                if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
+                if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
+                const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
+                gravity_pc(P, H, J, n, g, false);
+                if (merged) merge_acc_jerk(H, b, n, i);
+                const double t_c = t_s - H.time[g];
+                for (int ite = 0; ite < 2; ite++) {
+                    correct(H, g, t_c);
+                    gravity_pc(P, H, J, n, g, true);
+                    if (merged) merge_acc_jerk(H, b, n, i);
+                }
+                H.time[g] += t_c;
+                if (!flag_col) {
+                    double ad[3], jd[3], a2d[3], a3d[3], as[3], js[3], a2s[3], a3s[3];
+                    ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.a2_d, g, a2d); ld3(H.a3_d, g, a3d);
+                    ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js); ld3(H.a2_s, g, a2s); ld3(H.a3_s, g, a3s);
+                    H.dt[g] = dt_block(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
+                }
+                if (merged) {
+                    sync_twins(H, b, n, i);
+                    for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) H.dt[b + j] = H.dt[g];
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            if (flag_col) {
+                if (t == 0) ed += perfect_merge(P, H, b, n, ic, jc);
+                n_col++;
+                __threadfence_block();
+                __syncthreads();
+                for (int i = t; i < n; i += B) gravity_full(P, H, b, n, i);
+                __threadfence_block();
+                __syncthreads();
+                for (int i = t; i < n; i += B)
+                    if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
+                __threadfence_block();
+                __syncthreads();
+                for (int i = t; i < n; i += B) {
+                    const int g = b + i;
+                    if (!(H.flags[g] & GPLUM_FLAG_DEAD)) {
+                        double ad[3], jd[3], as[3], js[3];
+                        ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js);
+                        H.dt[g] = dt_initial(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, as, js);
+                    }
+                }
+                __threadfence_block();
+                __syncthreads();
+                for (int i = t; i < n; i += B)
+                    if (H.twin_of[b + i] >= 0) H.dt[b + i] = H.dt[b + H.twin_of[b + i]];
+                __threadfence_block();
+                __syncthreads();
+            }
+            // getSystemTime
+            double tm = 1.e10;
+            for (int i = t; i < n; i += B)
+                if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tm = dmin(tm, H.time[b + i]);
+            sys_time = block_min(R, tm);
+            loop++;
+        }
+        for (int i = t; i < n; i += B) gravity_full(P, H, b, n, i);
+        __threadfence_block();
+        __syncthreads();
+        if (n_col > 0) {
+            for (int i = t; i < n; i += B)
+                if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
+        }
+        if (t == 0) {
+            edisp[c] = ed;
+            ncol[c] = n_col;
+            nblock[c] = (sys_time < time_end) ? -1 : loop;   // -1: block-step cap reached
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- isolated
+
+// timeIntegrateKepler_isolated (hermite.h:787-815) / timeIntegrate_isolated
+// (hermite.h:734-785) chosen as in hard.h:803-811.
+struct IsoView {
+    double *pos, *vel, *acc_d, *jerk_d, *acc_s, *jerk_s, *phi_d, *phi_s, *time, *dt, *acc0;
+    const int32_t *number;
+    int32_t *n_cluster;
+};
+
+__device__ double solve_kepler(double l, double ecc) {
+    const double ecc2 = ecc * ecc, ecc3 = ecc2 * ecc, ecc4 = ecc2 * ecc2, ecc5 = ecc3 * ecc2, ecc6 = ecc3 * ecc3;
+    double u = l + (ecc - ecc3 / 8. + ecc5 / 192.) * sin(l) + (ecc2 / 2. - ecc4 / 6. + ecc6 / 48.) * sin(2. * l) +
+               (3. * ecc3 / 8. - 27. * ecc5 / 128.) * sin(3. * l) + (ecc4 / 3. - 4. * ecc6 / 15.) * sin(4. * l) +
+               125. * ecc5 / 384. * sin(5. * l) + 27. * ecc6 / 80. * sin(6. * l);
+    if (fabs((u - ecc * sin(u)) - l) > 1.e-15) {
+        double u0;
+        int loop = 0;
+        do {
+            u0 = u;
+            const double s = sin(u0), c = cos(u0);
+            u = u0 - ((u0 - ecc * s - l) / (1. - ecc * c));
+            loop++;
+        } while (fabs(u - u0) > 1.e-15 && loop < 10);
+    }
+    return u;
+}
+
+__device__ inline double eccentricity(const HP &P, const double x[3], const double v[3]) {
+    const double r = sqrt(dot3(x[0], x[1], x[2], x[0], x[1], x[2]));
+    const double rv = dot3(x[0], x[1], x[2], v[0], v[1], v[2]);
+    const double ax = 1.0 / (2.0 / r - dot3(v[0], v[1], v[2], v[0], v[1], v[2]) / P.m_sun);
+    const double ecccosu = 1. - r / ax;
+    const double eccsinu2 = rv * rv / (P.m_sun * ax);
+    return sqrt(ecccosu * ecccosu + eccsinu2);
+}
+
+__device__ void kepler_drift(const HP &P, double x[3], double v[3], double dtime) {
+    const double mu = P.m_sun;
+    const double r = sqrt(dot3(x[0], x[1], x[2], x[0], x[1], x[2]));
+    const double rinv = 1. / r;
+    const double v2 = dot3(v[0], v[1], v[2], v[0], v[1], v[2]);
+    const double rv = dot3(x[0], x[1], x[2], v[0], v[1], v[2]);
+    const double ax = 1.0 / (2.0 * rinv - v2 / mu);
+    const double ecccosu = 1. - r / ax;
+    const double eccsinu = rv / sqrt(mu * ax);
+    const double ecc = sqrt(ecccosu * ecccosu + eccsinu * eccsinu);
+    const double nn = sqrt(mu / (ax * ax * ax));
+    double u, cosu, sinu;
+    if (ecc != 0) { u = atan2(eccsinu, ecccosu); cosu = ecccosu / ecc; sinu = eccsinu / ecc; }
+    else { u = 0.; cosu = 1.; sinu = 0.; }
+    const double aninv = sqrt(ax / mu);
+    const double ecc_sq = sqrt(1. - ecc * ecc);
+    double Pv[3], Qv[3];
+    for (int k = 0; k < 3; k++) {
+        Pv[k] = rinv * cosu * x[k] - aninv * sinu * v[k];
+        Qv[k] = (rinv * sinu * x[k] + aninv * (cosu - ecc) * v[k]) / ecc_sq;
+    }
+    double l = u - ecc * sin(u);
+    l += nn * dtime;
+    u = solve_kepler(l, ecc);
+    const double c2 = cos(u), s2 = sin(u);
+    for (int k = 0; k < 3; k++) x[k] = ax * ((c2 - ecc) * Pv[k] + ecc_sq * s2 * Qv[k]);
+    const double ri2 = sqrt(1. / dot3(x[0], x[1], x[2], x[0], x[1], x[2]));
+    for (int k = 0; k < 3; k++) v[k] = ax * ax * nn * ri2 * (-s2 * Pv[k] + ecc_sq * c2 * Qv[k]);
+}
+
+__global__ void k_isolated(HP P, IsoView V, int n, double time_start, double time_end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || V.number[i] != 0) return;
+    double x[3], v[3], as[3], js[3], phs;
+    ld3(V.pos, i, x);
+    ld3(V.vel, i, v);
+    const double zero[3] = {0., 0., 0.};
+    if (eccentricity(P, x, v) < 0.8 && P.eps2_sun == 0.) {
+        kepler_drift(P, x, v, time_end - time_start);
+        V.time[i] += (time_end - time_start);
+        st3(V.pos, i, x);
+        st3(V.vel, i, v);
+        V.phi_d[i] = 0.;
+        st3(V.acc_d, i, zero);
+        st3(V.jerk_d, i, zero);
+        star_acc_jerk(P, x, v, as, js, &phs);
+        st3(V.acc_s, i, as);
+        st3(V.jerk_s, i, js);
+        V.phi_s[i] = phs;
+        V.dt[i] = dt_initial(P, V.time[i], V.dt[i], V.acc0[i], zero, zero, as, js);
+    } else {
+        // Sun-only Hermite in local time (FPHard: time_c = time, time = 0)
+        const double time_c = V.time[i];
+        double time = 0., dt = V.dt[i];
+        star_acc_jerk(P, x, v, as, js, nullptr);
+        double ad[3] = {0., 0., 0.}, jd[3] = {0., 0., 0.};
+        ld3(V.acc_d, i, ad);
+        ld3(V.jerk_d, i, jd);
+        double a2s[3], a3s[3], a2d[3], a3d[3];
+        for (long long it = 0; time < P.dt_tree && it < BLOCK_STEP_CAP; it++) {
+            const double Dt = dt;
+            double xp[3], vp[3], a0s[3], j0s[3], a0d[3], j0d[3];
+            for (int k = 0; k < 3; k++) {
+                a0s[k] = as[k]; j0s[k] = js[k]; a0d[k] = ad[k]; j0d[k] = jd[k];
+                const double acc_sd = as[k] + ad[k], jerk_sd = js[k] + jd[k];
+                xp[k] = x[k] + Dt * (v[k] + Dt * (SECONDORDER * acc_sd + THIRDORDER * Dt * jerk_sd));
+                vp[k] = v[k] + Dt * (acc_sd + Dt * SECONDORDER * jerk_sd);
+                ad[k] = 0.; jd[k] = 0.;
+            }
+            star_acc_jerk(P, xp, vp, as, js, nullptr);
+            for (int ite = 0; ite < 2; ite++) {
+                const double Dt2 = Dt * Dt, Dt_inv = 1. / Dt, Dt_inv2 = Dt_inv * Dt_inv, Dt_inv3 = Dt_inv2 * Dt_inv;
+                for (int k = 0; k < 3; k++) {
+                    const double Am_s = as[k] - a0s[k], J0_s = Dt * j0s[k], J1_s = Dt * js[k];
+                    const double A2_s = 3. * Am_s - (J1_s + 2. * J0_s), A3_s = -2. * Am_s + (J1_s + J0_s);
+                    a2s[k] = 2. * Dt_inv2 * (A2_s + 3. * A3_s);
+                    a3s[k] = 6. * Dt_inv3 * A3_s;
+                    const double Am_d = ad[k] - a0d[k], J0_d = j0d[k] * Dt, J1_d = jd[k] * Dt;
+                    const double A2_d = 3. * Am_d - (J1_d + 2. * J0_d), A3_d = -2. * Am_d + (J1_d + J0_d);
+                    a2d[k] = 2. * (A2_d + 3. * A3_d) * Dt_inv2;
+                    a3d[k] = 6. * A3_d * Dt_inv3;
+                    const double A2_sd = A2_s + A2_d, A3_sd = A3_s + A3_d;
+                    x[k] = xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+                    v[k] = vp[k] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
+                }
+                star_acc_jerk(P, x, v, as, js, nullptr);
+            }
+            time += dt;
+            dt = dt_block(P, time, dt, V.acc0[i], ad, jd, a2d, a3d, as, js, a2s, a3s);
+        }
+        star_acc_jerk(P, x, v, as, js, &phs);
+        st3(V.pos, i, x);
+        st3(V.vel, i, v);
+        st3(V.acc_s, i, as);
+        st3(V.jerk_s, i, js);
+        st3(V.acc_d, i, ad);
+        st3(V.jerk_d, i, jd);
+        V.phi_s[i] = phs;
+        V.phi_d[i] = 0.;
+        V.time[i] = time_c + time;
+        V.dt[i] = dt;
+    }
+    V.n_cluster[i] = 1;
+}
+
+// ----------------------------------------------------------------- host
+
+HP make_hp(const gplum_params &p) {
+    HP h;
+    h.m_sun = p.m_sun; h.eps2 = p.eps2; h.eps2_sun = p.eps2_sun; h.dt_tree = p.dt_tree;
+    h.dt_min = p.dt_min; h.eta = p.eta; h.eta_0 = p.eta_0; h.eta_sun = p.eta_sun;
+    h.eta_sun0 = p.eta_sun0; h.alpha2 = p.alpha2; h.cut = cutoff_of(p); h.collision = p.collision;
+    return h;
+}
+
+int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
+                const int32_t *d_order, bool persistent, int n_cu, hipStream_t st) {
+    if (ncl <= 0) return GPLUM_OK;
+    HP P = make_hp(p);
+    HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
+               hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
+               hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
+               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
+    GPLUM_ENSURE(hb.next, 2);
+    GPLUM_ENSURE(hb.edisp, ncl);
+    GPLUM_ENSURE(hb.ncol, ncl);
+    GPLUM_ENSURE(hb.nblock, ncl);
+    GPLUM_HIP(hipMemsetAsync(hb.next.p, 0, 2 * sizeof(int), st));
+    if (persistent) {
+        const int grid = std::min(ncl, std::max(1, n_cu) * 8);
+        hipLaunchKernelGGL((k_hermite<true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order,
+                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, 1);
+        if (hb.max_size > HERMITE_LDS)
+            hipLaunchKernelGGL((k_hermite<false>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off,
+                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, 1);
+    } else {
+        hipLaunchKernelGGL((k_hermite<true>), dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order,
+                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, 0);
+        if (hb.max_size > HERMITE_LDS)
+            hipLaunchKernelGGL((k_hermite<false>), dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off,
+                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, 0);
+    }
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, const int32_t *d_off,
+                       hipStream_t st) {
+    // one launch integrating cluster c only (the per-cluster-launch comparator)
+    HP P = make_hp(p);
+    HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
+               hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
+               hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
+               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
+    (void)c;
+    hipLaunchKernelGGL((k_hermite<true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
+                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, 0);
+    hipLaunchKernelGGL((k_hermite<false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
+                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, 0);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int isolated_launch(const gplum_params &p, const IsoArgs &a, int n, double t0, double t1, hipStream_t st) {
+    if (n <= 0) return GPLUM_OK;
+    HP P = make_hp(p);
+    IsoView V{a.pos, a.vel, a.acc_d, a.jerk_d, a.acc_s, a.jerk_s, a.phi_d, a.phi_s, a.time, a.dt, a.acc0,
+              a.number, a.n_cluster};
+    hipLaunchKernelGGL(k_isolated, dim3(ceil_div(n, 128)), dim3(128), 0, st, P, V, n, t0, t1);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+}  // namespace gplum

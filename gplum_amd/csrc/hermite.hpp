@@ -1,0 +1,54 @@
+// hermite.hpp -- batched FP64 Hermite (hard part) and isolated-particle drift.
+#pragma once
+#include "engine.hpp"
+
+namespace gplum {
+
+constexpr int HERMITE_BLOCK = 64;    // threads per cluster workgroup
+constexpr int HERMITE_LDS = 256;     // clusters up to this size keep j-data in LDS
+
+// Gathered cluster particles (SoA, vectors interleaved x,y,z).
+struct HardBuffers {
+    DArray<double> pos, vel, acc_s, jerk_s, acc_d, jerk_d, xp, vp, a0_s, j0_s, a0_d, j0_d;
+    DArray<double> a2_s, a3_s, a2_d, a3_d;
+    DArray<double> mass, rinv, time, dt, acc0, r_planet, f, phi_d, phi_s;
+    DArray<int64_t> id;
+    DArray<int32_t> flags, twin_of;
+    DArray<int> next, ncol;
+    DArray<double> edisp;
+    DArray<long long> nblock;
+    int max_size = 0;
+    int ensure(int n) {
+        GPLUM_ENSURE(pos, 3 * n); GPLUM_ENSURE(vel, 3 * n); GPLUM_ENSURE(acc_s, 3 * n);
+        GPLUM_ENSURE(jerk_s, 3 * n); GPLUM_ENSURE(acc_d, 3 * n); GPLUM_ENSURE(jerk_d, 3 * n);
+        GPLUM_ENSURE(xp, 3 * n); GPLUM_ENSURE(vp, 3 * n); GPLUM_ENSURE(a0_s, 3 * n);
+        GPLUM_ENSURE(j0_s, 3 * n); GPLUM_ENSURE(a0_d, 3 * n); GPLUM_ENSURE(j0_d, 3 * n);
+        GPLUM_ENSURE(a2_s, 3 * n); GPLUM_ENSURE(a3_s, 3 * n); GPLUM_ENSURE(a2_d, 3 * n);
+        GPLUM_ENSURE(a3_d, 3 * n);
+        GPLUM_ENSURE(mass, n); GPLUM_ENSURE(rinv, n); GPLUM_ENSURE(time, n); GPLUM_ENSURE(dt, n);
+        GPLUM_ENSURE(acc0, n); GPLUM_ENSURE(r_planet, n); GPLUM_ENSURE(f, n); GPLUM_ENSURE(phi_d, n);
+        GPLUM_ENSURE(phi_s, n); GPLUM_ENSURE(id, n); GPLUM_ENSURE(flags, n); GPLUM_ENSURE(twin_of, n);
+        return GPLUM_OK;
+    }
+};
+
+struct IsoArgs {
+    double *pos, *vel, *acc_d, *jerk_d, *acc_s, *jerk_s, *phi_d, *phi_s, *time, *dt, *acc0;
+    const int32_t *number;
+    int32_t *n_cluster;
+};
+
+// Integrates the ncl clusters whose particles are gathered in hb at offsets
+// d_off[0..ncl] (local time 0 -> dt_tree; caller sets time = 0).  d_order (optional)
+// gives the processing order (largest first).  persistent: one grid pulling
+// clusters from an atomic queue; otherwise one workgroup per cluster.
+int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
+                const int32_t *d_order, bool persistent, int n_cu, hipStream_t st);
+// One launch for cluster c only (the per-cluster-launch comparator).
+int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, const int32_t *d_off,
+                       hipStream_t st);
+// Isolated particles (number == 0): Kepler drift or Sun-only Hermite.
+int isolated_launch(const gplum_params &p, const IsoArgs &a, int n, double t0, double t1,
+                    hipStream_t st);
+
+}  // namespace gplum

@@ -1,0 +1,47 @@
+// neighbor.hpp -- FP64 neighbour finalize, neighbour CSR and hard clusters.
+#pragma once
+#include "engine.hpp"
+
+namespace gplum {
+
+struct Scratch {
+    DArray<unsigned char> tmp;     // hipcub temporary storage
+};
+
+struct NbParams {
+    double eps2, eps2_sun, m_sun, dt_tree, dt_min, R_search2, R_search3, eta_0, eta_sun0, alpha2;
+    Cutoff cut;
+};
+
+// Directed neighbour graph after the FP64 decision: row i lists, in ascending
+// index, the j that i accepted (NeighborList::n_list, src/neighbor.h:247-250).
+struct NbState {
+    DArray<uint64_t> keys_in, keys;
+    DArray<unsigned char> accepted;
+    DArray<int32_t> row_off, number_fp32, nbr_off, nbr_idx;
+    DArray<double> acc_d_new;
+    int n = 0, n_nbr = 0;
+};
+
+// Hard clusters: particles with neighbours sorted by (id_cluster, index)
+// (HardSystem::makeList, src/hard.h:321-438).
+struct ClusterList {
+    DArray<int32_t> parent, cflag, cscan, idx_sel, index, sflag, sscan, off;
+    DArray<uint64_t> ckey, key_sel, key_sorted;
+    DArray<int32_t> size_key, size_key_sorted, order_in, order;
+    int m = 0, ncl = 0, max_size = 0;
+};
+
+NbParams make_nb_params(const gplum_params &p);
+
+// correctForceLong[Initial] over the candidate pairs d_cand[0..ncand) emitted by the
+// FP32 EP-EP kernel; ps.number holds the FP32 count on entry and the FP64 count on
+// exit.  Fills nb (accepted-neighbour CSR).
+int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 *d_cand, int ncand,
+                      bool initial, NbState &nb, Scratch &s, hipStream_t st);
+
+// createNeighborCluster + makeList: id_cluster = min id of the component; clusters
+// of the particles with neighbours, offsets cl.off[0..ncl], order largest first.
+int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch &s, hipStream_t st);
+
+}  // namespace gplum

@@ -1,0 +1,165 @@
+// soft_capi.hip -- PIKG-functor-shaped soft-force entry points (host arrays in/out).
+//
+// Each call uploads the caller's EPI/EPJ/SPJ/Force arrays into per-thread device
+// buffers, runs one kernel over i-chunks of <= 256 particles (all relative to
+// epi[0], as the PIKG kernel is) and downloads the accumulated Force array.  Calls
+// from different host threads use different streams and buffers (FDPS calls the
+// functors from many OpenMP threads, src/gravity_kernel.hpp:8-23).
+#include <vector>
+
+#include "soft_kernels.hpp"
+
+namespace gplum {
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return GPLUM_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes < 4096 ? 4096 : bytes + bytes / 2;
+        GPLUM_HIP(hipMalloc(&p, want));
+        cap = want;
+        return GPLUM_OK;
+    }
+};
+
+struct CallCtx {
+    hipStream_t st = nullptr;
+    DevBuf epi, epj, spj, force, groups, acc, phi;
+    std::vector<SoftGroup> h_groups;
+    int init() {
+        if (st) return GPLUM_OK;
+        int count = 0;
+        GPLUM_HIP(hipGetDeviceCount(&count));
+        if (count <= 0) return fail(GPLUM_E_NODEVICE, "no HIP device");
+        GPLUM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        return GPLUM_OK;
+    }
+};
+
+thread_local CallCtx t_ctx;
+
+int make_groups(CallCtx &c, int ni, int nj, int ns) {
+    c.h_groups.clear();
+    for (int b = 0; b < ni; b += SOFT_BLOCK) {
+        SoftGroup g{};
+        g.i_begin = b;
+        g.ni = (ni - b) < SOFT_BLOCK ? (ni - b) : SOFT_BLOCK;
+        g.j_begin = 0;
+        g.nj = nj;
+        g.jf_begin = 0;
+        g.njf = 0;
+        g.s_begin = 0;
+        g.ns = ns;
+        g.i_ref = 0;
+        c.h_groups.push_back(g);
+    }
+    int rc = c.groups.ensure(sizeof(SoftGroup) * c.h_groups.size());
+    if (rc) return rc;
+    GPLUM_HIP(hipMemcpyAsync(c.groups.p, c.h_groups.data(), sizeof(SoftGroup) * c.h_groups.size(),
+                             hipMemcpyHostToDevice, c.st));
+    return GPLUM_OK;
+}
+
+template <class SP, class SPSRC, int MODE, bool F64>
+int run_call(const gplum_epi *epi, int ni, const void *jp, size_t jsize, int nj,
+             gplum_force *force, float eps2, double *acc64, double *phi64) {
+    if (ni < 0 || nj < 0 || (ni > 0 && (!epi || !force)) || (nj > 0 && !jp))
+        return fail(GPLUM_E_ARG, "null pointer or negative size");
+    if (ni == 0) return GPLUM_OK;
+    CallCtx &c = t_ctx;
+    int rc = c.init();
+    if (rc) return rc;
+    if ((rc = c.epi.ensure(sizeof(gplum_epi) * ni))) return rc;
+    if ((rc = c.epj.ensure(jsize * (size_t)(nj > 0 ? nj : 1)))) return rc;
+    if ((rc = c.force.ensure(sizeof(gplum_force) * ni))) return rc;
+    GPLUM_HIP(hipMemcpyAsync(c.epi.p, epi, sizeof(gplum_epi) * ni, hipMemcpyHostToDevice, c.st));
+    if (nj > 0)
+        GPLUM_HIP(hipMemcpyAsync(c.epj.p, jp, jsize * nj, hipMemcpyHostToDevice, c.st));
+    GPLUM_HIP(hipMemcpyAsync(c.force.p, force, sizeof(gplum_force) * ni, hipMemcpyHostToDevice, c.st));
+    const bool is_ep = (MODE & EP) != 0;
+    if ((rc = make_groups(c, ni, is_ep ? nj : 0, is_ep ? 0 : nj))) return rc;
+    const int ng = (int)c.h_groups.size();
+    SrcEPI is{(const gplum_epi *)c.epi.p};
+    CandBuf nocand{nullptr, nullptr, 0};
+    if (!F64) {
+        SinkForce out{(gplum_force *)c.force.p};
+        if (is_ep) {
+            launch_soft_f32<SrcEPI, SrcEPJ, SrcNone, SinkForce, MODE>(
+                (const SoftGroup *)c.groups.p, ng, is, SrcEPJ{(const gplum_epj *)c.epj.p},
+                SrcEPJ{(const gplum_epj *)c.epj.p}, SrcNone{},
+                out, eps2, nocand, c.st);
+        } else {
+            launch_soft_f32<SrcEPI, SrcNone, SPSRC, SinkForce, MODE>(
+                (const SoftGroup *)c.groups.p, ng, is, SrcNone{}, SrcNone{}, SPSRC{(const SP *)c.epj.p}, out,
+                eps2, nocand, c.st);
+        }
+    } else {
+        if ((rc = c.acc.ensure(sizeof(double) * 3 * ni))) return rc;
+        if ((rc = c.phi.ensure(sizeof(double) * ni))) return rc;
+        GPLUM_HIP(hipMemsetAsync(c.acc.p, 0, sizeof(double) * 3 * ni, c.st));
+        GPLUM_HIP(hipMemsetAsync(c.phi.p, 0, sizeof(double) * ni, c.st));
+        SinkF64 out{(double *)c.acc.p, (double *)c.phi.p, (gplum_force *)c.force.p};
+        if (is_ep) {
+            launch_soft_f64<SrcEPI, SrcEPJ, SrcNone, SinkF64, MODE>(
+                (const SoftGroup *)c.groups.p, ng, is, SrcEPJ{(const gplum_epj *)c.epj.p},
+                SrcEPJ{(const gplum_epj *)c.epj.p}, SrcNone{},
+                out, (double)eps2, c.st);
+        } else {
+            launch_soft_f64<SrcEPI, SrcNone, SPSRC, SinkF64, MODE>(
+                (const SoftGroup *)c.groups.p, ng, is, SrcNone{}, SrcNone{}, SPSRC{(const SP *)c.epj.p}, out,
+                (double)eps2, c.st);
+        }
+        if (acc64)
+            GPLUM_HIP(hipMemcpyAsync(acc64, c.acc.p, sizeof(double) * 3 * ni, hipMemcpyDeviceToHost, c.st));
+        if (phi64)
+            GPLUM_HIP(hipMemcpyAsync(phi64, c.phi.p, sizeof(double) * ni, hipMemcpyDeviceToHost, c.st));
+    }
+    GPLUM_HIP(hipGetLastError());
+    GPLUM_HIP(hipMemcpyAsync(force, c.force.p, sizeof(gplum_force) * ni, hipMemcpyDeviceToHost, c.st));
+    GPLUM_HIP(hipStreamSynchronize(c.st));
+    return GPLUM_OK;
+}
+
+}  // namespace
+}  // namespace gplum
+
+using namespace gplum;
+
+extern "C" {
+
+int gplum_calc_force_epep(const gplum_epi *epi, int ni, const gplum_epj *epj, int nj,
+                          gplum_force *force, float eps2) {
+    return run_call<gplum_spj_quad, SrcSPQ, EP, false>(epi, ni, epj, sizeof(gplum_epj), nj, force,
+                                                       eps2, nullptr, nullptr);
+}
+
+int gplum_calc_force_epsp_quad(const gplum_epi *epi, int ni, const gplum_spj_quad *spj, int nj,
+                               gplum_force *force, float eps2) {
+    return run_call<gplum_spj_quad, SrcSPQ, SPQ, false>(epi, ni, spj, sizeof(gplum_spj_quad), nj,
+                                                        force, eps2, nullptr, nullptr);
+}
+
+int gplum_calc_force_epsp_mono(const gplum_epi *epi, int ni, const gplum_spj_mono *spj, int nj,
+                               gplum_force *force, float eps2) {
+    return run_call<gplum_spj_mono, SrcSPM, SPM, false>(epi, ni, spj, sizeof(gplum_spj_mono), nj,
+                                                        force, eps2, nullptr, nullptr);
+}
+
+int gplum_calc_force_epep_f64(const gplum_epi *epi, int ni, const gplum_epj *epj, int nj,
+                              gplum_force *force, float eps2, double *acc64, double *phi64) {
+    return run_call<gplum_spj_quad, SrcSPQ, EP, true>(epi, ni, epj, sizeof(gplum_epj), nj, force,
+                                                      eps2, acc64, phi64);
+}
+
+int gplum_calc_force_epsp_quad_f64(const gplum_epi *epi, int ni, const gplum_spj_quad *spj, int nj,
+                                   gplum_force *force, float eps2, double *acc64, double *phi64) {
+    return run_call<gplum_spj_quad, SrcSPQ, SPQ, true>(epi, ni, spj, sizeof(gplum_spj_quad), nj,
+                                                       force, eps2, acc64, phi64);
+}
+
+}  // extern "C"

@@ -1,0 +1,777 @@
+// system.hip -- the device-resident P3T system and its C ABI (gplum_system_*).
+//
+// One soft step follows main_p3t.cpp:423-699 on one GPU:
+//   kick -> clusters (union-find over the neighbour graph) -> hard part (batched
+//   Hermite on clusters, Kepler drift on isolated particles) -> tree (Morton sort,
+//   particles physically reordered, nodes, moments, groups, walk) -> FP32 EP-EP /
+//   EP-SP with candidate emission -> FP64 neighbour finalize -> kick -> energy ->
+//   (after collisions) merge + full recompute.
+// Radii (setCutoffRadii) and the rare merge bookkeeping run on the host in FP64
+// with the reference's operation order; everything per-particle per-step is a
+// HIP kernel on one stream.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hermite.hpp"
+#include "neighbor.hpp"
+#include "soft_kernels.hpp"
+#include "system.hpp"
+#include "tree.hpp"
+
+namespace gplum {
+
+int ParticleSoA::ensure(int n) {
+    GPLUM_ENSURE(id, n); GPLUM_ENSURE(id_cluster, n);
+    GPLUM_ENSURE(mass, n); GPLUM_ENSURE(r_planet, n); GPLUM_ENSURE(f, n);
+    GPLUM_ENSURE(pos, 3 * n); GPLUM_ENSURE(vel, 3 * n); GPLUM_ENSURE(acc, 3 * n);
+    GPLUM_ENSURE(acc_d, 3 * n); GPLUM_ENSURE(jerk_d, 3 * n); GPLUM_ENSURE(acc_s, 3 * n);
+    GPLUM_ENSURE(jerk_s, 3 * n);
+    GPLUM_ENSURE(phi, n); GPLUM_ENSURE(phi_d, n); GPLUM_ENSURE(phi_s, n); GPLUM_ENSURE(r_out, n);
+    GPLUM_ENSURE(r_out_inv, n); GPLUM_ENSURE(r_search, n); GPLUM_ENSURE(v_disp, n);
+    GPLUM_ENSURE(time, n); GPLUM_ENSURE(dt, n); GPLUM_ENSURE(acc0, n);
+    GPLUM_ENSURE(number, n); GPLUM_ENSURE(rank, n); GPLUM_ENSURE(id_max, n); GPLUM_ENSURE(id_min, n);
+    GPLUM_ENSURE(n_cluster, n); GPLUM_ENSURE(flags, n);
+    return GPLUM_OK;
+}
+
+// ----------------------------------------------------------------- kernels
+
+template <class T, int K>
+__global__ void k_gather(int n, const int32_t *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = perm[i];
+#pragma unroll
+    for (int k = 0; k < K; k++) dst[K * i + k] = src[K * s + k];
+}
+
+template <class T, int K>
+__global__ void k_scatter(int n, const int32_t *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = perm[i];
+#pragma unroll
+    for (int k = 0; k < K; k++) dst[K * d + k] = src[K * i + k];
+}
+
+__global__ void k_jrec(int n, const double *__restrict__ pos, const double *__restrict__ mass,
+                       const double *__restrict__ r_out, const double *__restrict__ r_search, JRec *__restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    JRec q;
+    q.x = pos[3 * i]; q.y = pos[3 * i + 1]; q.z = pos[3 * i + 2]; q.m = mass[i];
+    q.ro = r_out[i]; q.rs = r_search[i]; q.id = i; q.rank = 0;
+    r[i] = q;
+}
+
+__global__ void k_soft_groups(int ng, const int32_t *__restrict__ gstart, const int32_t *__restrict__ gcount,
+                              const int32_t *__restrict__ ep_off, const int32_t *__restrict__ ep_cnt,
+                              const int32_t *__restrict__ epf_off, const int32_t *__restrict__ epf_cnt,
+                              const int32_t *__restrict__ sp_off, const int32_t *__restrict__ sp_cnt,
+                              SoftGroup *__restrict__ out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    SoftGroup s;
+    s.i_begin = gstart[g]; s.ni = gcount[g];
+    s.j_begin = ep_off[g]; s.nj = ep_cnt[g];
+    s.jf_begin = epf_off[g]; s.njf = epf_cnt[g];
+    s.s_begin = sp_off[g]; s.ns = sp_cnt[g];
+    s.i_ref = gstart[g]; s.pad = 0;
+    out[g] = s;
+}
+
+__global__ void k_kick(int n3, double hdt, const double *__restrict__ acc, double *__restrict__ vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    vel[i] += hdt * acc[i];
+}
+
+// Energy::calcEnergy partial sums (energy.h:29-63): per block {ekin, esun, ephi, ephid}
+__global__ void k_energy(int n, const double *__restrict__ mass, const double *__restrict__ vel,
+                         const double *__restrict__ phi, const double *__restrict__ phi_d,
+                         const double *__restrict__ phi_s, double *__restrict__ part) {
+    __shared__ double s[4][256];
+    double e[4] = {0., 0., 0., 0.};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double m = mass[i];
+        e[0] += (m * vel[3 * i]) * vel[3 * i] + (m * vel[3 * i + 1]) * vel[3 * i + 1] +
+                (m * vel[3 * i + 2]) * vel[3 * i + 2];
+        e[1] += m * phi_s[i];
+        e[2] += m * phi[i];
+        e[3] += m * phi_d[i];
+    }
+    for (int k = 0; k < 4; k++) s[k][threadIdx.x] = e[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 4; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[4 * blockIdx.x + k] = s[k][0];
+}
+
+// FPHard(const FPGrav&) for the cluster particles: time_c = time, local time 0
+// (particle.h:1096-1116).
+__global__ void k_hard_gather(int m, const int32_t *__restrict__ idx, HardPtrs H, PartPtrs P,
+                              double *__restrict__ time_c) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = idx[k];
+    for (int c = 0; c < 3; c++) {
+        H.pos[3 * k + c] = P.pos[3 * i + c];
+        H.vel[3 * k + c] = P.vel[3 * i + c];
+        H.acc_s[3 * k + c] = P.acc_s[3 * i + c];
+        H.jerk_s[3 * k + c] = P.jerk_s[3 * i + c];
+        H.acc_d[3 * k + c] = P.acc_d[3 * i + c];
+        H.jerk_d[3 * k + c] = P.jerk_d[3 * i + c];
+    }
+    H.mass[k] = P.mass[i];
+    H.rinv[k] = P.r_out_inv[i];
+    time_c[k] = P.time[i];
+    H.time[k] = 0.;
+    H.dt[k] = P.dt[i];
+    H.acc0[k] = P.acc0[i];
+    H.r_planet[k] = P.r_planet[i];
+    H.f[k] = P.f[i];
+    H.id[k] = P.id[i];
+    H.flags[k] = P.flags[i];
+}
+
+// FP_t(ptcl_multi[i][j]) after resetTime (hard.h:768-787).
+__global__ void k_hard_scatter(int m, const int32_t *__restrict__ idx, const int32_t *__restrict__ off,
+                               int ncl, HardPtrs H, PartPtrs P, const double *__restrict__ time_c) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int i = idx[k];
+    for (int c = 0; c < 3; c++) {
+        P.pos[3 * i + c] = H.pos[3 * k + c];
+        P.vel[3 * i + c] = H.vel[3 * k + c];
+        P.acc_s[3 * i + c] = H.acc_s[3 * k + c];
+        P.jerk_s[3 * i + c] = H.jerk_s[3 * k + c];
+        P.acc_d[3 * i + c] = H.acc_d[3 * k + c];
+        P.jerk_d[3 * i + c] = H.jerk_d[3 * k + c];
+    }
+    P.time[i] = time_c[k] + H.time[k];
+    P.dt[i] = H.dt[k];
+    P.r_planet[i] = H.r_planet[k];
+    P.id[i] = H.id[k];
+    P.flags[i] = H.flags[k];
+    P.phi_d[i] = H.phi_d[k];
+    P.phi_s[i] = H.phi_s[k];
+    // cluster size: binary search of k in off[0..ncl]
+    int lo = 0, hi = ncl;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= k) lo = mid; else hi = mid;
+    }
+    P.n_cluster[i] = off[lo + 1] - off[lo];
+}
+
+// ------------------------------------------------------------ host: radii
+
+// calcRandomVel (non-ISOTROPIC, particle.h:1319-1457) + setROutRSearch (:713-733)
+// in FP64 on the host; bin sums in ascending particle-id order.
+static void host_cutoff_radii(const gplum_params &p, int n, const double *pos, const double *vel,
+                              const double *mass, const int64_t *id, double *r_out, double *r_out_inv,
+                              double *r_search, double *v_disp) {
+    double r_max = 0., r_min = -DBL_MAX;
+    for (int i = 0; i < n; i++) {
+        const double r2 = pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1];
+        r_max = smax(r_max, r2);
+        r_min = smax(r_min, -r2);
+    }
+    r_max = std::sqrt(r_max) * 1.01;
+    r_min = std::sqrt(-r_min) * 0.99;
+    const int NB = 32;
+    const double dr = (r_max - r_min) / NB;
+    const double drinv = 1. / dr;
+    double vd[32];
+    int np[32];
+    for (int b = 0; b < NB; b++) { vd[b] = 0.; np[b] = 0; }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return id[a] < id[b]; });
+    for (int a = 0; a < n; a++) {
+        const int i = ord[a];
+        const double x = pos[3 * i], y = pos[3 * i + 1];
+        const double r2 = x * x + y * y;
+        const double ri = std::sqrt(r2);
+        int j = (int)((ri - r_min) * drinv);
+        if (j == NB) j = NB - 1;
+        const double vk = std::sqrt(p.m_sun / std::sqrt(x * x + y * y));
+        double vkx = -y / ri, vky = x / ri, vkz = 0.;
+        vkx *= vk; vky *= vk; vkz *= vk;
+        const double ux = vel[3 * i] - vkx, uy = vel[3 * i + 1] - vky, uz = vel[3 * i + 2] - vkz;
+        vd[j] += ux * ux + uy * uy + uz * uz;
+        np[j]++;
+    }
+    for (int b = 0; b < NB; b++) vd[b] = (np[b] > 0) ? std::sqrt(vd[b] / np[b]) : 0.;
+    for (int i = 0; i < n; i++) {
+        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        const double r2 = x * x + y * y;
+        double v_dispi = 0., ni = 0.;
+        for (int b = 0; b < NB; b++) {
+            const double ddr = r_min + (b + 0.5) * dr - std::sqrt(r2);
+            const double expr = std::exp(-ddr * ddr * drinv * drinv);
+            v_dispi += vd[b] * expr;
+            ni += (np[b] > 0) ? expr : 0.;
+        }
+        v_disp[i] = v_dispi / ni;
+        // getSemimajorAxis2 / getRHill (particle.h:660-699)
+        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        const double r = std::sqrt(x * x + y * y + z * z);
+        const double rv = x * vx + y * vy + z * vz;
+        const double ax = 1.0 / (2.0 / r - (vx * vx + vy * vy + vz * vz) / p.m_sun);
+        const double ecccosu = 1. - r / ax;
+        const double eccsinu2 = rv * rv / (p.m_sun * ax);
+        const double ecc = std::sqrt(ecccosu * ecccosu + eccsinu2);
+        const double ax2 = (ecc < 0.6) ? ax : std::sqrt(x * x + y * y + z * z);
+        const double rHill = std::pow(mass[i] / (3. * p.m_sun), 1. / 3.) * ax2;
+        const double r_out_i = smax(p.R_cut0 * std::pow(ax2, -p.p_cut) * rHill, p.R_cut1 * v_disp[i] * p.dt_tree);
+        double ro = smax(r_out_i, p.r_cut_min);
+        if (p.r_cut_max > 0.) ro = smin(p.r_cut_max, ro);
+        r_out[i] = ro;
+        r_out_inv[i] = 1. / ro;
+        r_search[i] = p.R_search0 * ro + p.R_search1 * v_disp[i] * p.dt_tree;
+    }
+}
+
+// ------------------------------------------------------------------ System
+
+System::System(const gplum_params &prm) : p(prm) {}
+
+System::~System() {
+    if (st) (void)hipStreamDestroy(st);
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+}
+
+int System::init_device() {
+    if (st) return GPLUM_OK;
+    int count = 0;
+    GPLUM_HIP(hipGetDeviceCount(&count));
+    if (count <= 0) return fail(GPLUM_E_NODEVICE, "no HIP device");
+    hipDeviceProp_t prop;
+    int dev = 0;
+    GPLUM_HIP(hipGetDevice(&dev));
+    GPLUM_HIP(hipGetDeviceProperties(&prop, dev));
+    n_cu = prop.multiProcessorCount;
+    GPLUM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (auto &e : ev) GPLUM_HIP(hipEventCreate(&e));
+    return GPLUM_OK;
+}
+
+PartPtrs System::ptrs(ParticleSoA &s) {
+    return PartPtrs{s.pos.p, s.vel.p, s.acc.p, s.acc_d.p, s.jerk_d.p, s.acc_s.p, s.jerk_s.p, s.mass.p,
+                    s.r_out.p, s.r_out_inv.p, s.r_search.p, s.time.p, s.dt.p, s.acc0.p, s.r_planet.p,
+                    s.f.p, s.phi.p, s.phi_d.p, s.phi_s.p, s.id.p, s.flags.p, s.n_cluster.p, s.number.p};
+}
+
+int System::set_particles(const gplum_particle *pp, int count, double time) {
+    int rc = init_device();
+    if (rc) return rc;
+    n = count;
+    time_sys = time;
+    if ((rc = A.ensure(n)) || (rc = B.ensure(n))) return rc;
+    std::vector<double> v3(3 * (size_t)n), v1(n);
+    std::vector<int64_t> i64(n);
+    std::vector<int32_t> i32(n);
+    auto put3 = [&](DArray<double> &d, size_t off) {
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) v3[3 * i + k] = *(const double *)((const char *)&pp[i] + off + 8 * k);
+        return hipMemcpy(d.p, v3.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    };
+    auto put1 = [&](DArray<double> &d, size_t off) {
+        for (int i = 0; i < n; i++) v1[i] = *(const double *)((const char *)&pp[i] + off);
+        return hipMemcpy(d.p, v1.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    };
+#define P3(f) GPLUM_HIP(put3(A.f, offsetof(gplum_particle, f)))
+#define P1(f) GPLUM_HIP(put1(A.f, offsetof(gplum_particle, f)))
+    P3(pos); P3(vel); P3(acc); P3(acc_d); P3(jerk_d); P3(acc_s); P3(jerk_s);
+    P1(mass); P1(r_planet); P1(f); P1(phi); P1(phi_d); P1(phi_s); P1(r_out); P1(r_out_inv); P1(r_search);
+    P1(v_disp); P1(time); P1(dt); P1(acc0);
+#undef P3
+#undef P1
+    for (int i = 0; i < n; i++) i64[i] = pp[i].id;
+    GPLUM_HIP(hipMemcpy(A.id.p, i64.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    for (int i = 0; i < n; i++) i64[i] = pp[i].id_cluster;
+    GPLUM_HIP(hipMemcpy(A.id_cluster.p, i64.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    auto puti = [&](DArray<int32_t> &d, auto getter) {
+        for (int i = 0; i < n; i++) i32[i] = getter(pp[i]);
+        return hipMemcpy(d.p, i32.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
+    };
+    GPLUM_HIP(puti(A.number, [](const gplum_particle &q) { return q.neighbor.number; }));
+    GPLUM_HIP(puti(A.rank, [](const gplum_particle &q) { return q.neighbor.rank; }));
+    GPLUM_HIP(puti(A.id_max, [](const gplum_particle &q) { return q.neighbor.id_max; }));
+    GPLUM_HIP(puti(A.id_min, [](const gplum_particle &q) { return q.neighbor.id_min; }));
+    GPLUM_HIP(puti(A.n_cluster, [](const gplum_particle &q) { return q.n_cluster; }));
+    GPLUM_HIP(puti(A.flags, [](const gplum_particle &q) { return q.flags; }));
+    nb.n = 0;
+    nb.n_nbr = 0;
+    have_nbr = false;
+    return GPLUM_OK;
+}
+
+int System::get_particles(gplum_particle *pp, int count) {
+    if (count < n) return fail(GPLUM_E_ARG, "output array too small");
+    GPLUM_HIP(hipStreamSynchronize(st));
+    std::vector<double> v3(3 * (size_t)n), v1(n);
+    std::vector<int64_t> i64(n);
+    std::vector<int32_t> i32(n);
+    std::memset(pp, 0, sizeof(gplum_particle) * (size_t)n);
+    auto get3 = [&](DArray<double> &d, size_t off) {
+        hipError_t e = hipMemcpy(v3.data(), d.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) *(double *)((char *)&pp[i] + off + 8 * k) = v3[3 * i + k];
+        return e;
+    };
+    auto get1 = [&](DArray<double> &d, size_t off) {
+        hipError_t e = hipMemcpy(v1.data(), d.p, sizeof(double) * n, hipMemcpyDeviceToHost);
+        for (int i = 0; i < n; i++) *(double *)((char *)&pp[i] + off) = v1[i];
+        return e;
+    };
+#define G3(f) GPLUM_HIP(get3(A.f, offsetof(gplum_particle, f)))
+#define G1(f) GPLUM_HIP(get1(A.f, offsetof(gplum_particle, f)))
+    G3(pos); G3(vel); G3(acc); G3(acc_d); G3(jerk_d); G3(acc_s); G3(jerk_s);
+    G1(mass); G1(r_planet); G1(f); G1(phi); G1(phi_d); G1(phi_s); G1(r_out); G1(r_out_inv); G1(r_search);
+    G1(v_disp); G1(time); G1(dt); G1(acc0);
+#undef G3
+#undef G1
+    GPLUM_HIP(hipMemcpy(i64.data(), A.id.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) pp[i].id = i64[i];
+    GPLUM_HIP(hipMemcpy(i64.data(), A.id_cluster.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) pp[i].id_cluster = i64[i];
+    auto geti = [&](DArray<int32_t> &d, auto setter) {
+        hipError_t e = hipMemcpy(i32.data(), d.p, 4 * (size_t)n, hipMemcpyDeviceToHost);
+        for (int i = 0; i < n; i++) setter(pp[i], i32[i]);
+        return e;
+    };
+    GPLUM_HIP(geti(A.number, [](gplum_particle &q, int v) { q.neighbor.number = v; }));
+    GPLUM_HIP(geti(A.rank, [](gplum_particle &q, int v) { q.neighbor.rank = v; }));
+    GPLUM_HIP(geti(A.id_max, [](gplum_particle &q, int v) { q.neighbor.id_max = v; }));
+    GPLUM_HIP(geti(A.id_min, [](gplum_particle &q, int v) { q.neighbor.id_min = v; }));
+    GPLUM_HIP(geti(A.n_cluster, [](gplum_particle &q, int v) { q.n_cluster = v; }));
+    GPLUM_HIP(geti(A.flags, [](gplum_particle &q, int v) { q.flags = v; }));
+    return GPLUM_OK;
+}
+
+int System::set_radii() {
+    std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n), mass(n), ro(n), roi(n), rs(n), vd(n);
+    std::vector<int64_t> id(n);
+    GPLUM_HIP(hipStreamSynchronize(st));
+    GPLUM_HIP(hipMemcpy(pos.data(), A.pos.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+    GPLUM_HIP(hipMemcpy(vel.data(), A.vel.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
+    GPLUM_HIP(hipMemcpy(mass.data(), A.mass.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    GPLUM_HIP(hipMemcpy(id.data(), A.id.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    host_cutoff_radii(p, n, pos.data(), vel.data(), mass.data(), id.data(), ro.data(), roi.data(), rs.data(),
+                      vd.data());
+    GPLUM_HIP(hipMemcpy(A.r_out.p, ro.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    GPLUM_HIP(hipMemcpy(A.r_out_inv.p, roi.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    GPLUM_HIP(hipMemcpy(A.r_search.p, rs.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    GPLUM_HIP(hipMemcpy(A.v_disp.p, vd.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    return GPLUM_OK;
+}
+
+// Morton sort + physical reorder of every particle array (A <- B gathered by perm).
+int System::reorder() {
+    int rc = tree_sort_keys(tree, A.pos.p, n, st);
+    if (rc) return rc;
+    const int32_t *perm = tree.perm.p;
+    const int gb = ceil_div(n, 256);
+#define GA3(f) hipLaunchKernelGGL((k_gather<double, 3>), dim3(gb), dim3(256), 0, st, n, perm, A.f.p, B.f.p)
+#define GA1(f) hipLaunchKernelGGL((k_gather<double, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.f.p, B.f.p)
+    GA3(pos); GA3(vel); GA3(acc); GA3(acc_d); GA3(jerk_d); GA3(acc_s); GA3(jerk_s);
+    GA1(mass); GA1(r_planet); GA1(f); GA1(phi); GA1(phi_d); GA1(phi_s); GA1(r_out); GA1(r_out_inv);
+    GA1(r_search); GA1(v_disp); GA1(time); GA1(dt); GA1(acc0);
+#undef GA3
+#undef GA1
+    hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.id.p, B.id.p);
+    hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.id_cluster.p, B.id_cluster.p);
+#define GI(f) hipLaunchKernelGGL((k_gather<int32_t, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.f.p, B.f.p)
+    GI(number); GI(rank); GI(id_max); GI(id_min); GI(n_cluster); GI(flags);
+#undef GI
+    GPLUM_HIP(hipGetLastError());
+    A.swap(B);
+    have_nbr = false;   // neighbour indices refer to the old order
+    return GPLUM_OK;
+}
+
+void System::tick(int k) {
+    if (timing) (void)hipEventRecord(ev[k], st);
+}
+
+double System::elapsed(int a, int b) {
+    float ms = 0.f;
+    if (timing && hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess) return ms;
+    return 0.;
+}
+
+// tree + FP32 soft force + FP64 correction (correctForceLong[Initial])
+int System::soft_force(bool initial, bool use_f64) {
+    int rc;
+    tick(EV_TREE0);
+    if ((rc = reorder())) return rc;
+    if ((rc = tree_build_nodes(tree, p, A.pos.p, A.mass.p, A.r_search.p, n, st))) return rc;
+    if ((rc = tree_walk(tree, p, A.pos.p, A.r_search.p, st))) return rc;
+    tick(EV_TREE1);
+    GPLUM_ENSURE(jrec, n);
+    hipLaunchKernelGGL(k_jrec, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.mass.p, A.r_out.p,
+                       A.r_search.p, jrec.p);
+    const int ng = tree.n_group;
+    GPLUM_ENSURE(groups, ng);
+    hipLaunchKernelGGL(k_soft_groups, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, tree.group_start.p,
+                       tree.group_count.p, tree.ep_off.p, tree.ep_cnt.p, tree.epf_off.p, tree.epf_cnt.p,
+                       tree.sp_off.p, tree.sp_cnt.p, groups.p);
+    // candidate buffer: sized by the previous step's count (or a fraction of n)
+    if (cand_cap == 0) cand_cap = std::max<unsigned>(1u << 16, (unsigned)(n / 4));
+    GPLUM_ENSURE(cand_count, 1);
+    SrcRecI is{jrec.p};
+    SrcRecJ jn{jrec.p, tree.ep_list.p};
+    SrcRecJ jf{jrec.p, tree.epf_list.p};
+    SrcNodeSP sp{tree.node_mass.p, tree.node_com.p, tree.node_quad.p, tree.sp_list.p};
+    SinkEngine out{A.acc.p, A.phi.p, A.number.p, A.rank.p, A.id_max.p, A.id_min.p};
+    unsigned ncand = 0;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        GPLUM_ENSURE(cand, cand_cap);
+        GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
+        CandBuf cb{cand.p, cand_count.p, cand_cap};
+        tick(EV_FORCE0);
+        if (!use_f64) {
+            if (p.quadrupole)
+                launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(
+                    groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
+            else
+                launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(
+                    groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
+        } else {
+            if (p.quadrupole)
+                launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ>(groups.p, ng, is, jn, jf, sp,
+                                                                                   out, p.eps2, st);
+            else
+                launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM>(groups.p, ng, is, jn, jf, sp,
+                                                                                   out, p.eps2, st);
+        }
+        tick(EV_FORCE1);
+        GPLUM_HIP(hipGetLastError());
+        GPLUM_HIP(hipMemcpyAsync(&ncand, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        if (use_f64 || ncand <= cand_cap) break;
+        cand_cap = ncand + ncand / 4 + 1024;   // too small: grow and redo the step
+        stats.cand_retries++;
+    }
+    stats.force_ms += elapsed(EV_FORCE0, EV_FORCE1);
+    stats.tree_ms += elapsed(EV_TREE0, EV_TREE1);
+    stats.n_force_calls++;
+    if (use_f64) {
+        // The FP64 twin emits no candidates; the neighbour finalize needs the FP32
+        // candidate set, so run the FP32 EP-EP pass once more for candidates only.
+        GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
+        DArray<double> tmp_acc, tmp_phi;
+        GPLUM_ENSURE(tmp_acc, 3 * (size_t)n);
+        GPLUM_ENSURE(tmp_phi, n);
+        SinkEngine out2{tmp_acc.p, tmp_phi.p, A.number.p, A.rank.p, A.id_max.p, A.id_min.p};
+        GPLUM_ENSURE(cand, cand_cap);
+        CandBuf cb{cand.p, cand_count.p, cand_cap};
+        launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | EMIT>(groups.p, ng, is, jn, jf, sp, out2,
+                                                                            (float)p.eps2, cb, st);
+        GPLUM_HIP(hipMemcpyAsync(&ncand, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        if (ncand > cand_cap) return fail(GPLUM_E_OVERFLOW, "candidate buffer overflow (f64 path)");
+    }
+    last_ncand = ncand;
+    tick(EV_NB0);
+    if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
+    tick(EV_NB1);
+    stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
+    have_nbr = true;
+    // interaction counts (sum over groups of ni*nj, ni*ns) for the roofline
+    std::vector<SoftGroup> hg(ng);
+    GPLUM_HIP(hipMemcpy(hg.data(), groups.p, sizeof(SoftGroup) * ng, hipMemcpyDeviceToHost));
+    long long ep = 0, epf = 0, sp_ = 0;
+    for (auto &g : hg) {
+        ep += (long long)g.ni * g.nj;
+        epf += (long long)g.ni * g.njf;
+        sp_ += (long long)g.ni * g.ns;
+    }
+    last_epep_near = ep;
+    last_epep_far = epf;
+    last_epsp = sp_;
+    last_groups = ng;
+    return GPLUM_OK;
+}
+
+int System::energy(double out[5]) {
+    const int nb_ = 256;
+    GPLUM_ENSURE(eparts, 4 * nb_);
+    hipLaunchKernelGGL(k_energy, dim3(nb_), dim3(256), 0, st, n, A.mass.p, A.vel.p, A.phi.p, A.phi_d.p,
+                       A.phi_s.p, eparts.p);
+    std::vector<double> h(4 * nb_);
+    GPLUM_HIP(hipMemcpyAsync(h.data(), eparts.p, sizeof(double) * 4 * nb_, hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    double e[4] = {0., 0., 0., 0.};
+    for (int b = 0; b < nb_; b++)
+        for (int k = 0; k < 4; k++) e[k] += h[4 * b + k];
+    out[1] = 0.5 * e[0];
+    out[2] = e[1];
+    out[3] = 0.5 * e[2];
+    out[4] = 0.5 * e[3];
+    out[0] = out[1] + out[2] + (out[3] + out[4]);
+    return GPLUM_OK;
+}
+
+// main_p3t.cpp:289-364: time, flags, r_planet/f defaults, radii, two rounds of
+// (tree force + correctForceLongInitial).
+int System::initialize() {
+    int rc = init_device();
+    if (rc) return rc;
+    {
+        std::vector<gplum_particle> h(n);
+        if ((rc = get_particles(h.data(), n))) return rc;
+        for (auto &q : h) {
+            q.time = time_sys;
+            q.neighbor.number = 0;
+            q.n_cluster = 0;
+            q.flags = 0;
+            if (q.r_planet <= 0.) q.r_planet = std::pow(0.75 * q.mass / (3.14159265358979323846 * p.dens), 1. / 3.);
+            if (q.f == 0.) q.f = p.increase_factor;
+        }
+        if ((rc = set_particles(h.data(), n, time_sys))) return rc;
+    }
+    if ((rc = set_radii())) return rc;
+    for (int r = 0; r < 2; r++)
+        if ((rc = soft_force(true, false))) return rc;
+    if ((rc = energy(e_init))) return rc;
+    return GPLUM_OK;
+}
+
+int System::kick() {
+    hipLaunchKernelGGL(k_kick, dim3(ceil_div(3 * n, 256)), dim3(256), 0, st, 3 * n, 0.5 * p.dt_tree, A.acc.p,
+                       A.vel.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+// hard part: clusters + isolated particles (HardSystem::timeIntegrate, hard.h:554-829)
+int System::hard_part(bool persistent) {
+    int rc;
+    if (!have_nbr) return fail(GPLUM_E_STATE, "neighbour graph missing: initialize first");
+    tick(EV_HARD0);
+    if ((rc = make_clusters(A, n, nb, cl, scratch, st))) return rc;
+    const int m = cl.m, ncl = cl.ncl;
+    stats.n_clusters = ncl;
+    stats.n_cluster_particles = m;
+    stats.max_cluster = cl.max_size;
+    step_ncol = 0;
+    step_edisp = 0.;
+    if (ncl > 0) {
+        if ((rc = hb.ensure(m))) return rc;
+        hb.max_size = cl.max_size;
+        GPLUM_ENSURE(time_c, m);
+        HardPtrs H = hard_ptrs();
+        PartPtrs P = ptrs(A);
+        hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
+        if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, persistent, n_cu, st))) return rc;
+        hipLaunchKernelGGL(k_hard_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, cl.off.p, ncl,
+                           H, P, time_c.p);
+        std::vector<int> nc(ncl);
+        std::vector<double> ed(ncl);
+        std::vector<long long> nbk(ncl);
+        GPLUM_HIP(hipMemcpyAsync(nc.data(), hb.ncol.p, sizeof(int) * ncl, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(ed.data(), hb.edisp.p, sizeof(double) * ncl, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(nbk.data(), hb.nblock.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        for (int c = 0; c < ncl; c++) {
+            if (nbk[c] < 0) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
+            step_ncol += nc[c];
+            step_edisp += ed[c];
+            stats.block_steps += nbk[c];
+        }
+    }
+    IsoArgs ia{A.pos.p, A.vel.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p, A.phi_d.p, A.phi_s.p, A.time.p,
+               A.dt.p, A.acc0.p, A.number.p, A.n_cluster.p};
+    if ((rc = isolated_launch(p, ia, n, time_sys, time_sys + p.dt_tree, st))) return rc;
+    tick(EV_HARD1);
+    GPLUM_HIP(hipStreamSynchronize(st));
+    stats.hard_ms += elapsed(EV_HARD0, EV_HARD1);
+    return GPLUM_OK;
+}
+
+HardPtrs System::hard_ptrs() {
+    return HardPtrs{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.mass.p, hb.rinv.p,
+                    hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p, hb.phi_s.p, hb.id.p,
+                    hb.flags.p};
+}
+
+// MergeParticle (func.h:160-207) on the host, then setCutoffRadii + force +
+// correctForceLongInitial (main_p3t.cpp:640-699).
+int System::merge_and_recompute() {
+    std::vector<gplum_particle> h(n);
+    int rc = get_particles(h.data(), n);
+    if (rc) return rc;
+    std::vector<char> rm(n, 0);
+    double ed = 0.;
+    for (int i = 0; i < n; i++) {
+        if (!((h[i].flags & GPLUM_FLAG_MERGED) && !(h[i].flags & GPLUM_FLAG_DEAD))) continue;
+        for (int j = 0; j < n; j++) {
+            if (h[j].id == h[i].id && i != j) {
+                const double mi = h[i].mass, mj = h[j].mass;
+                const double vr[3] = {h[j].vel[0] - h[i].vel[0], h[j].vel[1] - h[i].vel[1], h[j].vel[2] - h[i].vel[2]};
+                h[i].mass += mj;
+                for (int k = 0; k < 3; k++) h[i].vel[k] = (mi * h[i].vel[k] + mj * h[j].vel[k]) / (mi + mj);
+                h[i].phi = (mi * h[i].phi + mj * h[j].phi) / (mi + mj);
+                h[i].phi_d = (mi * h[i].phi_d + mj * h[j].phi_d) / (mi + mj);
+                const double s = 0.5 * mi * mj / (mi + mj);
+                ed -= (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
+                rm[j] = 1;
+            }
+        }
+        h[i].flags &= ~GPLUM_FLAG_MERGED;
+    }
+    std::vector<gplum_particle> k;
+    k.reserve(n);
+    for (int i = 0; i < n; i++) if (!rm[i]) k.push_back(h[i]);
+    edisp += ed;
+    if ((rc = set_particles(k.data(), (int)k.size(), time_sys))) return rc;
+    if ((rc = set_radii())) return rc;
+    return soft_force(true, false);
+}
+
+int System::step(bool persistent) {
+    int rc;
+    if ((rc = kick())) return rc;
+    if ((rc = hard_part(persistent))) return rc;
+    n_col_tot += step_ncol;
+    edisp += step_edisp;
+    if ((rc = soft_force(false, false))) return rc;
+    if ((rc = kick())) return rc;
+    if (step_ncol) {
+        if ((rc = merge_and_recompute())) return rc;
+    }
+    time_sys += p.dt_tree;
+    stats.steps++;
+    return GPLUM_OK;
+}
+
+}  // namespace gplum
+
+// ------------------------------------------------------------------ C ABI
+
+using gplum::System;
+
+extern "C" {
+
+int gplum_system_create(const gplum_params *p, gplum_system **out) {
+    if (!p || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
+    System *s = new System(*p);
+    int rc = s->init_device();
+    if (rc) { delete s; return rc; }
+    *out = reinterpret_cast<gplum_system *>(s);
+    return GPLUM_OK;
+}
+
+int gplum_system_destroy(gplum_system *s) {
+    delete reinterpret_cast<System *>(s);
+    return GPLUM_OK;
+}
+
+int gplum_system_set_params(gplum_system *s, const gplum_params *p) {
+    if (!s || !p) return gplum::fail(GPLUM_E_ARG, "null argument");
+    reinterpret_cast<System *>(s)->p = *p;
+    return GPLUM_OK;
+}
+
+int gplum_system_set_particles(gplum_system *s, const gplum_particle *pp, int n, double time_sys) {
+    if (!s || (n > 0 && !pp) || n < 0) return gplum::fail(GPLUM_E_ARG, "bad particle array");
+    return reinterpret_cast<System *>(s)->set_particles(pp, n, time_sys);
+}
+
+int gplum_system_get_particles(gplum_system *s, gplum_particle *pp, int n) {
+    if (!s || !pp) return gplum::fail(GPLUM_E_ARG, "null argument");
+    return reinterpret_cast<System *>(s)->get_particles(pp, n);
+}
+
+int gplum_system_count(gplum_system *s, int *n) {
+    if (!s || !n) return gplum::fail(GPLUM_E_ARG, "null argument");
+    *n = reinterpret_cast<System *>(s)->n;
+    return GPLUM_OK;
+}
+
+int gplum_system_initialize(gplum_system *s) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    return reinterpret_cast<System *>(s)->initialize();
+}
+
+int gplum_system_soft_force(gplum_system *s, int initial, int use_f64) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    return reinterpret_cast<System *>(s)->soft_force(initial != 0, use_f64 != 0);
+}
+
+int gplum_system_set_radii(gplum_system *s) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    return reinterpret_cast<System *>(s)->set_radii();
+}
+
+int gplum_system_step(gplum_system *s, int nsteps) {
+    if (!s || nsteps < 0) return gplum::fail(GPLUM_E_ARG, "bad arguments");
+    System *S = reinterpret_cast<System *>(s);
+    for (int k = 0; k < nsteps; k++) {
+        int rc = S->step(true);
+        if (rc) return rc;
+    }
+    return GPLUM_OK;
+}
+
+int gplum_system_energy(gplum_system *s, double out[6]) {
+    if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
+    System *S = reinterpret_cast<System *>(s);
+    int rc = S->energy(out);
+    out[5] = S->edisp;
+    return rc;
+}
+
+int gplum_system_neighbors(gplum_system *s, int32_t *offset, int32_t *index, int cap, int *total) {
+    if (!s || !offset || !total) return gplum::fail(GPLUM_E_ARG, "null argument");
+    System *S = reinterpret_cast<System *>(s);
+    if (!S->have_nbr) return gplum::fail(GPLUM_E_STATE, "no neighbour graph for the current order");
+    *total = S->nb.n_nbr;
+    GPLUM_HIP(hipMemcpy(offset, S->nb.nbr_off.p, sizeof(int32_t) * (S->n + 1), hipMemcpyDeviceToHost));
+    if (index && cap >= S->nb.n_nbr && S->nb.n_nbr > 0)
+        GPLUM_HIP(hipMemcpy(index, S->nb.nbr_idx.p, sizeof(int32_t) * S->nb.n_nbr, hipMemcpyDeviceToHost));
+    return GPLUM_OK;
+}
+
+int gplum_system_hard_part(gplum_system *s, int persistent) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    System *S = reinterpret_cast<System *>(s);
+    int rc = S->hard_part(persistent != 0);
+    if (!rc) { S->n_col_tot += S->step_ncol; S->edisp += S->step_edisp; }
+    return rc;
+}
+
+int gplum_system_stats(gplum_system *s, gplum_stats *out) {
+    if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
+    System *S = reinterpret_cast<System *>(s);
+    *out = S->stats;
+    out->n_epep_near = S->last_epep_near;
+    out->n_epep_far = S->last_epep_far;
+    out->n_epsp = S->last_epsp;
+    out->n_groups = S->last_groups;
+    out->n_candidates = S->last_ncand;
+    out->n_neighbors = S->nb.n_nbr;
+    out->n_collisions = S->n_col_tot;
+    out->time_sys = S->time_sys;
+    return GPLUM_OK;
+}
+
+int gplum_system_reset_stats(gplum_system *s, int timing) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    System *S = reinterpret_cast<System *>(s);
+    std::memset(&S->stats, 0, sizeof(S->stats));
+    S->timing = timing != 0;
+    return GPLUM_OK;
+}
+
+}  // extern "C"

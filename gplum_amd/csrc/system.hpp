@@ -1,0 +1,69 @@
+// system.hpp -- the device-resident P3T system (see system.hip).
+#pragma once
+#include "hermite.hpp"
+#include "neighbor.hpp"
+
+namespace gplum {
+
+struct PartPtrs {
+    double *pos, *vel, *acc, *acc_d, *jerk_d, *acc_s, *jerk_s, *mass, *r_out, *r_out_inv, *r_search;
+    double *time, *dt, *acc0, *r_planet, *f, *phi, *phi_d, *phi_s;
+    int64_t *id;
+    int32_t *flags, *n_cluster, *number;
+};
+
+struct HardPtrs {
+    double *pos, *vel, *acc_s, *jerk_s, *acc_d, *jerk_d, *mass, *rinv, *time, *dt, *acc0, *r_planet, *f;
+    double *phi_d, *phi_s;
+    int64_t *id;
+    int32_t *flags;
+};
+
+enum { EV_TREE0, EV_TREE1, EV_FORCE0, EV_FORCE1, EV_NB0, EV_NB1, EV_HARD0, EV_HARD1, EV_COUNT };
+
+struct System {
+    gplum_params p;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[EV_COUNT] = {};
+    int n_cu = 256;
+    int n = 0;
+    double time_sys = 0., edisp = 0., step_edisp = 0.;
+    int n_col_tot = 0, step_ncol = 0;
+    bool have_nbr = false, timing = false;
+    ParticleSoA A, B;
+    Tree tree;
+    DArray<JRec> jrec;
+    DArray<SoftGroup> groups;
+    DArray<int2> cand;
+    DArray<unsigned> cand_count;
+    unsigned cand_cap = 0, last_ncand = 0;
+    NbState nb;
+    ClusterList cl;
+    Scratch scratch;
+    HardBuffers hb;
+    DArray<double> time_c, eparts;
+    double e_init[5] = {0., 0., 0., 0., 0.};
+    long long last_epep_near = 0, last_epep_far = 0, last_epsp = 0, last_groups = 0;
+    gplum_stats stats{};
+
+    explicit System(const gplum_params &prm);
+    ~System();
+    int init_device();
+    int set_particles(const gplum_particle *pp, int count, double time);
+    int get_particles(gplum_particle *pp, int count);
+    int set_radii();
+    int reorder();
+    int soft_force(bool initial, bool use_f64);
+    int energy(double out[5]);
+    int initialize();
+    int kick();
+    int hard_part(bool persistent);
+    int merge_and_recompute();
+    int step(bool persistent);
+    PartPtrs ptrs(ParticleSoA &s);
+    HardPtrs hard_ptrs();
+    void tick(int k);
+    double elapsed(int a, int b);
+};
+
+}  // namespace gplum

@@ -1,0 +1,582 @@
+// tree.hip -- octree build, moments, i-groups and interaction-list walk on the GPU.
+//
+// FDPS (the reference's tree library) is not vendored; the construction rules are
+// the ones stated in oracle/orc_tree.c, implemented here level-synchronously:
+//   keys    : 63-bit Morton keys of the particle bbox cube, radix-sorted (stable, so
+//             ties keep the previous order);
+//   nodes   : breadth-first, one kernel pair per level (child count, child write);
+//   moments : bottom-up per level, one thread per node, FP64, FDPS MomentQuadrupole
+//             accumulation order (src/particle.h:1649-1715);
+//   groups  : maximal nodes with <= n_group_limit particles, in particle order;
+//   walk    : one wave64 per i-group, breadth-first queue in global memory, 64 nodes
+//             per step; opened leaves -> EP list, unopened nodes -> SP list, children
+//             of opened internal nodes appended in order (ballot + wave prefix sums),
+//             so the lists equal the sequential FIFO walk.
+#include <hipcub/hipcub.hpp>
+
+#include "tree.hpp"
+
+namespace gplum {
+
+constexpr int MAXLEVEL = 21;
+
+// ---------------------------------------------------------------- helpers
+
+__device__ inline uint64_t spread3(uint64_t v) {
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+__device__ inline int digit_at(uint64_t key, int level) {
+    return (int)((key >> (3 * (MAXLEVEL - 1 - level))) & 7ULL);
+}
+
+// Exclusive prefix sum over the 64 lanes of a wave; *total receives the wave sum.
+__device__ inline int wave_excl_scan(int v, int *total) {
+    const int lane = __lane_id();
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+// ------------------------------------------------------------ bbox + keys
+
+__global__ void k_bbox_partial(const double *__restrict__ pos, int n, double *__restrict__ part) {
+    __shared__ double s[6][256];
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; k++) {
+            const double x = pos[3 * i + k];
+            mn[k] = x < mn[k] ? x : mn[k];
+            mx[k] = x > mx[k] ? x : mx[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) { s[k][threadIdx.x] = mn[k]; s[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) {
+                const double a = s[k][threadIdx.x + w], b = s[3 + k][threadIdx.x + w];
+                if (a < s[k][threadIdx.x]) s[k][threadIdx.x] = a;
+                if (b > s[3 + k][threadIdx.x]) s[3 + k][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = s[k][0];
+}
+
+__global__ void k_bbox_final(const double *__restrict__ part, int nb, double *__restrict__ os) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            if (part[6 * b + k] < mn[k]) mn[k] = part[6 * b + k];
+            if (part[6 * b + 3 + k] > mx[k]) mx[k] = part[6 * b + 3 + k];
+        }
+    double ext = 0.;
+    for (int k = 0; k < 3; k++) {
+        const double e = mx[k] - mn[k];
+        if (e > ext) ext = e;
+    }
+    if (ext == 0.) ext = 1.;
+    const double side = ext * 1.0000001;
+    os[0] = mn[0]; os[1] = mn[1]; os[2] = mn[2];
+    os[3] = side;
+    os[4] = 2097152.0 / side;
+}
+
+__device__ inline uint64_t coord21(double x, double o, double scale) {
+    const double t = (x - o) * scale;
+    long long c = (long long)floor(t);
+    if (c < 0) c = 0;
+    if (c > 2097151) c = 2097151;
+    return (uint64_t)c;
+}
+
+__global__ void k_keys(const double *__restrict__ pos, int n, const double *__restrict__ os,
+                       uint64_t *__restrict__ key, int32_t *__restrict__ idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double scale = os[4];
+    const uint64_t cx = coord21(pos[3 * i], os[0], scale);
+    const uint64_t cy = coord21(pos[3 * i + 1], os[1], scale);
+    const uint64_t cz = coord21(pos[3 * i + 2], os[2], scale);
+    key[i] = (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
+    idx[i] = i;
+}
+
+// ------------------------------------------------------------------ nodes
+
+__global__ void k_root(int n, int32_t *start, int32_t *count, int32_t *level, int32_t *parent) {
+    start[0] = 0; count[0] = n; level[0] = 0; parent[0] = -1;
+}
+
+__global__ void k_count_children(const uint64_t *__restrict__ key, int lb, int le, int n_leaf,
+                                 const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                                 const int32_t *__restrict__ level, int32_t *__restrict__ nchild) {
+    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= le) return;
+    const int s = start[c], cnt = count[c], lv = level[c];
+    int nc = 0;
+    if (cnt > n_leaf && lv < MAXLEVEL) {
+        int b = s;
+        const int e = s + cnt;
+        while (b < e) {
+            const int d = digit_at(key[b], lv);
+            int lo = b, hi = e;              // first index with digit > d
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (digit_at(key[mid], lv) <= d) lo = mid + 1; else hi = mid;
+            }
+            nc++;
+            b = lo;
+        }
+    }
+    nchild[c - lb] = nc;
+}
+
+__global__ void k_write_children(const uint64_t *__restrict__ key, int lb, int le, int next_begin,
+                                 const int32_t *__restrict__ scan, int32_t *__restrict__ start,
+                                 int32_t *__restrict__ count, int32_t *__restrict__ level,
+                                 int32_t *__restrict__ parent, int32_t *__restrict__ child,
+                                 int32_t *__restrict__ nchild_out, const int32_t *__restrict__ nchild) {
+    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= le) return;
+    const int nc = nchild[c - lb];
+    nchild_out[c] = nc;
+    child[c] = nc ? next_begin + scan[c - lb] : -1;
+    if (!nc) return;
+    const int s = start[c], e = s + count[c], lv = level[c];
+    int b = s, w = next_begin + scan[c - lb];
+    while (b < e) {
+        const int d = digit_at(key[b], lv);
+        int lo = b, hi = e;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (digit_at(key[mid], lv) <= d) lo = mid + 1; else hi = mid;
+        }
+        start[w] = b; count[w] = lo - b; level[w] = lv + 1; parent[w] = c;
+        w++;
+        b = lo;
+    }
+}
+
+__global__ void k_leaf_tail(int lb, int le, int32_t *nchild_out, int32_t *child) {
+    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= le) return;
+    nchild_out[c] = 0;
+    child[c] = -1;
+}
+
+// ---------------------------------------------------------------- moments
+
+__global__ void k_moments(int lb, int le, const double *__restrict__ pos, const double *__restrict__ mass,
+                          const double *__restrict__ rsearch, const int32_t *__restrict__ start,
+                          const int32_t *__restrict__ count, const int32_t *__restrict__ child,
+                          const int32_t *__restrict__ nchild, double *__restrict__ nm, double *__restrict__ ncom,
+                          double *__restrict__ nq, double *__restrict__ bmin, double *__restrict__ bmax,
+                          double *__restrict__ nrs) {
+    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= le) return;
+    double m = 0., mp[3] = {0., 0., 0.}, q[6] = {0., 0., 0., 0., 0., 0.};
+    double bmn[3], bmx[3], rs = 0., com[3];
+    const int s = start[c], cnt = count[c];
+    if (nchild[c] == 0) {
+        for (int k = 0; k < 3; k++) { bmn[k] = pos[3 * s + k]; bmx[k] = pos[3 * s + k]; }
+        for (int a = s; a < s + cnt; a++) {
+            const double ma = mass[a];
+            m += ma;
+            for (int k = 0; k < 3; k++) {
+                const double x = pos[3 * a + k];
+                mp[k] += ma * x;
+                if (x < bmn[k]) bmn[k] = x;
+                if (x > bmx[k]) bmx[k] = x;
+            }
+            if (rsearch[a] > rs) rs = rsearch[a];
+        }
+        for (int k = 0; k < 3; k++) com[k] = (m != 0.) ? mp[k] / m : 0.;
+        for (int a = s; a < s + cnt; a++) {
+            const double ma = mass[a];
+            const double dx = pos[3 * a] - com[0], dy = pos[3 * a + 1] - com[1], dz = pos[3 * a + 2] - com[2];
+            const double cx = ma * dx, cy = ma * dy, cz = ma * dz;
+            q[0] += cx * dx; q[1] += cy * dy; q[2] += cz * dz;
+            q[3] += cx * dy; q[4] += cx * dz; q[5] += cy * dz;
+        }
+    } else {
+        const int c0 = child[c], nc = nchild[c];
+        for (int k = 0; k < 3; k++) { bmn[k] = bmin[3 * c0 + k]; bmx[k] = bmax[3 * c0 + k]; }
+        for (int ch = c0; ch < c0 + nc; ch++) {
+            m += nm[ch];
+            for (int k = 0; k < 3; k++) {
+                mp[k] += nm[ch] * ncom[3 * ch + k];
+                if (bmin[3 * ch + k] < bmn[k]) bmn[k] = bmin[3 * ch + k];
+                if (bmax[3 * ch + k] > bmx[k]) bmx[k] = bmax[3 * ch + k];
+            }
+            if (nrs[ch] > rs) rs = nrs[ch];
+        }
+        for (int k = 0; k < 3; k++) com[k] = (m != 0.) ? mp[k] / m : 0.;
+        for (int ch = c0; ch < c0 + nc; ch++) {
+            const double mt = nm[ch];
+            const double dx = ncom[3 * ch] - com[0], dy = ncom[3 * ch + 1] - com[1], dz = ncom[3 * ch + 2] - com[2];
+            const double cx = mt * dx, cy = mt * dy, cz = mt * dz;
+            const double *qc = &nq[6 * ch];
+            q[0] += cx * dx + qc[0]; q[1] += cy * dy + qc[1]; q[2] += cz * dz + qc[2];
+            q[3] += cx * dy + qc[3]; q[4] += cx * dz + qc[4]; q[5] += cy * dz + qc[5];
+        }
+    }
+    nm[c] = m;
+    for (int k = 0; k < 3; k++) { ncom[3 * c + k] = com[k]; bmin[3 * c + k] = bmn[k]; bmax[3 * c + k] = bmx[k]; }
+    for (int k = 0; k < 6; k++) nq[6 * c + k] = q[k];
+    nrs[c] = rs;
+}
+
+// ----------------------------------------------------------------- groups
+
+__global__ void k_group_flags(int n_node, int G, const int32_t *__restrict__ start,
+                              const int32_t *__restrict__ count, const int32_t *__restrict__ nchild,
+                              const int32_t *__restrict__ parent, int32_t *__restrict__ flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_node) return;
+    const int cnt = count[c];
+    const bool cand = (cnt <= G) || (nchild[c] == 0);
+    const bool top = (parent[c] < 0) || (count[parent[c]] > G);
+    if (!(cand && top)) return;
+    for (int off = 0; off < cnt; off += G) flag[start[c] + off] = c + 1;
+}
+
+__global__ void k_group_write(int n, int G, const int32_t *__restrict__ flag,
+                              const int32_t *__restrict__ scan, const int32_t *__restrict__ start,
+                              const int32_t *__restrict__ count, int32_t *__restrict__ gnode,
+                              int32_t *__restrict__ gstart, int32_t *__restrict__ gcount) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int f = flag[a];
+    if (!f) return;
+    const int c = f - 1;
+    const int g = scan[a];
+    gnode[g] = c;
+    gstart[g] = a;
+    const int rem = start[c] + count[c] - a;
+    gcount[g] = rem < G ? rem : G;
+}
+
+__global__ void k_flag01(int n, const int32_t *__restrict__ flag, int32_t *__restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) out[a] = flag[a] ? 1 : 0;
+}
+
+// ------------------------------------------------------------------- walk
+
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_walk(
+    int n_group, const int32_t *__restrict__ gstart, const int32_t *__restrict__ gcount,
+    const double *__restrict__ pos, const double *__restrict__ rsearch, const double *__restrict__ os,
+    double theta2, const int32_t *__restrict__ nstart, const int32_t *__restrict__ ncount,
+    const int32_t *__restrict__ nchild_first, const int32_t *__restrict__ nchild,
+    const int32_t *__restrict__ nlevel, const double *__restrict__ ncom, const double *__restrict__ bmin,
+    const double *__restrict__ bmax, const double *__restrict__ nrs, int32_t *__restrict__ queue_all,
+    int qcap, int32_t *__restrict__ ep_cnt, int32_t *__restrict__ epf_cnt, int32_t *__restrict__ epl_cnt,
+    int32_t *__restrict__ sp_cnt, const int32_t *__restrict__ ep_off, const int32_t *__restrict__ epf_off,
+    const int32_t *__restrict__ epl_off, const int32_t *__restrict__ sp_off, int32_t *__restrict__ ep_list,
+    int32_t *__restrict__ epf_list, int32_t *__restrict__ ep_leaf, int32_t *__restrict__ sp_list,
+    int32_t *__restrict__ err) {
+    const int lane = __lane_id();
+    int32_t *queue = queue_all + (size_t)blockIdx.x * qcap;
+    const double side = os[3];
+    for (int g = blockIdx.x; g < n_group; g += gridDim.x) {
+        const int s = gstart[g], cnt = gcount[g];
+        // group bbox and max r_search (wave reduction; exact)
+        double gmn[3] = {1e300, 1e300, 1e300}, gmx[3] = {-1e300, -1e300, -1e300}, grs = 0.;
+        for (int a = s + lane; a < s + cnt; a += 64) {
+            for (int k = 0; k < 3; k++) {
+                const double x = pos[3 * a + k];
+                gmn[k] = x < gmn[k] ? x : gmn[k];
+                gmx[k] = x > gmx[k] ? x : gmx[k];
+            }
+            grs = rsearch[a] > grs ? rsearch[a] : grs;
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            for (int k = 0; k < 3; k++) {
+                const double a = __shfl_xor(gmn[k], d), b = __shfl_xor(gmx[k], d);
+                gmn[k] = a < gmn[k] ? a : gmn[k];
+                gmx[k] = b > gmx[k] ? b : gmx[k];
+            }
+            const double r = __shfl_xor(grs, d);
+            grs = r > grs ? r : grs;
+        }
+        int head = 0, tail = 1;
+        if (lane == 0) queue[0] = 0;
+        int n_ep = 0, n_epf = 0, n_epl = 0, n_sp = 0;
+        const int base_ep = FILL ? ep_off[g] : 0;
+        const int base_epf = FILL ? epf_off[g] : 0;
+        const int base_epl = FILL ? epl_off[g] : 0;
+        const int base_sp = FILL ? sp_off[g] : 0;
+        bool overflow = false;
+        while (head < tail) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __syncthreads();
+            const int chunk = min(64, tail - head);
+            int c = -1;
+            bool open = false, leaf = false, near = false;
+            int nc = 0, cc = 0;
+            if (lane < chunk) {
+                c = queue[head + lane];
+                double d2 = 0., gap2 = 0.;
+                for (int k = 0; k < 3; k++) {
+                    const double cm = ncom[3 * c + k];
+                    double dd = 0.;
+                    if (cm < gmn[k]) dd = gmn[k] - cm;
+                    else if (cm > gmx[k]) dd = cm - gmx[k];
+                    d2 += dd * dd;
+                    double gg = 0.;
+                    if (bmax[3 * c + k] < gmn[k]) gg = gmn[k] - bmax[3 * c + k];
+                    else if (bmin[3 * c + k] > gmx[k]) gg = bmin[3 * c + k] - gmx[k];
+                    gap2 += gg * gg;
+                }
+                const double w = side / (double)(1ULL << nlevel[c]);
+                const double rsm = 1.1 * (grs > nrs[c] ? grs : nrs[c]);
+                near = gap2 < rsm * rsm;
+                open = (w * w > theta2 * d2) || near;
+                nc = nchild[c];
+                leaf = nc == 0;
+                cc = ncount[c];
+            }
+            const bool f_ep = (lane < chunk) && open && leaf;
+            const bool f_epn = f_ep && near;
+            const bool f_epf = f_ep && !near;
+            const bool f_in = (lane < chunk) && open && !leaf;
+            const bool f_sp = (lane < chunk) && !open;
+            int t_ep, t_epf, t_epl, t_sp, t_ch;
+            const int o_ep = wave_excl_scan(f_epn ? cc : 0, &t_ep);
+            const int o_epf = wave_excl_scan(f_epf ? cc : 0, &t_epf);
+            const int o_epl = wave_excl_scan(f_ep ? 1 : 0, &t_epl);
+            const int o_sp = wave_excl_scan(f_sp ? 1 : 0, &t_sp);
+            const int o_ch = wave_excl_scan(f_in ? nc : 0, &t_ch);
+            if (tail + t_ch > qcap) {
+                overflow = true;
+                break;
+            }
+            if (f_in) {
+                const int c0 = nchild_first[c];
+                for (int k = 0; k < nc; k++) queue[tail + o_ch + k] = c0 + k;
+            }
+            if (FILL) {
+                if (f_ep) ep_leaf[base_epl + n_epl + o_epl] = c;
+                if (f_epn) {
+                    const int st = nstart[c];
+                    for (int k = 0; k < cc; k++) ep_list[base_ep + n_ep + o_ep + k] = st + k;
+                }
+                if (f_epf) {
+                    const int st = nstart[c];
+                    for (int k = 0; k < cc; k++) epf_list[base_epf + n_epf + o_epf + k] = st + k;
+                }
+                if (f_sp) sp_list[base_sp + n_sp + o_sp] = c;
+            }
+            n_ep += t_ep;
+            n_epf += t_epf;
+            n_epl += t_epl;
+            n_sp += t_sp;
+            tail += t_ch;
+            head += chunk;
+        }
+        if (overflow) {
+            if (lane == 0) atomicExch(err, 1);
+        }
+        if (!FILL && lane == 0) {
+            ep_cnt[g] = n_ep;
+            epf_cnt[g] = n_epf;
+            epl_cnt[g] = n_epl;
+            sp_cnt[g] = n_sp;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- host side
+
+template <class T>
+static int excl_scan(Tree &t, const T *in, T *out, int n, hipStream_t st) {
+    size_t bytes = 0;
+    GPLUM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
+    GPLUM_ENSURE(t.scan_tmp, bytes + 16);
+    GPLUM_HIP(hipcub::DeviceScan::ExclusiveSum(t.scan_tmp.p, bytes, in, out, n, st));
+    return GPLUM_OK;
+}
+
+// Sum of the last element of an exclusive scan + the last input (total).
+template <class T>
+static int scan_total(const T *in, const T *scan, int n, T *total, hipStream_t st) {
+    T a = 0, b = 0;
+    if (n <= 0) { *total = 0; return GPLUM_OK; }
+    GPLUM_HIP(hipMemcpyAsync(&a, in + n - 1, sizeof(T), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipMemcpyAsync(&b, scan + n - 1, sizeof(T), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    *total = a + b;
+    return GPLUM_OK;
+}
+
+int tree_sort_keys(Tree &t, const double *d_pos, int n, hipStream_t st) {
+    const int nb = std::min(1024, std::max(1, ceil_div(n, 256)));
+    GPLUM_ENSURE(t.bbox, 6 * nb + 8);
+    GPLUM_ENSURE(t.origin_scale, 8);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, d_pos, n, t.bbox.p);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, t.bbox.p, nb, t.origin_scale.p);
+    GPLUM_ENSURE(t.key, n);
+    GPLUM_ENSURE(t.key_alt, n);
+    GPLUM_ENSURE(t.idx, n);
+    GPLUM_ENSURE(t.perm, n);
+    hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, st, d_pos, n, t.origin_scale.p,
+                       t.key_alt.p, t.idx.p);
+    size_t bytes = 0;
+    GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, t.key_alt.p, t.key.p, t.idx.p,
+                                                  t.perm.p, n, 0, 63, st));
+    GPLUM_ENSURE(t.scan_tmp, bytes + 16);
+    GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(t.scan_tmp.p, bytes, t.key_alt.p, t.key.p, t.idx.p,
+                                                  t.perm.p, n, 0, 63, st));
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const double *d_mass,
+                     const double *d_rsearch, int n, hipStream_t st) {
+    const int cap = 2 * n + 64;
+    GPLUM_ENSURE(t.node_start, cap);
+    GPLUM_ENSURE(t.node_count, cap);
+    GPLUM_ENSURE(t.node_child, cap);
+    GPLUM_ENSURE(t.node_nchild, cap);
+    GPLUM_ENSURE(t.node_level, cap);
+    GPLUM_ENSURE(t.node_parent, cap);
+    GPLUM_ENSURE(t.nchild_scan, cap);
+    GPLUM_ENSURE(t.grp_flag, cap);   // reused as per-level child counts
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, n, t.node_start.p, t.node_count.p,
+                       t.node_level.p, t.node_parent.p);
+    t.level_begin.assign(1, 0);
+    int lb = 0, le = 1;
+    while (le > lb) {
+        const int m = le - lb;
+        hipLaunchKernelGGL(k_count_children, dim3(ceil_div(m, 256)), dim3(256), 0, st, t.key.p, lb, le,
+                           p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p,
+                           t.grp_flag.p);
+        int rc = excl_scan(t, t.grp_flag.p, t.nchild_scan.p, m, st);
+        if (rc) return rc;
+        int total = 0;
+        if ((rc = scan_total(t.grp_flag.p, t.nchild_scan.p, m, &total, st))) return rc;
+        if (le + total > cap) return fail(GPLUM_E_OVERFLOW, "tree node capacity exceeded");
+        hipLaunchKernelGGL(k_write_children, dim3(ceil_div(m, 256)), dim3(256), 0, st, t.key.p, lb, le,
+                           le, t.nchild_scan.p, t.node_start.p, t.node_count.p, t.node_level.p,
+                           t.node_parent.p, t.node_child.p, t.node_nchild.p, t.grp_flag.p);
+        t.level_begin.push_back(le);
+        lb = le;
+        le = le + total;
+    }
+    t.n_level = (int)t.level_begin.size() - 1;
+    t.n_node = le;
+    GPLUM_ENSURE(t.node_mass, t.n_node);
+    GPLUM_ENSURE(t.node_com, 3 * t.n_node);
+    GPLUM_ENSURE(t.node_quad, 6 * t.n_node);
+    GPLUM_ENSURE(t.node_bmin, 3 * t.n_node);
+    GPLUM_ENSURE(t.node_bmax, 3 * t.n_node);
+    GPLUM_ENSURE(t.node_rs, t.n_node);
+    for (int l = t.n_level - 1; l >= 0; l--) {
+        const int b = t.level_begin[l], e = t.level_begin[l + 1];
+        hipLaunchKernelGGL(k_moments, dim3(ceil_div(e - b, 128)), dim3(128), 0, st, b, e, d_pos, d_mass,
+                           d_rsearch, t.node_start.p, t.node_count.p, t.node_child.p, t.node_nchild.p,
+                           t.node_mass.p, t.node_com.p, t.node_quad.p, t.node_bmin.p, t.node_bmax.p,
+                           t.node_rs.p);
+    }
+    // groups
+    GPLUM_ENSURE(t.grp_flag, n);
+    GPLUM_ENSURE(t.grp_scan, n);
+    GPLUM_ENSURE(t.nchild_scan, std::max(n, cap));
+    GPLUM_HIP(hipMemsetAsync(t.grp_flag.p, 0, sizeof(int32_t) * n, st));
+    hipLaunchKernelGGL(k_group_flags, dim3(ceil_div(t.n_node, 256)), dim3(256), 0, st, t.n_node,
+                       p.n_group_limit, t.node_start.p, t.node_count.p, t.node_nchild.p,
+                       t.node_parent.p, t.grp_flag.p);
+    hipLaunchKernelGGL(k_flag01, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, t.grp_flag.p,
+                       t.nchild_scan.p);
+    int rc = excl_scan(t, t.nchild_scan.p, t.grp_scan.p, n, st);
+    if (rc) return rc;
+    int ng = 0;
+    if ((rc = scan_total(t.nchild_scan.p, t.grp_scan.p, n, &ng, st))) return rc;
+    t.n_group = ng;
+    GPLUM_ENSURE(t.group_node, ng);
+    GPLUM_ENSURE(t.group_start, ng);
+    GPLUM_ENSURE(t.group_count, ng);
+    hipLaunchKernelGGL(k_group_write, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, p.n_group_limit,
+                       t.grp_flag.p, t.grp_scan.p, t.node_start.p, t.node_count.p, t.group_node.p,
+                       t.group_start.p, t.group_count.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double *d_rsearch,
+              hipStream_t st) {
+    const int ng = t.n_group;
+    GPLUM_ENSURE(t.ep_cnt, ng + 1);
+    GPLUM_ENSURE(t.epf_cnt, ng + 1);
+    GPLUM_ENSURE(t.sp_cnt, ng + 1);
+    GPLUM_ENSURE(t.epl_cnt, ng + 1);
+    GPLUM_ENSURE(t.ep_off, ng + 1);
+    GPLUM_ENSURE(t.epf_off, ng + 1);
+    GPLUM_ENSURE(t.sp_off, ng + 1);
+    GPLUM_ENSURE(t.epl_off, ng + 1);
+    GPLUM_ENSURE(t.walk_err, 1);
+    const int nblk = std::min(ng, 4096);
+    const double theta2 = p.theta * p.theta;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        GPLUM_ENSURE(t.walk_queue, (size_t)nblk * t.queue_cap);
+        GPLUM_HIP(hipMemsetAsync(t.walk_err.p, 0, sizeof(int32_t), st));
+        hipLaunchKernelGGL((k_walk<false>), dim3(nblk), dim3(64), 0, st, ng, t.group_start.p,
+                           t.group_count.p, d_pos, d_rsearch, t.origin_scale.p, theta2, t.node_start.p,
+                           t.node_count.p, t.node_child.p, t.node_nchild.p, t.node_level.p, t.node_com.p,
+                           t.node_bmin.p, t.node_bmax.p, t.node_rs.p, t.walk_queue.p, t.queue_cap,
+                           t.ep_cnt.p, t.epf_cnt.p, t.epl_cnt.p, t.sp_cnt.p, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, t.walk_err.p);
+        int err = 0;
+        GPLUM_HIP(hipMemcpyAsync(&err, t.walk_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        if (!err) break;
+        t.queue_cap *= 4;
+        if (attempt == 3) return fail(GPLUM_E_OVERFLOW, "tree walk queue overflow");
+    }
+    int rc;
+    int tot_ep = 0, tot_epf = 0, tot_sp = 0, tot_epl = 0;
+    if ((rc = excl_scan(t, t.ep_cnt.p, t.ep_off.p, ng, st))) return rc;
+    if ((rc = scan_total(t.ep_cnt.p, t.ep_off.p, ng, &tot_ep, st))) return rc;
+    if ((rc = excl_scan(t, t.epf_cnt.p, t.epf_off.p, ng, st))) return rc;
+    if ((rc = scan_total(t.epf_cnt.p, t.epf_off.p, ng, &tot_epf, st))) return rc;
+    if ((rc = excl_scan(t, t.sp_cnt.p, t.sp_off.p, ng, st))) return rc;
+    if ((rc = scan_total(t.sp_cnt.p, t.sp_off.p, ng, &tot_sp, st))) return rc;
+    if ((rc = excl_scan(t, t.epl_cnt.p, t.epl_off.p, ng, st))) return rc;
+    if ((rc = scan_total(t.epl_cnt.p, t.epl_off.p, ng, &tot_epl, st))) return rc;
+    t.n_ep_total = tot_ep;
+    t.n_epf_total = tot_epf;
+    t.n_sp_total = tot_sp;
+    t.n_epl_total = tot_epl;
+    GPLUM_ENSURE(t.ep_list, (size_t)tot_ep + 1);
+    GPLUM_ENSURE(t.epf_list, (size_t)tot_epf + 1);
+    GPLUM_ENSURE(t.sp_list, (size_t)tot_sp + 1);
+    GPLUM_ENSURE(t.ep_leaf, (size_t)tot_epl + 1);
+    hipLaunchKernelGGL((k_walk<true>), dim3(nblk), dim3(64), 0, st, ng, t.group_start.p,
+                       t.group_count.p, d_pos, d_rsearch, t.origin_scale.p, theta2, t.node_start.p,
+                       t.node_count.p, t.node_child.p, t.node_nchild.p, t.node_level.p, t.node_com.p,
+                       t.node_bmin.p, t.node_bmax.p, t.node_rs.p, t.walk_queue.p, t.queue_cap, t.ep_cnt.p,
+                       t.epf_cnt.p, t.epl_cnt.p, t.sp_cnt.p, t.ep_off.p, t.epf_off.p, t.epl_off.p, t.sp_off.p,
+                       t.ep_list.p, t.epf_list.p, t.ep_leaf.p, t.sp_list.p, t.walk_err.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+}  // namespace gplum

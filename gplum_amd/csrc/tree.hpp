@@ -1,0 +1,16 @@
+// tree.hpp -- octree build + interaction-list walk (see tree.hip).
+#pragma once
+#include "engine.hpp"
+
+namespace gplum {
+
+// Morton keys of d_pos[0..n) sorted; t.perm receives the sorted->input permutation.
+int tree_sort_keys(Tree &t, const double *d_pos, int n, hipStream_t st);
+// Nodes, moments and i-groups over particles already in sorted order.
+int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const double *d_mass,
+                     const double *d_rsearch, int n, hipStream_t st);
+// EP (near/far) and SP lists of every i-group.
+int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double *d_rsearch,
+              hipStream_t st);
+
+}  // namespace gplum

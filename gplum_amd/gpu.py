@@ -1,0 +1,226 @@
+"""ctypes binding of libgplum_gpu.so (the HIP implementation).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.  Build it with
+`python -m gplum_amd.build` (or __graft_entry__.build()).
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgplum_gpu.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+_lib = None
+
+
+class GplumError(RuntimeError):
+    pass
+
+
+def declared_symbols():
+    """Function names declared in include/*.h (the C ABI surface)."""
+    names = []
+    for h in sorted(os.listdir(INCLUDE_DIR)):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE_DIR, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(gplum_[a-z0-9_]+)\s*\(",
+                             text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GplumError("libgplum_gpu.so is not built (%s); run python -m gplum_amd.build"
+                             % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+        P = ctypes.POINTER(abi.Params)
+        L.gplum_last_error.restype = ctypes.c_char_p
+        L.gplum_params_default.argtypes = [P]
+        L.gplum_params_set_gamma.argtypes = [P, f64]
+        L.gplum_device_info.argtypes = [ctypes.c_char_p, i32, ctypes.POINTER(i32),
+                                        ctypes.POINTER(f64)]
+        for n in ("gplum_calc_force_epep", "gplum_calc_force_epsp_quad", "gplum_calc_force_epsp_mono"):
+            getattr(L, n).argtypes = [vp, i32, vp, i32, vp, f32]
+        for n in ("gplum_calc_force_epep_f64", "gplum_calc_force_epsp_quad_f64"):
+            getattr(L, n).argtypes = [vp, i32, vp, i32, vp, f32, vp, vp]
+        L.gplum_system_create.argtypes = [P, ctypes.POINTER(vp)]
+        L.gplum_system_destroy.argtypes = [vp]
+        L.gplum_system_set_params.argtypes = [vp, P]
+        L.gplum_system_set_particles.argtypes = [vp, vp, i32, f64]
+        L.gplum_system_get_particles.argtypes = [vp, vp, i32]
+        L.gplum_system_count.argtypes = [vp, ctypes.POINTER(i32)]
+        L.gplum_system_initialize.argtypes = [vp]
+        L.gplum_system_set_radii.argtypes = [vp]
+        L.gplum_system_soft_force.argtypes = [vp, i32, i32]
+        L.gplum_system_hard_part.argtypes = [vp, i32]
+        L.gplum_system_step.argtypes = [vp, i32]
+        L.gplum_system_energy.argtypes = [vp, vp]
+        L.gplum_system_neighbors.argtypes = [vp, vp, vp, i32, ctypes.POINTER(i32)]
+        L.gplum_system_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.gplum_system_reset_stats.argtypes = [vp, i32]
+        _lib = L
+    return _lib
+
+
+class Stats(ctypes.Structure):
+    """gplum_stats (include/gplum_system.h)."""
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "steps", "n_epep_near", "n_epep_far", "n_epsp", "n_groups", "n_candidates", "n_neighbors",
+        "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
+        "cand_retries", "n_force_calls")] + [(n, ctypes.c_double) for n in (
+            "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class System:
+    """Device-resident P3T system (include/gplum_system.h)."""
+
+    def __init__(self, p, particles=None, time_sys=0.0):
+        self.p = p
+        self.h = ctypes.c_void_p()
+        check(lib().gplum_system_create(ctypes.byref(p), ctypes.byref(self.h)))
+        if particles is not None:
+            self.set_particles(particles, time_sys)
+
+    def close(self):
+        if self.h:
+            lib().gplum_system_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        self.p = p
+        check(lib().gplum_system_set_params(self.h, ctypes.byref(p)))
+
+    def set_particles(self, pp, time_sys=0.0):
+        pp = np.ascontiguousarray(pp, dtype=abi.PARTICLE)
+        check(lib().gplum_system_set_particles(self.h, abi.ptr(pp), len(pp), time_sys))
+
+    def count(self):
+        n = ctypes.c_int(0)
+        check(lib().gplum_system_count(self.h, ctypes.byref(n)))
+        return n.value
+
+    def particles(self):
+        n = self.count()
+        pp = np.zeros(n, dtype=abi.PARTICLE)
+        check(lib().gplum_system_get_particles(self.h, abi.ptr(pp), n))
+        return pp
+
+    def initialize(self):
+        check(lib().gplum_system_initialize(self.h))
+
+    def set_radii(self):
+        check(lib().gplum_system_set_radii(self.h))
+
+    def soft_force(self, initial=False, use_f64=False):
+        check(lib().gplum_system_soft_force(self.h, int(initial), int(use_f64)))
+
+    def hard_part(self, persistent=True):
+        check(lib().gplum_system_hard_part(self.h, int(persistent)))
+
+    def step(self, nsteps=1):
+        check(lib().gplum_system_step(self.h, nsteps))
+
+    def energy(self):
+        out = np.zeros(6)
+        check(lib().gplum_system_energy(self.h, abi.ptr(out)))
+        return out
+
+    def neighbors(self):
+        """(offset[n+1], index) of the neighbour graph in the current device order."""
+        n = self.count()
+        off = np.zeros(n + 1, dtype=np.int32)
+        tot = ctypes.c_int(0)
+        check(lib().gplum_system_neighbors(self.h, abi.ptr(off), None, 0, ctypes.byref(tot)))
+        idx = np.zeros(max(tot.value, 1), dtype=np.int32)
+        check(lib().gplum_system_neighbors(self.h, abi.ptr(off), abi.ptr(idx), len(idx),
+                                           ctypes.byref(tot)))
+        return off, idx[: tot.value]
+
+    def stats(self):
+        s = Stats()
+        check(lib().gplum_system_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self, timing=True):
+        check(lib().gplum_system_reset_stats(self.h, int(timing)))
+
+
+def neighbor_sets_by_id(pp, off, idx):
+    """{id_i: sorted tuple of neighbour ids} from a CSR graph over array pp."""
+    ids = pp["id"]
+    return {int(ids[i]): tuple(sorted(int(ids[j]) for j in idx[off[i]:off[i + 1]]))
+            for i in range(len(pp))}
+
+
+def check(rc):
+    if rc != 0:
+        raise GplumError("gplum error %d: %s" % (rc, lib().gplum_last_error().decode()))
+
+
+def params(**kw):
+    p = abi.Params()
+    lib().gplum_params_default(ctypes.byref(p))
+    gamma = kw.pop("gamma", None)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    if gamma is not None:
+        lib().gplum_params_set_gamma(ctypes.byref(p), gamma)
+    return p
+
+
+def device_info():
+    name = ctypes.create_string_buffer(64)
+    cu = ctypes.c_int(0)
+    clk = ctypes.c_double(0.0)
+    check(lib().gplum_device_info(name, 64, ctypes.byref(cu), ctypes.byref(clk)))
+    return name.value.decode(), cu.value, clk.value
+
+
+def calc_force_epep(epi, epj, force, eps2=0.0):
+    check(lib().gplum_calc_force_epep(abi.ptr(epi), len(epi), abi.ptr(epj), len(epj),
+                                      abi.ptr(force), eps2))
+
+
+def calc_force_epsp_quad(epi, spj, force, eps2=0.0):
+    check(lib().gplum_calc_force_epsp_quad(abi.ptr(epi), len(epi), abi.ptr(spj), len(spj),
+                                           abi.ptr(force), eps2))
+
+
+def calc_force_epsp_mono(epi, spj, force, eps2=0.0):
+    check(lib().gplum_calc_force_epsp_mono(abi.ptr(epi), len(epi), abi.ptr(spj), len(spj),
+                                           abi.ptr(force), eps2))
+
+
+def calc_force_epep_f64(epi, epj, force, eps2=0.0):
+    acc = np.zeros((len(epi), 3))
+    phi = np.zeros(len(epi))
+    check(lib().gplum_calc_force_epep_f64(abi.ptr(epi), len(epi), abi.ptr(epj), len(epj),
+                                          abi.ptr(force), eps2, abi.ptr(acc), abi.ptr(phi)))
+    return acc, phi
+
+
+def calc_force_epsp_quad_f64(epi, spj, force, eps2=0.0):
+    acc = np.zeros((len(epi), 3))
+    phi = np.zeros(len(epi))
+    check(lib().gplum_calc_force_epsp_quad_f64(abi.ptr(epi), len(epi), abi.ptr(spj), len(spj),
+                                               abi.ptr(force), eps2, abi.ptr(acc), abi.ptr(phi)))
+    return acc, phi

@@ -34,6 +34,67 @@ typedef struct gplum_particle {
     int32_t flags;                  /* GPLUM_FLAG_*                            */
 } gplum_particle;
 
+/* Counters and timings of the system (timings accumulate while enabled by
+ * gplum_system_reset_stats(s, 1); HIP events on the system's stream). */
+typedef struct gplum_stats {
+    int64_t steps;
+    int64_t n_epep_near, n_epep_far, n_epsp;  /* pair evaluations of the last soft force */
+    int64_t n_groups, n_candidates, n_neighbors;
+    int64_t n_clusters, n_cluster_particles, max_cluster;
+    int64_t block_steps;                      /* Hermite block steps, summed over clusters */
+    int64_t n_collisions;
+    int64_t cand_retries;                     /* candidate-buffer regrowths                */
+    int64_t n_force_calls;
+    double force_ms, tree_ms, finalize_ms, hard_ms;
+    double time_sys;
+} gplum_stats;
+
+typedef struct gplum_system gplum_system;
+
+/* Lifetime. */
+int gplum_system_create(const gplum_params *p, gplum_system **out);
+int gplum_system_destroy(gplum_system *s);
+int gplum_system_set_params(gplum_system *s, const gplum_params *p);
+
+/* Host <-> device particle exchange.  The system keeps particles in Morton order,
+ * so get_particles returns them in the current device order (match by id). */
+int gplum_system_set_particles(gplum_system *s, const gplum_particle *pp, int n, double time_sys);
+int gplum_system_get_particles(gplum_system *s, gplum_particle *pp, int n);
+int gplum_system_count(gplum_system *s, int *n);
+
+/* Initialisation of main_p3t.cpp:289-364: time/flags/r_planet/f defaults,
+ * setCutoffRadii (particle.h:1528-1543), then two rounds of
+ * calcForceAllAndWriteBack + correctForceLongInitial (gravity_soft.h:374-528). */
+int gplum_system_initialize(gplum_system *s);
+
+/* setCutoffRadii alone (particle.h:1528-1543). */
+int gplum_system_set_radii(gplum_system *s);
+
+/* One tree force: replaces tree_grav.calcForceAllAndWriteBack(calcForceEPEPWithSearch,
+ * calcForceEPSP, ...) (main_p3t.cpp:583-588) followed by correctForceLong
+ * (gravity_soft.h:245-372), or correctForceLongInitial when initial != 0.
+ * use_f64 runs the FP64 twin kernels for acc/phi (comparison runs). */
+int gplum_system_soft_force(gplum_system *s, int initial, int use_f64);
+
+/* Hard part of one step on the current neighbour graph: createNeighborCluster +
+ * HardSystem::makeList + HardSystem::timeIntegrate (main_p3t.cpp:472-526;
+ * hard.h:321-829).  persistent: one persistent batched launch (else one
+ * workgroup per cluster). */
+int gplum_system_hard_part(gplum_system *s, int persistent);
+
+/* nsteps soft steps of the loop at main_p3t.cpp:423-699. */
+int gplum_system_step(gplum_system *s, int nsteps);
+
+/* Energy::calcEnergy (energy.h:29-63): out = {etot, ekin, ephi_sun, ephi, ephi_d, edisp}. */
+int gplum_system_energy(gplum_system *s, double out[6]);
+
+/* Neighbour graph of the current order (NeighborList::n_list, neighbor.h:247-250):
+ * offset[n+1]; index[offset[n]] when cap is large enough; *total = offset[n]. */
+int gplum_system_neighbors(gplum_system *s, int32_t *offset, int32_t *index, int cap, int *total);
+
+int gplum_system_stats(gplum_system *s, gplum_stats *out);
+int gplum_system_reset_stats(gplum_system *s, int timing);
+
 #ifdef __cplusplus
 }
 #endif

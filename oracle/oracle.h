@@ -24,6 +24,10 @@
 extern "C" {
 #endif
 
+/* Reference parameter defaults (particle.h:917-961) and FPGrav::setGamma (:619-633). */
+void orc_params_default(gplum_params *p);
+void orc_params_set_gamma(gplum_params *p, double g);
+
 /* ----------------------------------------------------------- src/cutfunc.h */
 double orc_cutoff_f(const gplum_params *p, double y);
 double orc_cutoff_W(const gplum_params *p, double rij, double r_out_inv);

@@ -28,8 +28,8 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
         P = ctypes.POINTER(abi.Params)
-        L.gplum_params_default.argtypes = [P]
-        L.gplum_params_set_gamma.argtypes = [P, f64]
+        L.orc_params_default.argtypes = [P]
+        L.orc_params_set_gamma.argtypes = [P, f64]
         for name in ("orc_epep_f32", "orc_epsp_quad_f32", "orc_epsp_mono_f32"):
             getattr(L, name).argtypes = [vp, i32, vp, i32, vp, f32]
         L.orc_epep_f64.argtypes = [vp, i32, vp, i32, f64, vp, vp, vp]
@@ -117,12 +117,12 @@ class Tree(ctypes.Structure):
 
 def params(**kw):
     p = abi.Params()
-    lib().gplum_params_default(ctypes.byref(p))
+    lib().orc_params_default(ctypes.byref(p))
     gamma = kw.pop("gamma", None)
     for k, v in kw.items():
         setattr(p, k, v)
     if gamma is not None:
-        lib().gplum_params_set_gamma(ctypes.byref(p), gamma)
+        lib().orc_params_set_gamma(ctypes.byref(p), gamma)
     return p
 
 

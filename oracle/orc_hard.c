@@ -17,6 +17,7 @@
 #define ALPHA_Q1     1.1666666666666667e-3   /* quirk Q1, particle.h:339 */
 #define ONE_TWELFTH  8.3333333333333333e-2
 #define ONE_SIXTIETH 1.6666666666666667e-2
+#define HALVE_CAP    160
 
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -93,12 +94,14 @@ void orc_calc_dt_initial(const gplum_params *p, gplum_particle *pi)
     double dt_next = 0.5 * p->dt_tree;
     double dt_1 = smin(calc_dt2(p->eta_0, p->alpha2, pi->acc0, pi->acc_d, pi->jerk_d),
                        calc_dt2(p->eta_sun0, p->alpha2, 0., pi->acc_s, pi->jerk_s));
+    /* loops capped at HALVE_CAP iterations like the HIP code (only non-power-of-two
+     * or NaN times ever reach the cap; the result is then dt_min) */
     double rem = fmod(pi->time, dt_next);
-    while (rem != 0.0) { dt_next *= 0.5; rem = fmod(pi->time, dt_next); }
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(pi->time, dt_next); }
     if (pi->dt > 0.)
-        while (2. * pi->dt < dt_next) dt_next *= 0.5;
-    while (dt_1 < dt_next) dt_next *= 0.5;
-    if (dt_next < 2. * p->dt_min) dt_next = p->dt_min;
+        for (int it = 0; 2. * pi->dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
+    for (int it = 0; dt_1 < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
+    if (dt_next < 2. * p->dt_min || rem != 0.0) dt_next = p->dt_min;
     pi->dt = dt_next;
 }
 
@@ -110,8 +113,9 @@ static void calc_dt(const gplum_params *p, orc_hp *h)
     double dt_1 = smin(calc_dt4(p->eta, p->alpha2, b->acc0, b->acc_d, b->jerk_d, h->a2_d, h->a3_d),
                        calc_dt4(p->eta_sun, p->alpha2, 0., b->acc_s, b->jerk_s, h->a2_s, h->a3_s));
     double rem = fmod(b->time, dt_next);
-    while (rem != 0.0) { dt_next *= 0.5; rem = fmod(b->time, dt_next); }
-    while (dt_1 < dt_next && 0.5 * b->dt < dt_next) dt_next *= 0.5;
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(b->time, dt_next); }
+    if (rem != 0.0) { b->dt = p->dt_min; return; }
+    for (int it = 0; dt_1 < dt_next && 0.5 * b->dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
     if (dt_next < 2. * p->dt_min) dt_next = p->dt_min;
     b->dt = dt_next;
 }

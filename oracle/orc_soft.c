@@ -16,7 +16,7 @@ static inline double smax(double a, double b) { return (a < b) ? b : a; }
 
 /* Reference defaults: static member initialisers of src/particle.h:917-961,
  * plus tree defaults (src/read.h: theta 0.5, n_leaf_limit 8, n_group_limit 256). */
-void gplum_params_default(gplum_params *p)
+void orc_params_default(gplum_params *p)
 {
     memset(p, 0, sizeof(*p));
     p->m_sun = 1.;
@@ -45,11 +45,11 @@ void gplum_params_default(gplum_params *p)
     p->n_group_limit = 256;
     p->quadrupole = 1;
     p->collision = 1;
-    gplum_params_set_gamma(p, 0.5);
+    orc_params_set_gamma(p, 0.5);
 }
 
 /* FPGrav::setGamma, src/particle.h:619-633. */
-void gplum_params_set_gamma(gplum_params *p, double g)
+void orc_params_set_gamma(gplum_params *p, double g)
 {
     p->gamma = g;
     p->g_1_inv = 1. / (g - 1.);

@@ -1,0 +1,105 @@
+"""GPU parity of the PIKG-shaped soft-force kernels against the oracle.
+
+Bar: FP32 neighbour number/id_min/id_max bit-identical to the oracle's FP32 kernel (same
+fixed operation order); FP32 acc/phi within the stated rms relative error of the FP64
+oracle; the FP64 GPU twin within 1e-12 of the FP64 oracle."""
+import numpy as np
+import pytest
+
+from gplum_amd import abi, gpu
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+ACC_RMS_TOL = 1e-6     # FP32 kernel vs FP64, rms relative |acc| error
+PHI_RMS_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def disk_tree(oracle_lib):
+    p, pp = helpers.disk_with_radii(oracle_lib, 40000, seed=1)
+    tree = oracle_lib.tree_build(p, pp)
+    yield p, pp, tree
+    tree.free()
+
+
+@pytest.mark.parametrize("which", ["first", "middle", "last"])
+def test_epep_epsp_group(oracle_lib, disk_tree, which):
+    p, pp, tree = disk_tree
+    g = {"first": 0, "middle": tree.n_group // 2, "last": tree.n_group - 1}[which]
+    epi, epj, spq, spm = helpers.group_lists(oracle_lib, p, pp, tree, g)
+    print("group", g, "ni", len(epi), "nj", len(epj), "ns", len(spq), flush=True)
+    f_gpu = abi.cleared_force(len(epi))
+    gpu.calc_force_epep(epi, epj, f_gpu)
+    f_orc = abi.cleared_force(len(epi))
+    oracle_lib.epep_f32(epi, epj, f_orc)
+    assert np.array_equal(f_gpu["neighbor"], f_orc["neighbor"])
+    gpu.calc_force_epsp_quad(epi, spq, f_gpu)
+    f64 = abi.cleared_force(len(epi))
+    a1, p1 = oracle_lib.epep_f64(epi, epj, force=f64)
+    a2, p2 = oracle_lib.epsp_quad_f64(epi, spq)
+    err_a = helpers.rel_err_rms(f_gpu["acc"], a1 + a2)
+    err_p = helpers.rel_err_rms(f_gpu["phi"], p1 + p2)
+    print("rms rel err acc %.3e phi %.3e" % (err_a, err_p))
+    assert err_a < ACC_RMS_TOL and err_p < PHI_RMS_TOL
+    # FP64 twins
+    g64 = abi.cleared_force(len(epi))
+    ga1, gp1 = gpu.calc_force_epep_f64(epi, epj, g64)
+    ga2, gp2 = gpu.calc_force_epsp_quad_f64(epi, spq, g64)
+    assert np.array_equal(g64["neighbor"], f64["neighbor"])
+    assert helpers.rel_err_rms(ga1 + ga2, a1 + a2) < 1e-12
+    assert helpers.rel_err_rms(gp1 + gp2, p1 + p2) < 1e-12
+    # monopole
+    fm = abi.cleared_force(len(epi))
+    gpu.calc_force_epsp_mono(epi, spm, fm)
+    am, pm = oracle_lib.epsp_mono_f64(epi, spm)
+    assert helpers.rel_err_rms(fm["acc"], am) < 1e-5
+
+
+def test_neighbour_predicate_at_threshold(oracle_lib):
+    """j placed within +-1e-6 relative of the 1.0201 r_search^2 threshold: the GPU's FP32
+    neighbour count equals the oracle's bit for bit (same operation order)."""
+    rng = np.random.default_rng(3)
+    ni, nj = 200, 3000
+    epi = np.zeros(ni, dtype=abi.EPI)
+    epi["id_local"] = np.arange(ni)
+    epi["pos"] = 1.0 + rng.normal(0, 1e-4, (ni, 3))
+    epi["r_out"] = 1e-4
+    epi["r_search"] = 2e-4
+    epj = np.zeros(nj, dtype=abi.EPJ)
+    epj["id_local"] = ni + np.arange(nj)
+    host = rng.integers(0, ni, nj)
+    d = rng.normal(0, 1, (nj, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rad = 2e-4 * np.sqrt(1.0201) * (1 + rng.uniform(-1e-6, 1e-6, nj))
+    epj["pos"] = epi["pos"][host] + d * rad[:, None]
+    epj["r_out"] = 1e-4
+    epj["r_search"] = 2e-4 * (1 + rng.uniform(-1e-7, 1e-7, nj))
+    epj["mass"] = 1e-12
+    f_gpu = abi.cleared_force(ni)
+    gpu.calc_force_epep(epi, epj, f_gpu)
+    f_orc = abi.cleared_force(ni)
+    oracle_lib.epep_f32(epi, epj, f_orc)
+    assert f_orc["neighbor"]["number"].sum() > 100
+    assert np.array_equal(f_gpu["neighbor"], f_orc["neighbor"])
+
+
+@pytest.mark.parametrize("ni,nj", [(1, 0), (1, 1), (63, 65), (257, 300), (1000, 17)])
+def test_ragged_sizes(oracle_lib, ni, nj):
+    rng = np.random.default_rng(ni * 7 + nj)
+    pp = np.zeros(max(ni, nj), dtype=abi.PARTICLE)
+    pp["pos"] = 1.0 + rng.normal(0, 1e-3, (len(pp), 3))
+    pp["r_out"] = 2e-4
+    pp["r_search"] = 3e-4
+    pp["mass"] = 1e-10
+    pp["id"] = np.arange(len(pp))
+    epi = helpers.to_epi(pp, np.arange(ni))
+    epj = helpers.to_epj(pp, np.arange(nj))
+    f_gpu = abi.cleared_force(ni)
+    gpu.calc_force_epep(epi, epj, f_gpu)
+    f_orc = abi.cleared_force(ni)
+    oracle_lib.epep_f32(epi, epj, f_orc)
+    assert np.array_equal(f_gpu["neighbor"], f_orc["neighbor"])
+    if nj:
+        a, ph = oracle_lib.epep_f64(epi, epj)
+        assert np.allclose(f_gpu["acc"], a, rtol=1e-5, atol=1e-12 * np.abs(a).max())

@@ -1,0 +1,125 @@
+"""GPU parity of the device-resident P3T system against the oracle.
+
+Bars: neighbour sets bit-exact (as sets of particle ids); radii bit-exact (same host
+arithmetic); FP64 tree force within 1e-10 of the oracle's (identical tree lists, only the
+summation order differs); FP32 tree force within ACC_RMS_TOL rms of the FP64 oracle;
+multi-step trajectories within a stated tolerance and identical neighbour graphs."""
+import numpy as np
+import pytest
+
+from gplum_amd import abi, gpu, ic
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+ACC_RMS_TOL = 1e-6
+CFG1 = dict(R_cut0=3.0 * 2 ** (1 / 3), R_cut1=8.0, R_search0=1.1, R_search1=6.0, R_search2=1.1,
+            R_search3=2.0, gamma=0.1, eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002,
+            dt_tree=2.0 ** -4, dt_min=2.0 ** -30, n_group_limit=64, theta=0.5)
+
+
+def by_id(pp):
+    return pp[np.argsort(pp["id"], kind="stable")]
+
+
+def ring(n, seed, mass=1e-9, a_in=0.98, a_out=1.02):
+    return ic.disk(n, mass, a_in, a_out, seed=seed)
+
+
+def both_params(oracle_lib, **kw):
+    po = oracle_lib.params(**kw)
+    pg = gpu.params(**kw)
+    return po, pg
+
+
+def test_init3000_golden(oracle_lib, init3000):
+    hdr, pp = init3000
+    p = gpu.params(dt_tree=2.0 ** -6)
+    s = gpu.System(p, pp)
+    s.initialize()
+    e = s.energy()
+    q = s.particles()
+    s.close()
+    assert abs(e[1] - hdr[4]) <= 1e-13 * abs(hdr[4])
+    assert abs(e[2] - hdr[5]) <= 1e-13 * abs(hdr[5])
+    assert abs(e[3] + e[4] - hdr[6]) <= 5e-5 * abs(hdr[6])
+    flagged = sorted(q["id"][q["neighbor"]["number"] > 0].tolist())
+    assert flagged == [624, 1728, 1742, 2165]
+
+
+@pytest.mark.parametrize("quad", [1, 0])
+def test_tree_force_and_neighbours_match_oracle(oracle_lib, quad):
+    po, pg = both_params(oracle_lib, quadrupole=quad, **CFG1)
+    pp = ring(6000, seed=21)
+    # radii: host arithmetic must agree bit for bit
+    ref = pp.copy()
+    oracle_lib.set_cutoff_radii(po, ref)
+    s = gpu.System(pg, pp)
+    s.set_radii()
+    got = by_id(s.particles())
+    for f in ("r_out", "r_out_inv", "r_search", "v_disp"):
+        assert np.array_equal(got[f], by_id(ref)[f]), f
+    # FP64 tree force + correctForceLongInitial
+    tree = oracle_lib.tree_build(po, ref)
+    oracle_lib.soft_force(po, ref, tree, use_f64=True)
+    tree.free()
+    off_o, idx_o = oracle_lib.correct_force_long(po, ref, initial=True)
+    s.soft_force(initial=True, use_f64=True)
+    g = s.particles()
+    off_g, idx_g = s.neighbors()
+    nset_o = gpu.neighbor_sets_by_id(ref, off_o, idx_o)
+    nset_g = gpu.neighbor_sets_by_id(g, off_g, idx_g)
+    assert sum(len(v) for v in nset_o.values()) > 20
+    assert nset_o == nset_g
+    go, gg = by_id(ref), by_id(g)
+    assert np.array_equal(go["neighbor"]["number"], gg["neighbor"]["number"])
+    assert np.array_equal(go["id_cluster"], gg["id_cluster"])
+    assert helpers.rel_err_rms(gg["acc"], go["acc"]) < 1e-10
+    assert helpers.rel_err_rms(gg["phi"], go["phi"]) < 1e-10
+    assert np.allclose(gg["acc0"], go["acc0"], rtol=1e-12, atol=0)
+    assert np.allclose(gg["acc_d"], go["acc_d"], rtol=1e-10, atol=1e-30)
+    assert np.array_equal(gg["dt"], go["dt"])
+    # FP32 soft force: accuracy vs the FP64 oracle, same neighbour graph
+    s.set_particles(pp)
+    s.set_radii()
+    s.soft_force(initial=True, use_f64=False)
+    g32 = by_id(s.particles())
+    off3, idx3 = s.neighbors()
+    assert gpu.neighbor_sets_by_id(s.particles(), off3, idx3) == nset_o
+    err = helpers.rel_err_rms(g32["acc"], go["acc"])
+    print("FP32 tree force rms rel err vs FP64: %.3e" % err)
+    assert err < ACC_RMS_TOL
+    s.close()
+
+
+def test_steps_match_oracle(oracle_lib):
+    """Four full P3T steps (dt_tree = 2^-4): identical neighbour graphs every step,
+    positions within 1e-9 relative, energy error within 1e-9 of the oracle's."""
+    po, pg = both_params(oracle_lib, **CFG1)
+    pp = ring(3000, seed=5)
+    run = oracle_lib.P3T(po, pp)
+    s = gpu.System(pg, pp)
+    s.initialize()
+    e0_o = oracle_lib.energy(run.pp)[0]
+    e0_g = s.energy()[0]
+    assert abs(e0_g - e0_o) < 1e-12 * abs(e0_o)
+    for k in range(4):
+        run.step()
+        s.step(1)
+        g = s.particles()
+        off_g, idx_g = s.neighbors()
+        off_o, idx_o = run.neighbor_graph()
+        assert gpu.neighbor_sets_by_id(g, off_g, idx_g) == gpu.neighbor_sets_by_id(run.pp, off_o, idx_o)
+        a, b = by_id(g), by_id(run.pp)
+        assert np.array_equal(a["id"], b["id"])
+        dx = np.linalg.norm(a["pos"] - b["pos"], axis=1) / np.linalg.norm(b["pos"], axis=1)
+        assert dx.max() < 1e-9, dx.max()
+        eo = (oracle_lib.energy(run.pp)[0] - e0_o - run.edisp.value) / e0_o
+        eg = s.energy()
+        eg = (eg[0] - e0_g - eg[5]) / e0_g
+        print("step", k, "dE oracle %.3e gpu %.3e" % (eo, eg))
+        assert abs(eg - eo) < 1e-9
+    st = s.stats()
+    print(st)
+    run.close()
+    s.close()
