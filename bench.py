@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Full P3T step throughput on one MI355X (particle-steps/s at N = 1,000,000).
+
+Workload: planetesimal disk, N = 1,000,000, m = 1e-12, a in [0.5, 2] AU with surface
+density ~ a^-1.5, Rayleigh e/i (rms e = 2 h, rms i = h), theta = 0.5 with quadrupole
+super-particles, n_group_limit 256, n_leaf_limit 8, r_out = 3 mutual Hill radii
+(R_cut0 = 3*2^(1/3)), r_in/r_out = 0.1, dt_tree = 2^-4 (the power-of-two soft step
+nearest below 2pi/64), eta = 0.01, perfect-merging collisions on, seed 1.
+A step is one full soft step: kick, clusters, batched Hermite + Kepler drift, tree,
+FP32 EP-EP/EP-SP, FP64 neighbour finalize, kick (energy after the timed region).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at N=1M (full P3T step); soft-force interactions/s vs FP64"
+PEAK_FP32_TFLOPS = 157.3          # MI355X published dense FP32 vector peak
+FLOP_EPEP = 28                    # per EP-EP pair (reference op list)
+FLOP_EPSP_QUAD = 60               # per EP-SP quadrupole pair
+FLOP_SPJ_QUAD = 12                # per staged quadrupole super-particle (hoisted terms)
+FLOP_EPSP_MONO = 25
+
+
+def params_dict():
+    return dict(R_cut0=3.0 * 2 ** (1 / 3), R_cut1=8.0, R_search0=1.1, R_search1=6.0, R_search2=1.1,
+                R_search3=2.0, gamma=0.1, eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002,
+                dt_tree=2.0 ** -4, dt_min=2.0 ** -30, n_group_limit=256, n_leaf_limit=8, theta=0.5,
+                quadrupole=1, collision=1)
+
+
+def make_disk(n, seed=1):
+    from gplum_amd import ic
+    return ic.disk(n, 1e-12, 0.5, 2.0, seed=seed)
+
+
+def cpu_baseline(n, seed, threads):
+    """Oracle full P3T step (CPU restatement, OpenMP) on the same workload: one step
+    after initialisation, timed on `threads` host threads."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import oracle as O
+    O.build()
+    p = O.params(**params_dict())
+    pp = make_disk(n, seed)
+    run = O.P3T(p, pp)
+    t0 = time.perf_counter()
+    run.step()
+    dt = time.perf_counter() - t0
+    run.close()
+    return {"value": n / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "one full soft step of the same N=%d disk after initialisation, "
+                      "oracle C restatement (PIKG-order FP32 soft kernels, FP64 correction/Hermite/"
+                      "Kepler, OpenMP over groups and clusters), %.2f s" % (n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from gplum_amd import gpu
+    name, n_cu, clk = gpu.device_info()
+    p = gpu.params(**params_dict())
+    pp = make_disk(args.n, args.seed)
+    s = gpu.System(p, pp)
+    t0 = time.perf_counter()
+    s.initialize()
+    t_init = time.perf_counter() - t0
+    print("# device %s, %d CUs; init %.2f s" % (name, n_cu, t_init), file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        s.step(1)
+        print("# warmup step %d done" % k, file=sys.stderr, flush=True)
+    s.reset_stats(timing=True)
+    e0 = s.energy()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.step(1)
+    t1 = time.perf_counter()
+    st = s.stats()
+    e1 = s.energy()
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.n * args.steps / elapsed
+
+    # dominant kernel: FP32 soft force (EP-EP + EP-SP), HIP-event timed on the system stream
+    ncalls = max(1, st["n_force_calls"])
+    force_ms = st["force_ms"] / ncalls
+    pairs_ep = st["n_epep_near"] + st["n_epep_far"]
+    pairs_sp = st["n_epsp"]
+    flops = FLOP_EPEP * pairs_ep + FLOP_EPSP_QUAD * pairs_sp + FLOP_SPJ_QUAD * st["n_sp_entries"]
+    achieved = flops / (force_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel": "k_soft_f32 (EP-EP near/far + EP-SP quadrupole, one launch per force)",
+                "flops_per_launch": flops, "avg_launch_ms": round(force_ms, 4)}
+
+    # FP64 twin on the same state: interactions/s and the FP32 error
+    q32 = s.particles()
+    s64 = gpu.System(p, q32, time_sys=st["time_sys"])
+    s64.reset_stats(timing=True)
+    s64.soft_force(initial=False, use_f64=True)
+    st64 = s64.stats()
+    q64 = s64.particles()
+    s32 = gpu.System(p, q32, time_sys=st["time_sys"])
+    s32.reset_stats(timing=True)
+    s32.soft_force(initial=False, use_f64=False)
+    st32 = s32.stats()
+    q32b = s32.particles()
+    a = q32b[np.argsort(q32b["id"])]["acc"]
+    b = q64[np.argsort(q64["id"])]["acc"]
+    rel = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+    acc_rms = float(np.sqrt(np.mean(rel ** 2)))
+    ips32 = (pairs_ep + pairs_sp) / (force_ms * 1e-3)
+    ips64 = (st64["n_epep_near"] + st64["n_epep_far"] + st64["n_epsp"]) / (st64["force_ms"] * 1e-3)
+    s64.close()
+    s32.close()
+
+    de = (e1[0] - e0[0] - (e1[5] - e0[5])) / e0[0]
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 soft force, f64 hard/correction",
+        "data": "synthetic",
+        "config": {"workload": "full P3T step: planetesimal disk N=%d, m=1e-12, a=0.5-2 AU, "
+                               "theta=0.5 quadrupole, n_group<=256, dt_tree=2^-4, collisions on" % args.n,
+                   "n": args.n, "seed": args.seed, "device": name},
+        "roofline": roofline,
+        "soft_force": {"interactions_per_s_fp32": round(ips32, 1), "interactions_per_s_fp64": round(ips64, 1),
+                       "fp32_over_fp64": round(ips32 / ips64, 3), "acc_rms_rel_err_fp32_vs_fp64": acc_rms,
+                       "epep_pairs": pairs_ep, "epsp_pairs": pairs_sp, "groups": st["n_groups"]},
+        "breakdown_ms_per_step": {k: round(st[k] / args.steps, 3) for k in ("force_ms", "tree_ms",
+                                                                             "finalize_ms", "hard_ms")},
+        "hard": {"clusters": st["n_clusters"], "cluster_particles": st["n_cluster_particles"],
+                 "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
+                 "collisions": st["n_collisions"]},
+        "energy_error_over_timed_steps": de,
+    }
+    s.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(args.n, args.seed, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -758,6 +758,7 @@ int gplum_system_stats(gplum_system *s, gplum_stats *out) {
     out->n_epep_near = S->last_epep_near;
     out->n_epep_far = S->last_epep_far;
     out->n_epsp = S->last_epsp;
+    out->n_sp_entries = S->tree.n_sp_total;
     out->n_groups = S->last_groups;
     out->n_candidates = S->last_ncand;
     out->n_neighbors = S->nb.n_nbr;

@@ -68,14 +68,30 @@ def lib():
         L.gplum_system_neighbors.argtypes = [vp, vp, vp, i32, ctypes.POINTER(i32)]
         L.gplum_system_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.gplum_system_reset_stats.argtypes = [vp, i32]
+        L.gplum_hard_integrate.argtypes = [P, vp, i32, vp, i32, i32, ctypes.POINTER(f64),
+                                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(f64)]
         _lib = L
     return _lib
+
+
+def hard_integrate(p, pp, offsets, persistent=True):
+    """Integrate clusters pp[offsets[c]:offsets[c+1]] over one soft step (in place).
+    Returns (edisp, block_steps, kernel_ms)."""
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    ed = ctypes.c_double(0.0)
+    nb = ctypes.c_int64(0)
+    ms = ctypes.c_double(0.0)
+    check(lib().gplum_hard_integrate(ctypes.byref(p), abi.ptr(pp), len(pp), abi.ptr(off), len(off) - 1,
+                                     1 if persistent else 0, ctypes.byref(ed), ctypes.byref(nb),
+                                     ctypes.byref(ms)))
+    return ed.value, nb.value, ms.value
 
 
 class Stats(ctypes.Structure):
     """gplum_stats (include/gplum_system.h)."""
     _fields_ = [(n, ctypes.c_int64) for n in (
-        "steps", "n_epep_near", "n_epep_far", "n_epsp", "n_groups", "n_candidates", "n_neighbors",
+        "steps", "n_epep_near", "n_epep_far", "n_epsp", "n_sp_entries", "n_groups", "n_candidates",
+        "n_neighbors",
         "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
         "cand_retries", "n_force_calls")] + [(n, ctypes.c_double) for n in (
             "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys")]

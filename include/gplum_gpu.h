@@ -144,6 +144,28 @@ int gplum_calc_force_epsp_quad_f64(const gplum_epi *epi, int ni,
                                    gplum_force *force, float eps2,
                                    double *acc64, double *phi64);
 
+/* ------------------------------------------------- FDPS MultiWalk interface */
+
+/* eps2 used by the MultiWalk kernels (the PIKG functors take it as constructor
+ * argument: CalcForceLongEPEP(FP_t::eps2), src/gravity_kernel.hpp:17-22). */
+int gplum_multiwalk_set_eps2(float eps2);
+
+/* Dispatch functor of PS::TreeForForce::calcForceAllAndWriteBackMultiWalk
+ * (called at the sites of calcForceAllAndWriteBack, src/main_p3t.cpp:351-356,
+ * 583-588, 672-677): n_walk i-groups with their EP and SP (quadrupole) lists are
+ * packed, uploaded and launched as one EP-EP + EP-SP kernel on the tag's stream;
+ * returns without waiting.  Same argument meaning as the FDPS dispatch signature
+ * S32 dispatch(S32 tag, S32 n_walk, const EPI**, const S32*, const EPJ**,
+ * const S32*, const SPJ**, const S32*). */
+int gplum_multiwalk_dispatch(int tag, int n_walk, const gplum_epi **epi, const int *n_epi,
+                             const gplum_epj **epj, const int *n_epj,
+                             const gplum_spj_quad **spj, const int *n_spj);
+
+/* Retrieve functor: waits for the tag's dispatch and assigns force[w][0..ni[w])
+ * (values start from ForceGrav::clear(), src/particle.h:81-85).  FDPS signature
+ * S32 retrieve(S32 tag, S32 n_walk, const S32* ni, Force** force). */
+int gplum_multiwalk_retrieve(int tag, int n_walk, const int *ni, gplum_force **force);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,7 @@ typedef struct gplum_particle {
 typedef struct gplum_stats {
     int64_t steps;
     int64_t n_epep_near, n_epep_far, n_epsp;  /* pair evaluations of the last soft force */
+    int64_t n_sp_entries;                     /* super-particles staged (sum over groups) */
     int64_t n_groups, n_candidates, n_neighbors;
     int64_t n_clusters, n_cluster_particles, max_cluster;
     int64_t block_steps;                      /* Hermite block steps, summed over clusters */
@@ -94,6 +95,18 @@ int gplum_system_neighbors(gplum_system *s, int32_t *offset, int32_t *index, int
 
 int gplum_system_stats(gplum_system *s, gplum_stats *out);
 int gplum_system_reset_stats(gplum_system *s, int timing);
+
+/* Hard integration of caller-provided clusters (host arrays): cluster c is
+ * pp[offsets[c] .. offsets[c+1]).  Each cluster advances one soft step dt_tree as
+ * timeIntegrate_multi does (src/hermite.h:224-479) from the particles' current
+ * pos/vel/dt/acc0/r_out_inv (FPHard(FPGrav) local time, src/particle.h:1096-1116),
+ * replacing the cluster loops of HardSystem::timeIntegrate (src/hard.h:583-788).
+ * mode 1: one persistent batched launch (largest cluster first); mode 0: one
+ * launch per cluster (plain comparator).  Outputs: edisp (collision energy),
+ * block_steps (summed), kernel_ms (HIP events around the launches). */
+int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, int n, const int32_t *offsets,
+                         int n_cluster, int mode, double *edisp, int64_t *block_steps,
+                         double *kernel_ms);
 
 #ifdef __cplusplus
 }

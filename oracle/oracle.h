@@ -105,6 +105,8 @@ void orc_calc_dt_initial(const gplum_params *p, gplum_particle *pi);
  * *edisp receives the energy dissipated (src/collisionA.h:358-669). */
 int orc_hermite_cluster(const gplum_params *p, gplum_particle *pp, int n,
                         double *edisp, int64_t *n_block_steps);
+/* Test helper: Sun + hard acc/jerk and calcDeltatInitial for one cluster. */
+void orc_cluster_prepare(const gplum_params *p, gplum_particle *pp, int n);
 /* timeIntegrateKepler_isolated (src/hermite.h:787-815), kepler.h:3-93. */
 void orc_kepler_isolated(const gplum_params *p, gplum_particle *pi,
                          double time_start, double time_end);

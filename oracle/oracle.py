@@ -56,6 +56,7 @@ def lib():
         L.orc_hermite_cluster.restype = i32
         L.orc_kepler_isolated.argtypes = [P, vp, f64, f64]
         L.orc_hermite_isolated.argtypes = [P, vp]
+        L.orc_cluster_prepare.argtypes = [P, vp, i32]
         L.orc_solve_kepler.argtypes = [f64, f64]
         L.orc_solve_kepler.restype = f64
         L.orc_set_cutoff_radii.argtypes = [P, vp, i32]
@@ -205,6 +206,27 @@ def hermite_cluster(p, pp):
     ncol = lib().orc_hermite_cluster(ctypes.byref(p), abi.ptr(pp), len(pp), ctypes.byref(ed),
                                      ctypes.byref(nb))
     return ncol, ed.value, nb.value
+
+
+def cluster_prepare(p, pp, offsets):
+    """Sun + hard acc/jerk and initial block steps for every cluster (in place)."""
+    for c in range(len(offsets) - 1):
+        a, b = int(offsets[c]), int(offsets[c + 1])
+        lib().orc_cluster_prepare(ctypes.byref(p), ctypes.c_void_p(pp.ctypes.data + a * pp.itemsize), b - a)
+
+
+def hermite_clusters(p, pp, offsets):
+    """orc_hermite_cluster over every cluster (in place); returns (n_col, edisp, block_steps)."""
+    nc, ed, nb = 0, 0.0, 0
+    for c in range(len(offsets) - 1):
+        a, b = int(offsets[c]), int(offsets[c + 1])
+        sub = pp[a:b].copy()
+        r = hermite_cluster(p, sub)
+        pp[a:b] = sub
+        nc += r[0]
+        ed += r[1]
+        nb += r[2]
+    return nc, ed, nb
 
 
 def kepler_isolated(p, pp, t0, t1):

@@ -486,6 +486,20 @@ int orc_hermite_cluster(const gplum_params *p, gplum_particle *part, int n,
     return n_col;
 }
 
+/* Test helper: state a cluster as correctForceLongInitial leaves it -- Sun and hard
+ * acc/jerk (calcGravity, gravity_hard.h:203-256) then calcDeltatInitial. */
+void orc_cluster_prepare(const gplum_params *p, gplum_particle *part, int n)
+{
+    orc_hp *pp = (orc_hp *)calloc((size_t)n, sizeof(orc_hp));
+    for (int i = 0; i < n; i++) { pp[i].b = part[i]; pp[i].twin_of = -1; }
+    for (int i = 0; i < n; i++) calc_gravity_full(p, pp, n, i);
+    for (int i = 0; i < n; i++) {
+        orc_calc_dt_initial(p, &pp[i].b);
+        part[i] = pp[i].b;
+    }
+    free(pp);
+}
+
 /* ------------------------------------------------------------- Kepler */
 
 /* solveKeplerEq, kepler.h:5-38 (Q8: fabs). */
