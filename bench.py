@@ -52,14 +52,16 @@ def cpu_baseline(n, seed, threads):
     p = O.params(**params_dict())
     pp = make_disk(n, seed)
     run = O.P3T(p, pp)
+    steps = 2
     t0 = time.perf_counter()
-    run.step()
+    for _ in range(steps):
+        run.step()
     dt = time.perf_counter() - t0
     run.close()
-    return {"value": n / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "one full soft step of the same N=%d disk after initialisation, "
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d full soft steps of the same N=%d disk after initialisation, "
                       "oracle C restatement (PIKG-order FP32 soft kernels, FP64 correction/Hermite/"
-                      "Kepler, OpenMP over groups and clusters), %.2f s" % (n, dt)}
+                      "Kepler, OpenMP over groups and clusters), %.2f s" % (steps, n, dt)}
 
 
 def main():
@@ -108,8 +110,11 @@ def main():
     pairs_sp = st["n_epsp"]
     flops = FLOP_EPEP * pairs_ep + FLOP_EPSP_QUAD * pairs_sp + FLOP_SPJ_QUAD * st["n_sp_entries"]
     achieved = flops / (force_ms * 1e-3) / 1e12
-    roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+    # compute-bound kernel: "mfma" is the compute roof of the schema; the arithmetic is FP32
+    # VALU (pairwise gravity is not a contraction), priced against the FP32 vector peak
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "compute": "fp32 VALU, no MFMA",
                 "kernel": "k_soft_f32 (EP-EP near/far + EP-SP quadrupole, one launch per force)",
                 "flops_per_launch": flops, "avg_launch_ms": round(force_ms, 4)}
 

@@ -8,6 +8,7 @@
 
 #include "../../include/gplum_gpu.h"
 #include "../../include/gplum_system.h"
+#include "../../include/gplum_forecast.h"
 
 static_assert(sizeof(gplum_neighbor_info) == 16, "NeighborInfo layout");
 static_assert(sizeof(gplum_force) == 32, "ForceGrav layout");

@@ -260,6 +260,26 @@ __global__ void k_segment_offsets(int m, const int32_t *__restrict__ flag, const
     (void)size_neg;
 }
 
+// Members of each cluster in ascending id (the reference's particle-array order on one
+// rank): rank of member a inside its cluster, then scatter.  Clusters are small, so
+// the O(n_c^2) rank count costs less than a segmented sort.
+__global__ void k_order_by_id(int m, const int32_t *__restrict__ flag, const int32_t *__restrict__ scan,
+                              const int32_t *__restrict__ off, const int32_t *__restrict__ index,
+                              const int64_t *__restrict__ id, int32_t *__restrict__ index_out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= m) return;
+    const int c = scan[a] + flag[a] - 1;
+    const int b0 = off[c], b1 = off[c + 1];
+    const int ia = index[a];
+    const int64_t ida = id[ia];
+    int r = 0;
+    for (int b = b0; b < b1; b++) {
+        const int64_t idb = id[index[b]];
+        r += (idb < ida || (idb == ida && b < a)) ? 1 : 0;
+    }
+    index_out[b0 + r] = ia;
+}
+
 __global__ void k_cluster_sizes(int ncl, const int32_t *__restrict__ off, int32_t *__restrict__ key,
                                 int32_t *__restrict__ idx) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -387,6 +407,10 @@ int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch 
     GPLUM_ENSURE(cl.off, ncl + 1);
     hipLaunchKernelGGL(k_segment_offsets, dim3(ceil_div(m + 1, 256)), dim3(256), 0, st, m, cl.sflag.p,
                        cl.sscan.p, cl.off.p, nullptr);
+    GPLUM_ENSURE(cl.index_tmp, m);
+    hipLaunchKernelGGL(k_order_by_id, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.sflag.p, cl.sscan.p,
+                       cl.off.p, cl.index.p, ps.id.p, cl.index_tmp.p);
+    cl.index.swap(cl.index_tmp);
     // processing order: largest cluster first
     GPLUM_ENSURE(cl.size_key, ncl);
     GPLUM_ENSURE(cl.size_key_sorted, ncl);

@@ -26,7 +26,7 @@ struct NbState {
 // Hard clusters: particles with neighbours sorted by (id_cluster, index)
 // (HardSystem::makeList, src/hard.h:321-438).
 struct ClusterList {
-    DArray<int32_t> parent, cflag, cscan, idx_sel, index, sflag, sscan, off;
+    DArray<int32_t> parent, cflag, cscan, idx_sel, index, index_tmp, sflag, sscan, off;
     DArray<uint64_t> ckey, key_sel, key_sorted;
     DArray<int32_t> size_key, size_key_sorted, order_in, order;
     int m = 0, ncl = 0, max_size = 0;
@@ -41,7 +41,8 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
                       bool initial, NbState &nb, Scratch &s, hipStream_t st);
 
 // createNeighborCluster + makeList: id_cluster = min id of the component; clusters
-// of the particles with neighbours, offsets cl.off[0..ncl], order largest first.
+// of the particles with neighbours (members by ascending id), offsets cl.off[0..ncl],
+// processing order largest first.
 int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch &s, hipStream_t st);
 
 }  // namespace gplum

@@ -41,13 +41,30 @@ struct IState {                 // i-particle registers
     int id, rank;
 };
 
-struct Acc32 {                  // per-tile FP32 partial sums
+// FP32 partial sums are folded into the FP64 accumulators every SOFT_FOLD pairs per
+// lane: with FP32 pair terms that largely cancel (a particle inside a disk), longer
+// FP32 runs double the rms error of the sum.
+constexpr int SOFT_FOLD = 32;
+
+struct Acc32 {                  // per-fold FP32 partial sums
     float x = 0.f, y = 0.f, z = 0.f, p = 0.f;
 };
 
 __device__ inline void split(double d, float &hi, float &lo) {
     hi = (float)d;
     lo = (float)(d - (double)hi);
+}
+
+// 1/sqrt(r2) to about 0.5 ulp: v_rsq_f32 (1 ulp) and one Newton step whose residual
+// 1 - r2 y^2 is formed exactly (r2*y split into h + eh with an FMA), so the step adds
+// only its final rounding.  Per pair this halves the error of rinv^3 against the
+// textbook step y (3 - r2 y^2) / 2.
+__device__ inline float rsqrt_refined(float r2) {
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float h = r2 * y;
+    const float eh = __builtin_fmaf(r2, y, -h);
+    const float e = __builtin_fmaf(-eh, y, __builtin_fmaf(-h, y, 1.0f));
+    return __builtin_fmaf(0.5f * y, e, y);
 }
 
 // EP-EP pair with neighbour predicate, r_out clamp and candidate emission.
@@ -84,9 +101,7 @@ __device__ inline void epep_near(const IState &I, bool valid, float4 pj, float4 
     const float rx = rxp + (lj.x - I.xl), ry = ryp + (lj.y - I.yl), rz = rzp + (lj.z - I.zl);
     const float r2f = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, eps2)));
     const float r2 = fmaxf(r2f, fmaxf(I.ro2, aj.x));
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    const float tmp = 3.0f - r2 * (rinv * rinv);
-    rinv *= (tmp * 0.5f);
+    const float rinv = rsqrt_refined(r2);
     const float mr = pj.w * rinv;
     const float mr3 = (rinv * rinv) * mr;
     a.x = __builtin_fmaf(mr3, rx, a.x);
@@ -101,9 +116,7 @@ __device__ inline void epep_far(const IState &I, float4 pj, float4 lj, float eps
     const float ry = (pj.y - I.yh) + (lj.y - I.yl);
     const float rz = (pj.z - I.zh) + (lj.z - I.zl);
     const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, eps2)));
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    const float tmp = 3.0f - r2 * (rinv * rinv);
-    rinv *= (tmp * 0.5f);
+    const float rinv = rsqrt_refined(r2);
     const float mr = pj.w * rinv;
     const float mr3 = (rinv * rinv) * mr;
     a.x = __builtin_fmaf(mr3, rx, a.x);
@@ -119,9 +132,7 @@ __device__ inline void epsp_pair(const IState &I, float4 pj, float4 lj, float4 q
     const float ry = (pj.y - I.yh) + (lj.y - I.yl);
     const float rz = (pj.z - I.zh) + (lj.z - I.zl);
     const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, eps2)));
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    const float tmp = 3.0f - r2 * (rinv * rinv);
-    rinv *= (tmp * 0.5f);
+    const float rinv = rsqrt_refined(r2);
     const float r2inv = rinv * rinv;
     if (QUAD) {
         const float r3inv = r2inv * rinv;
@@ -209,11 +220,14 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
             if (t < cnt) stage_j(js, G.j_begin + base + t);
             __syncthreads();
             if (S.in_split) {
-                Acc32 a;
-                for (int k = S.sp; k < cnt; k += S.nsplit)
-                    epep_near<(MODE & EMIT) != 0>(I, S.valid, s_a[k], s_b[k], s_l[k], eps2, a, ngb, rankf,
-                                                  idmax, idmin, cand);
-                ax += a.x; ay += a.y; az += a.z; pot += a.p;
+                for (int k0 = S.sp; k0 < cnt; k0 += SOFT_FOLD * S.nsplit) {
+                    Acc32 a;
+                    const int k1 = min(cnt, k0 + SOFT_FOLD * S.nsplit);
+                    for (int k = k0; k < k1; k += S.nsplit)
+                        epep_near<(MODE & EMIT) != 0>(I, S.valid, s_a[k], s_b[k], s_l[k], eps2, a, ngb,
+                                                      rankf, idmax, idmin, cand);
+                    ax += a.x; ay += a.y; az += a.z; pot += a.p;
+                }
             }
         }
         // far list (engine only): beyond 1.1 max(r_search) of the whole group
@@ -223,9 +237,12 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
             if (t < cnt) stage_j(jf, G.jf_begin + base + t);
             __syncthreads();
             if (S.in_split) {
-                Acc32 a;
-                for (int k = S.sp; k < cnt; k += S.nsplit) epep_far(I, s_a[k], s_l[k], eps2, a);
-                ax += a.x; ay += a.y; az += a.z; pot += a.p;
+                for (int k0 = S.sp; k0 < cnt; k0 += SOFT_FOLD * S.nsplit) {
+                    Acc32 a;
+                    const int k1 = min(cnt, k0 + SOFT_FOLD * S.nsplit);
+                    for (int k = k0; k < k1; k += S.nsplit) epep_far(I, s_a[k], s_l[k], eps2, a);
+                    ax += a.x; ay += a.y; az += a.z; pot += a.p;
+                }
             }
         }
     }
@@ -254,10 +271,13 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
             }
             __syncthreads();
             if (S.in_split) {
-                Acc32 a;
-                for (int k = S.sp; k < cnt; k += S.nsplit)
-                    epsp_pair<(MODE & SPQ) != 0>(I, s_a[k], s_l[k], s_b[k], s_c[k], eps2, a);
-                ax += a.x; ay += a.y; az += a.z; pot += a.p;
+                for (int k0 = S.sp; k0 < cnt; k0 += SOFT_FOLD * S.nsplit) {
+                    Acc32 a;
+                    const int k1 = min(cnt, k0 + SOFT_FOLD * S.nsplit);
+                    for (int k = k0; k < k1; k += S.nsplit)
+                        epsp_pair<(MODE & SPQ) != 0>(I, s_a[k], s_l[k], s_b[k], s_c[k], eps2, a);
+                    ax += a.x; ay += a.y; az += a.z; pot += a.p;
+                }
             }
         }
     }

@@ -70,6 +70,10 @@ def lib():
         L.gplum_system_reset_stats.argtypes = [vp, i32]
         L.gplum_hard_integrate.argtypes = [P, vp, i32, vp, i32, i32, ctypes.POINTER(f64),
                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(f64)]
+        L.gplum_mlp_forward.argtypes = [vp, vp, i32, vp]
+        L.gplum_mlp_loss_grad.argtypes = [vp, vp, vp, i32, ctypes.POINTER(f64), vp]
+        L.gplum_mlp_train.argtypes = [vp, vp, vp, i32, i32, f32, ctypes.POINTER(f64),
+                                      ctypes.POINTER(f64)]
         _lib = L
     return _lib
 
@@ -240,3 +244,37 @@ def calc_force_epsp_quad_f64(epi, spj, force, eps2=0.0):
     check(lib().gplum_calc_force_epsp_quad_f64(abi.ptr(epi), len(epi), abi.ptr(spj), len(spj),
                                                abi.ptr(force), eps2, abi.ptr(acc), abi.ptr(phi)))
     return acc, phi
+
+
+# ------------------------------------------------------------ forecaster network
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_forward(theta, x):
+    """rate[i] for features x[n, 4] (converted to the kernels' feature-major layout)."""
+    theta, xt = _f32(theta), _f32(np.asarray(x).T)
+    n = xt.shape[1]
+    out = np.zeros(n, np.float32)
+    check(lib().gplum_mlp_forward(abi.ptr(theta), abi.ptr(xt), n, abi.ptr(out)))
+    return out
+
+
+def mlp_loss_grad(theta, x, y):
+    theta, xt, y = _f32(theta), _f32(np.asarray(x).T), _f32(y)
+    n = xt.shape[1]
+    grad = np.zeros(len(theta), np.float32)
+    loss = ctypes.c_double(0.0)
+    check(lib().gplum_mlp_loss_grad(abi.ptr(theta), abi.ptr(xt), abi.ptr(y), n, ctypes.byref(loss),
+                                    abi.ptr(grad)))
+    return loss.value, grad
+
+
+def mlp_train(theta, x, y, n_steps, lr=3e-3):
+    """Full-batch Adam on the device; returns (theta, loss_before, loss_after)."""
+    theta, xt, y = _f32(theta).copy(), _f32(np.asarray(x).T), _f32(y)
+    l0, l1 = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    check(lib().gplum_mlp_train(abi.ptr(theta), abi.ptr(xt), abi.ptr(y), xt.shape[1], n_steps, lr,
+                                ctypes.byref(l0), ctypes.byref(l1)))
+    return theta, l0.value, l1.value

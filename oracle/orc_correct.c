@@ -203,23 +203,35 @@ void orc_neighbor_cluster(gplum_particle *pp, int n, const orc_nlist *nl)
     }
 }
 
-typedef struct { int64_t cid; int32_t idx; } cidkey;
+typedef struct { int64_t cid, id; int32_t idx; } cidkey;
 static int cmp_cid(const void *a, const void *b)
 {
     const cidkey *x = (const cidkey *)a, *y = (const cidkey *)b;
     if (x->cid < y->cid) return -1;
     if (x->cid > y->cid) return 1;
+    if (x->id < y->id) return -1;
+    if (x->id > y->id) return 1;
     return (x->idx < y->idx) ? -1 : (x->idx > y->idx);
 }
 
-/* HardSystem::makeList (hard.h:321-438) on one rank: keys of particles with
- * neighbours sorted by (id_cluster, index), segmented into clusters. */
+/* HardSystem::makeList (hard.h:321-438) on one rank: particles with neighbours,
+ * grouped by id_cluster.  The reference lists a cluster's members in particle-array
+ * order, which on one rank is the order of the initial conditions (ascending id)
+ * until FDPS removes merged particles; that order decides the merge target between
+ * equal masses (collisionDetermination, hermite.h:116-139) and the FP64 summation
+ * order inside the cluster.  Here members are listed by ascending id, so the result
+ * does not depend on how particles are laid out in memory. */
 int orc_make_cluster_list(const gplum_particle *pp, int n, int32_t *cl_offset, int32_t *cl_index)
 {
     cidkey *k = (cidkey *)malloc(sizeof(cidkey) * (n > 0 ? n : 1));
     int nk = 0;
     for (int i = 0; i < n; i++)
-        if (pp[i].neighbor.number) { k[nk].cid = pp[i].id_cluster; k[nk].idx = i; nk++; }
+        if (pp[i].neighbor.number) {
+            k[nk].cid = pp[i].id_cluster;
+            k[nk].id = pp[i].id;
+            k[nk].idx = i;
+            nk++;
+        }
     qsort(k, nk, sizeof(cidkey), cmp_cid);
     int nc = 0;
     for (int a = 0; a < nk; a++) {
