@@ -1,0 +1,273 @@
+// forecast.hip -- learned sizing of the EP-EP candidate buffer (see forecast.hpp).
+#include <cmath>
+#include <vector>
+
+#include "forecast.hpp"
+
+namespace gplum {
+
+constexpr int FC_SCAN_BLOCK = 1024;
+
+// Exclusive scan over one block of FC_SCAN_BLOCK threads (16 waves); *tot receives
+// the block total.  s_w holds 16 wave totals.
+__device__ inline long long block_excl_scan(long long v, long long *s_w, long long *tot) {
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    constexpr int NW = FC_SCAN_BLOCK / 64;
+    long long x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        long long t = lane < NW ? s_w[lane] : 0;
+        for (int d = 1; d < NW; d <<= 1) {
+            const long long y = __shfl_up(t, d);
+            if (lane >= d) t += y;
+        }
+        if (lane < NW) s_w[lane] = t;
+    }
+    __syncthreads();
+    const long long excl = x - v + (w > 0 ? s_w[w - 1] : 0);
+    *tot = s_w[NW - 1];
+    __syncthreads();
+    return excl;
+}
+
+// Features of the particles of group g (one 64-thread block per group):
+//  0  log density of the group's near list: (n_near + 1) / V, V = the group's
+//     particle box grown by 1.1 max r_search on every side
+//  1  log velocity dispersion, 2  log mass, 3  log Hill radius (m / 3 M_sun)^(1/3) |r|
+__global__ __launch_bounds__(64) void k_fc_features(int ng, FcGroupView gv, const double *__restrict__ pos,
+                                                    const double *__restrict__ mass,
+                                                    const double *__restrict__ vdisp, double m_sun, FcNorm nrm,
+                                                    int n, float *__restrict__ x) {
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int node = gv.gnode[g];
+    const double grow = 2.2 * gv.rs[node];
+    double vol = 1.;
+    for (int d = 0; d < 3; d++) vol *= (gv.bmax[3 * node + d] - gv.bmin[3 * node + d]) + grow;
+    const float f0 = (float)log(((double)gv.ep_cnt[g] + 1.) / fmax(vol, 1e-300));
+    const int i0 = gv.gstart[g], cnt = gv.gcount[g];
+    for (int k = (int)threadIdx.x; k < cnt; k += 64) {
+        const int i = i0 + k;
+        const double m = mass[i];
+        const double r = sqrt(pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1] + pos[3 * i + 2] * pos[3 * i + 2]);
+        const float f[MLP_NIN] = {f0, (float)log(fmax(vdisp[i], 1e-30)), (float)log(fmax(m, 1e-300)),
+                                  (float)log(fmax(cbrt(m / (3. * m_sun)) * r, 1e-300))};
+        for (int q = 0; q < MLP_NIN; q++) x[(size_t)q * n + i] = (f[q] - nrm.mu[q]) * nrm.inv_sd[q];
+    }
+}
+
+__global__ void k_fc_normalize(int n, FcNorm nrm, float *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = 0; q < MLP_NIN; q++) x[(size_t)q * n + i] = (x[(size_t)q * n + i] - nrm.mu[q]) * nrm.inv_sd[q];
+}
+
+// capacity of group g: expected count S plus three Poisson standard deviations and 8
+__global__ __launch_bounds__(64) void k_fc_capacity(int ng, const int32_t *__restrict__ gstart,
+                                                    const int32_t *__restrict__ gcount, const float *__restrict__ lam,
+                                                    int32_t *__restrict__ cap) {
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    double s = 0.;
+    for (int k = (int)threadIdx.x; k < gcount[g]; k += 64) s += (double)lam[gstart[g] + k];
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d);
+    if (threadIdx.x == 0) cap[g] = (int32_t)fmin(ceil(s + 3. * sqrt(s) + 8.), 1e9);
+}
+
+// one block: exclusive offsets of the capacities, clipped to the allocated slots
+__global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *__restrict__ cap,
+                                                              int32_t *__restrict__ off, long long slots,
+                                                              int64_t *__restrict__ info) {
+    __shared__ long long s_w[FC_SCAN_BLOCK / 64];
+    long long carry = 0;
+    for (int base = 0; base < ng; base += FC_SCAN_BLOCK) {
+        const int g = base + (int)threadIdx.x;
+        const long long v = g < ng ? cap[g] : 0;
+        long long tot;
+        const long long e = block_excl_scan(v, s_w, &tot);
+        if (g < ng) {
+            const long long o = carry + e;
+            off[g] = (int32_t)(o < slots ? o : slots);
+            const long long room = slots - o;
+            cap[g] = (int32_t)(room <= 0 ? 0 : (v < room ? v : room));
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) info[0] = carry;
+}
+
+// one block: groups whose exact count fits keep their segment (compacted offsets),
+// the others are listed for the candidate-only pass
+__global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int32_t *__restrict__ cnt,
+                                                              const int32_t *__restrict__ cap,
+                                                              int32_t *__restrict__ comp_off,
+                                                              int32_t *__restrict__ ovf_list,
+                                                              int64_t *__restrict__ info) {
+    __shared__ long long s_w[FC_SCAN_BLOCK / 64];
+    long long cs = 0, co = 0, ct = 0;
+    for (int base = 0; base < ng; base += FC_SCAN_BLOCK) {
+        const int g = base + (int)threadIdx.x;
+        const long long c = g < ng ? cnt[g] : 0;
+        const bool ovf = g < ng && c > cap[g];
+        long long ts, to, tt;
+        const long long es = block_excl_scan(ovf ? 0 : c, s_w, &ts);
+        const long long eo = block_excl_scan(ovf ? 1 : 0, s_w, &to);
+        (void)block_excl_scan(ovf ? c : 0, s_w, &tt);
+        if (g < ng) {
+            comp_off[g] = (int32_t)(cs + es);
+            if (ovf) ovf_list[co + eo] = g;
+        }
+        cs += ts; co += to; ct += tt;
+    }
+    if (threadIdx.x == 0) { info[1] = cs; info[2] = co; info[3] = ct; }
+}
+
+__global__ __launch_bounds__(64) void k_fc_compact(int ng, const int32_t *__restrict__ cnt,
+                                                   const int32_t *__restrict__ cap,
+                                                   const int32_t *__restrict__ seg_off,
+                                                   const int32_t *__restrict__ comp_off,
+                                                   const int2 *__restrict__ seg, int2 *__restrict__ out) {
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int c = cnt[g];
+    if (c > cap[g]) return;
+    for (int k = (int)threadIdx.x; k < c; k += 64) out[comp_off[g] + k] = seg[seg_off[g] + k];
+}
+
+__global__ void k_fc_gather_groups(int m, const int32_t *__restrict__ list, const SoftGroup *__restrict__ groups,
+                                   SoftGroup *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) out[k] = groups[list[k]];
+}
+
+__global__ void k_fc_sample(int m, int n, long long stride, long long phase, const float *__restrict__ x,
+                            const int32_t *__restrict__ cnt, float *__restrict__ xs, float *__restrict__ ys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    const int i = (int)((phase + (long long)s * stride) % n);
+    for (int q = 0; q < MLP_NIN; q++) xs[(size_t)q * m + s] = x[(size_t)q * n + i];
+    ys[s] = (float)cnt[i];
+}
+
+__global__ void k_fc_fill(int n, float v, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
+// ------------------------------------------------------------------ host
+
+int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, const double *mass,
+                const double *vdisp, double m_sun, int n, hipStream_t st) {
+    GPLUM_ENSURE(s.x, (size_t)MLP_NIN * n);
+    FcNorm nrm = s.normalized ? s.norm : FcNorm{{0.f, 0.f, 0.f, 0.f}, {1.f, 1.f, 1.f, 1.f}};
+    if (ng > 0)
+        hipLaunchKernelGGL(k_fc_features, dim3(ng), dim3(64), 0, st, ng, gv, pos, mass, vdisp, m_sun, nrm, n,
+                           s.x.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hipStream_t st) {
+    GPLUM_ENSURE(s.lam, n);
+    GPLUM_ENSURE(s.seg_off, ng + 1);
+    GPLUM_ENSURE(s.seg_cap, ng + 1);
+    GPLUM_ENSURE(s.seg_cnt, ng + 1);
+    GPLUM_ENSURE(s.info, 4);
+    if (s.seg_capacity > (size_t)INT32_MAX) s.seg_capacity = INT32_MAX;
+    GPLUM_ENSURE(s.pairs_seg, s.seg_capacity + 1);
+    if (s.uniform) {
+        hipLaunchKernelGGL(k_fc_fill, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.uniform_rate, s.lam.p);
+    } else {
+        int rc = s.net.forward(s.x.p, n, s.lam.p, st);
+        if (rc) return rc;
+    }
+    if (ng > 0) {
+        hipLaunchKernelGGL(k_fc_capacity, dim3(ng), dim3(64), 0, st, ng, gv.gstart, gv.gcount, s.lam.p,
+                           s.seg_cap.p);
+        hipLaunchKernelGGL(k_fc_offsets, dim3(1), dim3(FC_SCAN_BLOCK), 0, st, ng, s.seg_cap.p, s.seg_off.p,
+                           (long long)s.seg_capacity, s.info.p);
+    }
+    GPLUM_HIP(hipGetLastError());
+    CandBuf c{s.pairs_seg.p, nullptr, 0u};
+    c.seg_off = s.seg_off.p;
+    c.seg_cap = s.seg_cap.p;
+    c.seg_cnt = s.seg_cnt.p;
+    *cb = c;
+    return GPLUM_OK;
+}
+
+int fc_collect(CandSizer &s, int ng, int64_t out[4], hipStream_t st) {
+    GPLUM_ENSURE(s.comp_off, ng + 1);
+    GPLUM_ENSURE(s.ovf_list, ng + 1);
+    if (ng > 0)
+        hipLaunchKernelGGL(k_fc_summary, dim3(1), dim3(FC_SCAN_BLOCK), 0, st, ng, s.seg_cnt.p, s.seg_cap.p,
+                           s.comp_off.p, s.ovf_list.p, s.info.p);
+    GPLUM_HIP(hipGetLastError());
+    GPLUM_HIP(hipMemcpyAsync(out, s.info.p, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    return GPLUM_OK;
+}
+
+int fc_compact(CandSizer &s, int ng, int2 *out, hipStream_t st) {
+    if (ng > 0)
+        hipLaunchKernelGGL(k_fc_compact, dim3(ng), dim3(64), 0, st, ng, s.seg_cnt.p, s.seg_cap.p, s.seg_off.p,
+                           s.comp_off.p, s.pairs_seg.p, out);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int fc_overflow_groups(CandSizer &s, const SoftGroup *groups, int n_ovf, hipStream_t st) {
+    GPLUM_ENSURE(s.ovf_groups, n_ovf + 1);
+    if (n_ovf > 0)
+        hipLaunchKernelGGL(k_fc_gather_groups, dim3(ceil_div(n_ovf, 256)), dim3(256), 0, st, n_ovf, s.ovf_list.p,
+                           groups, s.ovf_groups.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int fc_train(CandSizer &s, const int32_t *count_fp32, int n, hipStream_t st) {
+    if (n <= 0) return GPLUM_OK;
+    int rc = s.net.init_params(st, 12345u);
+    if (rc) return rc;
+    if (!s.normalized) {
+        // standardise the raw features of this first step (host, once)
+        std::vector<float> h((size_t)MLP_NIN * n);
+        GPLUM_HIP(hipMemcpyAsync(h.data(), s.x.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        for (int q = 0; q < MLP_NIN; q++) {
+            double a = 0., b = 0.;
+            for (int i = 0; i < n; i++) a += h[(size_t)q * n + i];
+            a /= n;
+            for (int i = 0; i < n; i++) {
+                const double d = h[(size_t)q * n + i] - a;
+                b += d * d;
+            }
+            const double sd = std::sqrt(b / n);
+            s.norm.mu[q] = (float)a;
+            s.norm.inv_sd[q] = (float)(sd > 1e-6 ? 1. / sd : 1.);
+        }
+        hipLaunchKernelGGL(k_fc_normalize, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.norm, s.x.p);
+        s.normalized = true;
+    }
+    const int m = n < s.batch ? n : s.batch;
+    const long long stride = n / m;
+    GPLUM_ENSURE(s.xs, (size_t)MLP_NIN * m);
+    GPLUM_ENSURE(s.ys, m);
+    for (int k = 0; k < s.train_steps; k++) {
+        const long long phase = (long long)((s.phase++ * 2654435761u) % (unsigned)(stride > 0 ? stride : 1));
+        hipLaunchKernelGGL(k_fc_sample, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, n, stride, phase, s.x.p,
+                           count_fp32, s.xs.p, s.ys.p);
+        double *lp = (k == s.train_steps - 1) ? &s.last_loss : nullptr;
+        if ((rc = s.net.loss_grad(s.xs.p, s.ys.p, m, lp, st))) return rc;
+        if ((rc = s.net.adam_step(st))) return rc;
+    }
+    s.trained_rounds++;
+    return GPLUM_OK;
+}
+
+}  // namespace gplum

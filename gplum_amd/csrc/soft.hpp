@@ -6,10 +6,10 @@
 // combined through LDS at the end.  j-particles / super-particles are staged in LDS
 // tiles of 256 as FP32 coordinates relative to the group's reference particle
 // (difference formed in FP64, rounded once: gravity_kernel_epep.pikg:53,59) and
-// read back as wave-wide broadcasts.  FP32 partial sums of one tile are added into
-// FP64 accumulators at the tile boundary.  Neighbour candidates (pairs with
+// read back as wave-wide broadcasts.  FP32 partial sums are added into FP64
+// accumulators every 32 pairs per lane.  Neighbour candidates (pairs with
 // |r|^2+eps^2 < 1.0201 max(r_search)^2) are compacted per wave with a ballot and a
-// single atomic per wave into a global candidate buffer.
+// single atomic per wave, into a global buffer or into the group's own segment.
 #pragma once
 #include "common.hpp"
 
@@ -36,10 +36,17 @@ struct JRec {
 };
 static_assert(sizeof(JRec) == 56, "JRec");
 
+// Candidate output.  Global mode (seg_off == nullptr): one counter for the launch,
+// pairs[0..cap).  Segment mode: group g appends to pairs[seg_off[g] ..) through a
+// counter in LDS, stores at most seg_cap[g] pairs and writes its exact count to
+// seg_cnt[g] (a count above seg_cap[g] marks the group as overflowed).
 struct CandBuf {
     int2 *pairs;              // (i id_local, j id_local)
     unsigned int *count;      // number of candidates found (may exceed cap)
     unsigned int cap;
+    const int32_t *seg_off = nullptr, *seg_cap = nullptr;
+    int32_t *seg_cnt = nullptr;
+    unsigned int base = 0;    // first slot of this block's output (segment mode)
 };
 
 // ---------------------------------------------------------------- sources
@@ -152,6 +159,9 @@ struct SinkF64 {                      // FP64 sums (+ optional Force, PIKG accum
         }
     }
 };
+struct SinkDiscard {                  // candidate-only passes: results already written
+    __device__ void write(int, double, double, double, double, int, int, int, int, bool) const {}
+};
 struct SinkEngine {                   // engine particle arrays (assign, not accumulate)
     double *acc, *phi;
     int32_t *number, *rank, *id_max, *id_min;
@@ -163,7 +173,9 @@ struct SinkEngine {                   // engine particle arrays (assign, not acc
 };
 
 // --------------------------------------------------------- launch helpers
-enum SoftMode : int { EP = 1, SPQ = 2, SPM = 4, EMIT = 8 };
+// NOFAR: skip the far EP list (it holds no neighbour by construction); used by the
+// candidate-only pass that refills overflowed segments.
+enum SoftMode : int { EP = 1, SPQ = 2, SPM = 4, EMIT = 8, NOFAR = 16 };
 
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f32(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,

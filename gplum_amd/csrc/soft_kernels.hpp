@@ -94,7 +94,7 @@ __device__ inline void epep_near(const IState &I, bool valid, float4 pj, float4 
             b0 = (unsigned)__shfl((int)b0, first);
             if (nb) {
                 const unsigned int pos = b0 + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-                if (pos < cand.cap) cand.pairs[pos] = make_int2(I.id, idj);
+                if (pos < cand.cap) cand.pairs[cand.base + pos] = make_int2(I.id, idj);
             }
         }
     }
@@ -166,7 +166,12 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
     const int g = xcd_group(ngroups);
     if (g >= ngroups) return;
     const SoftGroup G = groups[g];
-    if (G.ni <= 0) return;
+    constexpr bool EMITC = (MODE & EMIT) != 0;
+    const bool seg = EMITC && cand.seg_off != nullptr;
+    if (G.ni <= 0) {
+        if (seg && threadIdx.x == 0) cand.seg_cnt[g] = 0;
+        return;
+    }
 
     __shared__ float4 s_a[SOFT_TILE];
     __shared__ float4 s_b[SOFT_TILE];
@@ -174,6 +179,14 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
     __shared__ float4 s_l[SOFT_TILE];
     __shared__ double s_red[4][SOFT_BLOCK];
     __shared__ int s_redi[4][SOFT_BLOCK];
+    __shared__ unsigned int s_cand;
+    CandBuf cb = cand;
+    if (seg) {
+        if (threadIdx.x == 0) s_cand = 0u;   // visible after the first tile barrier
+        cb.count = &s_cand;
+        cb.base = (unsigned int)cand.seg_off[g];
+        cb.cap = (unsigned int)cand.seg_cap[g];
+    }
 
     double x0, y0, z0;
     {
@@ -224,14 +237,14 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
                     Acc32 a;
                     const int k1 = min(cnt, k0 + SOFT_FOLD * S.nsplit);
                     for (int k = k0; k < k1; k += S.nsplit)
-                        epep_near<(MODE & EMIT) != 0>(I, S.valid, s_a[k], s_b[k], s_l[k], eps2, a, ngb,
-                                                      rankf, idmax, idmin, cand);
+                        epep_near<EMITC>(I, S.valid, s_a[k], s_b[k], s_l[k], eps2, a, ngb, rankf, idmax,
+                                         idmin, cb);
                     ax += a.x; ay += a.y; az += a.z; pot += a.p;
                 }
             }
         }
         // far list (engine only): beyond 1.1 max(r_search) of the whole group
-        for (int base = 0; base < G.njf; base += SOFT_TILE) {
+        for (int base = 0; base < ((MODE & NOFAR) ? 0 : G.njf); base += SOFT_TILE) {
             const int cnt = min(SOFT_TILE, G.njf - base);
             __syncthreads();
             if (t < cnt) stage_j(jf, G.jf_begin + base + t);
@@ -298,6 +311,10 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
     }
     if (S.sp == 0 && S.valid)
         out.write(G.i_begin + S.ii, ax, ay, az, pot, ngb, rankf, idmax, idmin, (MODE & EP) != 0);
+    if (seg) {
+        __syncthreads();
+        if (threadIdx.x == 0) cand.seg_cnt[g] = (int32_t)s_cand;
+    }
 }
 
 // FP64 twin: same structure, double throughout, r_inv = 1/sqrt(r2) (no Newton

@@ -425,42 +425,99 @@ int System::soft_force(bool initial, bool use_f64) {
     hipLaunchKernelGGL(k_soft_groups, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, tree.group_start.p,
                        tree.group_count.p, tree.ep_off.p, tree.ep_cnt.p, tree.epf_off.p, tree.epf_cnt.p,
                        tree.sp_off.p, tree.sp_cnt.p, groups.p);
-    // candidate buffer: sized by the previous step's count (or a fraction of n)
-    if (cand_cap == 0) cand_cap = std::max<unsigned>(1u << 16, (unsigned)(n / 4));
-    GPLUM_ENSURE(cand_count, 1);
     SrcRecI is{jrec.p};
     SrcRecJ jn{jrec.p, tree.ep_list.p};
     SrcRecJ jf{jrec.p, tree.epf_list.p};
     SrcNodeSP sp{tree.node_mass.p, tree.node_com.p, tree.node_quad.p, tree.sp_list.p};
     SinkEngine out{A.acc.p, A.phi.p, A.number.p, A.rank.p, A.id_max.p, A.id_min.p};
+    GPLUM_ENSURE(cand_count, 1);
     unsigned ncand = 0;
-    for (int attempt = 0; attempt < 3; attempt++) {
-        GPLUM_ENSURE(cand, cand_cap);
-        GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
-        CandBuf cb{cand.p, cand_count.p, cand_cap};
+    const bool fc_on = fc.enabled && !use_f64;
+    const FcGroupView gv{tree.group_start.p, tree.group_count.p, tree.group_node.p, tree.ep_cnt.p,
+                         tree.node_bmin.p,   tree.node_bmax.p,    tree.node_rs.p};
+    if (fc_on) {
+        tick(EV_FCA0);
+        if ((rc = fc_features(fc, gv, ng, A.pos.p, A.mass.p, A.v_disp.p, p.m_sun, n, st))) return rc;
+    }
+    if (fc_on && fc.active()) {
+        // learned sizing: one segment per i-group, overflowed groups redone exactly
+        if (fc.seg_capacity == 0) fc.seg_capacity = (size_t)last_ncand + last_ncand / 4 + 16 * (size_t)ng + 4096;
+        if (fc.uniform) fc.uniform_rate = n > 0 ? (float)((double)last_ncand / n) : 0.f;
+        CandBuf cb{nullptr, nullptr, 0u};
+        if ((rc = fc_plan(fc, gv, ng, n, &cb, st))) return rc;
+        tick(EV_FCA1);
         tick(EV_FORCE0);
-        if (!use_f64) {
-            if (p.quadrupole)
-                launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(
-                    groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
-            else
-                launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(
-                    groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
-        } else {
-            if (p.quadrupole)
-                launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ>(groups.p, ng, is, jn, jf, sp,
-                                                                                   out, p.eps2, st);
-            else
-                launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM>(groups.p, ng, is, jn, jf, sp,
-                                                                                   out, p.eps2, st);
-        }
+        if (p.quadrupole)
+            launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(groups.p, ng, is, jn, jf, sp,
+                                                                                  out, (float)p.eps2, cb, st);
+        else
+            launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(groups.p, ng, is, jn, jf, sp,
+                                                                                  out, (float)p.eps2, cb, st);
         tick(EV_FORCE1);
         GPLUM_HIP(hipGetLastError());
-        GPLUM_HIP(hipMemcpyAsync(&ncand, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipStreamSynchronize(st));
-        if (use_f64 || ncand <= cand_cap) break;
-        cand_cap = ncand + ncand / 4 + 1024;   // too small: grow and redo the step
-        stats.cand_retries++;
+        tick(EV_FCB0);
+        int64_t info[4] = {0, 0, 0, 0};
+        if ((rc = fc_collect(fc, ng, info, st))) return rc;
+        const int64_t requested = info[0], stored = info[1], n_ovf = info[2], ovf_total = info[3];
+        if (stored + ovf_total > (int64_t)INT32_MAX) return fail(GPLUM_E_OVERFLOW, "candidate count exceeds 2^31");
+        GPLUM_ENSURE(cand, (size_t)(stored + ovf_total) + 1);
+        if ((rc = fc_compact(fc, ng, cand.p, st))) return rc;
+        if (n_ovf > 0) {
+            if ((rc = fc_overflow_groups(fc, groups.p, (int)n_ovf, st))) return rc;
+            GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
+            CandBuf cb2{cand.p, cand_count.p, (unsigned)ovf_total};
+            cb2.base = (unsigned)stored;
+            launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkDiscard, EP | EMIT | NOFAR>(
+                fc.ovf_groups.p, (int)n_ovf, is, jn, jf, sp, SinkDiscard{}, (float)p.eps2, cb2, st);
+            unsigned got = 0;
+            GPLUM_HIP(hipMemcpyAsync(&got, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            GPLUM_HIP(hipStreamSynchronize(st));
+            if ((int64_t)got != ovf_total)
+                return fail(GPLUM_E_STATE, "candidate-only pass found " + std::to_string(got) + " of " +
+                                               std::to_string(ovf_total) + " candidates");
+        }
+        tick(EV_FCB1);
+        ncand = (unsigned)(stored + ovf_total);
+        stats.fc_calls++;
+        stats.fc_slots += std::min<int64_t>(requested, (int64_t)fc.seg_capacity);
+        stats.fc_ovf_groups += n_ovf;
+        stats.fc_ovf_candidates += ovf_total;
+        stats.fc_ms += elapsed(EV_FCA0, EV_FCA1) + elapsed(EV_FCB0, EV_FCB1);
+        // the plan did not fit the segment storage: grow it for the next call
+        if ((size_t)requested > fc.seg_capacity) fc.seg_capacity = (size_t)requested + (size_t)requested / 8;
+    } else {
+        if (fc_on) tick(EV_FCA1);
+        // candidate buffer: sized by the previous step's count (or a fraction of n)
+        if (cand_cap == 0) cand_cap = std::max<unsigned>(1u << 16, (unsigned)(n / 4));
+        for (int attempt = 0; attempt < 3; attempt++) {
+            GPLUM_ENSURE(cand, cand_cap);
+            GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
+            CandBuf cb{cand.p, cand_count.p, cand_cap};
+            tick(EV_FORCE0);
+            if (!use_f64) {
+                if (p.quadrupole)
+                    launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(
+                        groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
+                else
+                    launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(
+                        groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
+            } else {
+                if (p.quadrupole)
+                    launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ>(groups.p, ng, is, jn, jf,
+                                                                                       sp, out, p.eps2, st);
+                else
+                    launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM>(groups.p, ng, is, jn, jf,
+                                                                                       sp, out, p.eps2, st);
+            }
+            tick(EV_FORCE1);
+            GPLUM_HIP(hipGetLastError());
+            GPLUM_HIP(hipMemcpyAsync(&ncand, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            GPLUM_HIP(hipStreamSynchronize(st));
+            if (use_f64 || ncand <= cand_cap) break;
+            cand_cap = ncand + ncand / 4 + 1024;   // too small: grow and redo the step
+            stats.cand_retries++;
+        }
+        if (fc_on) stats.fc_ms += elapsed(EV_FCA0, EV_FCA1);
     }
     stats.force_ms += elapsed(EV_FORCE0, EV_FORCE1);
     stats.tree_ms += elapsed(EV_TREE0, EV_TREE1);
@@ -486,6 +543,15 @@ int System::soft_force(bool initial, bool use_f64) {
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
     tick(EV_NB1);
     stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
+    if (fc_on && !fc.uniform) {
+        // train on this call's exact FP32 candidate counts
+        tick(EV_FCC0);
+        if ((rc = fc_train(fc, nb.number_fp32.p, n, st))) return rc;
+        tick(EV_FCC1);
+        GPLUM_HIP(hipStreamSynchronize(st));
+        stats.fc_ms += elapsed(EV_FCC0, EV_FCC1);
+        stats.fc_loss = fc.last_loss;
+    }
     have_nbr = true;
     // interaction counts (sum over groups of ni*nj, ni*ns) for the roofline
     std::vector<SoftGroup> hg(ng);
@@ -772,6 +838,52 @@ int gplum_system_reset_stats(gplum_system *s, int timing) {
     System *S = reinterpret_cast<System *>(s);
     std::memset(&S->stats, 0, sizeof(S->stats));
     S->timing = timing != 0;
+    return GPLUM_OK;
+}
+
+int gplum_system_set_forecast(gplum_system *s, int mode, int train_steps, int batch, float lr) {
+    if (!s || mode < 0 || mode > 2 || train_steps < 0 || batch <= 0 || !(lr > 0.f))
+        return gplum::fail(GPLUM_E_ARG, "bad arguments");
+    System *S = reinterpret_cast<System *>(s);
+    S->fc.enabled = mode != 0;
+    S->fc.uniform = mode == 2;
+    S->fc.train_steps = train_steps;
+    S->fc.batch = batch;
+    S->fc.net.lr = lr;
+    return GPLUM_OK;
+}
+
+int gplum_system_get_forecast(gplum_system *s, float *theta, float *mu, float *inv_sd, int *trained) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    System *S = reinterpret_cast<System *>(s);
+    if (trained) *trained = S->fc.trained_rounds;
+    if (theta) {
+        if (!S->fc.net.initialized) return gplum::fail(GPLUM_E_STATE, "forecaster has not been trained");
+        GPLUM_HIP(hipMemcpy(theta, S->fc.net.theta.p, sizeof(float) * gplum::MLP_NPARAM, hipMemcpyDeviceToHost));
+    }
+    for (int q = 0; q < gplum::MLP_NIN; q++) {
+        if (mu) mu[q] = S->fc.norm.mu[q];
+        if (inv_sd) inv_sd[q] = S->fc.norm.inv_sd[q];
+    }
+    return GPLUM_OK;
+}
+
+int gplum_system_put_forecast(gplum_system *s, const float *theta, const float *mu, const float *inv_sd) {
+    if (!s || !theta || !mu || !inv_sd) return gplum::fail(GPLUM_E_ARG, "null argument");
+    System *S = reinterpret_cast<System *>(s);
+    int rc = S->fc.net.ensure(1);
+    if (rc) return rc;
+    GPLUM_HIP(hipMemcpy(S->fc.net.theta.p, theta, sizeof(float) * gplum::MLP_NPARAM, hipMemcpyHostToDevice));
+    GPLUM_HIP(hipMemset(S->fc.net.m.p, 0, sizeof(float) * gplum::MLP_NPARAM));
+    GPLUM_HIP(hipMemset(S->fc.net.v.p, 0, sizeof(float) * gplum::MLP_NPARAM));
+    S->fc.net.adam_t = 0;
+    S->fc.net.initialized = true;
+    for (int q = 0; q < gplum::MLP_NIN; q++) {
+        S->fc.norm.mu[q] = mu[q];
+        S->fc.norm.inv_sd[q] = inv_sd[q];
+    }
+    S->fc.normalized = true;
+    S->fc.trained_rounds = std::max(S->fc.trained_rounds, S->fc.min_rounds);
     return GPLUM_OK;
 }
 

@@ -1,5 +1,6 @@
 // system.hpp -- the device-resident P3T system (see system.hip).
 #pragma once
+#include "forecast.hpp"
 #include "hermite.hpp"
 #include "neighbor.hpp"
 
@@ -19,7 +20,10 @@ struct HardPtrs {
     int32_t *flags;
 };
 
-enum { EV_TREE0, EV_TREE1, EV_FORCE0, EV_FORCE1, EV_NB0, EV_NB1, EV_HARD0, EV_HARD1, EV_COUNT };
+enum {
+    EV_TREE0, EV_TREE1, EV_FORCE0, EV_FORCE1, EV_NB0, EV_NB1, EV_HARD0, EV_HARD1,
+    EV_FCA0, EV_FCA1, EV_FCB0, EV_FCB1, EV_FCC0, EV_FCC1, EV_COUNT
+};
 
 struct System {
     gplum_params p;
@@ -37,6 +41,7 @@ struct System {
     DArray<int2> cand;
     DArray<unsigned> cand_count;
     unsigned cand_cap = 0, last_ncand = 0;
+    CandSizer fc;                     // learned per-group candidate sizing
     NbState nb;
     ClusterList cl;
     Scratch scratch;

@@ -68,6 +68,9 @@ def lib():
         L.gplum_system_neighbors.argtypes = [vp, vp, vp, i32, ctypes.POINTER(i32)]
         L.gplum_system_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.gplum_system_reset_stats.argtypes = [vp, i32]
+        L.gplum_system_set_forecast.argtypes = [vp, i32, i32, i32, f32]
+        L.gplum_system_get_forecast.argtypes = [vp, vp, vp, vp, ctypes.POINTER(i32)]
+        L.gplum_system_put_forecast.argtypes = [vp, vp, vp, vp]
         L.gplum_hard_integrate.argtypes = [P, vp, i32, vp, i32, i32, ctypes.POINTER(f64),
                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(f64)]
         L.gplum_mlp_forward.argtypes = [vp, vp, i32, vp]
@@ -97,8 +100,9 @@ class Stats(ctypes.Structure):
         "steps", "n_epep_near", "n_epep_far", "n_epsp", "n_sp_entries", "n_groups", "n_candidates",
         "n_neighbors",
         "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
-        "cand_retries", "n_force_calls")] + [(n, ctypes.c_double) for n in (
-            "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys")]
+        "cand_retries", "n_force_calls", "fc_calls", "fc_slots", "fc_ovf_groups",
+        "fc_ovf_candidates")] + [(n, ctypes.c_double) for n in (
+            "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -182,6 +186,25 @@ class System:
 
     def reset_stats(self, timing=True):
         check(lib().gplum_system_reset_stats(self.h, int(timing)))
+
+    FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}
+
+    def set_forecast(self, mode="network", train_steps=4, batch=16384, lr=3e-3):
+        check(lib().gplum_system_set_forecast(self.h, self.FORECAST_MODES[mode], train_steps, batch, lr))
+
+    def get_forecast(self):
+        """(theta, mu, inv_sd, trained_rounds) of the system's forecaster network."""
+        theta = np.zeros(4545, np.float32)
+        mu = np.zeros(4, np.float32)
+        isd = np.zeros(4, np.float32)
+        tr = ctypes.c_int(0)
+        check(lib().gplum_system_get_forecast(self.h, abi.ptr(theta), abi.ptr(mu), abi.ptr(isd),
+                                              ctypes.byref(tr)))
+        return theta, mu, isd, tr.value
+
+    def put_forecast(self, theta, mu, inv_sd):
+        theta, mu, inv_sd = (np.ascontiguousarray(a, np.float32) for a in (theta, mu, inv_sd))
+        check(lib().gplum_system_put_forecast(self.h, abi.ptr(theta), abi.ptr(mu), abi.ptr(inv_sd)))
 
 
 def neighbor_sets_by_id(pp, off, idx):

@@ -46,8 +46,15 @@ typedef struct gplum_stats {
     int64_t n_collisions;
     int64_t cand_retries;                     /* candidate-buffer regrowths                */
     int64_t n_force_calls;
+    /* learned candidate sizing (all summed over soft-force calls) */
+    int64_t fc_calls;                         /* calls sized by the forecaster             */
+    int64_t fc_slots;                         /* candidate slots planned                   */
+    int64_t fc_ovf_groups;                    /* i-groups whose segment overflowed         */
+    int64_t fc_ovf_candidates;                /* candidates redone by the fallback pass    */
     double force_ms, tree_ms, finalize_ms, hard_ms;
     double time_sys;
+    double fc_loss;                           /* last training loss (Poisson NLL)          */
+    double fc_ms;                             /* features, forecast, fallback and training */
 } gplum_stats;
 
 typedef struct gplum_system gplum_system;
@@ -95,6 +102,21 @@ int gplum_system_neighbors(gplum_system *s, int32_t *offset, int32_t *index, int
 
 int gplum_system_stats(gplum_system *s, gplum_stats *out);
 int gplum_system_reset_stats(gplum_system *s, int timing);
+
+/* Learned candidate-buffer sizing (no counterpart in the reference, whose CPU lists
+ * grow as std::vectors; see include/gplum_forecast.h).  mode 1 trains the forecaster
+ * network on every soft-force call (train_steps Adam steps of batch samples,
+ * learning rate lr) and, once trained, sizes the EP-EP candidate buffer per i-group
+ * from its forecast; overflowed groups are redone exactly, so particles, neighbour
+ * lists and clusters are identical in every mode.  mode 2 is the same per-group
+ * scheme with the network replaced by the previous call's mean count per particle
+ * (the ablation the network is measured against); mode 0 (default) uses one global
+ * buffer sized by the previous call's total and redoes the force on overflow. */
+int gplum_system_set_forecast(gplum_system *s, int mode, int train_steps, int batch, float lr);
+/* Copy the network out (theta[4545] and the feature standardisation mu[4],
+ * inv_sd[4]) or into another system, e.g. one started from a new snapshot. */
+int gplum_system_get_forecast(gplum_system *s, float *theta, float *mu, float *inv_sd, int *trained);
+int gplum_system_put_forecast(gplum_system *s, const float *theta, const float *mu, const float *inv_sd);
 
 /* Hard integration of caller-provided clusters (host arrays): cluster c is
  * pp[offsets[c] .. offsets[c+1]).  Each cluster advances one soft step dt_tree as
