@@ -69,3 +69,10 @@ def rel_err_rms(a, ref):
         num = np.abs(a - ref)
         den = np.abs(ref)
     return float(np.sqrt(np.mean((num / den) ** 2)))
+
+
+# The reference's CPU run (sample/parameter.dat values; dt_tree 2^-4, the power of two
+# nearest below 2 pi / 64, because the reference's block steps need a power of two).
+CFG1 = dict(R_cut0=3.0 * 2 ** (1 / 3), R_cut1=8.0, R_search0=1.1, R_search1=6.0, R_search2=1.1,
+            R_search3=2.0, gamma=0.1, eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002,
+            dt_tree=2.0 ** -4, dt_min=2.0 ** -30, n_group_limit=64, theta=0.5)

@@ -13,9 +13,7 @@ from tests import helpers
 pytestmark = pytest.mark.gpu
 
 ACC_RMS_TOL = 1e-6
-CFG1 = dict(R_cut0=3.0 * 2 ** (1 / 3), R_cut1=8.0, R_search0=1.1, R_search1=6.0, R_search2=1.1,
-            R_search3=2.0, gamma=0.1, eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002,
-            dt_tree=2.0 ** -4, dt_min=2.0 ** -30, n_group_limit=64, theta=0.5)
+CFG1 = helpers.CFG1
 
 
 def by_id(pp):
@@ -121,5 +119,31 @@ def test_steps_match_oracle(oracle_lib):
         assert abs(eg - eo) < 1e-9
     st = s.stats()
     print(st)
+    run.close()
+    s.close()
+
+
+def test_config1_energy_drift(oracle_lib, init3000):
+    """INIT3000 over 64 soft steps (monopole, merging): the device system's max |dE|
+    stays within the oracle's own bar (1e-9, test_oracle_hermite) and within 10x the
+    oracle's drift on the same run."""
+    hdr, pp = init3000
+    po, pg = both_params(oracle_lib, quadrupole=0, collision=1, **CFG1)
+    run = oracle_lib.P3T(po, pp)
+    s = gpu.System(pg, pp)
+    s.initialize()
+    e0o = oracle_lib.energy(run.pp)[0]
+    e0g = s.energy()[0]
+    wo = wg = 0.0
+    for k in range(8):
+        for _ in range(8):
+            run.step()
+        s.step(8)
+        eg = s.energy()
+        wg = max(wg, abs((eg[0] - e0g - eg[5]) / e0g))
+        wo = max(wo, abs((oracle_lib.energy(run.pp)[0] - e0o - run.edisp.value) / e0o))
+    print("max |dE| gpu %.3e oracle %.3e" % (wg, wo))
+    assert wg < 1e-9
+    assert wg < 10 * wo + 1e-12
     run.close()
     s.close()
