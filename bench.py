@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -84,6 +85,7 @@ def main():
     p = gpu.params(**params_dict())
     pp = make_disk(args.n, args.seed)
     s = gpu.System(p, pp)
+    s.set_forecast("network")          # learned candidate sizing, trained online
     t0 = time.perf_counter()
     s.initialize()
     t_init = time.perf_counter() - t0
@@ -91,6 +93,8 @@ def main():
     for k in range(args.warmup):
         s.step(1)
         print("# warmup step %d done" % k, file=sys.stderr, flush=True)
+    warm_state = s.particles()
+    warm_time = s.stats()["time_sys"]
     s.reset_stats(timing=True)
     e0 = s.energy()
     t0 = time.perf_counter()
@@ -139,6 +143,26 @@ def main():
     s64.close()
     s32.close()
 
+    # plain comparator: FP64 soft force, one launch per cluster, no forecaster, from the
+    # same warmed-up state
+    plain = None
+    if args.plain_steps > 0:
+        sp_ = gpu.System(p, warm_state, time_sys=warm_time)
+        sp_.set_variant(fp64_soft=True, per_cluster_hard=True)
+        sp_.initialize()
+        sp_.step(1)
+        sp_.reset_stats(timing=True)
+        tp0 = time.perf_counter()
+        sp_.step(args.plain_steps)
+        tp1 = time.perf_counter()
+        vp = args.n * args.plain_steps / (tp1 - tp0)
+        plain = {"value": round(vp, 1), "unit": "particle-steps/s", "steps": args.plain_steps,
+                 "ms_per_step": round(1e3 * (tp1 - tp0) / args.plain_steps, 3),
+                 "speedup": round(value / vp, 3),
+                 "variant": "FP64 soft force, per-cluster Hermite launches, global candidate buffer"}
+        sp_.close()
+        print("# plain variant done", file=sys.stderr, flush=True)
+
     de = (e1[0] - e0[0] - (e1[5] - e0[5])) / e0[0]
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
@@ -158,6 +182,10 @@ def main():
                  "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
                  "collisions": st["n_collisions"]},
         "energy_error_over_timed_steps": de,
+        "plain_fp64_per_cluster": plain,
+        "forecast": {"calls": st["fc_calls"], "slots": st["fc_slots"], "overflowed_groups": st["fc_ovf_groups"],
+                     "redone_candidates": st["fc_ovf_candidates"], "loss": st["fc_loss"],
+                     "ms_per_step": round(st["fc_ms"] / args.steps, 3)},
     }
     s.close()
     if not args.no_cpu:

@@ -182,6 +182,6 @@ void launch_soft_f32(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf
                      float eps2, CandBuf cand, hipStream_t st);
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f64(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,
-                     double eps2, hipStream_t st);
+                     double eps2, CandBuf cand, hipStream_t st);
 
 }  // namespace gplum

@@ -108,11 +108,11 @@ int run_call(const gplum_epi *epi, int ni, const void *jp, size_t jsize, int nj,
             launch_soft_f64<SrcEPI, SrcEPJ, SrcNone, SinkF64, MODE>(
                 (const SoftGroup *)c.groups.p, ng, is, SrcEPJ{(const gplum_epj *)c.epj.p},
                 SrcEPJ{(const gplum_epj *)c.epj.p}, SrcNone{},
-                out, (double)eps2, c.st);
+                out, (double)eps2, CandBuf{nullptr, nullptr, 0u}, c.st);
         } else {
             launch_soft_f64<SrcEPI, SrcNone, SPSRC, SinkF64, MODE>(
                 (const SoftGroup *)c.groups.p, ng, is, SrcNone{}, SrcNone{}, SPSRC{(const SP *)c.epj.p}, out,
-                (double)eps2, c.st);
+                (double)eps2, CandBuf{nullptr, nullptr, 0u}, c.st);
         }
         if (acc64)
             GPLUM_HIP(hipMemcpyAsync(acc64, c.acc.p, sizeof(double) * 3 * ni, hipMemcpyDeviceToHost, c.st));

@@ -67,6 +67,23 @@ __device__ inline float rsqrt_refined(float r2) {
     return __builtin_fmaf(0.5f * y, e, y);
 }
 
+// Appends (idi, idj) for the lanes with nb set: one ballot, one atomic per wave.
+// Called by all lanes of a wave together.
+__device__ inline void emit_candidate(bool nb, int idi, int idj, const CandBuf &cand) {
+    const unsigned long long mask = __ballot(nb);
+    if (mask) {
+        const int lane = __lane_id();
+        const int first = __ffsll((long long)mask) - 1;
+        unsigned int b0 = 0;
+        if (lane == first) b0 = atomicAdd(cand.count, (unsigned)__popcll(mask));
+        b0 = (unsigned)__shfl((int)b0, first);
+        if (nb) {
+            const unsigned int pos = b0 + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            if (pos < cand.cap) cand.pairs[cand.base + pos] = make_int2(idi, idj);
+        }
+    }
+}
+
 // EP-EP pair with neighbour predicate, r_out clamp and candidate emission.
 template <bool EMITC>
 __device__ inline void epep_near(const IState &I, bool valid, float4 pj, float4 aj, float4 lj, float eps2,
@@ -84,20 +101,7 @@ __device__ inline void epep_near(const IState &I, bool valid, float4 pj, float4 
         idmax = max(idmax, idj);
         idmin = min(idmin, idj);
     }
-    if (EMITC) {
-        const unsigned long long mask = __ballot(nb);
-        if (mask) {
-            const int lane = __lane_id();
-            const int first = __ffsll((long long)mask) - 1;
-            unsigned int b0 = 0;
-            if (lane == first) b0 = atomicAdd(cand.count, (unsigned)__popcll(mask));
-            b0 = (unsigned)__shfl((int)b0, first);
-            if (nb) {
-                const unsigned int pos = b0 + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-                if (pos < cand.cap) cand.pairs[cand.base + pos] = make_int2(I.id, idj);
-            }
-        }
-    }
+    if (EMITC) emit_candidate(nb, I.id, idj, cand);
     const float rx = rxp + (lj.x - I.xl), ry = ryp + (lj.y - I.yl), rz = rzp + (lj.z - I.zl);
     const float r2f = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, eps2)));
     const float r2 = fmaxf(r2f, fmaxf(I.ro2, aj.x));
@@ -319,18 +323,25 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
 
 // FP64 twin: same structure, double throughout, r_inv = 1/sqrt(r2) (no Newton
 // step); the near/far split is irrelevant in FP64 (both lists use the near loop).
+// Without EMIT the neighbour predicate is evaluated in FP64 (the C-ABI twins).  With
+// EMIT (the device system's FP64 runs) the neighbour count and the emitted candidates
+// use the FP32 predicate of the FP32 kernel, so the neighbour decision that follows
+// sees the same candidate set as in an FP32 run.
 template <class IS, class JS, class SS, class OUT, int MODE>
 __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f64(const SoftGroup *__restrict__ groups,
                                                          int ngroups, IS is, JS js, JS jf, SS ss,
-                                                         OUT out, double eps2) {
+                                                         OUT out, double eps2, CandBuf cand) {
     const int g = xcd_group(ngroups);
     if (g >= ngroups) return;
     const SoftGroup G = groups[g];
     if (G.ni <= 0) return;
+    constexpr bool EMITC = (MODE & EMIT) != 0;
+    const float eps2f = (float)eps2;
 
     constexpr int T64 = 128;
     __shared__ double s_d[12][T64];
     __shared__ int s_id[2][T64];
+    __shared__ float s_h[4][T64];
     __shared__ double s_red[4][SOFT_BLOCK];
     __shared__ int s_redi[4][SOFT_BLOCK];
 
@@ -343,6 +354,7 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f64(const SoftGroup *__rest
     const SplitInfo S(G.ni);
     const int t = (int)threadIdx.x;
     double xi = 0., yi = 0., zi = 0., ro2i = 0., rs2i = -1.;
+    float xhi = 0.f, yhi = 0.f, zhi = 0.f, rs2fi = -1.f;
     int idi = -1, ranki = 0;
     if (S.valid) {
         double x, y, z, ro, rs;
@@ -350,6 +362,9 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f64(const SoftGroup *__rest
         xi = x - x0; yi = y - y0; zi = z - z0;
         ro2i = ro * ro;
         rs2i = (rs * rs) * 1.0201;
+        xhi = (float)xi; yhi = (float)yi; zhi = (float)zi;
+        const float rsf = (float)rs;
+        rs2fi = (rsf * rsf) * 1.0201f;
     }
     double ax = 0., ay = 0., az = 0., pot = 0.;
     int ngb = 0, rankf = 0, idmax = ID_MAX_CLEAR, idmin = 2147483647;
@@ -369,6 +384,11 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f64(const SoftGroup *__rest
                     s_d[0][t] = x - x0; s_d[1][t] = y - y0; s_d[2][t] = z - z0; s_d[3][t] = m;
                     s_d[4][t] = ro * ro; s_d[5][t] = (rs * rs) * 1.0201;
                     s_id[0][t] = id; s_id[1][t] = rk;
+                    if (EMITC) {
+                        const float rsf = (float)rs;
+                        s_h[0][t] = (float)(x - x0); s_h[1][t] = (float)(y - y0); s_h[2][t] = (float)(z - z0);
+                        s_h[3][t] = (rsf * rsf) * 1.0201f;
+                    }
                 }
                 __syncthreads();
                 if (S.in_split) {
@@ -379,13 +399,22 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f64(const SoftGroup *__rest
                         const double rs2 = smax(rs2i, s_d[5][k]);
                         const double r2 = smax(r2r, rout2);
                         const int idj = s_id[0][k], rkj = s_id[1][k];
-                        if (S.valid && (r2r < rs2) && (idi != idj || ranki != rkj)) {
+                        bool within = r2r < rs2;
+                        if (EMITC) {
+                            const float rxp = s_h[0][k] - xhi, ryp = s_h[1][k] - yhi, rzp = s_h[2][k] - zhi;
+                            const float r2f = __builtin_fmaf(rzp, rzp, __builtin_fmaf(ryp, ryp,
+                                                                                     __builtin_fmaf(rxp, rxp, eps2f)));
+                            within = r2f < fmaxf(rs2fi, s_h[3][k]);
+                        }
+                        const bool nb = S.valid && within && (idi != idj || ranki != rkj);
+                        if (nb) {
                             ngb++;
                             const int d = ranki - rkj;
                             rankf += d * d;
                             idmax = max(idmax, idj);
                             idmin = min(idmin, idj);
                         }
+                        if (EMITC) emit_candidate(nb, idi, idj, cand);
                         const double rinv = 1.0 / sqrt(r2);
                         const double mr = s_d[3][k] * rinv;
                         const double mr3 = rinv * rinv * mr;
@@ -469,10 +498,10 @@ void launch_soft_f32(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf
 }
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f64(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,
-                     double eps2, hipStream_t st) {
+                     double eps2, CandBuf cand, hipStream_t st) {
     if (ngroups <= 0) return;
     hipLaunchKernelGGL((k_soft_f64<IS, JS, SS, OUT, MODE>), dim3(xcd_grid(ngroups)), dim3(SOFT_BLOCK), 0,
-                       st, d_groups, ngroups, is, js, jf, ss, out, eps2);
+                       st, d_groups, ngroups, is, js, jf, ss, out, eps2, cand);
 }
 
 }  // namespace gplum

@@ -503,17 +503,17 @@ int System::soft_force(bool initial, bool use_f64) {
                         groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
             } else {
                 if (p.quadrupole)
-                    launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ>(groups.p, ng, is, jn, jf,
-                                                                                       sp, out, p.eps2, st);
+                    launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(
+                        groups.p, ng, is, jn, jf, sp, out, p.eps2, cb, st);
                 else
-                    launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM>(groups.p, ng, is, jn, jf,
-                                                                                       sp, out, p.eps2, st);
+                    launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(
+                        groups.p, ng, is, jn, jf, sp, out, p.eps2, cb, st);
             }
             tick(EV_FORCE1);
             GPLUM_HIP(hipGetLastError());
             GPLUM_HIP(hipMemcpyAsync(&ncand, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
             GPLUM_HIP(hipStreamSynchronize(st));
-            if (use_f64 || ncand <= cand_cap) break;
+            if (ncand <= cand_cap) break;
             cand_cap = ncand + ncand / 4 + 1024;   // too small: grow and redo the step
             stats.cand_retries++;
         }
@@ -522,22 +522,6 @@ int System::soft_force(bool initial, bool use_f64) {
     stats.force_ms += elapsed(EV_FORCE0, EV_FORCE1);
     stats.tree_ms += elapsed(EV_TREE0, EV_TREE1);
     stats.n_force_calls++;
-    if (use_f64) {
-        // The FP64 twin emits no candidates; the neighbour finalize needs the FP32
-        // candidate set, so run the FP32 EP-EP pass once more for candidates only.
-        GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
-        DArray<double> tmp_acc, tmp_phi;
-        GPLUM_ENSURE(tmp_acc, 3 * (size_t)n);
-        GPLUM_ENSURE(tmp_phi, n);
-        SinkEngine out2{tmp_acc.p, tmp_phi.p, A.number.p, A.rank.p, A.id_max.p, A.id_min.p};
-        GPLUM_ENSURE(cand, cand_cap);
-        CandBuf cb{cand.p, cand_count.p, cand_cap};
-        launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | EMIT>(groups.p, ng, is, jn, jf, sp, out2,
-                                                                            (float)p.eps2, cb, st);
-        GPLUM_HIP(hipMemcpyAsync(&ncand, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipStreamSynchronize(st));
-        if (ncand > cand_cap) return fail(GPLUM_E_OVERFLOW, "candidate buffer overflow (f64 path)");
-    }
     last_ncand = ncand;
     tick(EV_NB0);
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
@@ -708,10 +692,10 @@ int System::merge_and_recompute() {
 int System::step(bool persistent) {
     int rc;
     if ((rc = kick())) return rc;
-    if ((rc = hard_part(persistent))) return rc;
+    if ((rc = hard_part(persistent && !plain_per_cluster))) return rc;
     n_col_tot += step_ncol;
     edisp += step_edisp;
-    if ((rc = soft_force(false, false))) return rc;
+    if ((rc = soft_force(false, plain_f64))) return rc;
     if ((rc = kick())) return rc;
     if (step_ncol) {
         if ((rc = merge_and_recompute())) return rc;
@@ -838,6 +822,14 @@ int gplum_system_reset_stats(gplum_system *s, int timing) {
     System *S = reinterpret_cast<System *>(s);
     std::memset(&S->stats, 0, sizeof(S->stats));
     S->timing = timing != 0;
+    return GPLUM_OK;
+}
+
+int gplum_system_set_variant(gplum_system *s, int fp64_soft, int per_cluster_hard) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    System *S = reinterpret_cast<System *>(s);
+    S->plain_f64 = fp64_soft != 0;
+    S->plain_per_cluster = per_cluster_hard != 0;
     return GPLUM_OK;
 }
 

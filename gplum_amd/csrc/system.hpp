@@ -34,6 +34,7 @@ struct System {
     double time_sys = 0., edisp = 0., step_edisp = 0.;
     int n_col_tot = 0, step_ncol = 0;
     bool have_nbr = false, timing = false;
+    bool plain_f64 = false, plain_per_cluster = false;   // comparison variant of step()
     ParticleSoA A, B;
     Tree tree;
     DArray<JRec> jrec;

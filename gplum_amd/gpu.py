@@ -69,6 +69,7 @@ def lib():
         L.gplum_system_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.gplum_system_reset_stats.argtypes = [vp, i32]
         L.gplum_system_set_forecast.argtypes = [vp, i32, i32, i32, f32]
+        L.gplum_system_set_variant.argtypes = [vp, i32, i32]
         L.gplum_system_get_forecast.argtypes = [vp, vp, vp, vp, ctypes.POINTER(i32)]
         L.gplum_system_put_forecast.argtypes = [vp, vp, vp, vp]
         L.gplum_hard_integrate.argtypes = [P, vp, i32, vp, i32, i32, ctypes.POINTER(f64),
@@ -186,6 +187,9 @@ class System:
 
     def reset_stats(self, timing=True):
         check(lib().gplum_system_reset_stats(self.h, int(timing)))
+
+    def set_variant(self, fp64_soft=False, per_cluster_hard=False):
+        check(lib().gplum_system_set_variant(self.h, int(fp64_soft), int(per_cluster_hard)))
 
     FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}
 

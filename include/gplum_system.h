@@ -103,6 +103,13 @@ int gplum_system_neighbors(gplum_system *s, int32_t *offset, int32_t *index, int
 int gplum_system_stats(gplum_system *s, gplum_stats *out);
 int gplum_system_reset_stats(gplum_system *s, int timing);
 
+/* Comparison variant of gplum_system_step (no counterpart in the reference): fp64_soft
+ * runs the soft force with the FP64 kernels (neighbour candidates still from the
+ * FP32 predicate, so the neighbour graph is unchanged); per_cluster_hard integrates
+ * the clusters with one launch per cluster instead of the persistent batched launch.
+ * Both 0 (default) is the production path. */
+int gplum_system_set_variant(gplum_system *s, int fp64_soft, int per_cluster_hard);
+
 /* Learned candidate-buffer sizing (no counterpart in the reference, whose CPU lists
  * grow as std::vectors; see include/gplum_forecast.h).  mode 1 trains the forecaster
  * network on every soft-force call (train_steps Adam steps of batch samples,
