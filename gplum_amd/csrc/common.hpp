@@ -47,6 +47,39 @@ constexpr int32_t ID_MAX_CLEAR = -1;
 __host__ __device__ inline double smin(double a, double b) { return (b < a) ? b : a; }
 __host__ __device__ inline double smax(double a, double b) { return (a < b) ? b : a; }
 
+// x^(1/3) for finite x > 0 from IEEE-exact operations only (frexp, ldexp, +, -, *, /):
+// a quadratic first guess on [0.5, 4) and five Newton steps, within 1 ulp of the true
+// root.  The reference calls pow(x, 1./3.) (collisionA.h:251); library pow differs
+// by an ulp between the host libm and the device math library, and the merged radius
+// decides later collisions, so the oracle and the kernels share this function.
+__host__ __device__ inline double cbrt_exact_ops(double x) {
+    int e = 0;
+    const double f = frexp(x, &e);
+    const int k = (e >= 0) ? e / 3 : -((2 - e) / 3);
+    const double u = ldexp(f, e - 3 * k);
+    double y = (-0.0368 * u + 0.3769) * u + 0.6515;
+    for (int it = 0; it < 5; it++) y = y - (y * y * y - u) / (3.0 * y * y);
+    return ldexp(y, k);
+}
+
+// exp(x) from IEEE-exact operations only (rint, ldexp, +, -, *): k = rint(x / ln 2),
+// r = x - k ln2 in two parts (Cody-Waite; the high part has 21 trailing zero bits,
+// so k * LN2_HI is exact), e^r by its Taylor series to degree 13 (|r| <= 0.347:
+// truncation below 1e-17), scaled by 2^k; within a few ulp of the true value.
+// Shared with the oracle so that v_disp (a sum of 32 exponentials per particle,
+// particle.h:1436-1449) and the neighbour radii built on it agree bit for bit.
+__host__ __device__ inline double exp_exact_ops(double x) {
+    if (x < -745.2) return 0.;
+    if (x > 709.78) return __builtin_huge_val();
+    const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
+    const double INV_LN2 = 1.44269504088896338700e+00;
+    const double k = rint(x * INV_LN2);
+    const double r = (x - k * LN2_HI) - k * LN2_LO;
+    double s = 1. + r * (1. / 13.);
+    for (int m = 12; m >= 1; m--) s = 1. + (r / m) * s;
+    return ldexp(s, (int)k);
+}
+
 // Exact-order FP64 helpers (the library is compiled with -ffp-contract=off, so
 // these are plain IEEE mul/add in the written order).
 __host__ __device__ inline double dot3(double ax, double ay, double az,

@@ -54,7 +54,7 @@ using namespace gplum;
 
 extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, int n, const int32_t *offsets,
                                     int n_cluster, int mode, double *edisp, int64_t *block_steps,
-                                    double *kernel_ms) {
+                                    double *kernel_ms, int64_t *pair_evals) {
     if (!p || (n > 0 && (!pp || !offsets)) || n < 0 || n_cluster < 0)
         return fail(GPLUM_E_ARG, "bad arguments");
     if (n_cluster == 0) return GPLUM_OK;
@@ -104,8 +104,14 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     GPLUM_ENSURE(hb.edisp, n_cluster);
     GPLUM_ENSURE(hb.ncol, n_cluster);
     GPLUM_ENSURE(hb.nblock, n_cluster);
+    GPLUM_ENSURE(hb.npair, n_cluster);
+    if (mode & 2) {
+        // bare clusters: time is the local time 0 here, dt the caller's previous step
+        if ((rc = upload(hb.time, v1, c.st))) return rc;
+        if ((rc = hard_prepare(*p, hb, n_cluster, c.off.p, c.st))) return rc;
+    }
     GPLUM_HIP(hipEventRecord(c.e0, c.st));
-    if (mode == 1) {
+    if (mode & 1) {
         if ((rc = hard_launch(*p, hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st))) return rc;
     } else {
         for (int k = 0; k < n_cluster; k++)
@@ -120,6 +126,8 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     std::vector<long long> nb(n_cluster);
     if ((rc = download(ed, hb.edisp, c.st))) return rc;
     if ((rc = download(nb, hb.nblock, c.st))) return rc;
+    std::vector<long long> np(n_cluster);
+    if ((rc = download(np, hb.npair, c.st))) return rc;
     auto get3 = [&](DArray<double> &d, size_t fo) {
         int r = download(v3, d, c.st);
         if (r) return r;
@@ -145,11 +153,12 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     if ((rc = download(fl, hb.flags, c.st))) return rc;
     GPLUM_HIP(hipStreamSynchronize(c.st));
     double etot = 0.;
-    long long btot = 0;
+    long long btot = 0, ptot = 0;
     for (int k = 0; k < n_cluster; k++) {
         if (nb[k] < 0) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
         etot += ed[k];
         btot += nb[k];
+        ptot += np[k];
     }
     for (int k = 0; k < n_cluster; k++)
         for (int i = offsets[k]; i < offsets[k + 1]; i++) {
@@ -160,5 +169,6 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
         }
     if (edisp) *edisp = etot;
     if (block_steps) *block_steps = btot;
+    if (pair_evals) *pair_evals = ptot;
     return GPLUM_OK;
 }

@@ -324,7 +324,7 @@ __device__ double perfect_merge(const HP &P, HardView &H, int b, int n, int ic, 
     }
     const double mass_rem = mass_imp + mass_tar;
     H.r_planet[I] = 0.;
-    H.r_planet[T] *= pow(mass_rem / mass_tar, 1. / 3.);
+    H.r_planet[T] *= cbrt_exact_ops(mass_rem / mass_tar);
     const int64_t tar_id = H.id[T];
     H.id[I] = tar_id;
     H.flags[I] |= GPLUM_FLAG_DEAD;
@@ -429,11 +429,12 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                                                            const int32_t *__restrict__ order, int ncl,
                                                            int *__restrict__ next, double *__restrict__ edisp,
                                                            int *__restrict__ ncol, long long *__restrict__ nblock,
-                                                           int persistent) {
+                                                           long long *__restrict__ npair, int persistent) {
     __shared__ double s_x[3 * HERMITE_LDS], s_v[3 * HERMITE_LDS], s_m[HERMITE_LDS], s_ri[HERMITE_LDS];
     __shared__ int64_t s_id[HERMITE_LDS];
     __shared__ Reduce R;
     __shared__ int s_cl;
+    __shared__ unsigned long long s_pairs;
     const int t = threadIdx.x;
     const int B = blockDim.x;
     for (int iter = 0;; iter++) {
@@ -459,6 +460,9 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
         } else {
             J = JData{H.xp + 3 * b, H.vp + 3 * b, H.mass + b, H.rinv + b, H.id + b, 3};
         }
+        // pair evaluations of this cluster (hard_pair calls), for the roofline
+        long long pairs = (long long)(n > 0 ? n - 1 : 0) * ((t < n) ? (n - 1 - t) / B + 1 : 0);
+        if (t == 0) s_pairs = 0ull;
         // initial calcGravity for all, flags cleared (hermite.h:259-270)
         for (int i = t; i < n; i += B) {
             gravity_full(P, H, b, n, i);
@@ -525,6 +529,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                 if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
                 if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
                 const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
+                pairs += 3 * (long long)(n - 1);
                 gravity_pc(P, H, J, n, g, false);
                 if (merged) merge_acc_jerk(H, b, n, i);
                 const double t_c = t_s - H.time[g];
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                 n_col++;
                 __threadfence_block();
                 __syncthreads();
-                for (int i = t; i < n; i += B) gravity_full(P, H, b, n, i);
+                for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
                 __threadfence_block();
                 __syncthreads();
                 for (int i = t; i < n; i += B)
@@ -581,7 +586,8 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
             sys_time = block_min(R, tm);
             loop++;
         }
-        for (int i = t; i < n; i += B) gravity_full(P, H, b, n, i);
+        for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
+        atomicAdd(&s_pairs, (unsigned long long)pairs);
         __threadfence_block();
         __syncthreads();
         if (n_col > 0) {
@@ -592,6 +598,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
             edisp[c] = ed;
             ncol[c] = n_col;
             nblock[c] = (sys_time < time_end) ? -1 : loop;   // -1: block-step cap reached
+            npair[c] = (long long)s_pairs;
         }
         __syncthreads();
     }
@@ -761,21 +768,51 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
     GPLUM_ENSURE(hb.edisp, ncl);
     GPLUM_ENSURE(hb.ncol, ncl);
     GPLUM_ENSURE(hb.nblock, ncl);
+    GPLUM_ENSURE(hb.npair, ncl);
     GPLUM_HIP(hipMemsetAsync(hb.next.p, 0, 2 * sizeof(int), st));
     if (persistent) {
         const int grid = std::min(ncl, std::max(1, n_cu) * 8);
         hipLaunchKernelGGL((k_hermite<true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order,
-                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, 1);
+                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 1);
         if (hb.max_size > HERMITE_LDS)
             hipLaunchKernelGGL((k_hermite<false>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off,
-                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, 1);
+                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 1);
     } else {
         hipLaunchKernelGGL((k_hermite<true>), dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order,
-                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, 0);
+                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 0);
         if (hb.max_size > HERMITE_LDS)
             hipLaunchKernelGGL((k_hermite<false>), dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off,
-                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, 0);
+                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 0);
     }
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+// Sun + hard acc/jerk at the current state and the first block step
+// (calcGravity + calcDeltatInitial, gravity_hard.h:203-256, particle.h:886-914) of
+// every particle of clusters [d_off[c], d_off[c+1]): one 64-thread block per cluster.
+__global__ __launch_bounds__(HERMITE_BLOCK) void k_hard_prepare(HP P, HardView H, const int32_t *__restrict__ d_off,
+                                                                int ncl) {
+    const int c = blockIdx.x;
+    if (c >= ncl) return;
+    const int b = d_off[c], n = d_off[c + 1] - b;
+    for (int i = (int)threadIdx.x; i < n; i += HERMITE_BLOCK) {
+        gravity_full(P, H, b, n, i);
+        double ad[3], jd[3], as[3], js[3];
+        ld3(H.acc_d, b + i, ad); ld3(H.jerk_d, b + i, jd);
+        ld3(H.acc_s, b + i, as); ld3(H.jerk_s, b + i, js);
+        H.dt[b + i] = dt_initial(P, H.time[b + i], H.dt[b + i], H.acc0[b + i], ad, jd, as, js);
+    }
+}
+
+int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off, hipStream_t st) {
+    if (ncl <= 0) return GPLUM_OK;
+    HP P = make_hp(p);
+    HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
+               hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
+               hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
+               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
+    hipLaunchKernelGGL(k_hard_prepare, dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, ncl);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }
@@ -790,9 +827,9 @@ int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, const int3
                hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
     (void)c;
     hipLaunchKernelGGL((k_hermite<true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, 0);
+                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
     hipLaunchKernelGGL((k_hermite<false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, 0);
+                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

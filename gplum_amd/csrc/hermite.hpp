@@ -16,7 +16,7 @@ struct HardBuffers {
     DArray<int32_t> flags, twin_of;
     DArray<int> next, ncol;
     DArray<double> edisp;
-    DArray<long long> nblock;
+    DArray<long long> nblock, npair;    // block steps, hard_pair evaluations per cluster
     int max_size = 0;
     int ensure(int n) {
         GPLUM_ENSURE(pos, 3 * n); GPLUM_ENSURE(vel, 3 * n); GPLUM_ENSURE(acc_s, 3 * n);
@@ -44,6 +44,9 @@ struct IsoArgs {
 // clusters from an atomic queue; otherwise one workgroup per cluster.
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st);
+// Sun + hard acc/jerk and the initial block step of every cluster particle (as the
+// initial force correction leaves them), for callers that bring bare clusters.
+int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off, hipStream_t st);
 // One launch for cluster c only (the per-cluster-launch comparator).
 int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, const int32_t *d_off,
                        hipStream_t st);

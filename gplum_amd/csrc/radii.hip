@@ -1,0 +1,187 @@
+// radii.hip -- setCutoffRadii on the device: calcRandomVel (non-isotropic branch,
+// src/particle.h:1319-1457) then setROutRSearch (src/particle.h:713-733).
+//
+// Every expression keeps the operation order of the reference (and of the oracle),
+// so r_out / r_search come out bit-identical to the oracle's: the radial range is a
+// max-reduction (exact in any order), the 32 bin sums run sequentially over each bin's
+// particles in ascending id (after a radix sort by (bin, id)), and the transcendental
+// functions are the shared exact-operation versions (exp_exact_ops, cbrt_exact_ops).
+#include <cfloat>
+
+#include <hipcub/hipcub.hpp>
+
+#include "system.hpp"
+
+namespace gplum {
+
+constexpr int RADII_NB = 32;
+constexpr int RADII_ID_BITS = 58;
+
+__global__ void k_radii_range(int n, const double *__restrict__ pos, double *__restrict__ part) {
+    __shared__ double s_a[256], s_b[256];
+    const int t = threadIdx.x;
+    double a = 0., b = -DBL_MAX;
+    for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x) {
+        const double r2 = pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1];
+        a = smax(a, r2);
+        b = smax(b, -r2);
+    }
+    s_a[t] = a;
+    s_b[t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s_a[t] = smax(s_a[t], s_a[t + w]);
+            s_b[t] = smax(s_b[t], s_b[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[2 * blockIdx.x] = s_a[0];
+        part[2 * blockIdx.x + 1] = s_b[0];
+    }
+}
+
+// range[0] = r_min, range[1] = dr, range[2] = 1/dr
+__global__ void k_radii_range_final(int nb, const double *__restrict__ part, double *__restrict__ range) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double r_max = 0., r_min = -DBL_MAX;
+    for (int k = 0; k < nb; k++) {
+        r_max = smax(r_max, part[2 * k]);
+        r_min = smax(r_min, part[2 * k + 1]);
+    }
+    r_max = sqrt(r_max) * 1.01;
+    r_min = sqrt(-r_min) * 0.99;
+    const double dr = (r_max - r_min) / RADII_NB;
+    range[0] = r_min;
+    range[1] = dr;
+    range[2] = 1. / dr;
+}
+
+__global__ void k_radii_bin(int n, const double *__restrict__ pos, const double *__restrict__ vel,
+                            const int64_t *__restrict__ id, double m_sun, const double *__restrict__ range,
+                            uint64_t *__restrict__ key, double *__restrict__ val, int *__restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r_min = range[0], drinv = range[2];
+    const double x = pos[3 * i], y = pos[3 * i + 1];
+    const double r2 = x * x + y * y;
+    const double ri = sqrt(r2);
+    int j = (int)((ri - r_min) * drinv);
+    if (j == RADII_NB) j = RADII_NB - 1;
+    const double vk = sqrt(m_sun / sqrt(x * x + y * y));
+    double vkx = -y / ri, vky = x / ri, vkz = 0.;
+    vkx *= vk; vky *= vk; vkz *= vk;
+    const double ux = vel[3 * i] - vkx, uy = vel[3 * i + 1] - vky, uz = vel[3 * i + 2] - vkz;
+    const int64_t idi = id[i];
+    if (idi < 0 || idi >= (int64_t(1) << RADII_ID_BITS) || j < 0 || j >= RADII_NB) atomicOr(err, 1);
+    key[i] = ((uint64_t)(j & 31) << RADII_ID_BITS) | ((uint64_t)idi & ((uint64_t(1) << RADII_ID_BITS) - 1));
+    val[i] = ux * ux + uy * uy + uz * uz;
+}
+
+// one thread per bin: sequential sum over the bin's particles in ascending id
+__global__ void k_radii_binsum(int n, const uint64_t *__restrict__ key, const double *__restrict__ val,
+                               double *__restrict__ vd, int *__restrict__ np) {
+    const int b = threadIdx.x;
+    if (b >= RADII_NB) return;
+    auto lower = [&](uint64_t k) {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (key[mid] < k) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    const int a0 = lower((uint64_t)b << RADII_ID_BITS);
+    const int a1 = lower((uint64_t)(b + 1) << RADII_ID_BITS);
+    double s = 0.;
+    for (int a = a0; a < a1; a++) s += val[a];
+    const int c = a1 - a0;
+    np[b] = c;
+    vd[b] = (c > 0) ? sqrt(s / c) : 0.;
+}
+
+struct RadiiParams {
+    double m_sun, dt_tree, R_cut0, R_cut1, p_cut, r_cut_min, r_cut_max, R_search0, R_search1;
+};
+
+__global__ void k_radii_apply(int n, RadiiParams P, const double *__restrict__ range, const double *__restrict__ vdb,
+                              const int *__restrict__ npb, const double *__restrict__ pos,
+                              const double *__restrict__ vel, const double *__restrict__ mass,
+                              double *__restrict__ v_disp, double *__restrict__ r_out,
+                              double *__restrict__ r_out_inv, double *__restrict__ r_search) {
+    __shared__ double s_vd[RADII_NB];
+    __shared__ int s_np[RADII_NB];
+    if (threadIdx.x < RADII_NB) { s_vd[threadIdx.x] = vdb[threadIdx.x]; s_np[threadIdx.x] = npb[threadIdx.x]; }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r_min = range[0], dr = range[1], drinv = range[2];
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const double r2 = x * x + y * y;
+    double vd = 0., ni = 0.;
+    for (int b = 0; b < RADII_NB; b++) {
+        const double ddr = r_min + (b + 0.5) * dr - sqrt(r2);
+        const double expr = exp_exact_ops(-ddr * ddr * drinv * drinv);
+        vd += s_vd[b] * expr;
+        ni += (s_np[b] > 0) ? expr : 0.;
+    }
+    const double vdi = vd / ni;
+    v_disp[i] = vdi;
+    // getEccentricity / getSemimajorAxis2 / getRHill (particle.h:660-699)
+    const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+    const double r = sqrt(x * x + y * y + z * z);
+    const double rv = x * vx + y * vy + z * vz;
+    const double ax = 1.0 / (2.0 / r - (vx * vx + vy * vy + vz * vz) / P.m_sun);
+    const double ecccosu = 1. - r / ax;
+    const double eccsinu2 = rv * rv / (P.m_sun * ax);
+    const double ecc = sqrt(ecccosu * ecccosu + eccsinu2);
+    const double ax2 = (ecc < 0.6) ? ax : sqrt(x * x + y * y + z * z);
+    const double rHill = cbrt_exact_ops(mass[i] / (3. * P.m_sun)) * ax2;
+    const double pc = (P.p_cut == 0.) ? 1. : pow(ax2, -P.p_cut);
+    const double r_out_i = smax(P.R_cut0 * pc * rHill, P.R_cut1 * vdi * P.dt_tree);
+    double ro = smax(r_out_i, P.r_cut_min);
+    if (P.r_cut_max > 0.) ro = smin(P.r_cut_max, ro);
+    r_out[i] = ro;
+    r_out_inv[i] = 1. / ro;
+    r_search[i] = P.R_search0 * ro + P.R_search1 * vdi * P.dt_tree;
+}
+
+int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScratch &R, hipStream_t st) {
+    if (n <= 0) return GPLUM_OK;
+    const int nb = std::min(1024, ceil_div(n, 256));
+    GPLUM_ENSURE(R.part, 2 * (size_t)nb);
+    GPLUM_ENSURE(R.range, 4);
+    GPLUM_ENSURE(R.key, n);
+    GPLUM_ENSURE(R.key_sorted, n);
+    GPLUM_ENSURE(R.val, n);
+    GPLUM_ENSURE(R.val_sorted, n);
+    GPLUM_ENSURE(R.vd, RADII_NB);
+    GPLUM_ENSURE(R.np, RADII_NB);
+    GPLUM_ENSURE(R.err, 1);
+    GPLUM_HIP(hipMemsetAsync(R.err.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_radii_range, dim3(nb), dim3(256), 0, st, n, A.pos.p, R.part.p);
+    hipLaunchKernelGGL(k_radii_range_final, dim3(1), dim3(64), 0, st, nb, R.part.p, R.range.p);
+    hipLaunchKernelGGL(k_radii_bin, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.vel.p, A.id.p, p.m_sun,
+                       R.range.p, R.key.p, R.val.p, R.err.p);
+    size_t bytes = 0;
+    GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, R.key.p, R.key_sorted.p, R.val.p, R.val_sorted.p, n,
+                                                  0, 64, st));
+    GPLUM_ENSURE(R.tmp, bytes + 16);
+    GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(R.tmp.p, bytes, R.key.p, R.key_sorted.p, R.val.p, R.val_sorted.p, n,
+                                                  0, 64, st));
+    hipLaunchKernelGGL(k_radii_binsum, dim3(1), dim3(RADII_NB), 0, st, n, R.key_sorted.p, R.val_sorted.p, R.vd.p,
+                       R.np.p);
+    RadiiParams P{p.m_sun, p.dt_tree, p.R_cut0, p.R_cut1, p.p_cut, p.r_cut_min, p.r_cut_max, p.R_search0,
+                  p.R_search1};
+    hipLaunchKernelGGL(k_radii_apply, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, P, R.range.p, R.vd.p, R.np.p,
+                       A.pos.p, A.vel.p, A.mass.p, A.v_disp.p, A.r_out.p, A.r_out_inv.p, A.r_search.p);
+    GPLUM_HIP(hipGetLastError());
+    int err = 0;
+    GPLUM_HIP(hipMemcpyAsync(&err, R.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    if (err) return fail(GPLUM_E_ARG, "setCutoffRadii: particle ids must lie in [0, 2^58)");
+    return GPLUM_OK;
+}
+
+}  // namespace gplum

@@ -15,6 +15,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "hermite.hpp"
 #include "neighbor.hpp"
 #include "soft_kernels.hpp"
@@ -173,73 +175,6 @@ __global__ void k_hard_scatter(int m, const int32_t *__restrict__ idx, const int
 
 // ------------------------------------------------------------ host: radii
 
-// calcRandomVel (non-ISOTROPIC, particle.h:1319-1457) + setROutRSearch (:713-733)
-// in FP64 on the host; bin sums in ascending particle-id order.
-static void host_cutoff_radii(const gplum_params &p, int n, const double *pos, const double *vel,
-                              const double *mass, const int64_t *id, double *r_out, double *r_out_inv,
-                              double *r_search, double *v_disp) {
-    double r_max = 0., r_min = -DBL_MAX;
-    for (int i = 0; i < n; i++) {
-        const double r2 = pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1];
-        r_max = smax(r_max, r2);
-        r_min = smax(r_min, -r2);
-    }
-    r_max = std::sqrt(r_max) * 1.01;
-    r_min = std::sqrt(-r_min) * 0.99;
-    const int NB = 32;
-    const double dr = (r_max - r_min) / NB;
-    const double drinv = 1. / dr;
-    double vd[32];
-    int np[32];
-    for (int b = 0; b < NB; b++) { vd[b] = 0.; np[b] = 0; }
-    std::vector<int> ord(n);
-    for (int i = 0; i < n; i++) ord[i] = i;
-    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return id[a] < id[b]; });
-    for (int a = 0; a < n; a++) {
-        const int i = ord[a];
-        const double x = pos[3 * i], y = pos[3 * i + 1];
-        const double r2 = x * x + y * y;
-        const double ri = std::sqrt(r2);
-        int j = (int)((ri - r_min) * drinv);
-        if (j == NB) j = NB - 1;
-        const double vk = std::sqrt(p.m_sun / std::sqrt(x * x + y * y));
-        double vkx = -y / ri, vky = x / ri, vkz = 0.;
-        vkx *= vk; vky *= vk; vkz *= vk;
-        const double ux = vel[3 * i] - vkx, uy = vel[3 * i + 1] - vky, uz = vel[3 * i + 2] - vkz;
-        vd[j] += ux * ux + uy * uy + uz * uz;
-        np[j]++;
-    }
-    for (int b = 0; b < NB; b++) vd[b] = (np[b] > 0) ? std::sqrt(vd[b] / np[b]) : 0.;
-    for (int i = 0; i < n; i++) {
-        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-        const double r2 = x * x + y * y;
-        double v_dispi = 0., ni = 0.;
-        for (int b = 0; b < NB; b++) {
-            const double ddr = r_min + (b + 0.5) * dr - std::sqrt(r2);
-            const double expr = std::exp(-ddr * ddr * drinv * drinv);
-            v_dispi += vd[b] * expr;
-            ni += (np[b] > 0) ? expr : 0.;
-        }
-        v_disp[i] = v_dispi / ni;
-        // getSemimajorAxis2 / getRHill (particle.h:660-699)
-        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
-        const double r = std::sqrt(x * x + y * y + z * z);
-        const double rv = x * vx + y * vy + z * vz;
-        const double ax = 1.0 / (2.0 / r - (vx * vx + vy * vy + vz * vz) / p.m_sun);
-        const double ecccosu = 1. - r / ax;
-        const double eccsinu2 = rv * rv / (p.m_sun * ax);
-        const double ecc = std::sqrt(ecccosu * ecccosu + eccsinu2);
-        const double ax2 = (ecc < 0.6) ? ax : std::sqrt(x * x + y * y + z * z);
-        const double rHill = std::pow(mass[i] / (3. * p.m_sun), 1. / 3.) * ax2;
-        const double r_out_i = smax(p.R_cut0 * std::pow(ax2, -p.p_cut) * rHill, p.R_cut1 * v_disp[i] * p.dt_tree);
-        double ro = smax(r_out_i, p.r_cut_min);
-        if (p.r_cut_max > 0.) ro = smin(p.r_cut_max, ro);
-        r_out[i] = ro;
-        r_out_inv[i] = 1. / ro;
-        r_search[i] = p.R_search0 * ro + p.R_search1 * v_disp[i] * p.dt_tree;
-    }
-}
-
 // ------------------------------------------------------------------ System
 
 System::System(const gplum_params &prm) : p(prm) {}
@@ -358,43 +293,38 @@ int System::get_particles(gplum_particle *pp, int count) {
     return GPLUM_OK;
 }
 
-int System::set_radii() {
-    std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n), mass(n), ro(n), roi(n), rs(n), vd(n);
-    std::vector<int64_t> id(n);
-    GPLUM_HIP(hipStreamSynchronize(st));
-    GPLUM_HIP(hipMemcpy(pos.data(), A.pos.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
-    GPLUM_HIP(hipMemcpy(vel.data(), A.vel.p, 24 * (size_t)n, hipMemcpyDeviceToHost));
-    GPLUM_HIP(hipMemcpy(mass.data(), A.mass.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
-    GPLUM_HIP(hipMemcpy(id.data(), A.id.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
-    host_cutoff_radii(p, n, pos.data(), vel.data(), mass.data(), id.data(), ro.data(), roi.data(), rs.data(),
-                      vd.data());
-    GPLUM_HIP(hipMemcpy(A.r_out.p, ro.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    GPLUM_HIP(hipMemcpy(A.r_out_inv.p, roi.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    GPLUM_HIP(hipMemcpy(A.r_search.p, rs.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    GPLUM_HIP(hipMemcpy(A.v_disp.p, vd.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    return GPLUM_OK;
-}
+int System::set_radii() { return device_cutoff_radii(p, A, n, rad, st); }
 
 // Morton sort + physical reorder of every particle array (A <- B gathered by perm).
 int System::reorder() {
     int rc = tree_sort_keys(tree, A.pos.p, n, st);
     if (rc) return rc;
-    const int32_t *perm = tree.perm.p;
-    const int gb = ceil_div(n, 256);
-#define GA3(f) hipLaunchKernelGGL((k_gather<double, 3>), dim3(gb), dim3(256), 0, st, n, perm, A.f.p, B.f.p)
-#define GA1(f) hipLaunchKernelGGL((k_gather<double, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.f.p, B.f.p)
-    GA3(pos); GA3(vel); GA3(acc); GA3(acc_d); GA3(jerk_d); GA3(acc_s); GA3(jerk_s);
-    GA1(mass); GA1(r_planet); GA1(f); GA1(phi); GA1(phi_d); GA1(phi_s); GA1(r_out); GA1(r_out_inv);
-    GA1(r_search); GA1(v_disp); GA1(time); GA1(dt); GA1(acc0);
+    return gather_all(tree.perm.p, n);
+}
+
+// A <- A[perm[0..m)] for every particle array (through B); n becomes m.
+int System::gather_all(const int32_t *perm, int m) {
+    int rc = B.ensure(std::max(m, 1));
+    if (rc) return rc;
+    const int gb = ceil_div(m, 256);
+    if (m > 0) {
+#define GA3(f) hipLaunchKernelGGL((k_gather<double, 3>), dim3(gb), dim3(256), 0, st, m, perm, A.f.p, B.f.p)
+#define GA1(f) hipLaunchKernelGGL((k_gather<double, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.f.p, B.f.p)
+        GA3(pos); GA3(vel); GA3(acc); GA3(acc_d); GA3(jerk_d); GA3(acc_s); GA3(jerk_s);
+        GA1(mass); GA1(r_planet); GA1(f); GA1(phi); GA1(phi_d); GA1(phi_s); GA1(r_out); GA1(r_out_inv);
+        GA1(r_search); GA1(v_disp); GA1(time); GA1(dt); GA1(acc0);
 #undef GA3
 #undef GA1
-    hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.id.p, B.id.p);
-    hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.id_cluster.p, B.id_cluster.p);
-#define GI(f) hipLaunchKernelGGL((k_gather<int32_t, 1>), dim3(gb), dim3(256), 0, st, n, perm, A.f.p, B.f.p)
-    GI(number); GI(rank); GI(id_max); GI(id_min); GI(n_cluster); GI(flags);
+        hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.id.p, B.id.p);
+        hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.id_cluster.p,
+                           B.id_cluster.p);
+#define GI(f) hipLaunchKernelGGL((k_gather<int32_t, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.f.p, B.f.p)
+        GI(number); GI(rank); GI(id_max); GI(id_min); GI(n_cluster); GI(flags);
 #undef GI
+    }
     GPLUM_HIP(hipGetLastError());
     A.swap(B);
+    n = m;
     have_nbr = false;   // neighbour indices refer to the old order
     return GPLUM_OK;
 }
@@ -623,7 +553,18 @@ int System::hard_part(bool persistent) {
         HardPtrs H = hard_ptrs();
         PartPtrs P = ptrs(A);
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
-        if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, persistent, n_cu, st))) return rc;
+        if (persistent) {
+            if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st))) return rc;
+        } else {
+            // the comparator: one launch per cluster
+            GPLUM_ENSURE(hb.next, 2);
+            GPLUM_ENSURE(hb.edisp, ncl);
+            GPLUM_ENSURE(hb.ncol, ncl);
+            GPLUM_ENSURE(hb.nblock, ncl);
+            GPLUM_ENSURE(hb.npair, ncl);
+            for (int c = 0; c < ncl; c++)
+                if ((rc = hard_launch_single(p, hb, c, cl.off.p, st))) return rc;
+        }
         hipLaunchKernelGGL(k_hard_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, cl.off.p, ncl,
                            H, P, time_c.p);
         std::vector<int> nc(ncl);
@@ -632,12 +573,15 @@ int System::hard_part(bool persistent) {
         GPLUM_HIP(hipMemcpyAsync(nc.data(), hb.ncol.p, sizeof(int) * ncl, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipMemcpyAsync(ed.data(), hb.edisp.p, sizeof(double) * ncl, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipMemcpyAsync(nbk.data(), hb.nblock.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
+        std::vector<long long> npr(ncl);
+        GPLUM_HIP(hipMemcpyAsync(npr.data(), hb.npair.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
         for (int c = 0; c < ncl; c++) {
             if (nbk[c] < 0) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
             step_ncol += nc[c];
             step_edisp += ed[c];
             stats.block_steps += nbk[c];
+            stats.hard_pairs += npr[c];
         }
     }
     IsoArgs ia{A.pos.p, A.vel.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p, A.phi_d.p, A.phi_s.p, A.time.p,
@@ -657,34 +601,120 @@ HardPtrs System::hard_ptrs() {
 
 // MergeParticle (func.h:160-207) on the host, then setCutoffRadii + force +
 // correctForceLongInitial (main_p3t.cpp:640-699).
-int System::merge_and_recompute() {
-    std::vector<gplum_particle> h(n);
-    int rc = get_particles(h.data(), n);
-    if (rc) return rc;
-    std::vector<char> rm(n, 0);
-    double ed = 0.;
-    for (int i = 0; i < n; i++) {
-        if (!((h[i].flags & GPLUM_FLAG_MERGED) && !(h[i].flags & GPLUM_FLAG_DEAD))) continue;
-        for (int j = 0; j < n; j++) {
-            if (h[j].id == h[i].id && i != j) {
-                const double mi = h[i].mass, mj = h[j].mass;
-                const double vr[3] = {h[j].vel[0] - h[i].vel[0], h[j].vel[1] - h[i].vel[1], h[j].vel[2] - h[i].vel[2]};
-                h[i].mass += mj;
-                for (int k = 0; k < 3; k++) h[i].vel[k] = (mi * h[i].vel[k] + mj * h[j].vel[k]) / (mi + mj);
-                h[i].phi = (mi * h[i].phi + mj * h[j].phi) / (mi + mj);
-                h[i].phi_d = (mi * h[i].phi_d + mj * h[j].phi_d) / (mi + mj);
-                const double s = 0.5 * mi * mj / (mi + mj);
-                ed -= (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
-                rm[j] = 1;
-            }
-        }
-        h[i].flags &= ~GPLUM_FLAG_MERGED;
+// MergeParticle (func.h:160-207) on the device, then the recomputation of
+// main_p3t.cpp:640-699 (setCutoffRadii, tree force, correctForceLongInitial).  After
+// the hard part a merged body is its target (MERGED) plus dead twins carrying the
+// target's id; only those few particles are listed on the host to pair them up.
+__global__ void k_merge_flag(int n, const int32_t *__restrict__ flags, int32_t *__restrict__ sel,
+                             int32_t *__restrict__ keep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int f = flags[i];
+    sel[i] = (f & (GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED)) ? 1 : 0;
+    keep[i] = (f & GPLUM_FLAG_DEAD) ? 0 : 1;
+}
+
+__global__ void k_compact_idx(int n, const int32_t *__restrict__ flag, const int32_t *__restrict__ scan,
+                              int32_t *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) out[scan[i]] = i;
+}
+
+// one thread per target: its twins in ascending index, the reference's arithmetic
+__global__ void k_merge_apply(int ntg, const int32_t *__restrict__ tgt, const int32_t *__restrict__ twin_off,
+                              const int32_t *__restrict__ twin, double *__restrict__ mass, double *__restrict__ vel,
+                              double *__restrict__ phi, double *__restrict__ phi_d, int32_t *__restrict__ flags,
+                              double *__restrict__ ed) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ntg) return;
+    const int i = tgt[k];
+    double e = 0.;
+    for (int a = twin_off[k]; a < twin_off[k + 1]; a++) {
+        const int j = twin[a];
+        const double mi = mass[i], mj = mass[j];
+        const double vr[3] = {vel[3 * j] - vel[3 * i], vel[3 * j + 1] - vel[3 * i + 1], vel[3 * j + 2] - vel[3 * i + 2]};
+        mass[i] += mj;
+        for (int c = 0; c < 3; c++) vel[3 * i + c] = (mi * vel[3 * i + c] + mj * vel[3 * j + c]) / (mi + mj);
+        phi[i] = (mi * phi[i] + mj * phi[j]) / (mi + mj);
+        phi_d[i] = (mi * phi_d[i] + mj * phi_d[j]) / (mi + mj);
+        const double s = 0.5 * mi * mj / (mi + mj);   // `0.5*mi*mj/(mi+mj) * vrel*vrel`: (s v).v
+        e -= (s * vr[0]) * vr[0] + (s * vr[1]) * vr[1] + (s * vr[2]) * vr[2];
     }
-    std::vector<gplum_particle> k;
-    k.reserve(n);
-    for (int i = 0; i < n; i++) if (!rm[i]) k.push_back(h[i]);
-    edisp += ed;
-    if ((rc = set_particles(k.data(), (int)k.size(), time_sys))) return rc;
+    flags[i] &= ~GPLUM_FLAG_MERGED;
+    ed[k] = e;
+}
+
+template <class T>
+static int excl_sum(DArray<unsigned char> &tmp, const T *in, T *out, int n, hipStream_t st) {
+    size_t bytes = 0;
+    GPLUM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st));
+    GPLUM_ENSURE(tmp, bytes + 16);
+    GPLUM_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, st));
+    return GPLUM_OK;
+}
+
+int System::merge_and_recompute() {
+    MergeScratch &M = mrg;
+    GPLUM_ENSURE(M.flag, n + 1);
+    GPLUM_ENSURE(M.scan, n + 1);
+    GPLUM_ENSURE(M.keep, n + 1);
+    GPLUM_ENSURE(M.kscan, n + 1);
+    hipLaunchKernelGGL(k_merge_flag, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.flags.p, M.flag.p, M.keep.p);
+    int rc = excl_sum(rad.tmp, M.flag.p, M.scan.p, n, st);
+    if (rc) return rc;
+    if ((rc = excl_sum(rad.tmp, M.keep.p, M.kscan.p, n, st))) return rc;
+    int32_t tail[4] = {0, 0, 0, 0};
+    GPLUM_HIP(hipMemcpyAsync(&tail[0], M.flag.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipMemcpyAsync(&tail[1], M.scan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipMemcpyAsync(&tail[2], M.keep.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipMemcpyAsync(&tail[3], M.kscan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    const int k = tail[0] + tail[1], m = tail[2] + tail[3];
+    if (k == 0) return GPLUM_OK;
+    GPLUM_ENSURE(M.sel, k);
+    hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.flag.p, M.scan.p, M.sel.p);
+    std::vector<int32_t> sel(k), fl(k);
+    std::vector<int64_t> ids(k);
+    GPLUM_HIP(hipMemcpyAsync(sel.data(), M.sel.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    for (int a = 0; a < k; a++) {
+        GPLUM_HIP(hipMemcpyAsync(&fl[a], A.flags.p + sel[a], sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(&ids[a], A.id.p + sel[a], sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    }
+    GPLUM_HIP(hipStreamSynchronize(st));
+    // pair twins with their target (same id), both in ascending index
+    std::vector<int32_t> tgt, off{0}, twin;
+    for (int a = 0; a < k; a++) {
+        if ((fl[a] & GPLUM_FLAG_DEAD) || !(fl[a] & GPLUM_FLAG_MERGED)) continue;
+        tgt.push_back(sel[a]);
+        for (int b = 0; b < k; b++)
+            if ((fl[b] & GPLUM_FLAG_DEAD) && ids[b] == ids[a]) twin.push_back(sel[b]);
+        off.push_back((int32_t)twin.size());
+    }
+    int n_dead = 0;
+    for (int a = 0; a < k; a++) n_dead += (fl[a] & GPLUM_FLAG_DEAD) ? 1 : 0;
+    if (n_dead != (int)twin.size()) return fail(GPLUM_E_STATE, "a merged particle has no live target");
+    const int ntg = (int)tgt.size();
+    if (ntg > 0) {
+        GPLUM_ENSURE(M.tgt, ntg);
+        GPLUM_ENSURE(M.twin_off, ntg + 1);
+        GPLUM_ENSURE(M.twin, twin.size() + 1);
+        GPLUM_ENSURE(M.ed, ntg);
+        GPLUM_HIP(hipMemcpyAsync(M.tgt.p, tgt.data(), sizeof(int32_t) * ntg, hipMemcpyHostToDevice, st));
+        GPLUM_HIP(hipMemcpyAsync(M.twin_off.p, off.data(), sizeof(int32_t) * (ntg + 1), hipMemcpyHostToDevice, st));
+        if (!twin.empty())
+            GPLUM_HIP(hipMemcpyAsync(M.twin.p, twin.data(), sizeof(int32_t) * twin.size(), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_merge_apply, dim3(ceil_div(ntg, 64)), dim3(64), 0, st, ntg, M.tgt.p, M.twin_off.p,
+                           M.twin.p, A.mass.p, A.vel.p, A.phi.p, A.phi_d.p, A.flags.p, M.ed.p);
+        std::vector<double> ed(ntg);
+        GPLUM_HIP(hipMemcpyAsync(ed.data(), M.ed.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        for (int a = 0; a < ntg; a++) edisp += ed[a];
+    }
+    // removeParticle: keep the live particles in their order
+    GPLUM_ENSURE(M.perm, m + 1);
+    hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.keep.p, M.kscan.p, M.perm.p);
+    if ((rc = gather_all(M.perm.p, m))) return rc;
     if ((rc = set_radii())) return rc;
     return soft_force(true, false);
 }

@@ -25,6 +25,20 @@ enum {
     EV_FCA0, EV_FCA1, EV_FCB0, EV_FCB1, EV_FCC0, EV_FCC1, EV_COUNT
 };
 
+struct RadiiScratch {                // setCutoffRadii on the device (radii.hip)
+    DArray<double> part, range, val, val_sorted, vd;
+    DArray<uint64_t> key, key_sorted;
+    DArray<int> np, err;
+    DArray<unsigned char> tmp;
+};
+int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScratch &R, hipStream_t st);
+
+struct MergeScratch {                 // MergeParticle on the device (system.hip)
+    DArray<int32_t> flag, scan, sel, keep, kscan, perm;
+    DArray<int32_t> tgt, twin_off, twin;
+    DArray<double> ed;
+};
+
 struct System {
     gplum_params p;
     hipStream_t st = nullptr;
@@ -36,6 +50,8 @@ struct System {
     bool have_nbr = false, timing = false;
     bool plain_f64 = false, plain_per_cluster = false;   // comparison variant of step()
     ParticleSoA A, B;
+    RadiiScratch rad;
+    MergeScratch mrg;
     Tree tree;
     DArray<JRec> jrec;
     DArray<SoftGroup> groups;
@@ -59,6 +75,7 @@ struct System {
     int get_particles(gplum_particle *pp, int count);
     int set_radii();
     int reorder();
+    int gather_all(const int32_t *perm, int m);
     int soft_force(bool initial, bool use_f64);
     int energy(double out[5]);
     int initialize();

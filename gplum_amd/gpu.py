@@ -73,7 +73,8 @@ def lib():
         L.gplum_system_get_forecast.argtypes = [vp, vp, vp, vp, ctypes.POINTER(i32)]
         L.gplum_system_put_forecast.argtypes = [vp, vp, vp, vp]
         L.gplum_hard_integrate.argtypes = [P, vp, i32, vp, i32, i32, ctypes.POINTER(f64),
-                                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(f64)]
+                                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(f64),
+                                           ctypes.POINTER(ctypes.c_int64)]
         L.gplum_mlp_forward.argtypes = [vp, vp, i32, vp]
         L.gplum_mlp_loss_grad.argtypes = [vp, vp, vp, i32, ctypes.POINTER(f64), vp]
         L.gplum_mlp_train.argtypes = [vp, vp, vp, i32, i32, f32, ctypes.POINTER(f64),
@@ -82,16 +83,21 @@ def lib():
     return _lib
 
 
-def hard_integrate(p, pp, offsets, persistent=True):
+def hard_integrate(p, pp, offsets, persistent=True, prepare=False, counts=False):
     """Integrate clusters pp[offsets[c]:offsets[c+1]] over one soft step (in place).
-    Returns (edisp, block_steps, kernel_ms)."""
+    prepare: compute acc/jerk and the first block step on the device first.
+    Returns (edisp, block_steps, kernel_ms), plus pair evaluations when counts."""
     off = np.ascontiguousarray(offsets, dtype=np.int32)
     ed = ctypes.c_double(0.0)
     nb = ctypes.c_int64(0)
     ms = ctypes.c_double(0.0)
-    check(lib().gplum_hard_integrate(ctypes.byref(p), abi.ptr(pp), len(pp), abi.ptr(off), len(off) - 1,
-                                     1 if persistent else 0, ctypes.byref(ed), ctypes.byref(nb),
-                                     ctypes.byref(ms)))
+    npair = ctypes.c_int64(0)
+    mode = (1 if persistent else 0) | (2 if prepare else 0)
+    check(lib().gplum_hard_integrate(ctypes.byref(p), abi.ptr(pp), len(pp), abi.ptr(off), len(off) - 1, mode,
+                                     ctypes.byref(ed), ctypes.byref(nb), ctypes.byref(ms),
+                                     ctypes.byref(npair)))
+    if counts:
+        return ed.value, nb.value, ms.value, npair.value
     return ed.value, nb.value, ms.value
 
 
@@ -102,7 +108,7 @@ class Stats(ctypes.Structure):
         "n_neighbors",
         "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
         "cand_retries", "n_force_calls", "fc_calls", "fc_slots", "fc_ovf_groups",
-        "fc_ovf_candidates")] + [(n, ctypes.c_double) for n in (
+        "fc_ovf_candidates", "hard_pairs")] + [(n, ctypes.c_double) for n in (
             "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms")]
 
     def as_dict(self):

@@ -136,6 +136,17 @@ def clusters(n_clusters, seed=2, size_min=2, size_max=256, m_lo=1e-12, m_hi=1e-8
     return pp, off
 
 
+def cluster_acc0(pp, off):
+    """acc0 as correctForceLong leaves it for a cluster whose members are all
+    neighbours (gravity_soft.h:330-340): (n - 1) / sum_j max(r_out_i, r_out_j)^2 / m_j."""
+    for c in range(len(off) - 1):
+        a, b = int(off[c]), int(off[c + 1])
+        ro = pp["r_out"][a:b]
+        w = np.maximum.outer(ro, ro) ** 2 / pp["mass"][a:b][None, :]
+        np.fill_diagonal(w, 0.0)
+        pp["acc0"][a:b] = (b - a - 1) / w.sum(1)
+
+
 def read_snapshot(path):
     """ASCII snapshot (FPGrav::readAscii, src/particle.h:829-843) -> (header, particles).
 

@@ -51,6 +51,7 @@ typedef struct gplum_stats {
     int64_t fc_slots;                         /* candidate slots planned                   */
     int64_t fc_ovf_groups;                    /* i-groups whose segment overflowed         */
     int64_t fc_ovf_candidates;                /* candidates redone by the fallback pass    */
+    int64_t hard_pairs;                       /* Hermite pair evaluations (hard_pair calls) */
     double force_ms, tree_ms, finalize_ms, hard_ms;
     double time_sys;
     double fc_loss;                           /* last training loss (Poisson NLL)          */
@@ -86,8 +87,8 @@ int gplum_system_soft_force(gplum_system *s, int initial, int use_f64);
 
 /* Hard part of one step on the current neighbour graph: createNeighborCluster +
  * HardSystem::makeList + HardSystem::timeIntegrate (main_p3t.cpp:472-526;
- * hard.h:321-829).  persistent: one persistent batched launch (else one
- * workgroup per cluster). */
+ * hard.h:321-829).  persistent: one persistent batched launch (else one launch
+ * per cluster). */
 int gplum_system_hard_part(gplum_system *s, int persistent);
 
 /* nsteps soft steps of the loop at main_p3t.cpp:423-699. */
@@ -130,12 +131,16 @@ int gplum_system_put_forecast(gplum_system *s, const float *theta, const float *
  * timeIntegrate_multi does (src/hermite.h:224-479) from the particles' current
  * pos/vel/dt/acc0/r_out_inv (FPHard(FPGrav) local time, src/particle.h:1096-1116),
  * replacing the cluster loops of HardSystem::timeIntegrate (src/hard.h:583-788).
- * mode 1: one persistent batched launch (largest cluster first); mode 0: one
- * launch per cluster (plain comparator).  Outputs: edisp (collision energy),
- * block_steps (summed), kernel_ms (HIP events around the launches). */
+ * mode bit 0 set: one persistent batched launch (largest cluster first); clear: one
+ * launch per cluster (plain comparator).  mode bit 1 set: the clusters are bare
+ * (pos, vel, mass, r_out_inv, acc0, dt only): Sun and hard acc/jerk and the first
+ * block step are computed on the device first, as the initial force correction
+ * would leave them (not timed).  Outputs (each may be null): edisp (collision
+ * energy), block_steps (summed), kernel_ms (HIP events around the integration
+ * launches), pair_evals (Hermite pair-force evaluations, summed). */
 int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, int n, const int32_t *offsets,
                          int n_cluster, int mode, double *edisp, int64_t *block_steps,
-                         double *kernel_ms);
+                         double *kernel_ms, int64_t *pair_evals);
 
 #ifdef __cplusplus
 }

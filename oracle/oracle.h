@@ -114,6 +114,8 @@ void orc_kepler_isolated(const gplum_params *p, gplum_particle *pi,
  * e >= 0.8 or eps2_sun != 0 (src/hard.h:803-811). */
 void orc_hermite_isolated(const gplum_params *p, gplum_particle *pi);
 /* Kepler-equation pieces, src/kepler.h:3-93 (exposed for tests). */
+double orc_cbrt_exact_ops(double x);
+double orc_exp_exact_ops(double x);
 double orc_solve_kepler(double l, double ecc);
 void orc_posvel_to_elements(const double pos[3], const double vel[3], double mu,
                             double *ax, double *ecc, double *n, double *u,

@@ -29,6 +29,10 @@ def lib():
         vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
         P = ctypes.POINTER(abi.Params)
         L.orc_params_default.argtypes = [P]
+        L.orc_cbrt_exact_ops.argtypes = [f64]
+        L.orc_cbrt_exact_ops.restype = f64
+        L.orc_exp_exact_ops.argtypes = [f64]
+        L.orc_exp_exact_ops.restype = f64
         L.orc_params_set_gamma.argtypes = [P, f64]
         for name in ("orc_epep_f32", "orc_epsp_quad_f32", "orc_epsp_mono_f32"):
             getattr(L, name).argtypes = [vp, i32, vp, i32, vp, f32]

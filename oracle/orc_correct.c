@@ -268,7 +268,8 @@ static int cmp_id(const void *a, const void *b)
 /* calcRandomVel (non-ISOTROPIC branch, particle.h:1319-1457) then setROutRSearch
  * (particle.h:713-733).  The bin sums are taken in ascending particle-id order
  * (the reference's OpenMP reduction order is unspecified; a fixed order makes the
- * radii reproducible across particle orderings). */
+ * radii reproducible across particle orderings).  exp and the cube root are the
+ * exact-operation versions of orc_math.c, shared bit for bit with the kernels. */
 void orc_set_cutoff_radii(const gplum_params *p, gplum_particle *pp, int n)
 {
     double r_max = 0., r_min = -DBL_MAX;
@@ -309,7 +310,7 @@ void orc_set_cutoff_radii(const gplum_params *p, gplum_particle *pp, int n)
         double vd = 0., ni = 0.;
         for (int b = 0; b < NB; b++) {
             double ddr = r_min + (b + 0.5) * dr - sqrt(r2);
-            double expr = exp(-ddr * ddr * drinv * drinv);
+            double expr = orc_exp_exact_ops(-ddr * ddr * drinv * drinv);
             vd += v_disp[b] * expr;
             ni += (n_ptcl[b] > 0) ? expr : 0.;
         }
@@ -318,8 +319,11 @@ void orc_set_cutoff_radii(const gplum_params *p, gplum_particle *pp, int n)
         double ax;
         double ecc = orc_eccentricity(p, pi, &ax);
         double ax2 = (ecc < 0.6) ? ax : sqrt(pi->pos[0] * pi->pos[0] + pi->pos[1] * pi->pos[1] + pi->pos[2] * pi->pos[2]);
-        double rHill = pow(pi->mass / (3. * p->m_sun), 1. / 3.) * ax2;
-        double r_out_i = smax(p->R_cut0 * pow(ax2, -p->p_cut) * rHill, p->R_cut1 * pi->v_disp * p->dt_tree);
+        double rHill = orc_cbrt_exact_ops(pi->mass / (3. * p->m_sun)) * ax2;
+        /* pow(ax2, -p_cut) is exactly 1 for the default p_cut = 0 (library pow may
+         * differ by an ulp between host and device otherwise) */
+        double pc = (p->p_cut == 0.) ? 1. : pow(ax2, -p->p_cut);
+        double r_out_i = smax(p->R_cut0 * pc * rHill, p->R_cut1 * pi->v_disp * p->dt_tree);
         double r_out = smax(r_out_i, p->r_cut_min);
         if (p->r_cut_max > 0.) r_out = smin(p->r_cut_max, r_out);
         pi->r_out = r_out;

@@ -22,6 +22,7 @@
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+
 /* FPHard (particle.h:970-1316) = FPGrav + the fields below. */
 typedef struct orc_hp {
     gplum_particle b;
@@ -319,7 +320,7 @@ static double perfect_merge(const gplum_params *p, orc_hp *pp, int n, int ic, in
     /* setParticle (flag_merge, mass_frag = 0) */
     const double mass_rem = mass_imp + mass_tar;
     pp[ic].b.r_planet = 0.;
-    pp[jc].b.r_planet *= pow(mass_rem / mass_tar, 1. / 3.);
+    pp[jc].b.r_planet *= orc_cbrt_exact_ops(mass_rem / mass_tar);
     const int64_t tar_id = pp[jc].b.id;
     pp[ic].b.id = tar_id;
     pp[ic].b.flags |= GPLUM_FLAG_DEAD;

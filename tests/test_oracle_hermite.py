@@ -237,3 +237,27 @@ def test_config1_run_energy_drift(oracle_lib, init3000):
     run.close()
     print("max |dE| over 64 steps %.3e" % worst)
     assert worst < 1e-9
+
+
+def test_cube_root_within_one_ulp(oracle_lib):
+    """The merged-radius cube root (shared with the kernels) is within 1 ulp of the
+    true root over the range mass ratios take, and exact on perfect cubes."""
+    rng = np.random.default_rng(0)
+    x = np.concatenate([10 ** rng.uniform(-6, 6, 20000), [1.0, 8.0, 27.0, 0.125, 2.0, 1e-300, 1e300]])
+    got = np.array([oracle_lib.lib().orc_cbrt_exact_ops(float(v)) for v in x])
+    ref = np.cbrt(x)
+    ulp = np.spacing(ref)
+    assert np.all(np.abs(got - ref) <= ulp)
+    assert got[-7] == 1.0 and got[-6] == 2.0 and got[-5] == 3.0 and got[-4] == 0.5
+
+
+def test_exp_within_one_ulp(oracle_lib):
+    """exp used by the velocity-dispersion smoothing (shared with the kernels) is within
+    1 ulp of libm's over the arguments it takes (-(ddr/dr)^2 <= 0), exact at 0, and 0
+    below the subnormal range."""
+    rng = np.random.default_rng(1)
+    x = np.concatenate([-rng.uniform(0, 1, 20000), -rng.uniform(0, 740, 20000), [0.0, -1e-300, -800.0]])
+    got = np.array([oracle_lib.lib().orc_exp_exact_ops(float(v)) for v in x])
+    ref = np.exp(x)
+    assert np.all(np.abs(got - ref) <= np.spacing(ref))
+    assert got[-3] == 1.0 and got[-2] == 1.0 and got[-1] == 0.0
