@@ -237,6 +237,53 @@ __device__ inline void correct(HardView &H, int g, double Dt) {
     }
 }
 
+// Register-resident twins of predict() and correct() (same operation order), for the
+// one-particle-per-lane path of clusters that fit one wave.
+__device__ inline void predict_regs(double Dt, const double pos[3], const double vel[3], const double as[3],
+                                    const double js[3], const double ad[3], const double jd[3], double a0s[3],
+                                    double j0s[3], double a0d[3], double j0d[3], double xp[3], double vp[3]) {
+    for (int k = 0; k < 3; k++) {
+        a0s[k] = as[k];
+        j0s[k] = js[k];
+        j0d[k] = jd[k];
+        a0d[k] = ad[k];
+        const double acc_sd = as[k] + ad[k];
+        const double jerk_sd = js[k] + jd[k];
+        xp[k] = pos[k] + Dt * (vel[k] + Dt * (SECONDORDER * acc_sd + THIRDORDER * Dt * jerk_sd));
+        vp[k] = vel[k] + Dt * (acc_sd + Dt * SECONDORDER * jerk_sd);
+    }
+}
+
+__device__ inline void correct_regs(double Dt, const double xp[3], const double vp[3], const double as[3],
+                                    const double a0s[3], const double j0s[3], const double js[3], const double ad[3],
+                                    const double a0d[3], const double j0d[3], const double jd[3], double pos[3],
+                                    double vel[3], double a2s[3], double a3s[3], double a2d[3], double a3d[3]) {
+    const double Dt2 = Dt * Dt;
+    const double Dt_inv = 1. / Dt;
+    const double Dt_inv2 = Dt_inv * Dt_inv;
+    const double Dt_inv3 = Dt_inv2 * Dt_inv;
+    for (int k = 0; k < 3; k++) {
+        const double Am_s = as[k] - a0s[k];
+        const double J0_s = Dt * j0s[k];
+        const double J1_s = Dt * js[k];
+        const double A2_s = 3. * Am_s - (J1_s + 2. * J0_s);
+        const double A3_s = -2. * Am_s + (J1_s + J0_s);
+        a2s[k] = 2. * Dt_inv2 * (A2_s + 3. * A3_s);
+        a3s[k] = 6. * Dt_inv3 * A3_s;
+        const double Am_d = ad[k] - a0d[k];
+        const double J0_d = j0d[k] * Dt;
+        const double J1_d = jd[k] * Dt;
+        const double A2_d = 3. * Am_d - (J1_d + 2. * J0_d);
+        const double A3_d = -2. * Am_d + (J1_d + J0_d);
+        a2d[k] = 2. * (A2_d + 3. * A3_d) * Dt_inv2;
+        a3d[k] = 6. * A3_d * Dt_inv3;
+        const double A2_sd = A2_s + A2_d;
+        const double A3_sd = A3_s + A3_d;
+        pos[k] = xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+        vel[k] = vp[k] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
+    }
+}
+
 // gravity_p / gravity_c of particle g (cluster-local i) against the predicted j-data.
 __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int n, int g,
                                   bool corrected) {
@@ -423,6 +470,275 @@ __device__ inline double block_argmin(Reduce &R, double v, int i, int j, int *bi
     return r;
 }
 
+// wave-wide min and (value, i)-lexicographic argmin (the workgroup is one wave)
+__device__ inline double wave_min(double v) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const double o = __shfl_xor(v, d);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+__device__ inline void wave_argmin(double &v, int &i, int &j) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const double ov = __shfl_xor(v, d);
+        const int oi = __shfl_xor(i, d), oj = __shfl_xor(j, d);
+        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; j = oj; }
+    }
+}
+
+// Block steps of one cluster from (sys_time, loop) to the end of the soft step on the
+// global-memory state H (j-data through J), then the final calcGravity.  Every
+// thread of the workgroup calls it; collisions and twins handled here.
+template <bool LDSJ>
+__device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
+                               int64_t *s_id, int b, int n, double &sys_time, long long &loop, double &ed,
+                               int &n_col, long long &pairs) {
+    const int t = threadIdx.x;
+    const int B = blockDim.x;
+    const double time_end = P.dt_tree;
+    while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
+        // makeActiveList: t_s = min(time + dt) over live particles
+        double tl = 1.e10;
+        for (int i = t; i < n; i += B)
+            if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tl = dmin(tl, H.time[b + i] + H.dt[b + i]);
+        const double t_s = block_min(R, tl);
+        // predict all (hermite.h:277-283)
+        for (int i = t; i < n; i += B) {
+            predict(H, b + i, t_s - H.time[b + i]);
+            if (LDSJ) {
+                for (int k = 0; k < 3; k++) {
+                    s_x[3 * i + k] = H.xp[3 * (b + i) + k];
+                    s_v[3 * i + k] = H.vp[3 * (b + i) + k];
+                }
+                s_id[i] = H.id[b + i];
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        // collisionDetermination (hermite.h:98-196)
+        int flag_col = 0, ic = -1, jc = -1;
+        if (P.collision) {
+            double rmin = 1.;
+            int bi = 0x7fffffff, bj = -1;
+            for (int i = t; i < n; i += B) {
+                if (H.flags[b + i] & GPLUM_FLAG_DEAD) continue;
+                for (int j = 0; j < n; j++) {
+                    if (i == j || (H.flags[b + j] & GPLUM_FLAG_DEAD)) continue;
+                    const double dx = H.xp[3 * (b + i)] - H.xp[3 * (b + j)];
+                    const double dy = H.xp[3 * (b + i) + 1] - H.xp[3 * (b + j) + 1];
+                    const double dz = H.xp[3 * (b + i) + 2] - H.xp[3 * (b + j) + 2];
+                    const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                    const double r2 = H.f[b + i] * H.r_planet[b + i] + H.f[b + j] * H.r_planet[b + j];
+                    const double Rr = r1 / r2;
+                    if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
+                }
+            }
+            int gi, gj;
+            const double rm = block_argmin(R, rmin, bi, bj, &gi, &gj);
+            if (rm < 1.) {
+                flag_col = 1;
+                if (H.mass[b + gi] < H.mass[b + gj]) { ic = gi; jc = gj; } else { ic = gj; jc = gi; }
+            }
+            if (flag_col)
+                for (int i = t; i < n; i += B) H.dt[b + i] = t_s - H.time[b + i];
+            __threadfence_block();
+            __syncthreads();
+        }
+        // correct the active particles (hermite.h:300-375)
+        for (int i = t; i < n; i += B) {
+            const int g = b + i;
+            if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
+            if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
+            const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
+            pairs += 3 * (long long)(n - 1);
+            gravity_pc(P, H, J, n, g, false);
+            if (merged) merge_acc_jerk(H, b, n, i);
+            const double t_c = t_s - H.time[g];
+            for (int ite = 0; ite < 2; ite++) {
+                correct(H, g, t_c);
+                gravity_pc(P, H, J, n, g, true);
+                if (merged) merge_acc_jerk(H, b, n, i);
+            }
+            H.time[g] += t_c;
+            if (!flag_col) {
+                double ad[3], jd[3], a2d[3], a3d[3], as[3], js[3], a2s[3], a3s[3];
+                ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.a2_d, g, a2d); ld3(H.a3_d, g, a3d);
+                ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js); ld3(H.a2_s, g, a2s); ld3(H.a3_s, g, a3s);
+                H.dt[g] = dt_block(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
+            }
+            if (merged) {
+                sync_twins(H, b, n, i);
+                for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) H.dt[b + j] = H.dt[g];
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (flag_col) {
+            if (t == 0) ed += perfect_merge(P, H, b, n, ic, jc);
+            n_col++;
+            __threadfence_block();
+            __syncthreads();
+            for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
+            __threadfence_block();
+            __syncthreads();
+            for (int i = t; i < n; i += B)
+                if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
+            __threadfence_block();
+            __syncthreads();
+            for (int i = t; i < n; i += B) {
+                const int g = b + i;
+                if (!(H.flags[g] & GPLUM_FLAG_DEAD)) {
+                    double ad[3], jd[3], as[3], js[3];
+                    ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js);
+                    H.dt[g] = dt_initial(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, as, js);
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            for (int i = t; i < n; i += B)
+                if (H.twin_of[b + i] >= 0) H.dt[b + i] = H.dt[b + H.twin_of[b + i]];
+            __threadfence_block();
+            __syncthreads();
+        }
+        // getSystemTime
+        double tm = 1.e10;
+        for (int i = t; i < n; i += B)
+            if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tm = dmin(tm, H.time[b + i]);
+        sys_time = block_min(R, tm);
+        loop++;
+    }
+    for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
+    __threadfence_block();
+    __syncthreads();
+    if (n_col > 0) {
+        for (int i = t; i < n; i += B)
+            if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
+    }
+    __syncthreads();
+}
+
+// Clusters that fit one wave: lane i keeps particle i's whole state in registers
+// and only the predicted j-data (and f r_planet) goes through LDS, so a block step
+// costs a few LDS round trips and wave shuffles instead of global-memory round
+// trips.  Same operations in the same order as integrate_from.  At the first block
+// step with a collision the state is written back to H and integrate_from takes
+// over from the start of that block step (it predicts the same positions and finds
+// the same pair), so merging and twins need no register path.
+__device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
+                               double *s_m, double *s_ri, int64_t *s_id, double *s_fr, int b, int n,
+                               double &sys_time, long long &loop, double &ed, int &n_col, long long &pairs) {
+    const int t = threadIdx.x;
+    const bool own = t < n;
+    const int g = b + t;
+    double x[3] = {0., 0., 0.}, v[3] = {0., 0., 0.}, as[3], js[3], ad[3], jd[3];
+    double time = 0., dt = 0., acc0 = 0., rinv = 0.;
+    int64_t id = -1;
+    if (own) {
+        ld3(H.pos, g, x);
+        ld3(H.vel, g, v);
+        time = H.time[g];
+        dt = H.dt[g];
+        acc0 = H.acc0[g];
+        rinv = H.rinv[g];
+        id = H.id[g];
+        s_m[t] = H.mass[g];
+        s_ri[t] = rinv;
+        s_id[t] = id;
+        s_fr[t] = H.f[g] * H.r_planet[g];
+        st3(s_x, t, x);
+        st3(s_v, t, v);
+    }
+    __syncthreads();
+    // initial calcGravity (hermite.h:259-270)
+    if (own) {
+        star_acc_jerk(P, x, v, as, js, nullptr);
+        gravity_loop(P, J, n, id, x, v, rinv, ad, jd, nullptr);
+    }
+    const double time_end = P.dt_tree;
+    double a0s[3], j0s[3], a0d[3], j0d[3], xp[3], vp[3], a2s[3], a3s[3], a2d[3], a3d[3];
+    while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
+        const double t_s = wave_min(own ? time + dt : 1.e10);
+        __syncthreads();   // the previous step's j-data reads are done
+        if (own) {
+            predict_regs(t_s - time, x, v, as, js, ad, jd, a0s, j0s, a0d, j0d, xp, vp);
+            st3(s_x, t, xp);
+            st3(s_v, t, vp);
+        }
+        __syncthreads();
+        if (P.collision) {
+            double rmin = 1.;
+            int bi = 0x7fffffff, bj = -1;
+            if (own) {
+                for (int j = 0; j < n; j++) {
+                    if (j == t) continue;
+                    const double dx = s_x[3 * t] - s_x[3 * j];
+                    const double dy = s_x[3 * t + 1] - s_x[3 * j + 1];
+                    const double dz = s_x[3 * t + 2] - s_x[3 * j + 2];
+                    const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                    const double r2 = s_fr[t] + s_fr[j];
+                    const double Rr = r1 / r2;
+                    if (Rr < rmin) { rmin = Rr; bi = t; bj = j; }
+                }
+            }
+            wave_argmin(rmin, bi, bj);
+            if (rmin < 1.) {
+                // hand the cluster to the global-memory path at the start of this step
+                if (own) {
+                    st3(H.pos, g, x); st3(H.vel, g, v);
+                    st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
+                    st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
+                    H.time[g] = time;
+                    H.dt[g] = dt;
+                    H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+                    H.twin_of[g] = -1;
+                }
+                __threadfence_block();
+                __syncthreads();
+                integrate_from<true>(P, H, J, R, s_x, s_v, s_id, b, n, sys_time, loop, ed, n_col, pairs);
+                return;
+            }
+        }
+        if (own && time + dt == t_s) {
+            pairs += 3 * (long long)(n - 1);
+            star_acc_jerk(P, xp, vp, as, js, nullptr);
+            gravity_loop(P, J, n, id, xp, vp, rinv, ad, jd, nullptr);
+            const double t_c = t_s - time;
+            for (int ite = 0; ite < 2; ite++) {
+                correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                star_acc_jerk(P, x, v, as, js, nullptr);
+                gravity_loop(P, J, n, id, x, v, rinv, ad, jd, nullptr);
+            }
+            time += t_c;
+            dt = dt_block(P, time, dt, acc0, ad, jd, a2d, a3d, as, js, a2s, a3s);
+        }
+        sys_time = wave_min(own ? time : 1.e10);
+        loop++;
+    }
+    // final calcGravity with potentials (hermite.h:466-470), current positions
+    __syncthreads();
+    if (own) {
+        st3(s_x, t, x);
+        st3(s_v, t, v);
+    }
+    __syncthreads();
+    if (own) {
+        double phs = 0., phd = 0.;
+        star_acc_jerk(P, x, v, as, js, &phs);
+        gravity_loop(P, J, n, id, x, v, rinv, ad, jd, &phd);
+        pairs += n - 1;
+        st3(H.pos, g, x); st3(H.vel, g, v);
+        st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
+        st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
+        H.time[g] = time;
+        H.dt[g] = dt;
+        H.phi_s[g] = phs;
+        H.phi_d[g] = phd;
+        H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+        H.twin_of[g] = -1;
+    }
+    __syncthreads();
+}
+
 template <bool LDSJ>
 __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                                                            const int32_t *__restrict__ cl_off,
@@ -432,6 +748,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                                                            long long *__restrict__ npair, int persistent) {
     __shared__ double s_x[3 * HERMITE_LDS], s_v[3 * HERMITE_LDS], s_m[HERMITE_LDS], s_ri[HERMITE_LDS];
     __shared__ int64_t s_id[HERMITE_LDS];
+    __shared__ double s_fr[HERMITE_BLOCK];
     __shared__ Reduce R;
     __shared__ int s_cl;
     __shared__ unsigned long long s_pairs;
@@ -463,137 +780,25 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
         // pair evaluations of this cluster (hard_pair calls), for the roofline
         long long pairs = (long long)(n > 0 ? n - 1 : 0) * ((t < n) ? (n - 1 - t) / B + 1 : 0);
         if (t == 0) s_pairs = 0ull;
-        // initial calcGravity for all, flags cleared (hermite.h:259-270)
-        for (int i = t; i < n; i += B) {
-            gravity_full(P, H, b, n, i);
-            H.flags[b + i] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
-            H.twin_of[b + i] = -1;
-        }
-        __syncthreads();
         double ed = 0.;
         int n_col = 0;
         long long loop = 0;
         double sys_time = 0.;
-        const double time_end = P.dt_tree;
-        while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
-            // makeActiveList: t_s = min(time + dt) over live particles
-            double tl = 1.e10;
-            for (int i = t; i < n; i += B)
-                if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tl = dmin(tl, H.time[b + i] + H.dt[b + i]);
-            const double t_s = block_min(R, tl);
-            // predict all (hermite.h:277-283)
+        if (LDSJ && n <= B && B == 64) {
+            integrate_regs(P, H, J, R, s_x, s_v, s_m, s_ri, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
+        } else {
+            // initial calcGravity for all, flags cleared (hermite.h:259-270)
             for (int i = t; i < n; i += B) {
-                predict(H, b + i, t_s - H.time[b + i]);
-                if (LDSJ) {
-                    for (int k = 0; k < 3; k++) {
-                        s_x[3 * i + k] = H.xp[3 * (b + i) + k];
-                        s_v[3 * i + k] = H.vp[3 * (b + i) + k];
-                    }
-                    s_id[i] = H.id[b + i];
-                }
+                gravity_full(P, H, b, n, i);
+                H.flags[b + i] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+                H.twin_of[b + i] = -1;
             }
-            __threadfence_block();
             __syncthreads();
-            // collisionDetermination (hermite.h:98-196)
-            int flag_col = 0, ic = -1, jc = -1;
-            if (P.collision) {
-                double rmin = 1.;
-                int bi = 0x7fffffff, bj = -1;
-                for (int i = t; i < n; i += B) {
-                    if (H.flags[b + i] & GPLUM_FLAG_DEAD) continue;
-                    for (int j = 0; j < n; j++) {
-                        if (i == j || (H.flags[b + j] & GPLUM_FLAG_DEAD)) continue;
-                        const double dx = H.xp[3 * (b + i)] - H.xp[3 * (b + j)];
-                        const double dy = H.xp[3 * (b + i) + 1] - H.xp[3 * (b + j) + 1];
-                        const double dz = H.xp[3 * (b + i) + 2] - H.xp[3 * (b + j) + 2];
-                        const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-                        const double r2 = H.f[b + i] * H.r_planet[b + i] + H.f[b + j] * H.r_planet[b + j];
-                        const double Rr = r1 / r2;
-                        if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
-                    }
-                }
-                int gi, gj;
-                const double rm = block_argmin(R, rmin, bi, bj, &gi, &gj);
-                if (rm < 1.) {
-                    flag_col = 1;
-                    if (H.mass[b + gi] < H.mass[b + gj]) { ic = gi; jc = gj; } else { ic = gj; jc = gi; }
-                }
-                if (flag_col)
-                    for (int i = t; i < n; i += B) H.dt[b + i] = t_s - H.time[b + i];
-                __threadfence_block();
-                __syncthreads();
-            }
-            // correct the active particles (hermite.h:300-375)
-            for (int i = t; i < n; i += B) {
-                const int g = b + i;
-                if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
-                if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
-                const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
-                pairs += 3 * (long long)(n - 1);
-                gravity_pc(P, H, J, n, g, false);
-                if (merged) merge_acc_jerk(H, b, n, i);
-                const double t_c = t_s - H.time[g];
-                for (int ite = 0; ite < 2; ite++) {
-                    correct(H, g, t_c);
-                    gravity_pc(P, H, J, n, g, true);
-                    if (merged) merge_acc_jerk(H, b, n, i);
-                }
-                H.time[g] += t_c;
-                if (!flag_col) {
-                    double ad[3], jd[3], a2d[3], a3d[3], as[3], js[3], a2s[3], a3s[3];
-                    ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.a2_d, g, a2d); ld3(H.a3_d, g, a3d);
-                    ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js); ld3(H.a2_s, g, a2s); ld3(H.a3_s, g, a3s);
-                    H.dt[g] = dt_block(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
-                }
-                if (merged) {
-                    sync_twins(H, b, n, i);
-                    for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) H.dt[b + j] = H.dt[g];
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
-            if (flag_col) {
-                if (t == 0) ed += perfect_merge(P, H, b, n, ic, jc);
-                n_col++;
-                __threadfence_block();
-                __syncthreads();
-                for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
-                __threadfence_block();
-                __syncthreads();
-                for (int i = t; i < n; i += B)
-                    if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
-                __threadfence_block();
-                __syncthreads();
-                for (int i = t; i < n; i += B) {
-                    const int g = b + i;
-                    if (!(H.flags[g] & GPLUM_FLAG_DEAD)) {
-                        double ad[3], jd[3], as[3], js[3];
-                        ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js);
-                        H.dt[g] = dt_initial(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, as, js);
-                    }
-                }
-                __threadfence_block();
-                __syncthreads();
-                for (int i = t; i < n; i += B)
-                    if (H.twin_of[b + i] >= 0) H.dt[b + i] = H.dt[b + H.twin_of[b + i]];
-                __threadfence_block();
-                __syncthreads();
-            }
-            // getSystemTime
-            double tm = 1.e10;
-            for (int i = t; i < n; i += B)
-                if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tm = dmin(tm, H.time[b + i]);
-            sys_time = block_min(R, tm);
-            loop++;
+            integrate_from<LDSJ>(P, H, J, R, s_x, s_v, s_id, b, n, sys_time, loop, ed, n_col, pairs);
         }
-        for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
         atomicAdd(&s_pairs, (unsigned long long)pairs);
-        __threadfence_block();
         __syncthreads();
-        if (n_col > 0) {
-            for (int i = t; i < n; i += B)
-                if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
-        }
+        const double time_end = P.dt_tree;
         if (t == 0) {
             edisp[c] = ed;
             ncol[c] = n_col;
