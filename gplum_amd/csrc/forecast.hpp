@@ -24,8 +24,8 @@ struct CandSizer {
     bool enabled = false, normalized = false;
     bool uniform = false;             // ablation: every particle gets uniform_rate
     float uniform_rate = 0.f;         //   (the previous call's mean count)
-    int train_steps = 4;              // Adam steps per soft step
-    int batch = 16384;                // samples per Adam step
+    int train_steps = 2;              // Adam steps per soft step
+    int batch = 8192;                 // samples per Adam step
     int trained_rounds = 0;           // soft steps with training so far
     int min_rounds = 1;               // forecast used once this many rounds trained
     unsigned phase = 0;

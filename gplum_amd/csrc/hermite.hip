@@ -75,8 +75,8 @@ __device__ inline double dt_initial(const HP &P, double time, double dt, double 
     double dt_next = 0.5 * P.dt_tree;
     const double dt_1 = dmin(calc_dt2(P.eta_0, P.alpha2, acc0, acc_d, jerk_d),
                              calc_dt2(P.eta_sun0, P.alpha2, 0., acc_s, jerk_s));
-    double rem = fmod(time, dt_next);
-    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(time, dt_next); }
+    double rem = fmod_block(time, dt_next);
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(time, dt_next); }
     if (rem != 0.0) return P.dt_min;
     if (dt > 0.)
         for (int it = 0; 2. * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
@@ -93,8 +93,8 @@ __device__ inline double dt_block(const HP &P, double time, double dt, double ac
     double dt_next = dmin(2. * dt, 0.5 * P.dt_tree);
     const double dt_1 = dmin(calc_dt4(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
                              calc_dt4(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
-    double rem = fmod(time, dt_next);
-    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(time, dt_next); }
+    double rem = fmod_block(time, dt_next);
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(time, dt_next); }
     if (rem != 0.0) return P.dt_min;
     for (int it = 0; dt_1 < dt_next && 0.5 * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
     if (dt_next < 2. * P.dt_min) dt_next = P.dt_min;

@@ -21,14 +21,13 @@ constexpr float MLP_EPS = 1e-6f;
 __device__ inline float softplus_f(float z) { return z > 20.f ? z : log1pf(expf(z)); }
 __device__ inline float sigmoid_f(float z) { return 1.f / (1.f + expf(-z)); }
 
-__device__ inline void stage_params(const float *__restrict__ theta, float *s) {
-    for (int k = threadIdx.x; k < MLP_NPARAM; k += blockDim.x) s[k] = theta[k];
-    __syncthreads();
-}
-
-// per-sample forward pass; h1, h2 in registers
-__device__ inline float mlp_eval(const float *s, const float x[MLP_NIN], float h1[MLP_NH], float h2[MLP_NH],
-                                 float *zout) {
+// per-sample forward pass; h1, h2 in registers.  The weights are read with
+// wave-uniform addresses straight from global memory, which the compiler turns into
+// scalar loads: every lane uses the same weight, so no LDS traffic per FMA.
+__device__ inline float mlp_eval(const float *__restrict__ s, const float x[MLP_NIN], float h1[MLP_NH],
+                                 float h2[MLP_NH], float *zout) {
+    // every loop fully unrolled: h1/h2 are indexed by compile-time constants and stay
+    // in registers (a runtime index would put them in scratch memory)
 #pragma unroll
     for (int p = 0; p < MLP_NH; p++) {
         float a = s[OFF_B1 + p];
@@ -36,9 +35,10 @@ __device__ inline float mlp_eval(const float *s, const float x[MLP_NIN], float h
         for (int q = 0; q < MLP_NIN; q++) a = __builtin_fmaf(s[OFF_W1 + p * MLP_NIN + q], x[q], a);
         h1[p] = tanhf(a);
     }
+#pragma unroll
     for (int p = 0; p < MLP_NH; p++) {
         float a = s[OFF_B2 + p];
-#pragma unroll 16
+#pragma unroll
         for (int q = 0; q < MLP_NH; q++) a = __builtin_fmaf(s[OFF_W2 + p * MLP_NH + q], h1[q], a);
         h2[p] = tanhf(a);
     }
@@ -51,13 +51,12 @@ __device__ inline float mlp_eval(const float *s, const float x[MLP_NIN], float h
 
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ theta, const float *__restrict__ x,
                                                      int n, float *__restrict__ lam) {
-    __shared__ float s[MLP_NPARAM];
-    stage_params(theta, s);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float xi[MLP_NIN], h1[MLP_NH], h2[MLP_NH], z;
+#pragma unroll
     for (int q = 0; q < MLP_NIN; q++) xi[q] = x[(size_t)q * n + i];
-    lam[i] = mlp_eval(s, xi, h1, h2, &z);
+    lam[i] = mlp_eval(theta, xi, h1, h2, &z);
 }
 
 // per-sample backward: stores h1, h2, d1, d2 (sample-major, [n][64]) and dz[n]
@@ -66,13 +65,13 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const float *__restrict__ 
                                                       float *__restrict__ H2, float *__restrict__ D1,
                                                       float *__restrict__ D2, float *__restrict__ DZ,
                                                       double *__restrict__ loss_part) {
-    __shared__ float s[MLP_NPARAM];
     __shared__ double sl[256];
-    stage_params(theta, s);
+    const float *__restrict__ s = theta;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double li = 0.;
     if (i < n) {
         float xi[MLP_NIN], h1[MLP_NH], h2[MLP_NH], z;
+#pragma unroll
         for (int q = 0; q < MLP_NIN; q++) xi[q] = x[(size_t)q * n + i];
         const float lam = mlp_eval(s, xi, h1, h2, &z);
         const float yi = y[i];
@@ -80,14 +79,17 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const float *__restrict__ 
         const float dz = (1.f - yi / (lam + MLP_EPS)) * sigmoid_f(z) / (float)n;
         DZ[i] = dz;
         float d2[MLP_NH];
+#pragma unroll
         for (int p = 0; p < MLP_NH; p++) {
             d2[p] = dz * s[OFF_W3 + p] * (1.f - h2[p] * h2[p]);
             H1[(size_t)i * MLP_NH + p] = h1[p];
             H2[(size_t)i * MLP_NH + p] = h2[p];
             D2[(size_t)i * MLP_NH + p] = d2[p];
         }
+#pragma unroll
         for (int q = 0; q < MLP_NH; q++) {
             float a = 0.f;
+#pragma unroll
             for (int p = 0; p < MLP_NH; p++) a = __builtin_fmaf(s[OFF_W2 + p * MLP_NH + q], d2[p], a);
             D1[(size_t)i * MLP_NH + q] = a * (1.f - h1[q] * h1[q]);
         }

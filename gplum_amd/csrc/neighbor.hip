@@ -167,8 +167,8 @@ __global__ void k_finalize(NbParams P, PView V, int n, int initial, const int32_
         const double dt_1 = smin(calc_dt2(P.eta_0, P.alpha2, a0, ad_old, a.jerk),
                                  calc_dt2(P.eta_sun0, P.alpha2, 0., as, js));
         const double tm = time[i];
-        double rem = fmod(tm, dt_next);
-        for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod(tm, dt_next); }
+        double rem = fmod_block(tm, dt_next);
+        for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(tm, dt_next); }
         const double dt0 = dt[i];
         if (dt0 > 0.)
             for (int it = 0; 2. * dt0 < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;

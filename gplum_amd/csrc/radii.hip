@@ -3,8 +3,8 @@
 //
 // Every expression keeps the operation order of the reference (and of the oracle),
 // so r_out / r_search come out bit-identical to the oracle's: the radial range is a
-// max-reduction (exact in any order), the 32 bin sums run sequentially over each bin's
-// particles in ascending id (after a radix sort by (bin, id)), and the transcendental
+// max-reduction (exact in any order), the 32 bin sums run over each bin's particles in
+// ascending id in a fixed blocked order (after a radix sort by (bin, id)), and the transcendental
 // functions are the shared exact-operation versions (exp_exact_ops, cbrt_exact_ops).
 #include <cfloat>
 
@@ -42,14 +42,28 @@ __global__ void k_radii_range(int n, const double *__restrict__ pos, double *__r
     }
 }
 
-// range[0] = r_min, range[1] = dr, range[2] = 1/dr
-__global__ void k_radii_range_final(int nb, const double *__restrict__ part, double *__restrict__ range) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double r_max = 0., r_min = -DBL_MAX;
-    for (int k = 0; k < nb; k++) {
-        r_max = smax(r_max, part[2 * k]);
-        r_min = smax(r_min, part[2 * k + 1]);
+// range[0] = r_min, range[1] = dr, range[2] = 1/dr (one block of 256 threads)
+__global__ __launch_bounds__(256) void k_radii_range_final(int nb, const double *__restrict__ part,
+                                                           double *__restrict__ range) {
+    __shared__ double s_a[256], s_b[256];
+    const int t = threadIdx.x;
+    double a = 0., b = -DBL_MAX;
+    for (int k = t; k < nb; k += 256) {
+        a = smax(a, part[2 * k]);
+        b = smax(b, part[2 * k + 1]);
     }
+    s_a[t] = a;
+    s_b[t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s_a[t] = smax(s_a[t], s_a[t + w]);
+            s_b[t] = smax(s_b[t], s_b[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    double r_max = s_a[0], r_min = s_b[0];
     r_max = sqrt(r_max) * 1.01;
     r_min = sqrt(-r_min) * 0.99;
     const double dr = (r_max - r_min) / RADII_NB;
@@ -79,26 +93,60 @@ __global__ void k_radii_bin(int n, const double *__restrict__ pos, const double 
     val[i] = ux * ux + uy * uy + uz * uz;
 }
 
-// one thread per bin: sequential sum over the bin's particles in ascending id
-__global__ void k_radii_binsum(int n, const uint64_t *__restrict__ key, const double *__restrict__ val,
-                               double *__restrict__ vd, int *__restrict__ np) {
+// Bin sums in a fixed blocked order: each bin's values in ascending id, summed in
+// consecutive chunks of RADII_CHUNK (left to right), then the chunk sums left to
+// right.  The oracle sums in the same blocked order, so v_disp agrees bit for bit.
+constexpr int RADII_CHUNK = 64;
+
+// one thread per bin: range of the bin in the (bin, id)-sorted keys and its chunks
+__global__ void k_radii_binrange(int n, const uint64_t *__restrict__ key, int *__restrict__ range,
+                                 int *__restrict__ cbase) {
     const int b = threadIdx.x;
-    if (b >= RADII_NB) return;
-    auto lower = [&](uint64_t k) {
-        int lo = 0, hi = n;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (key[mid] < k) lo = mid + 1; else hi = mid;
+    if (b < RADII_NB) {
+        auto lower = [&](uint64_t k) {
+            int lo = 0, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (key[mid] < k) lo = mid + 1; else hi = mid;
+            }
+            return lo;
+        };
+        range[2 * b] = lower((uint64_t)b << RADII_ID_BITS);
+        range[2 * b + 1] = lower((uint64_t)(b + 1) << RADII_ID_BITS);
+    }
+    __syncthreads();
+    if (b == 0) {
+        int c = 0;
+        for (int q = 0; q < RADII_NB; q++) {
+            cbase[q] = c;
+            c += (range[2 * q + 1] - range[2 * q] + RADII_CHUNK - 1) / RADII_CHUNK;
         }
-        return lo;
-    };
-    const int a0 = lower((uint64_t)b << RADII_ID_BITS);
-    const int a1 = lower((uint64_t)(b + 1) << RADII_ID_BITS);
+        cbase[RADII_NB] = c;
+    }
+}
+
+__global__ void k_radii_chunksum(const int *__restrict__ range, const int *__restrict__ cbase,
+                                 const double *__restrict__ val, double *__restrict__ csum) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cbase[RADII_NB]) return;
+    int b = 0;
+    while (b + 1 < RADII_NB && cbase[b + 1] <= c) b++;
+    const int a0 = range[2 * b] + (c - cbase[b]) * RADII_CHUNK;
+    const int a1 = min(a0 + RADII_CHUNK, range[2 * b + 1]);
     double s = 0.;
     for (int a = a0; a < a1; a++) s += val[a];
-    const int c = a1 - a0;
-    np[b] = c;
-    vd[b] = (c > 0) ? sqrt(s / c) : 0.;
+    csum[c] = s;
+}
+
+__global__ void k_radii_binfinal(const int *__restrict__ range, const int *__restrict__ cbase,
+                                 const double *__restrict__ csum, double *__restrict__ vd, int *__restrict__ np) {
+    const int b = threadIdx.x;
+    if (b >= RADII_NB) return;
+    double s = 0.;
+    for (int c = cbase[b]; c < cbase[b + 1]; c++) s += csum[c];
+    const int cnt = range[2 * b + 1] - range[2 * b];
+    np[b] = cnt;
+    vd[b] = (cnt > 0) ? sqrt(s / cnt) : 0.;
 }
 
 struct RadiiParams {
@@ -161,7 +209,7 @@ int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScrat
     GPLUM_ENSURE(R.err, 1);
     GPLUM_HIP(hipMemsetAsync(R.err.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_radii_range, dim3(nb), dim3(256), 0, st, n, A.pos.p, R.part.p);
-    hipLaunchKernelGGL(k_radii_range_final, dim3(1), dim3(64), 0, st, nb, R.part.p, R.range.p);
+    hipLaunchKernelGGL(k_radii_range_final, dim3(1), dim3(256), 0, st, nb, R.part.p, R.range.p);
     hipLaunchKernelGGL(k_radii_bin, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.vel.p, A.id.p, p.m_sun,
                        R.range.p, R.key.p, R.val.p, R.err.p);
     size_t bytes = 0;
@@ -170,8 +218,14 @@ int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScrat
     GPLUM_ENSURE(R.tmp, bytes + 16);
     GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(R.tmp.p, bytes, R.key.p, R.key_sorted.p, R.val.p, R.val_sorted.p, n,
                                                   0, 64, st));
-    hipLaunchKernelGGL(k_radii_binsum, dim3(1), dim3(RADII_NB), 0, st, n, R.key_sorted.p, R.val_sorted.p, R.vd.p,
-                       R.np.p);
+    GPLUM_ENSURE(R.brange, 2 * RADII_NB);
+    GPLUM_ENSURE(R.cbase, RADII_NB + 1);
+    const int max_chunks = ceil_div(n, RADII_CHUNK) + RADII_NB;
+    GPLUM_ENSURE(R.csum, max_chunks);
+    hipLaunchKernelGGL(k_radii_binrange, dim3(1), dim3(64), 0, st, n, R.key_sorted.p, R.brange.p, R.cbase.p);
+    hipLaunchKernelGGL(k_radii_chunksum, dim3(ceil_div(max_chunks, 256)), dim3(256), 0, st, R.brange.p, R.cbase.p,
+                       R.val_sorted.p, R.csum.p);
+    hipLaunchKernelGGL(k_radii_binfinal, dim3(1), dim3(64), 0, st, R.brange.p, R.cbase.p, R.csum.p, R.vd.p, R.np.p);
     RadiiParams P{p.m_sun, p.dt_tree, p.R_cut0, p.R_cut1, p.p_cut, p.r_cut_min, p.r_cut_max, p.R_search0,
                   p.R_search1};
     hipLaunchKernelGGL(k_radii_apply, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, P, R.range.p, R.vd.p, R.np.p,

@@ -335,7 +335,8 @@ void System::tick(int k) {
 
 double System::elapsed(int a, int b) {
     float ms = 0.f;
-    if (timing && hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess) return ms;
+    if (timing && hipEventSynchronize(ev[b]) == hipSuccess && hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess)
+        return ms;
     return 0.;
 }
 

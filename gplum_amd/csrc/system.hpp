@@ -26,9 +26,9 @@ enum {
 };
 
 struct RadiiScratch {                // setCutoffRadii on the device (radii.hip)
-    DArray<double> part, range, val, val_sorted, vd;
+    DArray<double> part, range, val, val_sorted, vd, csum;
     DArray<uint64_t> key, key_sorted;
-    DArray<int> np, err;
+    DArray<int> np, err, brange, cbase;
     DArray<unsigned char> tmp;
 };
 int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScratch &R, hipStream_t st);

@@ -77,14 +77,28 @@ __global__ void k_bbox_partial(const double *__restrict__ pos, int n, double *__
         for (int k = 0; k < 6; k++) part[6 * blockIdx.x + k] = s[k][0];
 }
 
-__global__ void k_bbox_final(const double *__restrict__ part, int nb, double *__restrict__ os) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one block of 256 threads: min/max over the per-block partials (exact in any order)
+__global__ __launch_bounds__(256) void k_bbox_final(const double *__restrict__ part, int nb, double *__restrict__ os) {
+    __shared__ double s[6][256];
+    const int t = threadIdx.x;
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-    for (int b = 0; b < nb; b++)
+    for (int b = t; b < nb; b += 256)
         for (int k = 0; k < 3; k++) {
             if (part[6 * b + k] < mn[k]) mn[k] = part[6 * b + k];
             if (part[6 * b + 3 + k] > mx[k]) mx[k] = part[6 * b + 3 + k];
         }
+    for (int k = 0; k < 3; k++) { s[k][t] = mn[k]; s[3 + k][t] = mx[k]; }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 3; k++) {
+                if (s[k][t + w] < s[k][t]) s[k][t] = s[k][t + w];
+                if (s[3 + k][t + w] > s[3 + k][t]) s[3 + k][t] = s[3 + k][t + w];
+            }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    for (int k = 0; k < 3; k++) { mn[k] = s[k][0]; mx[k] = s[3 + k][0]; }
     double ext = 0.;
     for (int k = 0; k < 3; k++) {
         const double e = mx[k] - mn[k];
@@ -432,7 +446,7 @@ int tree_sort_keys(Tree &t, const double *d_pos, int n, hipStream_t st) {
     GPLUM_ENSURE(t.bbox, 6 * nb + 8);
     GPLUM_ENSURE(t.origin_scale, 8);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, d_pos, n, t.bbox.p);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, t.bbox.p, nb, t.origin_scale.p);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, t.bbox.p, nb, t.origin_scale.p);
     GPLUM_ENSURE(t.key, n);
     GPLUM_ENSURE(t.key_alt, n);
     GPLUM_ENSURE(t.idx, n);

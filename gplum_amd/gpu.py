@@ -199,7 +199,7 @@ class System:
 
     FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}
 
-    def set_forecast(self, mode="network", train_steps=4, batch=16384, lr=3e-3):
+    def set_forecast(self, mode="network", train_steps=2, batch=8192, lr=3e-3):
         check(lib().gplum_system_set_forecast(self.h, self.FORECAST_MODES[mode], train_steps, batch, lr))
 
     def get_forecast(self):

@@ -266,9 +266,10 @@ static int cmp_id(const void *a, const void *b)
 }
 
 /* calcRandomVel (non-ISOTROPIC branch, particle.h:1319-1457) then setROutRSearch
- * (particle.h:713-733).  The bin sums are taken in ascending particle-id order
- * (the reference's OpenMP reduction order is unspecified; a fixed order makes the
- * radii reproducible across particle orderings).  exp and the cube root are the
+ * (particle.h:713-733).  The bin sums are taken in ascending particle-id order, in
+ * chunks of 64 (the reference's OpenMP reduction order is unspecified; a fixed order
+ * makes the radii reproducible across particle orderings and lets the kernels sum the
+ * chunks in parallel with the same result).  exp and the cube root are the
  * exact-operation versions of orc_math.c, shared bit for bit with the kernels. */
 void orc_set_cutoff_radii(const gplum_params *p, gplum_particle *pp, int n)
 {
@@ -283,9 +284,13 @@ void orc_set_cutoff_radii(const gplum_params *p, gplum_particle *pp, int n)
     enum { NB = 32 };
     const double dr = (r_max - r_min) / NB;
     const double drinv = 1. / dr;
-    double v_disp[NB];
-    int32_t n_ptcl[NB];
-    for (int b = 0; b < NB; b++) { v_disp[b] = 0.; n_ptcl[b] = 0; }
+    /* bin sums in a fixed blocked order: chunks of 64 consecutive values (ascending
+     * id) summed left to right, then the chunk sums left to right (the kernels sum
+     * the same way, in parallel over chunks) */
+    enum { CHUNK = 64 };
+    double v_disp[NB], chunk[NB];
+    int32_t n_ptcl[NB], in_chunk[NB];
+    for (int b = 0; b < NB; b++) { v_disp[b] = 0.; n_ptcl[b] = 0; chunk[b] = 0.; in_chunk[b] = 0; }
     idkey *ord = (idkey *)malloc(sizeof(idkey) * (n > 0 ? n : 1));
     for (int i = 0; i < n; i++) { ord[i].id = pp[i].id; ord[i].idx = i; }
     qsort(ord, n, sizeof(idkey), cmp_id);
@@ -299,9 +304,11 @@ void orc_set_cutoff_radii(const gplum_params *p, gplum_particle *pp, int n)
         double vkx = -pi->pos[1] / ri, vky = pi->pos[0] / ri, vkz = 0.;
         vkx *= vk; vky *= vk; vkz *= vk;
         double vx = pi->vel[0] - vkx, vy = pi->vel[1] - vky, vz = pi->vel[2] - vkz;
-        v_disp[j] += vx * vx + vy * vy + vz * vz;
+        chunk[j] += vx * vx + vy * vy + vz * vz;
         n_ptcl[j]++;
+        if (++in_chunk[j] == CHUNK) { v_disp[j] += chunk[j]; chunk[j] = 0.; in_chunk[j] = 0; }
     }
+    for (int b = 0; b < NB; b++) if (in_chunk[b]) v_disp[b] += chunk[b];
     free(ord);
     for (int b = 0; b < NB; b++) v_disp[b] = (n_ptcl[b] > 0) ? sqrt(v_disp[b] / n_ptcl[b]) : 0.;
     for (int i = 0; i < n; i++) {

@@ -71,9 +71,22 @@ def test_hermite_perfect_merging_matches_oracle(oracle_lib):
     dead_ref = (ref["flags"] & 1) != 0
     dead_got = (got["flags"] & 1) != 0
     print("collisions (oracle) %d, dead %d, edisp %.3e vs %.3e" % (nc, dead_ref.sum(), ed, ed_g))
+    # cluster by cluster, so that a mismatch names its cluster
+    bad = []
+    for c in range(len(off) - 1):
+        a, b = int(off[c]), int(off[c + 1])
+        r = pp[a:b].copy()
+        nc_c, ed_c, nb_c = oracle_lib.hermite_cluster(po, r)
+        g = pp[a:b].copy()
+        ed_gc, nb_gc, _ = gpu.hard_integrate(pg, g, np.array([0, b - a]), persistent=True)
+        dx = np.abs(g["pos"] - r["pos"]).max() / np.abs(r["pos"]).max()
+        if nb_gc != nb_c or dx > 1e-12:
+            bad.append((c, b - a, nc_c, nb_c, nb_gc, float(dx), float(np.abs(g["dt"] - r["dt"]).max())))
+    print("clusters differing (c, n, collisions, steps oracle, steps gpu, pos diff, dt diff):", bad)
     assert nc > 0
     assert np.array_equal(dead_got, dead_ref)
     assert np.array_equal(got["id"], ref["id"])
+    assert not bad
     assert nb_g == nb_ref
     assert abs(ed_g - ed) <= 1e-10 * abs(ed)
     rel = np.abs(got["pos"] - ref["pos"]).max() / np.abs(ref["pos"]).max()
