@@ -10,6 +10,7 @@ A step is one full soft step: kick, clusters, batched Hermite + Kepler drift, tr
 FP32 EP-EP/EP-SP, FP64 neighbour finalize, kick (energy after the timed region).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+    python bench.py --workload hard [--clusters 50000]     (batched hard part alone)
 """
 import argparse
 import json
@@ -64,6 +65,83 @@ def cpu_baseline(n, seed, threads):
                       "Kepler, OpenMP over groups and clusters), %.2f s" % (steps, n, dt)}
 
 
+PEAK_FP64_TFLOPS = 78.6           # MI355X published dense FP64 vector peak
+FLOP_HARD_PAIR = 75               # FP64 flops per hard_pair evaluation (op list in the kernel docs)
+
+
+def hard_params():
+    # perfect merging on: without it, bound pairs of 1e-8 bodies in these dense groups
+    # shrink their block step towards dt_min = 2^-30 (up to 2^26 block steps per cluster)
+    return dict(eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002, dt_tree=2.0 ** -4, dt_min=2.0 ** -30,
+                gamma=0.1, collision=1)
+
+
+def bench_hard(args):
+    """Batched hard part: 50,000 clusters (sizes log-uniform in [2, 256], Gaussian members
+    with sigma = 2 Hill radii of m = 1e-10 around circular 1 AU orbits, masses log-uniform
+    in [1e-12, 1e-8]) over one soft step 2^-4 with perfect merging; persistent launch,
+    compared with one launch per cluster on a sample of the clusters; the oracle on a
+    bounded sample of the same clusters as the CPU baseline."""
+    from gplum_amd import gpu, ic
+    n_cl = args.clusters
+    p = gpu.params(**hard_params())
+    pp, off = ic.clusters(n_cl, seed=2)
+    ic.cluster_acc0(pp, off)
+    n = len(pp)
+    print("# %d clusters, %d particles" % (n_cl, n), file=sys.stderr, flush=True)
+    warm = pp.copy()
+    gpu.hard_integrate(p, warm, off, persistent=True, prepare=True)
+    a = pp.copy()
+    ed, nb, ms_p, npair = gpu.hard_integrate(p, a, off, persistent=True, prepare=True, counts=True)
+    print("# persistent: %.2f ms" % ms_p, file=sys.stderr, flush=True)
+    # per-cluster launches run the clusters one after another: compare on a sample
+    k = min(args.compare_clusters, n_cl)
+    s_off = off[: k + 1]
+    sub = pp[: s_off[-1]]
+    sa, sb = sub.copy(), sub.copy()
+    _, nb_a, ms_sp = gpu.hard_integrate(p, sa, s_off, persistent=True, prepare=True)
+    _, nb_b, ms_c = gpu.hard_integrate(p, sb, s_off, persistent=False, prepare=True)
+    print("# sample of %d clusters: persistent %.2f ms, per-cluster launches %.2f ms" % (k, ms_sp, ms_c),
+          file=sys.stderr, flush=True)
+    same = bool(np.array_equal(sa["pos"], sb["pos"]) and np.array_equal(sa["vel"], sb["vel"]) and nb_a == nb_b)
+    value = n / (ms_p * 1e-3)
+    achieved = FLOP_HARD_PAIR * npair / (ms_p * 1e-3) / 1e12
+    out = {
+        "metric": "batched hard part: particle-steps/s (one soft step, block-timestep Hermite)",
+        "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(ms_p, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched hard: %d clusters, sizes log-uniform 2-256, sigma 2 r_H(1e-10) at 1 AU, "
+                               "m log-uniform 1e-12..1e-8, dt_tree 2^-4, eta 0.01, perfect merging" % n_cl,
+                   "clusters": n_cl, "particles": n, "seed": 2},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "compute": "fp64 VALU, no MFMA", "kernel": "k_hermite (persistent)",
+                     "flops_per_launch": FLOP_HARD_PAIR * npair, "avg_launch_ms": round(ms_p, 3)},
+        "per_cluster_launches": {"sample_clusters": k, "ms": round(ms_c, 3), "persistent_ms": round(ms_sp, 3),
+                                 "speedup_persistent": round(ms_c / ms_sp, 3), "identical_results": same},
+        "block_steps": nb, "pair_evaluations": npair,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        po = O.params(**hard_params())
+        k = args.cpu_clusters
+        sub_off = off[: k + 1]
+        sub = pp[: sub_off[-1]].copy()
+        O.cluster_prepare(po, sub, sub_off)
+        t0 = time.perf_counter()
+        O.hermite_clusters(po, sub, sub_off)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(int(sub_off[-1]) / dt, 1), "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "first %d of the same clusters (%d particles), oracle C restatement "
+                                         "of timeIntegrate_multi, one thread, %.2f s" % (k, int(sub_off[-1]), dt)}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,11 +152,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--workload", choices=["step", "hard"], default="step",
+                    help="step: full P3T step at N (default); hard: batched hard part alone")
+    ap.add_argument("--clusters", type=int, default=50000)
+    ap.add_argument("--cpu-clusters", type=int, default=20)
+    ap.add_argument("--compare-clusters", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "hard":
+        bench_hard(args)
+        return
 
     from gplum_amd import gpu
     name, n_cu, clk = gpu.device_info()
