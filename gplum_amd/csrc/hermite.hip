@@ -298,23 +298,20 @@ __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int 
     st3(H.jerk_d, g, jd);
 }
 
-// mergeAccJerk (hermite.h:36-74) for target t.  A twin sits at the target's
-// position with the target's id, so its own freshly computed acc_d/jerk_d equal the
-// target's bit for bit; the mass-weighted mean is formed from the target's raw
-// values in twin index order.
+// mergeAccJerk (hermite.h:36-74) for target t: the mass-weighted mean of the target's
+// and its twins' acc_d/jerk_d (each computed with its own cut-off radius), in twin
+// index order, written to all of them.
 __device__ inline void merge_acc_jerk(HardView &H, int b, int n, int t) {
-    double acc[3], jerk[3], at[3], jt[3], mass = H.mass[b + t];
-    ld3(H.acc_d, b + t, at);
-    ld3(H.jerk_d, b + t, jt);
+    double acc[3], jerk[3], mass = H.mass[b + t];
     for (int k = 0; k < 3; k++) {
-        acc[k] = H.mass[b + t] * at[k];
-        jerk[k] = H.mass[b + t] * jt[k];
+        acc[k] = H.mass[b + t] * H.acc_d[3 * (b + t) + k];
+        jerk[k] = H.mass[b + t] * H.jerk_d[3 * (b + t) + k];
     }
     for (int j = 0; j < n; j++) {
         if (H.twin_of[b + j] != t) continue;
         for (int k = 0; k < 3; k++) {
-            acc[k] += H.mass[b + j] * at[k];
-            jerk[k] += H.mass[b + j] * jt[k];
+            acc[k] += H.mass[b + j] * H.acc_d[3 * (b + j) + k];
+            jerk[k] += H.mass[b + j] * H.jerk_d[3 * (b + j) + k];
         }
         mass += H.mass[b + j];
     }
@@ -326,28 +323,6 @@ __device__ inline void merge_acc_jerk(HardView &H, int b, int n, int t) {
         if (H.twin_of[b + j] != t) continue;
         st3(H.acc_d, b + j, acc);
         st3(H.jerk_d, b + j, jerk);
-    }
-}
-
-// copy the integrated state of target t onto its twins (identical inputs give
-// identical outputs, so this equals integrating them separately)
-__device__ inline void sync_twins(HardView &H, int b, int n, int t) {
-    for (int j = 0; j < n; j++) {
-        if (H.twin_of[b + j] != t) continue;
-        const int s = b + t, d = b + j;
-        for (int k = 0; k < 3; k++) {
-            H.pos[3 * d + k] = H.pos[3 * s + k];
-            H.vel[3 * d + k] = H.vel[3 * s + k];
-            H.acc_s[3 * d + k] = H.acc_s[3 * s + k];
-            H.jerk_s[3 * d + k] = H.jerk_s[3 * s + k];
-            H.acc_d[3 * d + k] = H.acc_d[3 * s + k];
-            H.jerk_d[3 * d + k] = H.jerk_d[3 * s + k];
-            H.a2_s[3 * d + k] = H.a2_s[3 * s + k];
-            H.a3_s[3 * d + k] = H.a3_s[3 * s + k];
-            H.a2_d[3 * d + k] = H.a2_d[3 * s + k];
-            H.a3_d[3 * d + k] = H.a3_d[3 * s + k];
-        }
-        H.time[d] = H.time[s];
     }
 }
 
@@ -549,26 +524,38 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
             const int g = b + i;
             if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
             if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
+            // a merged target integrates its dead twins alongside (hermite.h:300-375): each
+            // twin feels the hard force with its own r_out (cut-off), and mergeAccJerk
+            // averages target and twins by mass
             const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
             pairs += 3 * (long long)(n - 1);
             gravity_pc(P, H, J, n, g, false);
-            if (merged) merge_acc_jerk(H, b, n, i);
+            if (merged) {
+                for (int j = 0; j < n; j++)
+                    if (H.twin_of[b + j] == i) { gravity_pc(P, H, J, n, b + j, false); pairs += 3 * (long long)(n - 1); }
+                merge_acc_jerk(H, b, n, i);
+            }
             const double t_c = t_s - H.time[g];
             for (int ite = 0; ite < 2; ite++) {
                 correct(H, g, t_c);
+                if (merged)
+                    for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) correct(H, b + j, t_c);
                 gravity_pc(P, H, J, n, g, true);
-                if (merged) merge_acc_jerk(H, b, n, i);
+                if (merged) {
+                    for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) gravity_pc(P, H, J, n, b + j, true);
+                    merge_acc_jerk(H, b, n, i);
+                }
             }
             H.time[g] += t_c;
+            if (merged)
+                for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) H.time[b + j] += t_c;
             if (!flag_col) {
                 double ad[3], jd[3], a2d[3], a3d[3], as[3], js[3], a2s[3], a3s[3];
                 ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.a2_d, g, a2d); ld3(H.a3_d, g, a3d);
                 ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js); ld3(H.a2_s, g, a2s); ld3(H.a3_s, g, a3s);
                 H.dt[g] = dt_block(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
-            }
-            if (merged) {
-                sync_twins(H, b, n, i);
-                for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) H.dt[b + j] = H.dt[g];
+                if (merged)
+                    for (int j = 0; j < n; j++) if (H.twin_of[b + j] == i) H.dt[b + j] = H.dt[g];
             }
         }
         __threadfence_block();
