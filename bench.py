@@ -154,9 +154,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--workload", choices=["step", "hard"], default="step",
                     help="step: full P3T step at N (default); hard: batched hard part alone")
-    ap.add_argument("--clusters", type=int, default=50000)
+    ap.add_argument("--clusters", type=int, default=5000)
     ap.add_argument("--cpu-clusters", type=int, default=20)
-    ap.add_argument("--compare-clusters", type=int, default=500)
+    ap.add_argument("--compare-clusters", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -466,8 +466,8 @@ __device__ inline void wave_argmin(double &v, int &i, int &j) {
 // thread of the workgroup calls it; collisions and twins handled here.
 template <bool LDSJ>
 __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
-                               int64_t *s_id, int b, int n, double &sys_time, long long &loop, double &ed,
-                               int &n_col, long long &pairs) {
+                               int64_t *s_id, double *s_fr, int b, int n, double &sys_time, long long &loop,
+                               double &ed, int &n_col, long long &pairs) {
     const int t = threadIdx.x;
     const int B = blockDim.x;
     const double time_end = P.dt_tree;
@@ -486,6 +486,8 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                     s_v[3 * i + k] = H.vp[3 * (b + i) + k];
                 }
                 s_id[i] = H.id[b + i];
+                // f R of the live particles for the collision test (-1: dead)
+                s_fr[i] = (H.flags[b + i] & GPLUM_FLAG_DEAD) ? -1. : H.f[b + i] * H.r_planet[b + i];
             }
         }
         __threadfence_block();
@@ -496,16 +498,34 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
             double rmin = 1.;
             int bi = 0x7fffffff, bj = -1;
             for (int i = t; i < n; i += B) {
-                if (H.flags[b + i] & GPLUM_FLAG_DEAD) continue;
-                for (int j = 0; j < n; j++) {
-                    if (i == j || (H.flags[b + j] & GPLUM_FLAG_DEAD)) continue;
-                    const double dx = H.xp[3 * (b + i)] - H.xp[3 * (b + j)];
-                    const double dy = H.xp[3 * (b + i) + 1] - H.xp[3 * (b + j) + 1];
-                    const double dz = H.xp[3 * (b + i) + 2] - H.xp[3 * (b + j) + 2];
-                    const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-                    const double r2 = H.f[b + i] * H.r_planet[b + i] + H.f[b + j] * H.r_planet[b + j];
-                    const double Rr = r1 / r2;
-                    if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
+                if (LDSJ) {
+                    // predicted positions and f R from LDS
+                    const double fri = s_fr[i];
+                    if (fri < 0.) continue;
+                    const double xi = s_x[3 * i], yi = s_x[3 * i + 1], zi = s_x[3 * i + 2];
+                    for (int j = 0; j < n; j++) {
+                        const double frj = s_fr[j];
+                        if (i == j || frj < 0.) continue;
+                        const double dx = xi - s_x[3 * j];
+                        const double dy = yi - s_x[3 * j + 1];
+                        const double dz = zi - s_x[3 * j + 2];
+                        const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                        const double r2 = fri + frj;
+                        const double Rr = r1 / r2;
+                        if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
+                    }
+                } else {
+                    if (H.flags[b + i] & GPLUM_FLAG_DEAD) continue;
+                    for (int j = 0; j < n; j++) {
+                        if (i == j || (H.flags[b + j] & GPLUM_FLAG_DEAD)) continue;
+                        const double dx = H.xp[3 * (b + i)] - H.xp[3 * (b + j)];
+                        const double dy = H.xp[3 * (b + i) + 1] - H.xp[3 * (b + j) + 1];
+                        const double dz = H.xp[3 * (b + i) + 2] - H.xp[3 * (b + j) + 2];
+                        const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                        const double r2 = H.f[b + i] * H.r_planet[b + i] + H.f[b + j] * H.r_planet[b + j];
+                        const double Rr = r1 / r2;
+                        if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
+                    }
                 }
             }
             int gi, gj;
@@ -681,7 +701,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
                 }
                 __threadfence_block();
                 __syncthreads();
-                integrate_from<true>(P, H, J, R, s_x, s_v, s_id, b, n, sys_time, loop, ed, n_col, pairs);
+                integrate_from<true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
                 return;
             }
         }
@@ -735,7 +755,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                                                            long long *__restrict__ npair, int persistent) {
     __shared__ double s_x[3 * HERMITE_LDS], s_v[3 * HERMITE_LDS], s_m[HERMITE_LDS], s_ri[HERMITE_LDS];
     __shared__ int64_t s_id[HERMITE_LDS];
-    __shared__ double s_fr[HERMITE_BLOCK];
+    __shared__ double s_fr[HERMITE_LDS];
     __shared__ Reduce R;
     __shared__ int s_cl;
     __shared__ unsigned long long s_pairs;
@@ -781,7 +801,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                 H.twin_of[b + i] = -1;
             }
             __syncthreads();
-            integrate_from<LDSJ>(P, H, J, R, s_x, s_v, s_id, b, n, sys_time, loop, ed, n_col, pairs);
+            integrate_from<LDSJ>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
         }
         atomicAdd(&s_pairs, (unsigned long long)pairs);
         __syncthreads();

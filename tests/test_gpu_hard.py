@@ -1,9 +1,10 @@
 """GPU parity of the batched FP64 Hermite (persistent and per-cluster launches) against the
 oracle's timeIntegrate_multi restatement, cluster by cluster.
 
-Both sides use the reference's operation order without fused multiply-add, so block-step
-sizes and times must agree exactly and positions/velocities to round-off (1e-13 relative);
-collisions (perfect merging) must happen in the same block step between the same bodies."""
+Both sides use the reference's operation order without fused multiply-add, so the whole
+state (block steps, times, positions, velocities, forces, potentials) must agree bit for
+bit; collisions (perfect merging) must happen in the same block step between the same
+bodies, with the same merged radii and dissipated energy."""
 import numpy as np
 import pytest
 
@@ -56,8 +57,11 @@ def test_batched_hermite_matches_oracle(no_collision_case, persistent):
     rel = np.abs(got["pos"] - ref["pos"]).max() / np.abs(ref["pos"]).max()
     relv = np.abs(got["vel"] - ref["vel"]).max() / np.abs(ref["vel"]).max()
     print("max rel diff pos %.3e vel %.3e" % (rel, relv))
-    assert rel < 1e-13 and relv < 1e-13
-    assert np.allclose(got["acc_d"], ref["acc_d"], rtol=1e-10, atol=1e-12 * np.abs(ref["acc_d"]).max())
+    # same operations in the same order on both sides: bit for bit (phi_d goes through
+    # the library log of the changeover potential, which may differ in the last bit)
+    for f in ("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_s"):
+        assert np.array_equal(got[f], ref[f]), f
+    assert np.allclose(got["phi_d"], ref["phi_d"], rtol=1e-13, atol=0.0)
 
 
 def test_hermite_perfect_merging_matches_oracle(oracle_lib):
@@ -80,7 +84,7 @@ def test_hermite_perfect_merging_matches_oracle(oracle_lib):
         g = pp[a:b].copy()
         ed_gc, nb_gc, _ = gpu.hard_integrate(pg, g, np.array([0, b - a]), persistent=True)
         dx = np.abs(g["pos"] - r["pos"]).max() / np.abs(r["pos"]).max()
-        if nb_gc != nb_c or dx > 1e-12:
+        if nb_gc != nb_c or dx > 0.:
             bad.append((c, b - a, nc_c, nb_c, nb_gc, float(dx), float(np.abs(g["dt"] - r["dt"]).max())))
     print("clusters differing (c, n, collisions, steps oracle, steps gpu, pos diff, dt diff):", bad)
     assert nc > 0
@@ -88,6 +92,6 @@ def test_hermite_perfect_merging_matches_oracle(oracle_lib):
     assert np.array_equal(got["id"], ref["id"])
     assert not bad
     assert nb_g == nb_ref
-    assert abs(ed_g - ed) <= 1e-10 * abs(ed)
-    rel = np.abs(got["pos"] - ref["pos"]).max() / np.abs(ref["pos"]).max()
-    assert rel < 1e-12
+    assert ed_g == ed
+    for f in ("pos", "vel", "acc_d", "jerk_d", "mass", "r_planet", "dt", "time"):
+        assert np.array_equal(got[f], ref[f]), f
