@@ -13,8 +13,8 @@ namespace gplum {
 namespace {
 
 struct HardCtx {
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st = nullptr, st2 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, fork = nullptr, join = nullptr;
     HardBuffers hb;
     DArray<int32_t> off, order;
     int n_cu = 256;
@@ -28,6 +28,9 @@ struct HardCtx {
         GPLUM_HIP(hipGetDeviceProperties(&prop, dev));
         n_cu = prop.multiProcessorCount;
         GPLUM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        GPLUM_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        GPLUM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        GPLUM_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
         GPLUM_HIP(hipEventCreate(&e0));
         GPLUM_HIP(hipEventCreate(&e1));
         return GPLUM_OK;
@@ -100,7 +103,7 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     GPLUM_ENSURE(c.order, n_cluster);
     if ((rc = upload(c.off, offs, c.st))) return rc;
     if ((rc = upload(c.order, order, c.st))) return rc;
-    GPLUM_ENSURE(hb.next, 2);
+    GPLUM_ENSURE(hb.next, 3);
     GPLUM_ENSURE(hb.edisp, n_cluster);
     GPLUM_ENSURE(hb.ncol, n_cluster);
     GPLUM_ENSURE(hb.nblock, n_cluster);
@@ -112,10 +115,11 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     }
     GPLUM_HIP(hipEventRecord(c.e0, c.st));
     if (mode & 1) {
-        if ((rc = hard_launch(*p, hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st))) return rc;
+        if ((rc = hard_launch(*p, hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st, c.st2, c.fork, c.join)))
+            return rc;
     } else {
         for (int k = 0; k < n_cluster; k++)
-            if ((rc = hard_launch_single(*p, hb, k, c.off.p, c.st))) return rc;
+            if ((rc = hard_launch_single(*p, hb, k, sizes[k], c.off.p, c.st))) return rc;
     }
     GPLUM_HIP(hipEventRecord(c.e1, c.st));
     GPLUM_HIP(hipStreamSynchronize(c.st));

@@ -31,6 +31,12 @@ struct HP {                                  // physics constants for the kernel
 
 __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
 
+// collisionDetermination looks for R = |dx| / (f_i R_i + f_j R_j) < 1.  Pairs with
+// |dx|^2 >= (1 + 1e-10) (f_i R_i + f_j R_j)^2 have R >= 1 even after the rounding of
+// sqrt and the division, so they are skipped before computing R; the minimum and its
+// pair are unchanged.
+constexpr double COL_FILTER = 1.0 + 1e-10;
+
 // ---------------------------------------------------------- Sun gravity
 
 __device__ inline void star_acc_jerk(const HP &P, const double x[3], const double v[3],
@@ -298,6 +304,27 @@ __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int 
     st3(H.jerk_d, g, jd);
 }
 
+// Hard force on one particle with all 64 lanes of the (one-wave) workgroup: lane l
+// sums the pairs j = l, l+64, ... in ascending j, then a butterfly over the lanes adds
+// the 64 partial sums (every lane ends with the same total).  This fixed order is the
+// hard-force summation order of clusters above HARD_SEQ_MAX particles (the oracle sums
+// the same way); smaller clusters sum sequentially in j.
+__device__ inline void coop_pair_sum(const HP &P, const JData &J, int n, int64_t idi, const double xi[3],
+                                     const double vi[3], double rinv_i, double acc[3], double jerk[3]) {
+    double a[3] = {0., 0., 0.}, jk[3] = {0., 0., 0.};
+    for (int j = (int)threadIdx.x; j < n; j += 64) {
+        if (J.id[j] == idi) continue;
+        hard_pair(P, xi, vi, rinv_i, J.x[3 * j], J.x[3 * j + 1], J.x[3 * j + 2], J.v[3 * j], J.v[3 * j + 1],
+                  J.v[3 * j + 2], J.m[j], J.ri[j], a, jk, nullptr);
+    }
+    for (int d = 32; d > 0; d >>= 1)
+        for (int k = 0; k < 3; k++) {
+            a[k] += __shfl_xor(a[k], d);
+            jk[k] += __shfl_xor(jk[k], d);
+        }
+    for (int k = 0; k < 3; k++) { acc[k] = a[k]; jerk[k] = jk[k]; }
+}
+
 // mergeAccJerk (hermite.h:36-74) for target t: the mass-weighted mean of the target's
 // and its twins' acc_d/jerk_d (each computed with its own cut-off radius), in twin
 // index order, written to all of them.
@@ -461,6 +488,108 @@ __device__ inline void wave_argmin(double &v, int &i, int &j) {
     }
 }
 
+// The active-particle correction of integrate_from for clusters above HARD_SEQ_MAX
+// particles: particles are corrected one after the other (ascending index), each with
+// all 64 lanes evaluating its hard force (coop_pair_sum); every lane carries the same
+// particle state in registers and lane 0 writes it back.  Twins of a merged target are
+// integrated alongside with their own cut-off radius, as in the per-lane path.
+__device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
+                                    int flag_col, long long &pairs) {
+    const bool lead = threadIdx.x == 0;
+    for (int i = 0; i < n; i++) {
+        const int g = b + i;
+        if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
+        if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
+        const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
+        if (lead) pairs += 3 * (long long)(n - 1);
+        // state of g (predicted by the predict pass)
+        double xp[3], vp[3], a0s[3], j0s[3], a0d[3], j0d[3], as[3], js[3], ad[3], jd[3];
+        double x[3], v[3], a2s[3], a3s[3], a2d[3], a3d[3];
+        ld3(H.xp, g, xp); ld3(H.vp, g, vp);
+        ld3(H.a0_s, g, a0s); ld3(H.j0_s, g, j0s); ld3(H.a0_d, g, a0d); ld3(H.j0_d, g, j0d);
+        const double rinv = H.rinv[g];
+        const int64_t id = H.id[g];
+        star_acc_jerk(P, xp, vp, as, js, nullptr);
+        coop_pair_sum(P, J, n, id, xp, vp, rinv, ad, jd);
+        if (merged) {
+            // mergeAccJerk over the target and its twins (twin index order)
+            double sa[3], sj[3], mass = H.mass[g];
+            for (int k = 0; k < 3; k++) { sa[k] = H.mass[g] * ad[k]; sj[k] = H.mass[g] * jd[k]; }
+            for (int j = 0; j < n; j++) {
+                if (H.twin_of[b + j] != i) continue;
+                double txp[3], tvp[3], tad[3], tjd[3], tas[3], tjs[3];
+                ld3(H.xp, b + j, txp); ld3(H.vp, b + j, tvp);
+                star_acc_jerk(P, txp, tvp, tas, tjs, nullptr);
+                coop_pair_sum(P, J, n, H.id[b + j], txp, tvp, H.rinv[b + j], tad, tjd);
+                if (lead) { st3(H.acc_s, b + j, tas); st3(H.jerk_s, b + j, tjs); pairs += 3 * (long long)(n - 1); }
+                for (int k = 0; k < 3; k++) { sa[k] += H.mass[b + j] * tad[k]; sj[k] += H.mass[b + j] * tjd[k]; }
+                mass += H.mass[b + j];
+            }
+            const double minv = 1. / mass;
+            for (int k = 0; k < 3; k++) { ad[k] = sa[k] * minv; jd[k] = sj[k] * minv; }
+        }
+        const double t_c = t_s - H.time[g];
+        for (int ite = 0; ite < 2; ite++) {
+            correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+            if (merged && lead)
+                for (int j = 0; j < n; j++)
+                    if (H.twin_of[b + j] == i) {
+                        // the twin's correction from its own predicted state and the merged acc/jerk
+                        double txp[3], tvp[3], ta0s[3], tj0s[3], ta0d[3], tj0d[3], tas[3], tjs[3];
+                        double tx[3], tv[3], t2s[3], t3s[3], t2d[3], t3d[3];
+                        ld3(H.xp, b + j, txp); ld3(H.vp, b + j, tvp);
+                        ld3(H.a0_s, b + j, ta0s); ld3(H.j0_s, b + j, tj0s);
+                        ld3(H.a0_d, b + j, ta0d); ld3(H.j0_d, b + j, tj0d);
+                        ld3(H.acc_s, b + j, tas); ld3(H.jerk_s, b + j, tjs);
+                        correct_regs(t_c, txp, tvp, tas, ta0s, tj0s, tjs, ad, ta0d, tj0d, jd, tx, tv, t2s, t3s, t2d,
+                                     t3d);
+                        st3(H.pos, b + j, tx); st3(H.vel, b + j, tv);
+                    }
+            __threadfence_block();
+            __syncthreads();
+            star_acc_jerk(P, x, v, as, js, nullptr);
+            coop_pair_sum(P, J, n, id, x, v, rinv, ad, jd);
+            if (merged) {
+                double sa[3], sj[3], mass = H.mass[g];
+                for (int k = 0; k < 3; k++) { sa[k] = H.mass[g] * ad[k]; sj[k] = H.mass[g] * jd[k]; }
+                for (int j = 0; j < n; j++) {
+                    if (H.twin_of[b + j] != i) continue;
+                    double tx[3], tv[3], tad[3], tjd[3], tas[3], tjs[3];
+                    ld3(H.pos, b + j, tx); ld3(H.vel, b + j, tv);
+                    star_acc_jerk(P, tx, tv, tas, tjs, nullptr);
+                    coop_pair_sum(P, J, n, H.id[b + j], tx, tv, H.rinv[b + j], tad, tjd);
+                    __syncthreads();
+                    if (lead) { st3(H.acc_s, b + j, tas); st3(H.jerk_s, b + j, tjs); }
+                    for (int k = 0; k < 3; k++) { sa[k] += H.mass[b + j] * tad[k]; sj[k] += H.mass[b + j] * tjd[k]; }
+                    mass += H.mass[b + j];
+                }
+                const double minv = 1. / mass;
+                for (int k = 0; k < 3; k++) { ad[k] = sa[k] * minv; jd[k] = sj[k] * minv; }
+            }
+        }
+        double time = H.time[g] + t_c;
+        double dt = H.dt[g];
+        if (!flag_col) dt = dt_block(P, time, dt, H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
+        __syncthreads();
+        if (lead) {
+            st3(H.pos, g, x); st3(H.vel, g, v);
+            st3(H.acc_s, g, as); st3(H.jerk_s, g, js); st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
+            st3(H.a2_s, g, a2s); st3(H.a3_s, g, a3s); st3(H.a2_d, g, a2d); st3(H.a3_d, g, a3d);
+            H.time[g] = time;
+            if (!flag_col) H.dt[g] = dt;
+            if (merged)
+                for (int j = 0; j < n; j++)
+                    if (H.twin_of[b + j] == i) {
+                        st3(H.acc_d, b + j, ad); st3(H.jerk_d, b + j, jd);
+                        H.time[b + j] += t_c;
+                        if (!flag_col) H.dt[b + j] = dt;
+                    }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
 // Block steps of one cluster from (sys_time, loop) to the end of the soft step on the
 // global-memory state H (j-data through J), then the final calcGravity.  Every
 // thread of the workgroup calls it; collisions and twins handled here.
@@ -509,9 +638,10 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                         const double dx = xi - s_x[3 * j];
                         const double dy = yi - s_x[3 * j + 1];
                         const double dz = zi - s_x[3 * j + 2];
-                        const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                        const double d2 = dot3(dx, dy, dz, dx, dy, dz);
                         const double r2 = fri + frj;
-                        const double Rr = r1 / r2;
+                        if (!(d2 < COL_FILTER * r2 * r2)) continue;
+                        const double Rr = sqrt(d2) / r2;
                         if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
                     }
                 } else {
@@ -521,9 +651,10 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                         const double dx = H.xp[3 * (b + i)] - H.xp[3 * (b + j)];
                         const double dy = H.xp[3 * (b + i) + 1] - H.xp[3 * (b + j) + 1];
                         const double dz = H.xp[3 * (b + i) + 2] - H.xp[3 * (b + j) + 2];
-                        const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                        const double d2 = dot3(dx, dy, dz, dx, dy, dz);
                         const double r2 = H.f[b + i] * H.r_planet[b + i] + H.f[b + j] * H.r_planet[b + j];
-                        const double Rr = r1 / r2;
+                        if (!(d2 < COL_FILTER * r2 * r2)) continue;
+                        const double Rr = sqrt(d2) / r2;
                         if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
                     }
                 }
@@ -540,6 +671,9 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
             __syncthreads();
         }
         // correct the active particles (hermite.h:300-375)
+        if (n > HARD_SEQ_MAX) {
+            coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs);
+        } else
         for (int i = t; i < n; i += B) {
             const int g = b + i;
             if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
@@ -681,9 +815,10 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
                     const double dx = s_x[3 * t] - s_x[3 * j];
                     const double dy = s_x[3 * t + 1] - s_x[3 * j + 1];
                     const double dz = s_x[3 * t + 2] - s_x[3 * j + 2];
-                    const double r1 = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+                    const double d2 = dot3(dx, dy, dz, dx, dy, dz);
                     const double r2 = s_fr[t] + s_fr[j];
-                    const double Rr = r1 / r2;
+                    if (!(d2 < COL_FILTER * r2 * r2)) continue;   // R >= 1: never the minimum
+                    const double Rr = sqrt(d2) / r2;
                     if (Rr < rmin) { rmin = Rr; bi = t; bj = j; }
                 }
             }
@@ -746,7 +881,13 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     __syncthreads();
 }
 
-template <bool LDSJ>
+// Three instantiations, one per cluster-size class (each skips the others):
+// REG: n <= HARD_SEQ_MAX, particle state in registers (integrate_regs);
+// LDSJ && !REG: HARD_SEQ_MAX < n <= HERMITE_LDS, j-data in LDS (integrate_from);
+// !LDSJ: n > HERMITE_LDS, j-data in global memory.
+// Separate kernels keep the register-heavy small-cluster path from lowering the
+// occupancy of the others.
+template <bool LDSJ, bool REG>
 __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
                                                            const int32_t *__restrict__ cl_off,
                                                            const int32_t *__restrict__ order, int ncl,
@@ -775,7 +916,8 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
         const int c = order ? order[q] : q;
         const int b = cl_off[c];
         const int n = cl_off[c + 1] - b;
-        if (LDSJ && n > HERMITE_LDS) continue;   // handled by the global-memory variant
+        if (REG && n > HARD_SEQ_MAX) continue;
+        if (LDSJ && !REG && (n <= HARD_SEQ_MAX || n > HERMITE_LDS)) continue;
         if (!LDSJ && n <= HERMITE_LDS) continue;
         JData J;
         if (LDSJ) {
@@ -791,7 +933,7 @@ __global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
         int n_col = 0;
         long long loop = 0;
         double sys_time = 0.;
-        if (LDSJ && n <= B && B == 64) {
+        if (REG) {
             integrate_regs(P, H, J, R, s_x, s_v, s_m, s_ri, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
         } else {
             // initial calcGravity for all, flags cleared (hermite.h:259-270)
@@ -968,33 +1110,46 @@ HP make_hp(const gplum_params &p) {
     return h;
 }
 
+static HardView hard_view(HardBuffers &hb) {
+    return HardView{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
+                    hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
+                    hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
+                    hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
+}
+
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
-                const int32_t *d_order, bool persistent, int n_cu, hipStream_t st) {
+                const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2,
+                hipEvent_t ev_fork, hipEvent_t ev_join) {
     if (ncl <= 0) return GPLUM_OK;
     HP P = make_hp(p);
-    HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
-               hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
-               hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
-               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
-    GPLUM_ENSURE(hb.next, 2);
+    HardView H = hard_view(hb);
+    GPLUM_ENSURE(hb.next, 3);
     GPLUM_ENSURE(hb.edisp, ncl);
     GPLUM_ENSURE(hb.ncol, ncl);
     GPLUM_ENSURE(hb.nblock, ncl);
     GPLUM_ENSURE(hb.npair, ncl);
-    GPLUM_HIP(hipMemsetAsync(hb.next.p, 0, 2 * sizeof(int), st));
-    if (persistent) {
-        const int grid = std::min(ncl, std::max(1, n_cu) * 8);
-        hipLaunchKernelGGL((k_hermite<true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order,
-                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 1);
-        if (hb.max_size > HERMITE_LDS)
-            hipLaunchKernelGGL((k_hermite<false>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off,
-                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 1);
-    } else {
-        hipLaunchKernelGGL((k_hermite<true>), dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order,
-                           ncl, hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 0);
-        if (hb.max_size > HERMITE_LDS)
-            hipLaunchKernelGGL((k_hermite<false>), dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off,
-                               d_order, ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, 0);
+    GPLUM_HIP(hipMemsetAsync(hb.next.p, 0, 3 * sizeof(int), st));
+    const int pers = persistent ? 1 : 0;
+    const int grid = persistent ? std::min(ncl, std::max(1, n_cu) * 8) : ncl;
+    // clusters above HARD_SEQ_MAX on a second stream (when given), concurrently with
+    // the register-path kernel
+    const bool big = hb.max_size > HARD_SEQ_MAX;
+    hipStream_t sb = (st2 && big) ? st2 : st;
+    if (sb != st) {
+        GPLUM_HIP(hipEventRecord(ev_fork, st));
+        GPLUM_HIP(hipStreamWaitEvent(sb, ev_fork, 0));
+    }
+    if (hb.max_size > HERMITE_LDS)
+        hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BLOCK), 0, sb, P, H, d_off, d_order,
+                           ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
+    if (big)
+        hipLaunchKernelGGL((k_hermite<true, false>), dim3(grid), dim3(HERMITE_BLOCK), 0, sb, P, H, d_off, d_order,
+                           ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
+    hipLaunchKernelGGL((k_hermite<true, true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order, ncl,
+                       hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
+    if (sb != st) {
+        GPLUM_HIP(hipEventRecord(ev_join, sb));
+        GPLUM_HIP(hipStreamWaitEvent(st, ev_join, 0));
     }
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
@@ -1029,19 +1184,20 @@ int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t 
     return GPLUM_OK;
 }
 
-int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, const int32_t *d_off,
+int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, const int32_t *d_off,
                        hipStream_t st) {
-    // one launch integrating cluster c only (the per-cluster-launch comparator)
+    // one launch integrating cluster c (n_c particles) only: the per-cluster-launch comparator
     HP P = make_hp(p);
-    HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
-               hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
-               hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
-               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
-    (void)c;
-    hipLaunchKernelGGL((k_hermite<true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
-    hipLaunchKernelGGL((k_hermite<false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                       hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
+    HardView H = hard_view(hb);
+    if (n_c <= HARD_SEQ_MAX)
+        hipLaunchKernelGGL((k_hermite<true, true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
+                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
+    else if (n_c <= HERMITE_LDS)
+        hipLaunchKernelGGL((k_hermite<true, false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
+                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
+    else
+        hipLaunchKernelGGL((k_hermite<false, false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr,
+                           1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

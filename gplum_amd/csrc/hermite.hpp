@@ -6,6 +6,10 @@ namespace gplum {
 
 constexpr int HERMITE_BLOCK = 64;    // threads per cluster workgroup
 constexpr int HERMITE_LDS = 256;     // clusters up to this size keep j-data in LDS
+// Clusters up to HARD_SEQ_MAX particles sum each particle's hard force sequentially in
+// j (one lane per particle); larger ones with 64 lane-strided partial sums and a
+// butterfly (all lanes on one particle), an order the oracle reproduces.
+constexpr int HARD_SEQ_MAX = 64;
 
 // Gathered cluster particles (SoA, vectors interleaved x,y,z).
 struct HardBuffers {
@@ -42,13 +46,16 @@ struct IsoArgs {
 // d_off[0..ncl] (local time 0 -> dt_tree; caller sets time = 0).  d_order (optional)
 // gives the processing order (largest first).  persistent: one grid pulling
 // clusters from an atomic queue; otherwise one workgroup per cluster.
+// st2/ev_fork/ev_join (optional): clusters above HARD_SEQ_MAX run on st2 concurrently
+// with the small-cluster kernel on st, and st waits for st2.
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
-                const int32_t *d_order, bool persistent, int n_cu, hipStream_t st);
+                const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2 = nullptr,
+                hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr);
 // Sun + hard acc/jerk and the initial block step of every cluster particle (as the
 // initial force correction leaves them), for callers that bring bare clusters.
 int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off, hipStream_t st);
-// One launch for cluster c only (the per-cluster-launch comparator).
-int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, const int32_t *d_off,
+// One launch for cluster c (n_c particles) only (the per-cluster-launch comparator).
+int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, const int32_t *d_off,
                        hipStream_t st);
 // Isolated particles (number == 0): Kepler drift or Sun-only Hermite.
 int isolated_launch(const gplum_params &p, const IsoArgs &a, int n, double t0, double t1,

@@ -181,6 +181,9 @@ System::System(const gplum_params &prm) : p(prm) {}
 
 System::~System() {
     if (st) (void)hipStreamDestroy(st);
+    if (st2) (void)hipStreamDestroy(st2);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     for (auto &e : ev) if (e) (void)hipEventDestroy(e);
 }
 
@@ -195,6 +198,9 @@ int System::init_device() {
     GPLUM_HIP(hipGetDeviceProperties(&prop, dev));
     n_cu = prop.multiProcessorCount;
     GPLUM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    GPLUM_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    GPLUM_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    GPLUM_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     for (auto &e : ev) GPLUM_HIP(hipEventCreate(&e));
     return GPLUM_OK;
 }
@@ -555,16 +561,20 @@ int System::hard_part(bool persistent) {
         PartPtrs P = ptrs(A);
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
         if (persistent) {
-            if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st))) return rc;
+            if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join)))
+                return rc;
         } else {
             // the comparator: one launch per cluster
-            GPLUM_ENSURE(hb.next, 2);
+            GPLUM_ENSURE(hb.next, 3);
             GPLUM_ENSURE(hb.edisp, ncl);
             GPLUM_ENSURE(hb.ncol, ncl);
             GPLUM_ENSURE(hb.nblock, ncl);
             GPLUM_ENSURE(hb.npair, ncl);
+            std::vector<int32_t> hoff(ncl + 1);
+            GPLUM_HIP(hipMemcpyAsync(hoff.data(), cl.off.p, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, st));
+            GPLUM_HIP(hipStreamSynchronize(st));
             for (int c = 0; c < ncl; c++)
-                if ((rc = hard_launch_single(p, hb, c, cl.off.p, st))) return rc;
+                if ((rc = hard_launch_single(p, hb, c, hoff[c + 1] - hoff[c], cl.off.p, st))) return rc;
         }
         hipLaunchKernelGGL(k_hard_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, cl.off.p, ncl,
                            H, P, time_c.p);

@@ -41,7 +41,8 @@ struct MergeScratch {                 // MergeParticle on the device (system.hip
 
 struct System {
     gplum_params p;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr, st2 = nullptr;   // st2: large clusters of the hard part
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev[EV_COUNT] = {};
     int n_cu = 256;
     int n = 0;

@@ -220,6 +220,13 @@ static void calc_gravity_full(const gplum_params *p, orc_hp *pp, int n, int i)
 
 /* calcGravity_p (gravity_hard.h:258-311): predicted i vs predicted j.
  * calcGravity_c (:313-367): corrected i vs predicted j. */
+/* Clusters above HARD_SEQ_MAX particles sum the hard force in 64 partial sums
+ * (partial l over j = l, l+64, ... ascending) added by a butterfly (partial l plus
+ * partial l^d for d = 32, 16, ..., 1): the order in which the kernels evaluate one
+ * particle's force with all lanes of a wave.  Smaller clusters sum sequentially in j,
+ * as calcGravity_p/_c do. */
+enum { HARD_SEQ_MAX = 64, HARD_LANES = 64 };
+
 static void calc_gravity_pc(const gplum_params *p, orc_hp *pp, int n, int i, int corrected)
 {
     orc_hp *h = &pp[i];
@@ -227,10 +234,33 @@ static void calc_gravity_pc(const gplum_params *p, orc_hp *pp, int n, int i, int
     const double *vi = corrected ? h->b.vel : h->vp;
     star_acc_jerk(p, xi, vi, h->b.acc_s, h->b.jerk_s);
     double acc[3] = {0., 0., 0.}, jerk[3] = {0., 0., 0.};
-    for (int j = 0; j < n; j++) {
-        if (h->b.id == pp[j].b.id) continue;
-        hard_pair(p, xi, vi, h->b.r_out_inv, pp[j].xp, pp[j].vp, pp[j].b.mass,
-                  pp[j].b.r_out_inv, acc, jerk, NULL);
+    if (n <= HARD_SEQ_MAX) {
+        for (int j = 0; j < n; j++) {
+            if (h->b.id == pp[j].b.id) continue;
+            hard_pair(p, xi, vi, h->b.r_out_inv, pp[j].xp, pp[j].vp, pp[j].b.mass,
+                      pp[j].b.r_out_inv, acc, jerk, NULL);
+        }
+    } else {
+        double pa[HARD_LANES][3], pj[HARD_LANES][3];
+        for (int l = 0; l < HARD_LANES; l++) {
+            for (int k = 0; k < 3; k++) pa[l][k] = pj[l][k] = 0.;
+            for (int j = l; j < n; j += HARD_LANES) {
+                if (h->b.id == pp[j].b.id) continue;
+                hard_pair(p, xi, vi, h->b.r_out_inv, pp[j].xp, pp[j].vp, pp[j].b.mass,
+                          pp[j].b.r_out_inv, pa[l], pj[l], NULL);
+            }
+        }
+        for (int d = HARD_LANES / 2; d > 0; d >>= 1) {
+            double qa[HARD_LANES][3], qj[HARD_LANES][3];
+            for (int l = 0; l < HARD_LANES; l++)
+                for (int k = 0; k < 3; k++) {
+                    qa[l][k] = pa[l][k] + pa[l ^ d][k];
+                    qj[l][k] = pj[l][k] + pj[l ^ d][k];
+                }
+            memcpy(pa, qa, sizeof(pa));
+            memcpy(pj, qj, sizeof(pj));
+        }
+        for (int k = 0; k < 3; k++) { acc[k] = pa[0][k]; jerk[k] = pj[0][k]; }
     }
     for (int k = 0; k < 3; k++) { h->b.acc_d[k] = acc[k]; h->b.jerk_d[k] = jerk[k]; }
 }
