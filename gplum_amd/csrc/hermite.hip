@@ -109,10 +109,12 @@ __device__ inline double dt_block(const HP &P, double time, double dt, double ac
 
 // ------------------------------------------------------------- hard pair
 
-// gravity_hard.h:219-255: acc_d, jerk_d (and phi_d) of i from j.
-__device__ inline void hard_pair(const HP &P, const double xi[3], const double vi[3], double rinv_i,
+// gravity_hard.h:219-255: the summands of acc_d, jerk_d (and phi_d) of i from j,
+// returned separately so that several j can be evaluated side by side and added in
+// ascending j afterwards (the sums stay those of a sequential loop).
+__device__ inline void pair_term(const HP &P, const double xi[3], const double vi[3], double rinv_i,
                                  double xj0, double xj1, double xj2, double vj0, double vj1, double vj2,
-                                 double mj, double rinv_j, double acc[3], double jerk[3], double *phi) {
+                                 double mj, double rinv_j, double ta[3], double tj[3], double *tphi) {
     const double dr[3] = {xj0 - xi[0], xj1 - xi[1], xj2 - xi[2]};
     double dr2 = dot3(dr[0], dr[1], dr[2], dr[0], dr[1], dr[2]);
     dr2 += P.eps2;
@@ -127,14 +129,14 @@ __device__ inline void hard_pair(const HP &P, const double xi[3], const double v
     const double _K = 1. - cutoff_K(P.cut, rij, r_out_inv);
     const double dKdt = cutoff_dKdt(P.cut, rij, r_out_inv, alpha);
     const double alpha_c = alpha * _K;
-    if (phi) {
+    if (tphi) {
         const double _W = 1. - cutoff_W(P.cut, rij, r_out_inv);
-        *phi -= mj * rinv * _W;
+        *tphi = mj * rinv * _W;
     }
     const double c2 = 3. * alpha_c + dKdt;
     for (int k = 0; k < 3; k++) {
-        acc[k] += mj_rij3 * _K * dr[k];
-        jerk[k] += mj_rij3 * (_K * dv[k] - c2 * dr[k]);
+        ta[k] = mj_rij3 * _K * dr[k];
+        tj[k] = mj_rij3 * (_K * dv[k] - c2 * dr[k]);
     }
 }
 
@@ -145,7 +147,7 @@ struct HardView {          // SoA arrays of the gathered cluster particles
     double *a2_s, *a3_s, *a2_d, *a3_d;                       // 3 per particle
     double *mass, *rinv, *time, *dt, *acc0, *r_planet, *f, *phi_d, *phi_s;
     int64_t *id;
-    int32_t *flags, *twin_of;
+    int32_t *flags, *twin_of, *act;     // act: active in the current block step
 };
 
 __device__ inline void ld3(const double *a, int i, double v[3]) {
@@ -162,16 +164,49 @@ struct JData {
     int stride;            // 3 for interleaved global arrays
 };
 
+// pairs evaluated side by side per lane (independent dependency chains; the one wave
+// of a small cluster has no other wave to hide FP64 latency behind)
+constexpr int HARD_ILP = 4;
+
+__device__ inline void pair_term_j(const HP &P, const JData &J, int j, const double xi[3], const double vi[3],
+                                   double rinv_i, double ta[3], double tj[3], double *tphi) {
+    pair_term(P, xi, vi, rinv_i, J.x[3 * j], J.x[3 * j + 1], J.x[3 * j + 2], J.v[3 * j], J.v[3 * j + 1],
+              J.v[3 * j + 2], J.m[j], J.ri[j], ta, tj, tphi);
+}
+
+// Sum over j = j0, j0 + s, j0 + 2s, ... < n (j != i by id) in ascending j, HARD_ILP
+// pairs evaluated at a time.  The self pair is evaluated too (its summands may be
+// NaN) and not added.
+__device__ inline void pair_sum_strided(const HP &P, const JData &J, int j0, int s, int n, int64_t idi,
+                                        const double xi[3], const double vi[3], double rinv_i, double acc[3],
+                                        double jerk[3], double *phi) {
+    int j = j0;
+    for (; j + (HARD_ILP - 1) * s < n; j += HARD_ILP * s) {
+        double ta[HARD_ILP][3], tj[HARD_ILP][3], tp[HARD_ILP];
+#pragma unroll
+        for (int u = 0; u < HARD_ILP; u++) pair_term_j(P, J, j + u * s, xi, vi, rinv_i, ta[u], tj[u], phi ? &tp[u] : nullptr);
+#pragma unroll
+        for (int u = 0; u < HARD_ILP; u++) {
+            if (J.id[j + u * s] == idi) continue;
+            for (int k = 0; k < 3; k++) { acc[k] += ta[u][k]; jerk[k] += tj[u][k]; }
+            if (phi) *phi -= tp[u];
+        }
+    }
+    for (; j < n; j += s) {
+        if (J.id[j] == idi) continue;
+        double ta[3], tj[3], tp;
+        pair_term_j(P, J, j, xi, vi, rinv_i, ta, tj, phi ? &tp : nullptr);
+        for (int k = 0; k < 3; k++) { acc[k] += ta[k]; jerk[k] += tj[k]; }
+        if (phi) *phi -= tp;
+    }
+}
+
 __device__ inline void gravity_loop(const HP &P, const JData &J, int n, int64_t idi,
                                     const double xi[3], const double vi[3], double rinv_i,
                                     double acc[3], double jerk[3], double *phi) {
     acc[0] = acc[1] = acc[2] = 0.;
     jerk[0] = jerk[1] = jerk[2] = 0.;
-    for (int j = 0; j < n; j++) {
-        if (J.id[j] == idi) continue;
-        hard_pair(P, xi, vi, rinv_i, J.x[3 * j], J.x[3 * j + 1], J.x[3 * j + 2], J.v[3 * j],
-                  J.v[3 * j + 1], J.v[3 * j + 2], J.m[j], J.ri[j], acc, jerk, phi);
-    }
+    pair_sum_strided(P, J, 0, 1, n, idi, xi, vi, rinv_i, acc, jerk, phi);
 }
 
 // calcGravity (full, with phi_d and Sun phi), gravity_hard.h:203-256, current pos/vel.
@@ -180,15 +215,8 @@ __device__ void gravity_full(const HP &P, HardView &H, int b, int n, int i) {
     ld3(H.pos, b + i, x);
     ld3(H.vel, b + i, v);
     star_acc_jerk(P, x, v, as, js, &phs);
-    ad[0] = ad[1] = ad[2] = jd[0] = jd[1] = jd[2] = 0.;
-    const int64_t idi = H.id[b + i];
-    const double ri = H.rinv[b + i];
-    for (int j = 0; j < n; j++) {
-        if (H.id[b + j] == idi) continue;
-        hard_pair(P, x, v, ri, H.pos[3 * (b + j)], H.pos[3 * (b + j) + 1], H.pos[3 * (b + j) + 2],
-                  H.vel[3 * (b + j)], H.vel[3 * (b + j) + 1], H.vel[3 * (b + j) + 2], H.mass[b + j],
-                  H.rinv[b + j], ad, jd, &phd);
-    }
+    const JData Jc{H.pos + 3 * b, H.vel + 3 * b, H.mass + b, H.rinv + b, H.id + b, 3};
+    gravity_loop(P, Jc, n, H.id[b + i], x, v, H.rinv[b + i], ad, jd, &phd);
     st3(H.acc_s, b + i, as);
     st3(H.jerk_s, b + i, js);
     st3(H.acc_d, b + i, ad);
@@ -312,11 +340,7 @@ __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int 
 __device__ inline void coop_pair_sum(const HP &P, const JData &J, int n, int64_t idi, const double xi[3],
                                      const double vi[3], double rinv_i, double acc[3], double jerk[3]) {
     double a[3] = {0., 0., 0.}, jk[3] = {0., 0., 0.};
-    for (int j = (int)threadIdx.x; j < n; j += 64) {
-        if (J.id[j] == idi) continue;
-        hard_pair(P, xi, vi, rinv_i, J.x[3 * j], J.x[3 * j + 1], J.x[3 * j + 2], J.v[3 * j], J.v[3 * j + 1],
-                  J.v[3 * j + 2], J.m[j], J.ri[j], a, jk, nullptr);
-    }
+    pair_sum_strided(P, J, (int)(threadIdx.x & 63), 64, n, idi, xi, vi, rinv_i, a, jk, nullptr);
     for (int d = 32; d > 0; d >>= 1)
         for (int k = 0; k < 3; k++) {
             a[k] += __shfl_xor(a[k], d);
@@ -431,9 +455,9 @@ __device__ double perfect_merge(const HP &P, HardView &H, int b, int n, int ic, 
 // --------------------------------------------------------------- kernel
 
 struct Reduce {              // block-level reductions through LDS
-    double d[HERMITE_BLOCK];
-    int i[HERMITE_BLOCK];
-    int j[HERMITE_BLOCK];
+    double d[HERMITE_BIG_BLOCK];
+    int i[HERMITE_BIG_BLOCK];
+    int j[HERMITE_BIG_BLOCK];
 };
 
 __device__ inline double block_min(Reduce &R, double v) {
@@ -488,18 +512,34 @@ __device__ inline void wave_argmin(double &v, int &i, int &j) {
     }
 }
 
+// Orders the wave's own global-memory writes before its later reads (lane 0 writes
+// state that all lanes read back): the wave's memory operations are complete at the
+// fence, and the barrier keeps the compiler from moving accesses across it.
+__device__ inline void wave_sync() {
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+}
+
 // The active-particle correction of integrate_from for clusters above HARD_SEQ_MAX
-// particles: particles are corrected one after the other (ascending index), each with
-// all 64 lanes evaluating its hard force (coop_pair_sum); every lane carries the same
-// particle state in registers and lane 0 writes it back.  Twins of a merged target are
+// particles (H.act marks the active ones): the k-th active particle (ascending index)
+// goes to wave k mod (waves per workgroup), and the wave's 64 lanes evaluate its hard
+// force together (coop_pair_sum); every lane carries the same particle state in
+// registers and lane 0 writes it back.  Active particles only read the predicted
+// j-data, so the waves need not wait for each other.  Twins of a merged target are
 // integrated alongside with their own cut-off radius, as in the per-lane path.
 __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
                                     int flag_col, long long &pairs) {
-    const bool lead = threadIdx.x == 0;
-    for (int i = 0; i < n; i++) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)(threadIdx.x >> 6), n_wave = (int)(blockDim.x >> 6);
+    const bool lead = lane == 0;
+    int rank = 0;                          // active particles seen so far (wave-uniform)
+    for (int i0 = 0; i0 < n; i0 += 64) {
+    unsigned long long mask = __ballot(i0 + lane < n && H.act[b + i0 + lane] != 0);
+    while (mask) {
+        const int i = i0 + __ffsll(mask) - 1;
+        mask &= mask - 1;
+        if ((rank++) % n_wave != wave) continue;
         const int g = b + i;
-        if (H.flags[g] & GPLUM_FLAG_DEAD) continue;
-        if (!flag_col && H.time[g] + H.dt[g] != t_s) continue;
         const bool merged = (H.flags[g] & GPLUM_FLAG_MERGED) != 0;
         if (lead) pairs += 3 * (long long)(n - 1);
         // state of g (predicted by the predict pass)
@@ -545,8 +585,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
                                      t3d);
                         st3(H.pos, b + j, tx); st3(H.vel, b + j, tv);
                     }
-            __threadfence_block();
-            __syncthreads();
+            wave_sync();
             star_acc_jerk(P, x, v, as, js, nullptr);
             coop_pair_sum(P, J, n, id, x, v, rinv, ad, jd);
             if (merged) {
@@ -558,7 +597,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
                     ld3(H.pos, b + j, tx); ld3(H.vel, b + j, tv);
                     star_acc_jerk(P, tx, tv, tas, tjs, nullptr);
                     coop_pair_sum(P, J, n, H.id[b + j], tx, tv, H.rinv[b + j], tad, tjd);
-                    __syncthreads();
+                    wave_sync();
                     if (lead) { st3(H.acc_s, b + j, tas); st3(H.jerk_s, b + j, tjs); }
                     for (int k = 0; k < 3; k++) { sa[k] += H.mass[b + j] * tad[k]; sj[k] += H.mass[b + j] * tjd[k]; }
                     mass += H.mass[b + j];
@@ -570,7 +609,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
         double time = H.time[g] + t_c;
         double dt = H.dt[g];
         if (!flag_col) dt = dt_block(P, time, dt, H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
-        __syncthreads();
+        wave_sync();
         if (lead) {
             st3(H.pos, g, x); st3(H.vel, g, v);
             st3(H.acc_s, g, as); st3(H.jerk_s, g, js); st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
@@ -585,8 +624,8 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
                         if (!flag_col) H.dt[b + j] = dt;
                     }
         }
-        __threadfence_block();
-        __syncthreads();
+        wave_sync();
+    }
     }
 }
 
@@ -672,6 +711,11 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         }
         // correct the active particles (hermite.h:300-375)
         if (n > HARD_SEQ_MAX) {
+            for (int i = t; i < n; i += B)
+                H.act[b + i] = !(H.flags[b + i] & GPLUM_FLAG_DEAD) &&
+                               (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
+            __threadfence_block();
+            __syncthreads();
             coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs);
         } else
         for (int i = t; i < n; i += B) {
@@ -885,10 +929,12 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
 // REG: n <= HARD_SEQ_MAX, particle state in registers (integrate_regs);
 // LDSJ && !REG: HARD_SEQ_MAX < n <= HERMITE_LDS, j-data in LDS (integrate_from);
 // !LDSJ: n > HERMITE_LDS, j-data in global memory.
+// The REG kernel runs one wave per workgroup; the other two run HERMITE_BIG_BLOCK
+// threads, whose waves share the active particles of a block step.
 // Separate kernels keep the register-heavy small-cluster path from lowering the
 // occupancy of the others.
 template <bool LDSJ, bool REG>
-__global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H,
+__global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_hermite(HP P, HardView H,
                                                            const int32_t *__restrict__ cl_off,
                                                            const int32_t *__restrict__ order, int ncl,
                                                            int *__restrict__ next, double *__restrict__ edisp,
@@ -1114,7 +1160,7 @@ static HardView hard_view(HardBuffers &hb) {
     return HardView{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
                     hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
                     hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
-                    hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
+                    hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p, hb.act.p};
 }
 
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
@@ -1140,10 +1186,10 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
         GPLUM_HIP(hipStreamWaitEvent(sb, ev_fork, 0));
     }
     if (hb.max_size > HERMITE_LDS)
-        hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BLOCK), 0, sb, P, H, d_off, d_order,
+        hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
                            ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
     if (big)
-        hipLaunchKernelGGL((k_hermite<true, false>), dim3(grid), dim3(HERMITE_BLOCK), 0, sb, P, H, d_off, d_order,
+        hipLaunchKernelGGL((k_hermite<true, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
                            ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
     hipLaunchKernelGGL((k_hermite<true, true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order, ncl,
                        hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
@@ -1178,7 +1224,7 @@ int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t 
     HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
                hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
                hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
-               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p};
+               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p, hb.act.p};
     hipLaunchKernelGGL(k_hard_prepare, dim3(ncl), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, ncl);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
@@ -1193,10 +1239,10 @@ int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, c
         hipLaunchKernelGGL((k_hermite<true, true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
                            hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
     else if (n_c <= HERMITE_LDS)
-        hipLaunchKernelGGL((k_hermite<true, false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
+        hipLaunchKernelGGL((k_hermite<true, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
                            hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
     else
-        hipLaunchKernelGGL((k_hermite<false, false>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr,
+        hipLaunchKernelGGL((k_hermite<false, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr,
                            1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;

@@ -4,7 +4,8 @@
 
 namespace gplum {
 
-constexpr int HERMITE_BLOCK = 64;    // threads per cluster workgroup
+constexpr int HERMITE_BLOCK = 64;      // threads per workgroup, clusters up to HARD_SEQ_MAX
+constexpr int HERMITE_BIG_BLOCK = 256; // threads per workgroup (4 waves), larger clusters
 constexpr int HERMITE_LDS = 256;     // clusters up to this size keep j-data in LDS
 // Clusters up to HARD_SEQ_MAX particles sum each particle's hard force sequentially in
 // j (one lane per particle); larger ones with 64 lane-strided partial sums and a
@@ -17,7 +18,7 @@ struct HardBuffers {
     DArray<double> a2_s, a3_s, a2_d, a3_d;
     DArray<double> mass, rinv, time, dt, acc0, r_planet, f, phi_d, phi_s;
     DArray<int64_t> id;
-    DArray<int32_t> flags, twin_of;
+    DArray<int32_t> flags, twin_of, act;
     DArray<int> next, ncol;
     DArray<double> edisp;
     DArray<long long> nblock, npair;    // block steps, hard_pair evaluations per cluster
@@ -32,6 +33,7 @@ struct HardBuffers {
         GPLUM_ENSURE(mass, n); GPLUM_ENSURE(rinv, n); GPLUM_ENSURE(time, n); GPLUM_ENSURE(dt, n);
         GPLUM_ENSURE(acc0, n); GPLUM_ENSURE(r_planet, n); GPLUM_ENSURE(f, n); GPLUM_ENSURE(phi_d, n);
         GPLUM_ENSURE(phi_s, n); GPLUM_ENSURE(id, n); GPLUM_ENSURE(flags, n); GPLUM_ENSURE(twin_of, n);
+        GPLUM_ENSURE(act, n);
         return GPLUM_OK;
     }
 };
