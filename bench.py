@@ -142,6 +142,26 @@ def bench_hard(args):
     print(json.dumps(out))
 
 
+def bench_copy(args):
+    """A plain device-to-device copy of known size (torch's copy kernel): the reference
+    point for the FETCH_SIZE / WRITE_SIZE counters (a copy of B bytes reads B and writes
+    B bytes of HBM)."""
+    import torch
+    nbytes = args.copy_mib << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.warmup)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "device copy bandwidth (read + write bytes / s)", "value": round(2 * nbytes / dt / 1e9, 1),
+                      "unit": "GB/s", "bytes_copied": nbytes, "steps": args.steps, "ms_per_copy": round(dt * 1e3, 4)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,8 +172,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--workload", choices=["step", "hard"], default="step",
-                    help="step: full P3T step at N (default); hard: batched hard part alone")
+    ap.add_argument("--workload", choices=["step", "hard", "copy"], default="step",
+                    help="step: full P3T step at N (default); hard: batched hard part alone; "
+                         "copy: plain device copy of --copy-mib MiB (calibrates the HBM byte counters)")
+    ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--clusters", type=int, default=5000)
     ap.add_argument("--cpu-clusters", type=int, default=20)
     ap.add_argument("--compare-clusters", type=int, default=100)
@@ -165,6 +187,9 @@ def main():
 
     if args.workload == "hard":
         bench_hard(args)
+        return
+    if args.workload == "copy":
+        bench_copy(args)
         return
 
     from gplum_amd import gpu

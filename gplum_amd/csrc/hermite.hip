@@ -496,13 +496,43 @@ __device__ inline double block_argmin(Reduce &R, double v, int i, int j, int *bi
     return r;
 }
 
-// wave-wide min and (value, i)-lexicographic argmin (the workgroup is one wave)
+// Lane permutations within rows of 16 lanes (DPP): quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror.  Applied in this order with a min, they
+// leave every lane of a row holding the row's minimum.
+constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Minimum over the 64 lanes of the wave (every lane active; values not NaN): four DPP
+// steps inside each row of 16 lanes, then the four row minima read as scalars.  Far
+// shorter latency than a shuffle butterfly through the LDS crossbar.
 __device__ inline double wave_min(double v) {
-    for (int d = 32; d > 0; d >>= 1) {
-        const double o = __shfl_xor(v, d);
-        v = (o < v) ? o : v;
-    }
-    return v;
+    v = dmin(v, dpp_f64<DPP_QUAD_1032>(v));
+    v = dmin(v, dpp_f64<DPP_QUAD_2301>(v));
+    v = dmin(v, dpp_f64<DPP_ROW_HALF_MIRROR>(v));
+    v = dmin(v, dpp_f64<DPP_ROW_MIRROR>(v));
+    return dmin(dmin(readlane_f64(v, 0), readlane_f64(v, 16)), dmin(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+// two minima at once (independent chains)
+__device__ inline void wave_min2(double &u, double &v) {
+    u = dmin(u, dpp_f64<DPP_QUAD_1032>(u)); v = dmin(v, dpp_f64<DPP_QUAD_1032>(v));
+    u = dmin(u, dpp_f64<DPP_QUAD_2301>(u)); v = dmin(v, dpp_f64<DPP_QUAD_2301>(v));
+    u = dmin(u, dpp_f64<DPP_ROW_HALF_MIRROR>(u)); v = dmin(v, dpp_f64<DPP_ROW_HALF_MIRROR>(v));
+    u = dmin(u, dpp_f64<DPP_ROW_MIRROR>(u)); v = dmin(v, dpp_f64<DPP_ROW_MIRROR>(v));
+    u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, 16)), dmin(readlane_f64(u, 32), readlane_f64(u, 48)));
+    v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, 16)), dmin(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 __device__ inline void wave_argmin(double &v, int &i, int &j) {
     for (int d = 32; d > 0; d >>= 1) {
@@ -841,8 +871,10 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     }
     const double time_end = P.dt_tree;
     double a0s[3], j0s[3], a0d[3], j0d[3], xp[3], vp[3], a2s[3], a3s[3], a2d[3], a3d[3];
+    // t_s: the next block time, min(time + dt) (makeActiveList); computed together with
+    // the system time at the end of each step
+    double t_s = wave_min(own ? time + dt : 1.e10);
     while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
-        const double t_s = wave_min(own ? time + dt : 1.e10);
         __syncthreads();   // the previous step's j-data reads are done
         if (own) {
             predict_regs(t_s - time, x, v, as, js, ad, jd, a0s, j0s, a0d, j0d, xp, vp);
@@ -866,8 +898,9 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
                     if (Rr < rmin) { rmin = Rr; bi = t; bj = j; }
                 }
             }
-            wave_argmin(rmin, bi, bj);
-            if (rmin < 1.) {
+            // the argmin only when some pair is below 1 (rare)
+            if (__ballot(rmin < 1.)) wave_argmin(rmin, bi, bj);
+            if (__ballot(rmin < 1.)) {
                 // hand the cluster to the global-memory path at the start of this step
                 if (own) {
                     st3(H.pos, g, x); st3(H.vel, g, v);
@@ -897,7 +930,9 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             time += t_c;
             dt = dt_block(P, time, dt, acc0, ad, jd, a2d, a3d, as, js, a2s, a3s);
         }
-        sys_time = wave_min(own ? time : 1.e10);
+        sys_time = own ? time : 1.e10;
+        t_s = own ? time + dt : 1.e10;
+        wave_min2(sys_time, t_s);
         loop++;
     }
     // final calcGravity with potentials (hermite.h:466-470), current positions
