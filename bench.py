@@ -44,6 +44,35 @@ def make_disk(n, seed=1):
     return ic.disk(n, 1e-12, 0.5, 2.0, seed=seed)
 
 
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from its committed counter summary under
+    profiles/ (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy); None if absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
+def cpu_soft_force(n, seed, threads, pp=None):
+    """Oracle FP32 soft force (PIKG-order kernels, OpenMP over groups) on the same disk:
+    interactions per second on `threads` host threads."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import oracle as O
+    O.build()
+    p = O.params(**params_dict())
+    pp = make_disk(n, seed) if pp is None else pp.copy()
+    O.set_cutoff_radii(p, pp)
+    tree = O.tree_build(p, pp)
+    t0 = time.perf_counter()
+    nep, nsp = O.soft_force(p, pp, tree)
+    dt = time.perf_counter() - t0
+    tree.free()
+    return {"interactions_per_s": round((nep + nsp) / dt, 1), "epep_pairs": int(nep), "epsp_pairs": int(nsp),
+            "seconds": round(dt, 3), "cores": threads}
+
+
 def cpu_baseline(n, seed, threads):
     """Oracle full P3T step (CPU restatement, OpenMP) on the same workload: one step
     after initialisation, timed on `threads` host threads."""
@@ -228,8 +257,9 @@ def main():
     achieved = flops / (force_ms * 1e-3) / 1e12
     # compute-bound kernel: "mfma" is the compute roof of the schema; the arithmetic is FP32
     # VALU (pairwise gravity is not a contraction), priced against the FP32 vector peak
+    traffic = measured_traffic("k_soft_f32_n1m_traffic.md") if args.n == 1_000_000 else None
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                 "compute": "fp32 VALU, no MFMA",
                 "kernel": "k_soft_f32 (EP-EP near/far + EP-SP quadrupole, one launch per force)",
                 "flops_per_launch": flops, "avg_launch_ms": round(force_ms, 4)}
@@ -250,6 +280,9 @@ def main():
     b = q64[np.argsort(q64["id"])]["acc"]
     rel = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
     acc_rms = float(np.sqrt(np.mean(rel ** 2)))
+    pa = q32b[np.argsort(q32b["id"])]["phi"]
+    pb = q64[np.argsort(q64["id"])]["phi"]
+    phi_max = float(np.max(np.abs(pa - pb) / np.abs(pb)))
     ips32 = (pairs_ep + pairs_sp) / (force_ms * 1e-3)
     ips64 = (st64["n_epep_near"] + st64["n_epep_far"] + st64["n_epsp"]) / (st64["force_ms"] * 1e-3)
     s64.close()
@@ -287,21 +320,30 @@ def main():
         "roofline": roofline,
         "soft_force": {"interactions_per_s_fp32": round(ips32, 1), "interactions_per_s_fp64": round(ips64, 1),
                        "fp32_over_fp64": round(ips32 / ips64, 3), "acc_rms_rel_err_fp32_vs_fp64": acc_rms,
+                       "phi_max_rel_err_fp32_vs_fp64": phi_max,
                        "epep_pairs": pairs_ep, "epsp_pairs": pairs_sp, "groups": st["n_groups"]},
         "breakdown_ms_per_step": {k: round(st[k] / args.steps, 3) for k in ("force_ms", "tree_ms",
                                                                              "finalize_ms", "hard_ms")},
         "hard": {"clusters": st["n_clusters"], "cluster_particles": st["n_cluster_particles"],
                  "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
-                 "collisions": st["n_collisions"]},
+                 "max_block_steps_one_cluster": st["max_block_steps"], "collisions": st["n_collisions"]},
         "energy_error_over_timed_steps": de,
         "plain_fp64_per_cluster": plain,
         "forecast": {"calls": st["fc_calls"], "slots": st["fc_slots"], "overflowed_groups": st["fc_ovf_groups"],
                      "redone_candidates": st["fc_ovf_candidates"], "loss": st["fc_loss"],
                      "ms_per_step": round(st["fc_ms"] / args.steps, 3)},
     }
+    # list building (neighbour finalize): HBM-bound; bytes from the k_finalize op list,
+    # 116 B per particle + 105 B per FP32 candidate, over the finalize stage's event time
+    lb_bytes = 116 * args.n + 105 * st["n_candidates"]
+    fin_ms = st["finalize_ms"] / max(1, st["n_force_calls"])
+    out["list_build"] = {"bytes_per_call": lb_bytes, "finalize_ms_per_call": round(fin_ms, 4),
+                         "achieved_GBps": round(lb_bytes / (fin_ms * 1e-3) / 1e9, 1), "peak_GBps": 8000.0,
+                         "note": "finalize stage (candidate sort, k_finalize, CSR build) timed as a whole"}
     s.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["soft_force"]["interactions_per_s_cpu_fp32"] = cpu_soft_force(args.n, args.seed, threads, warm_state)
         out["cpu_baseline"] = cpu_baseline(args.n, args.seed, threads)
     else:
         out["cpu_baseline"] = None

@@ -874,6 +874,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     // t_s: the next block time, min(time + dt) (makeActiveList); computed together with
     // the system time at the end of each step
     double t_s = wave_min(own ? time + dt : 1.e10);
+    bool handover = false;
     while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
         __syncthreads();   // the previous step's j-data reads are done
         if (own) {
@@ -900,22 +901,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             }
             // the argmin only when some pair is below 1 (rare)
             if (__ballot(rmin < 1.)) wave_argmin(rmin, bi, bj);
-            if (__ballot(rmin < 1.)) {
-                // hand the cluster to the global-memory path at the start of this step
-                if (own) {
-                    st3(H.pos, g, x); st3(H.vel, g, v);
-                    st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
-                    st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
-                    H.time[g] = time;
-                    H.dt[g] = dt;
-                    H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
-                    H.twin_of[g] = -1;
-                }
-                __threadfence_block();
-                __syncthreads();
-                integrate_from<true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
-                return;
-            }
+            if (__ballot(rmin < 1.)) { handover = true; break; }
         }
         if (own && time + dt == t_s) {
             pairs += 3 * (long long)(n - 1);
@@ -934,6 +920,23 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         t_s = own ? time + dt : 1.e10;
         wave_min2(sys_time, t_s);
         loop++;
+    }
+    if (handover) {
+        // a collision in this block step: hand the cluster to the global-memory path
+        // at the start of the step (kept out of the loop, which then holds less state)
+        if (own) {
+            st3(H.pos, g, x); st3(H.vel, g, v);
+            st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
+            st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
+            H.time[g] = time;
+            H.dt[g] = dt;
+            H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+            H.twin_of[g] = -1;
+        }
+        __threadfence_block();
+        __syncthreads();
+        integrate_from<true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
+        return;
     }
     // final calcGravity with potentials (hermite.h:466-470), current positions
     __syncthreads();
@@ -969,7 +972,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
 // Separate kernels keep the register-heavy small-cluster path from lowering the
 // occupancy of the others.
 template <bool LDSJ, bool REG>
-__global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_hermite(HP P, HardView H,
+__global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_hermite(HP P_arg, HardView H,
                                                            const int32_t *__restrict__ cl_off,
                                                            const int32_t *__restrict__ order, int ncl,
                                                            int *__restrict__ next, double *__restrict__ edisp,
@@ -981,8 +984,14 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
     __shared__ Reduce R;
     __shared__ int s_cl;
     __shared__ unsigned long long s_pairs;
+    // the physics constants are read from LDS: as kernel arguments they would occupy
+    // scalar registers for the whole kernel (and be spilled and reloaded in the loops)
+    __shared__ HP s_P;
     const int t = threadIdx.x;
     const int B = blockDim.x;
+    if (t == 0) s_P = P_arg;
+    __syncthreads();
+    const HP &P = s_P;
     for (int iter = 0;; iter++) {
         if (persistent) {
             if (t == 0) s_cl = atomicAdd(next, 1);

@@ -592,6 +592,7 @@ int System::hard_part(bool persistent) {
             step_ncol += nc[c];
             step_edisp += ed[c];
             stats.block_steps += nbk[c];
+            stats.max_block_steps = std::max<int64_t>(stats.max_block_steps, nbk[c]);
             stats.hard_pairs += npr[c];
         }
     }
