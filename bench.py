@@ -121,8 +121,16 @@ def bench_hard(args):
     warm = pp.copy()
     gpu.hard_integrate(p, warm, off, persistent=True, prepare=True)
     a = pp.copy()
-    ed, nb, ms_p, npair = gpu.hard_integrate(p, a, off, persistent=True, prepare=True, counts=True)
-    print("# persistent: %.2f ms" % ms_p, file=sys.stderr, flush=True)
+    ed, nb, ms_e, npair = gpu.hard_integrate(p, a, off, persistent=True, prepare=True, counts=True)
+    print("# persistent (eager launch): %.2f ms" % ms_e, file=sys.stderr, flush=True)
+    # the same launch captured into a hipGraph and replayed (each replay restores the
+    # uploaded state inside the graph, so every replay integrates the same step)
+    g = pp.copy()
+    ed_g, nb_g, ms_p, mism = gpu.hard_integrate_graph(p, g, off, replays=args.replays, prepare=True)
+    graph_same = bool(mism == 0 and nb_g == nb and np.array_equal(g["pos"], a["pos"]) and
+                      np.array_equal(g["vel"], a["vel"]))
+    print("# persistent (graph replay, mean of %d): %.2f ms, identical %s" % (args.replays, ms_p, graph_same),
+          file=sys.stderr, flush=True)
     # per-cluster launches run the clusters one after another: compare on a sample
     k = min(args.compare_clusters, n_cl)
     s_off = off[: k + 1]
@@ -137,7 +145,7 @@ def bench_hard(args):
     achieved = FLOP_HARD_PAIR * npair / (ms_p * 1e-3) / 1e12
     out = {
         "metric": "batched hard part: particle-steps/s (one soft step, block-timestep Hermite)",
-        "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": args.replays, "warmup": 1,
         "ms_per_step": round(ms_p, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batched hard: %d clusters, sizes log-uniform 2-256, sigma 2 r_H(1e-10) at 1 AU, "
@@ -145,10 +153,12 @@ def bench_hard(args):
                    "clusters": n_cl, "particles": n, "seed": 2},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "compute": "fp64 VALU, no MFMA", "kernel": "k_hermite (persistent)",
+                     "compute": "fp64 VALU, no MFMA", "kernel": "k_hermite size classes (persistent, graph replay)",
                      "flops_per_launch": FLOP_HARD_PAIR * npair, "avg_launch_ms": round(ms_p, 3)},
         "per_cluster_launches": {"sample_clusters": k, "ms": round(ms_c, 3), "persistent_ms": round(ms_sp, 3),
                                  "speedup_persistent": round(ms_c / ms_sp, 3), "identical_results": same},
+        "graph": {"replays": args.replays, "ms_per_replay": round(ms_p, 3), "eager_launch_ms": round(ms_e, 3),
+                  "identical_to_eager_and_across_replays": graph_same},
         "block_steps": nb, "pair_evaluations": npair,
     }
     if not args.no_cpu:
@@ -208,6 +218,7 @@ def main():
     ap.add_argument("--clusters", type=int, default=5000)
     ap.add_argument("--cpu-clusters", type=int, default=20)
     ap.add_argument("--compare-clusters", type=int, default=100)
+    ap.add_argument("--replays", type=int, default=3, help="graph replays (hard workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -984,14 +984,17 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
     __shared__ Reduce R;
     __shared__ int s_cl;
     __shared__ unsigned long long s_pairs;
-    // the physics constants are read from LDS: as kernel arguments they would occupy
-    // scalar registers for the whole kernel (and be spilled and reloaded in the loops)
+    // the register path reads the physics constants from LDS: as kernel arguments they
+    // would occupy scalar registers for the whole kernel (and be spilled and reloaded
+    // in its loop); the other paths keep them as arguments
     __shared__ HP s_P;
     const int t = threadIdx.x;
     const int B = blockDim.x;
-    if (t == 0) s_P = P_arg;
-    __syncthreads();
-    const HP &P = s_P;
+    if (REG) {
+        if (t == 0) s_P = P_arg;
+        __syncthreads();
+    }
+    const HP &P = REG ? s_P : P_arg;
     for (int iter = 0;; iter++) {
         if (persistent) {
             if (t == 0) s_cl = atomicAdd(next, 1);
@@ -1207,6 +1210,12 @@ static HardView hard_view(HardBuffers &hb) {
                     hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p, hb.act.p};
 }
 
+// zeroes the three cluster-queue counters (a kernel, not a memset, so that the launch
+// sequence can be captured into a graph and replayed)
+__global__ void k_reset_queue(int *next) {
+    if (threadIdx.x < 3) next[threadIdx.x] = 0;
+}
+
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2,
                 hipEvent_t ev_fork, hipEvent_t ev_join) {
@@ -1218,7 +1227,7 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
     GPLUM_ENSURE(hb.ncol, ncl);
     GPLUM_ENSURE(hb.nblock, ncl);
     GPLUM_ENSURE(hb.npair, ncl);
-    GPLUM_HIP(hipMemsetAsync(hb.next.p, 0, 3 * sizeof(int), st));
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, hb.next.p);
     const int pers = persistent ? 1 : 0;
     const int grid = persistent ? std::min(ncl, std::max(1, n_cu) * 8) : ncl;
     // clusters above HARD_SEQ_MAX on a second stream (when given), concurrently with

@@ -101,6 +101,21 @@ def hard_integrate(p, pp, offsets, persistent=True, prepare=False, counts=False)
     return ed.value, nb.value, ms.value
 
 
+def hard_integrate_graph(p, pp, offsets, replays=4, prepare=False):
+    """gplum_hard_integrate_graph: the persistent launch captured into a hipGraph and
+    replayed `replays` times from the same uploaded state (in place, first replay's
+    result).  Returns (edisp, block_steps, ms_per_replay, mismatches)."""
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    ed = ctypes.c_double(0.0)
+    nb = ctypes.c_int64(0)
+    ms = ctypes.c_double(0.0)
+    bad = ctypes.c_int(-1)
+    check(lib().gplum_hard_integrate_graph(ctypes.byref(p), abi.ptr(pp), len(pp), abi.ptr(off), len(off) - 1,
+                                           int(prepare), int(replays), ctypes.byref(ed), ctypes.byref(nb),
+                                           ctypes.byref(ms), ctypes.byref(bad)))
+    return ed.value, nb.value, ms.value, bad.value
+
+
 class Stats(ctypes.Structure):
     """gplum_stats (include/gplum_system.h)."""
     _fields_ = [(n, ctypes.c_int64) for n in (

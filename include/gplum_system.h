@@ -143,6 +143,18 @@ int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, int n, const
                          int n_cluster, int mode, double *edisp, int64_t *block_steps,
                          double *kernel_ms, int64_t *pair_evals);
 
+/* The persistent form of gplum_hard_integrate captured into a hipGraph and replayed
+ * `replays` times (no counterpart in the reference; same clusters, results and
+ * outputs as mode 1, for callers that replay a fixed batch).  The graph restores the
+ * uploaded state, resets the cluster queue with a kernel and launches the size-class
+ * kernels on two streams; every replay therefore integrates the same step.  prepare
+ * as mode bit 1.  Outputs (each may be null): ms_per_replay (HIP events around each
+ * replay, mean), mismatches (replays whose outputs differ bit for bit from the first
+ * replay's: 0 expected).  pp receives the first replay's result. */
+int gplum_hard_integrate_graph(const gplum_params *p, gplum_particle *pp, int n, const int32_t *offsets,
+                               int n_cluster, int prepare, int replays, double *edisp, int64_t *block_steps,
+                               double *ms_per_replay, int *mismatches);
+
 #ifdef __cplusplus
 }
 #endif

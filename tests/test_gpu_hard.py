@@ -95,3 +95,37 @@ def test_hermite_perfect_merging_matches_oracle(oracle_lib):
     assert ed_g == ed
     for f in ("pos", "vel", "acc_d", "jerk_d", "mass", "r_planet", "dt", "time"):
         assert np.array_equal(got[f], ref[f]), f
+
+
+def _same_state(a, b, fields=("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_d", "phi_s", "dt", "time",
+                              "mass", "r_planet", "id", "flags")):
+    return [f for f in fields if not np.array_equal(a[f], b[f])]
+
+
+def test_graph_replay_matches_eager(no_collision_case):
+    """The persistent launch captured into a hipGraph (state restore, queue reset by a
+    kernel, size-class kernels on two streams): every replay reproduces the first bit
+    for bit (checked inside the call with synchronisations between replays), and the
+    first equals the eager launch."""
+    pg, pp, off, ref, nb_ref = no_collision_case
+    eager = pp.copy()
+    ed_e, nb_e, ms_e = gpu.hard_integrate(pg, eager, off, persistent=True)
+    got = pp.copy()
+    ed_g, nb_g, ms_g, bad = gpu.hard_integrate_graph(pg, got, off, replays=3)
+    print("eager %.2f ms, graph replay %.2f ms, mismatching replays %d" % (ms_e, ms_g, bad))
+    assert bad == 0
+    assert nb_g == nb_e == nb_ref
+    assert _same_state(got, eager) == []
+
+
+def test_graph_replay_with_merging(oracle_lib):
+    """Replays restore ids, flags and radii changed by perfect merging."""
+    po, pg, pp, off = make_clusters(oracle_lib, 24, seed=9, size_max=24, collision=1, f=300.0, m_hi=1e-8)
+    eager = pp.copy()
+    ed_e, nb_e, _ = gpu.hard_integrate(pg, eager, off, persistent=True)
+    assert ((eager["flags"] & 1) != 0).any()
+    got = pp.copy()
+    ed_g, nb_g, _, bad = gpu.hard_integrate_graph(pg, got, off, replays=3)
+    assert bad == 0
+    assert nb_g == nb_e and ed_g == ed_e
+    assert _same_state(got, eager) == []
