@@ -219,6 +219,7 @@ def main():
     ap.add_argument("--cpu-clusters", type=int, default=20)
     ap.add_argument("--compare-clusters", type=int, default=100)
     ap.add_argument("--replays", type=int, default=3, help="graph replays (hard workload)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -242,6 +243,12 @@ def main():
     s.initialize()
     t_init = time.perf_counter() - t0
     print("# device %s, %d CUs; init %.2f s" % (name, n_cu, t_init), file=sys.stderr, flush=True)
+    tuning = None
+    if not args.no_tune:
+        # the project's autotuner picks i-group size, soft j-tile, Hermite grid and LDS
+        # binning threshold on this state (untimed, like initialisation)
+        tuning = s.autotune(trials=2)
+        print("# autotune: %s" % tuning, file=sys.stderr, flush=True)
     for k in range(args.warmup):
         s.step(1)
         print("# warmup step %d done" % k, file=sys.stderr, flush=True)
@@ -329,6 +336,7 @@ def main():
                                "theta=0.5 quadrupole, n_group<=256, dt_tree=2^-4, collisions on" % args.n,
                    "n": args.n, "seed": args.seed, "device": name},
         "roofline": roofline,
+        "autotune": tuning,
         "soft_force": {"interactions_per_s_fp32": round(ips32, 1), "interactions_per_s_fp64": round(ips64, 1),
                        "fp32_over_fp64": round(ips32 / ips64, 3), "acc_rms_rel_err_fp32_vs_fp64": acc_rms,
                        "phi_max_rel_err_fp32_vs_fp64": phi_max,

@@ -977,7 +977,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
                                                            const int32_t *__restrict__ order, int ncl,
                                                            int *__restrict__ next, double *__restrict__ edisp,
                                                            int *__restrict__ ncol, long long *__restrict__ nblock,
-                                                           long long *__restrict__ npair, int persistent) {
+                                                           long long *__restrict__ npair, int persistent,
+                                                           int lds_max) {
     __shared__ double s_x[3 * HERMITE_LDS], s_v[3 * HERMITE_LDS], s_m[HERMITE_LDS], s_ri[HERMITE_LDS];
     __shared__ int64_t s_id[HERMITE_LDS];
     __shared__ double s_fr[HERMITE_LDS];
@@ -1010,8 +1011,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         const int b = cl_off[c];
         const int n = cl_off[c + 1] - b;
         if (REG && n > HARD_SEQ_MAX) continue;
-        if (LDSJ && !REG && (n <= HARD_SEQ_MAX || n > HERMITE_LDS)) continue;
-        if (!LDSJ && n <= HERMITE_LDS) continue;
+        if (LDSJ && !REG && (n <= HARD_SEQ_MAX || n > lds_max)) continue;
+        if (!LDSJ && n <= lds_max) continue;
         JData J;
         if (LDSJ) {
             J = JData{s_x, s_v, s_m, s_ri, s_id, 3};
@@ -1218,7 +1219,7 @@ __global__ void k_reset_queue(int *next) {
 
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2,
-                hipEvent_t ev_fork, hipEvent_t ev_join) {
+                hipEvent_t ev_fork, hipEvent_t ev_join, int wg_per_cu, int lds_max) {
     if (ncl <= 0) return GPLUM_OK;
     HP P = make_hp(p);
     HardView H = hard_view(hb);
@@ -1229,7 +1230,8 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
     GPLUM_ENSURE(hb.npair, ncl);
     hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, hb.next.p);
     const int pers = persistent ? 1 : 0;
-    const int grid = persistent ? std::min(ncl, std::max(1, n_cu) * 8) : ncl;
+    lds_max = std::min(HERMITE_LDS, std::max(HARD_SEQ_MAX, lds_max));
+    const int grid = persistent ? std::min(ncl, std::max(1, n_cu) * std::max(1, wg_per_cu)) : ncl;
     // clusters above HARD_SEQ_MAX on a second stream (when given), concurrently with
     // the register-path kernel
     const bool big = hb.max_size > HARD_SEQ_MAX;
@@ -1238,14 +1240,14 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
         GPLUM_HIP(hipEventRecord(ev_fork, st));
         GPLUM_HIP(hipStreamWaitEvent(sb, ev_fork, 0));
     }
-    if (hb.max_size > HERMITE_LDS)
+    if (hb.max_size > lds_max)
         hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
-                           ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
+                           ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max);
     if (big)
         hipLaunchKernelGGL((k_hermite<true, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
-                           ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
+                           ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max);
     hipLaunchKernelGGL((k_hermite<true, true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order, ncl,
-                       hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers);
+                       hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max);
     if (sb != st) {
         GPLUM_HIP(hipEventRecord(ev_join, sb));
         GPLUM_HIP(hipStreamWaitEvent(st, ev_join, 0));
@@ -1290,13 +1292,13 @@ int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, c
     HardView H = hard_view(hb);
     if (n_c <= HARD_SEQ_MAX)
         hipLaunchKernelGGL((k_hermite<true, true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
+                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS);
     else if (n_c <= HERMITE_LDS)
         hipLaunchKernelGGL((k_hermite<true, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
+                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS);
     else
         hipLaunchKernelGGL((k_hermite<false, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr,
-                           1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0);
+                           1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

@@ -49,10 +49,13 @@ struct IsoArgs {
 // gives the processing order (largest first).  persistent: one grid pulling
 // clusters from an atomic queue; otherwise one workgroup per cluster.
 // st2/ev_fork/ev_join (optional): clusters above HARD_SEQ_MAX run on st2 concurrently
-// with the small-cluster kernel on st, and st waits for st2.
+// with the small-cluster kernel on st, and st waits for st2.  wg_per_cu: persistent
+// workgroups per CU; lds_max (<= HERMITE_LDS): the largest cluster whose j-data is
+// staged in LDS (larger ones read it from global memory).  Neither changes results.
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2 = nullptr,
-                hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr);
+                hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr, int wg_per_cu = 8,
+                int lds_max = HERMITE_LDS);
 // Sun + hard acc/jerk and the initial block step of every cluster particle (as the
 // initial force correction leaves them), for callers that bring bare clusters.
 int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off, hipStream_t st);

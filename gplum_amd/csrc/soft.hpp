@@ -177,9 +177,12 @@ struct SinkEngine {                   // engine particle arrays (assign, not acc
 // candidate-only pass that refills overflowed segments.
 enum SoftMode : int { EP = 1, SPQ = 2, SPM = 4, EMIT = 8, NOFAR = 16 };
 
+// tile: j-particles staged in LDS per pass (a multiple of 64 up to SOFT_TILE; the
+// autotuner's choice).  The FP32 partial sums restart at tile boundaries, so the tile
+// width moves results within the FP32 tolerance; neighbour flags do not change.
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f32(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,
-                     float eps2, CandBuf cand, hipStream_t st);
+                     float eps2, CandBuf cand, hipStream_t st, int tile = SOFT_TILE);
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f64(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,
                      double eps2, CandBuf cand, hipStream_t st);

@@ -166,7 +166,7 @@ __device__ inline void epsp_pair(const IState &I, float4 pj, float4 lj, float4 q
 template <class IS, class JS, class SS, class OUT, int MODE>
 __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__restrict__ groups,
                                                          int ngroups, IS is, JS js, JS jf, SS ss,
-                                                         OUT out, float eps2, CandBuf cand) {
+                                                         OUT out, float eps2, CandBuf cand, int tile) {
     const int g = xcd_group(ngroups);
     if (g >= ngroups) return;
     const SoftGroup G = groups[g];
@@ -231,8 +231,8 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
 
     if (MODE & EP) {
         // near list: predicate + clamp + candidates
-        for (int base = 0; base < G.nj; base += SOFT_TILE) {
-            const int cnt = min(SOFT_TILE, G.nj - base);
+        for (int base = 0; base < G.nj; base += tile) {
+            const int cnt = min(tile, G.nj - base);
             __syncthreads();
             if (t < cnt) stage_j(js, G.j_begin + base + t);
             __syncthreads();
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
             }
         }
         // far list (engine only): beyond 1.1 max(r_search) of the whole group
-        for (int base = 0; base < ((MODE & NOFAR) ? 0 : G.njf); base += SOFT_TILE) {
-            const int cnt = min(SOFT_TILE, G.njf - base);
+        for (int base = 0; base < ((MODE & NOFAR) ? 0 : G.njf); base += tile) {
+            const int cnt = min(tile, G.njf - base);
             __syncthreads();
             if (t < cnt) stage_j(jf, G.jf_begin + base + t);
             __syncthreads();
@@ -265,8 +265,8 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
     }
 
     if (MODE & (SPQ | SPM)) {
-        for (int base = 0; base < G.ns; base += SOFT_TILE) {
-            const int cnt = min(SOFT_TILE, G.ns - base);
+        for (int base = 0; base < G.ns; base += tile) {
+            const int cnt = min(tile, G.ns - base);
             __syncthreads();
             if (t < cnt) {
                 double m, x, y, z, q[6];
@@ -491,10 +491,11 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f64(const SoftGroup *__rest
 
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f32(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,
-                     float eps2, CandBuf cand, hipStream_t st) {
+                     float eps2, CandBuf cand, hipStream_t st, int tile) {
     if (ngroups <= 0) return;
+    tile = std::min(SOFT_TILE, std::max(64, tile & ~63));
     hipLaunchKernelGGL((k_soft_f32<IS, JS, SS, OUT, MODE>), dim3(xcd_grid(ngroups)), dim3(SOFT_BLOCK), 0,
-                       st, d_groups, ngroups, is, js, jf, ss, out, eps2, cand);
+                       st, d_groups, ngroups, is, js, jf, ss, out, eps2, cand, tile);
 }
 template <class IS, class JS, class SS, class OUT, int MODE>
 void launch_soft_f64(const SoftGroup *d_groups, int ngroups, IS is, JS js, JS jf, SS ss, OUT out,

@@ -386,10 +386,10 @@ int System::soft_force(bool initial, bool use_f64) {
         tick(EV_FORCE0);
         if (p.quadrupole)
             launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(groups.p, ng, is, jn, jf, sp,
-                                                                                  out, (float)p.eps2, cb, st);
+                                                                                  out, (float)p.eps2, cb, st, soft_tile);
         else
             launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(groups.p, ng, is, jn, jf, sp,
-                                                                                  out, (float)p.eps2, cb, st);
+                                                                                  out, (float)p.eps2, cb, st, soft_tile);
         tick(EV_FORCE1);
         GPLUM_HIP(hipGetLastError());
         tick(EV_FCB0);
@@ -405,7 +405,7 @@ int System::soft_force(bool initial, bool use_f64) {
             CandBuf cb2{cand.p, cand_count.p, (unsigned)ovf_total};
             cb2.base = (unsigned)stored;
             launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkDiscard, EP | EMIT | NOFAR>(
-                fc.ovf_groups.p, (int)n_ovf, is, jn, jf, sp, SinkDiscard{}, (float)p.eps2, cb2, st);
+                fc.ovf_groups.p, (int)n_ovf, is, jn, jf, sp, SinkDiscard{}, (float)p.eps2, cb2, st, soft_tile);
             unsigned got = 0;
             GPLUM_HIP(hipMemcpyAsync(&got, cand_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
             GPLUM_HIP(hipStreamSynchronize(st));
@@ -434,10 +434,10 @@ int System::soft_force(bool initial, bool use_f64) {
             if (!use_f64) {
                 if (p.quadrupole)
                     launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(
-                        groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
+                        groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st, soft_tile);
                 else
                     launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPM | EMIT>(
-                        groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st);
+                        groups.p, ng, is, jn, jf, sp, out, (float)p.eps2, cb, st, soft_tile);
             } else {
                 if (p.quadrupole)
                     launch_soft_f64<SrcRecI, SrcRecJ, SrcNodeSP, SinkEngine, EP | SPQ | EMIT>(
@@ -464,7 +464,7 @@ int System::soft_force(bool initial, bool use_f64) {
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
     tick(EV_NB1);
     stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
-    if (fc_on && !fc.uniform) {
+    if (fc_on && !fc.uniform && !tuning) {
         // train on this call's exact FP32 candidate counts
         tick(EV_FCC0);
         if ((rc = fc_train(fc, nb.number_fp32.p, n, st))) return rc;
@@ -561,7 +561,8 @@ int System::hard_part(bool persistent) {
         PartPtrs P = ptrs(A);
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
         if (persistent) {
-            if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join)))
+            if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join,
+                                  hard_wg_per_cu, hard_lds_max)))
                 return rc;
         } else {
             // the comparator: one launch per cluster
@@ -731,6 +732,93 @@ int System::merge_and_recompute() {
     return soft_force(true, false);
 }
 
+// One timed run of the batched Hermite kernels on the current clusters (gathered from
+// the system, integrated in the gathered copy, not scattered back): the system's state
+// is unchanged.
+int System::hard_trial(int wg_per_cu, int lds_max, double *ms) {
+    int rc;
+    *ms = 0.;
+    if ((rc = make_clusters(A, n, nb, cl, scratch, st))) return rc;
+    const int m = cl.m, ncl = cl.ncl;
+    if (ncl == 0) return GPLUM_OK;
+    if ((rc = hb.ensure(m))) return rc;
+    hb.max_size = cl.max_size;
+    GPLUM_ENSURE(time_c, m);
+    HardPtrs H = hard_ptrs();
+    PartPtrs P = ptrs(A);
+    hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
+    tick(EV_HARD0);
+    if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join, wg_per_cu,
+                          lds_max)))
+        return rc;
+    tick(EV_HARD1);
+    *ms = elapsed(EV_HARD0, EV_HARD1);
+    GPLUM_HIP(hipStreamSynchronize(st));
+    return GPLUM_OK;
+}
+
+// The project's autotuner: times the soft stage (tree, FP32 force, neighbour finalize)
+// for each i-group size x j-tile width and the Hermite kernels for each persistent grid
+// x LDS binning threshold on the system's current state, best of `trials` runs each,
+// and keeps the fastest pair of each stage.  The i-group size changes which tree nodes
+// are opened (the tree approximation, as the reference's n_group_limit does) and the
+// tile width the FP32 partial sums; neighbour lists are decided in FP64 and do not
+// change, and the FP32 force stays within its tolerance of the FP64 force on the same
+// lists.  The hard knobs change no result.  The soft force is recomputed with
+// the chosen shape at the end, so the state is that of a soft force with the choice.
+int System::autotune(int trials, gplum_tuning *out) {
+    if (!have_nbr) return fail(GPLUM_E_STATE, "autotune needs an initialised system");
+    trials = std::max(1, trials);
+    const gplum_stats saved = stats;
+    const bool timing_saved = timing;
+    timing = true;
+    tuning = true;
+    int rc = GPLUM_OK;
+    const int groups_c[3] = {64, 128, 256}, tiles_c[3] = {64, 128, 256};
+    const int wg_c[4] = {2, 4, 8, 16}, lds_c[2] = {128, 256};
+    double best_soft = 1e300, best_hard = 1e300;
+    int b_group = p.n_group_limit, b_tile = soft_tile, b_wg = hard_wg_per_cu, b_lds = hard_lds_max;
+    for (int gi = 0; gi < 3 && !rc; gi++)
+        for (int ti = 0; ti < 3 && !rc; ti++) {
+            p.n_group_limit = groups_c[gi];
+            soft_tile = tiles_c[ti];
+            double t = 1e300;
+            for (int r = 0; r < trials && !rc; r++) {
+                if ((rc = soft_force(false, false))) break;
+                t = std::min(t, elapsed(EV_TREE0, EV_NB1));
+            }
+            if (!rc && t < best_soft) { best_soft = t; b_group = groups_c[gi]; b_tile = tiles_c[ti]; }
+        }
+    for (int wi = 0; wi < 4 && !rc; wi++)
+        for (int li = 0; li < 2 && !rc; li++) {
+            double t = 1e300;
+            for (int r = 0; r < trials && !rc; r++) {
+                double ms = 0.;
+                if ((rc = hard_trial(wg_c[wi], lds_c[li], &ms))) break;
+                t = std::min(t, ms);
+            }
+            if (!rc && t < best_hard) { best_hard = t; b_wg = wg_c[wi]; b_lds = lds_c[li]; }
+        }
+    p.n_group_limit = b_group;
+    soft_tile = b_tile;
+    hard_wg_per_cu = b_wg;
+    hard_lds_max = b_lds;
+    if (!rc) rc = soft_force(false, false);
+    tuning = false;
+    timing = timing_saved;
+    stats = saved;
+    if (rc) return rc;
+    if (out) {
+        out->n_group_limit = b_group;
+        out->soft_tile = b_tile;
+        out->hard_wg_per_cu = b_wg;
+        out->hard_lds_max = b_lds;
+        out->soft_ms = best_soft;
+        out->hard_ms = best_hard < 1e300 ? best_hard : 0.;
+    }
+    return GPLUM_OK;
+}
+
 int System::step(bool persistent) {
     int rc;
     if ((rc = kick())) return rc;
@@ -864,6 +952,33 @@ int gplum_system_reset_stats(gplum_system *s, int timing) {
     System *S = reinterpret_cast<System *>(s);
     std::memset(&S->stats, 0, sizeof(S->stats));
     S->timing = timing != 0;
+    return GPLUM_OK;
+}
+
+int gplum_system_autotune(gplum_system *s, int trials, gplum_tuning *out) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    return reinterpret_cast<System *>(s)->autotune(trials, out);
+}
+
+int gplum_system_get_tuning(gplum_system *s, gplum_tuning *out) {
+    if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
+    System *S = reinterpret_cast<System *>(s);
+    *out = gplum_tuning{S->p.n_group_limit, S->soft_tile, S->hard_wg_per_cu, S->hard_lds_max, 0., 0.};
+    return GPLUM_OK;
+}
+
+int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t) {
+    if (!s || !t) return gplum::fail(GPLUM_E_ARG, "null argument");
+    if (t->n_group_limit < 1 || t->n_group_limit > 256) return gplum::fail(GPLUM_E_ARG, "n_group_limit must be 1..256");
+    if (t->soft_tile < 64 || t->soft_tile > 256 || t->soft_tile % 64)
+        return gplum::fail(GPLUM_E_ARG, "soft_tile must be 64, 128, 192 or 256");
+    if (t->hard_wg_per_cu < 1 || t->hard_wg_per_cu > 64) return gplum::fail(GPLUM_E_ARG, "hard_wg_per_cu must be 1..64");
+    if (t->hard_lds_max < 64 || t->hard_lds_max > 256) return gplum::fail(GPLUM_E_ARG, "hard_lds_max must be 64..256");
+    System *S = reinterpret_cast<System *>(s);
+    S->p.n_group_limit = t->n_group_limit;
+    S->soft_tile = t->soft_tile;
+    S->hard_wg_per_cu = t->hard_wg_per_cu;
+    S->hard_lds_max = t->hard_lds_max;
     return GPLUM_OK;
 }
 

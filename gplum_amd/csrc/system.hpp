@@ -50,6 +50,11 @@ struct System {
     int n_col_tot = 0, step_ncol = 0;
     bool have_nbr = false, timing = false;
     bool plain_f64 = false, plain_per_cluster = false;   // comparison variant of step()
+    // launch shapes picked by autotune() (defaults until it runs)
+    int soft_tile = SOFT_TILE;          // j-particles staged per LDS pass of the soft kernel
+    int hard_wg_per_cu = 8;             // persistent Hermite workgroups per CU
+    int hard_lds_max = HERMITE_LDS;     // largest cluster whose j-data is staged in LDS
+    bool tuning = false;                // inside autotune(): no training, no statistics
     ParticleSoA A, B;
     RadiiScratch rad;
     MergeScratch mrg;
@@ -84,6 +89,8 @@ struct System {
     int hard_part(bool persistent);
     int merge_and_recompute();
     int step(bool persistent);
+    int hard_trial(int wg_per_cu, int lds_max, double *ms);
+    int autotune(int trials, gplum_tuning *out);
     PartPtrs ptrs(ParticleSoA &s);
     HardPtrs hard_ptrs();
     void tick(int k);

@@ -130,6 +130,16 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    """gplum_tuning (include/gplum_system.h)."""
+    _fields_ = [("n_group_limit", ctypes.c_int32), ("soft_tile", ctypes.c_int32),
+                ("hard_wg_per_cu", ctypes.c_int32), ("hard_lds_max", ctypes.c_int32),
+                ("soft_ms", ctypes.c_double), ("hard_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class System:
     """Device-resident P3T system (include/gplum_system.h)."""
 
@@ -211,6 +221,22 @@ class System:
 
     def set_variant(self, fp64_soft=False, per_cluster_hard=False):
         check(lib().gplum_system_set_variant(self.h, int(fp64_soft), int(per_cluster_hard)))
+
+    def autotune(self, trials=2):
+        """Pick the launch shapes (i-group size, soft j-tile, Hermite grid and LDS
+        binning threshold) by timing them on the current state; returns the choice."""
+        t = Tuning()
+        check(lib().gplum_system_autotune(self.h, int(trials), ctypes.byref(t)))
+        return t.as_dict()
+
+    def get_tuning(self):
+        t = Tuning()
+        check(lib().gplum_system_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, n_group_limit=256, soft_tile=256, hard_wg_per_cu=8, hard_lds_max=256):
+        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, 0.0, 0.0)
+        check(lib().gplum_system_set_tuning(self.h, ctypes.byref(t)))
 
     FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}
 

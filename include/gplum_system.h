@@ -112,6 +112,27 @@ int gplum_system_reset_stats(gplum_system *s, int timing);
  * Both 0 (default) is the production path. */
 int gplum_system_set_variant(gplum_system *s, int fp64_soft, int per_cluster_hard);
 
+/* Launch shapes of the device system (no counterpart in the reference, whose i-group
+ * size is the n_group_limit parameter of src/read.h:104).  gplum_system_autotune times,
+ * on the system's current state and best of `trials` runs each, the soft stage (tree,
+ * FP32 force, neighbour finalize) for n_group_limit in {64, 128, 256} x soft_tile in
+ * {64, 128, 256}, and the Hermite kernels for hard_wg_per_cu in {2, 4, 8, 16} x
+ * hard_lds_max in {128, 256}, keeps the fastest of each stage and recomputes the soft
+ * force with it.  n_group_limit changes which tree nodes are opened (the tree
+ * approximation, as the reference parameter does) and soft_tile the FP32 partial sums;
+ * neighbour lists, clusters and every hard-part result are unchanged.  get/set read or apply a tuning (e.g. one cached from an earlier run);
+ * soft_ms / hard_ms report the autotuner's measurements (0 from get). */
+typedef struct gplum_tuning {
+    int32_t n_group_limit;   /* i-group size of the tree walk                        */
+    int32_t soft_tile;       /* j-particles per LDS pass of the FP32 soft kernel      */
+    int32_t hard_wg_per_cu;  /* persistent Hermite workgroups per CU                  */
+    int32_t hard_lds_max;    /* clusters up to this size stage their j-data in LDS    */
+    double soft_ms, hard_ms; /* best measured soft stage / Hermite kernels, ms        */
+} gplum_tuning;
+int gplum_system_autotune(gplum_system *s, int trials, gplum_tuning *out);
+int gplum_system_get_tuning(gplum_system *s, gplum_tuning *out);
+int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t);
+
 /* Learned candidate-buffer sizing (no counterpart in the reference, whose CPU lists
  * grow as std::vectors; see include/gplum_forecast.h).  mode 1 trains the forecaster
  * network on every soft-force call (train_steps Adam steps of batch samples,
