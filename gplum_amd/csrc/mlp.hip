@@ -3,8 +3,8 @@
 //
 // Parameter layout (float32, 4545 values): W1[64][4], b1[64], W2[64][64], b2[64],
 // w3[64], b3.  Loss: mean Poisson NLL  lambda - y log(lambda + 1e-6).
-// Forward/backward: one thread per sample, parameters staged in LDS (18 KB) and read
-// as wave-wide broadcasts.  Gradient: per-sample activations and deltas are stored
+// Forward/backward: one thread per sample, parameters read with wave-uniform addresses
+// (scalar loads, no per-lane traffic).  Gradient: per-sample activations and deltas are stored
 // sample-major ([n][64]); one lane per parameter sums its products over a chunk of
 // samples in FP64 (lanes of a wave read one contiguous 256-byte row per sample), and a
 // second pass adds the chunk partials in chunk order, so the gradient is deterministic.
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const float *__restrict__ 
     if (threadIdx.x == 0) loss_part[blockIdx.x] = sl[0];
 }
 
-constexpr int MLP_CHUNK = 1024;   // samples per partial sum
+constexpr int MLP_CHUNK = 128;    // samples per partial sum (fixed: the sum order)
 
 // grid (ceil(NPARAM/64), nchunk), one wave per block: lane = parameter, y = sample chunk.
 // x is feature-major [4][n] (the kernels' input layout), the rest sample-major.
@@ -117,20 +117,26 @@ __global__ __launch_bounds__(64) void k_mlp_grad_part(const float *__restrict__ 
     double acc = 0.;
     if (k < OFF_B1) {
         const int p = k / MLP_NIN, q = k % MLP_NIN;
+#pragma unroll 4
         for (int s = s0; s < s1; s++) acc += (double)D1[(size_t)s * MLP_NH + p] * (double)x[(size_t)q * n + s];
     } else if (k < OFF_W2) {
         const int p = k - OFF_B1;
+#pragma unroll 4
         for (int s = s0; s < s1; s++) acc += (double)D1[(size_t)s * MLP_NH + p];
     } else if (k < OFF_B2) {
         const int p = (k - OFF_W2) / MLP_NH, q = (k - OFF_W2) % MLP_NH;
+#pragma unroll 4
         for (int s = s0; s < s1; s++) acc += (double)D2[(size_t)s * MLP_NH + p] * (double)H1[(size_t)s * MLP_NH + q];
     } else if (k < OFF_W3) {
         const int p = k - OFF_B2;
+#pragma unroll 4
         for (int s = s0; s < s1; s++) acc += (double)D2[(size_t)s * MLP_NH + p];
     } else if (k < OFF_B3) {
         const int p = k - OFF_W3;
+#pragma unroll 4
         for (int s = s0; s < s1; s++) acc += (double)DZ[s] * (double)H2[(size_t)s * MLP_NH + p];
     } else {
+#pragma unroll 4
         for (int s = s0; s < s1; s++) acc += (double)DZ[s];
     }
     part[(size_t)blockIdx.y * MLP_NPARAM + k] = acc;

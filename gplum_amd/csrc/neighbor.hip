@@ -280,12 +280,25 @@ __global__ void k_order_by_id(int m, const int32_t *__restrict__ flag, const int
     index_out[b0 + r] = ia;
 }
 
-__global__ void k_cluster_sizes(int ncl, const int32_t *__restrict__ off, int32_t *__restrict__ key,
-                                int32_t *__restrict__ idx) {
+// Processing order of the clusters: the costliest first, the cost estimated as block
+// steps (dt_tree over the smallest current step of the members) times (4 + size), so that
+// the cluster with the longest chain of block steps starts at once rather than after the
+// bulk of the small ones.  Also the largest size (for the size-class launches).
+__global__ void k_cluster_cost(int ncl, const int32_t *__restrict__ off, const int32_t *__restrict__ index,
+                               const double *__restrict__ dt, double dt_tree, int32_t *__restrict__ key,
+                               int32_t *__restrict__ idx, int32_t *__restrict__ max_size) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncl) return;
-    key[c] = -(off[c + 1] - off[c]);    // ascending sort of -size = largest first
+    const int b = off[c], e = off[c + 1], sz = e - b;
+    double dmin = dt_tree;
+    for (int a = b; a < e; a++) {
+        const double d = dt[index[a]];
+        if (d > 0. && d < dmin) dmin = d;
+    }
+    const double cost = (dt_tree / dmin) * (4.0 + sz);
+    key[c] = -(int32_t)fmin(2.0e9, 64.0 * log2(cost));   // ascending sort: costliest first
     idx[c] = c;
+    atomicMax(max_size, sz);
 }
 
 // ------------------------------------------------------------------ host
@@ -367,7 +380,8 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
     return GPLUM_OK;
 }
 
-int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch &s, hipStream_t st) {
+int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch &s, double dt_tree,
+                  hipStream_t st) {
     GPLUM_ENSURE(cl.parent, n);
     GPLUM_ENSURE(cl.ckey, n);
     GPLUM_ENSURE(cl.cflag, n);
@@ -411,23 +425,25 @@ int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch 
     hipLaunchKernelGGL(k_order_by_id, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.sflag.p, cl.sscan.p,
                        cl.off.p, cl.index.p, ps.id.p, cl.index_tmp.p);
     cl.index.swap(cl.index_tmp);
-    // processing order: largest cluster first
+    // processing order: costliest cluster first
     GPLUM_ENSURE(cl.size_key, ncl);
     GPLUM_ENSURE(cl.size_key_sorted, ncl);
     GPLUM_ENSURE(cl.order_in, ncl);
     GPLUM_ENSURE(cl.order, ncl);
-    hipLaunchKernelGGL(k_cluster_sizes, dim3(ceil_div(ncl, 256)), dim3(256), 0, st, ncl, cl.off.p,
-                       cl.size_key.p, cl.order_in.p);
+    GPLUM_ENSURE(cl.max_dev, 1);
+    GPLUM_HIP(hipMemsetAsync(cl.max_dev.p, 0, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_cluster_cost, dim3(ceil_div(ncl, 256)), dim3(256), 0, st, ncl, cl.off.p, cl.index.p,
+                       ps.dt.p, dt_tree, cl.size_key.p, cl.order_in.p, cl.max_dev.p);
     bytes = 0;
     GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, cl.size_key.p, cl.size_key_sorted.p,
                                                   cl.order_in.p, cl.order.p, ncl, 0, 32, st));
     GPLUM_ENSURE(s.tmp, bytes + 16);
     GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, bytes, cl.size_key.p, cl.size_key_sorted.p,
                                                   cl.order_in.p, cl.order.p, ncl, 0, 32, st));
-    int neg_max = 0;
-    GPLUM_HIP(hipMemcpyAsync(&neg_max, cl.size_key_sorted.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    int max_size = 0;
+    GPLUM_HIP(hipMemcpyAsync(&max_size, cl.max_dev.p, sizeof(int), hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
-    cl.max_size = -neg_max;
+    cl.max_size = max_size;
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

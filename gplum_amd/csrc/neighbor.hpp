@@ -28,7 +28,7 @@ struct NbState {
 struct ClusterList {
     DArray<int32_t> parent, cflag, cscan, idx_sel, index, index_tmp, sflag, sscan, off;
     DArray<uint64_t> ckey, key_sel, key_sorted;
-    DArray<int32_t> size_key, size_key_sorted, order_in, order;
+    DArray<int32_t> size_key, size_key_sorted, order_in, order, max_dev;
     int m = 0, ncl = 0, max_size = 0;
 };
 
@@ -43,6 +43,9 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
 // createNeighborCluster + makeList: id_cluster = min id of the component; clusters
 // of the particles with neighbours (members by ascending id), offsets cl.off[0..ncl],
 // processing order largest first.
-int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch &s, hipStream_t st);
+// Clusters (connected components of the neighbour graph), members by ascending id, and
+// their processing order for the Hermite kernels (estimated cost, costliest first).
+int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch &s, double dt_tree,
+                  hipStream_t st);
 
 }  // namespace gplum

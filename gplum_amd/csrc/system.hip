@@ -464,7 +464,7 @@ int System::soft_force(bool initial, bool use_f64) {
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
     tick(EV_NB1);
     stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
-    if (fc_on && !fc.uniform && !tuning) {
+    if (fc_on && !fc.uniform && !tuning && !skip_train) {
         // train on this call's exact FP32 candidate counts
         tick(EV_FCC0);
         if ((rc = fc_train(fc, nb.number_fp32.p, n, st))) return rc;
@@ -546,7 +546,7 @@ int System::hard_part(bool persistent) {
     int rc;
     if (!have_nbr) return fail(GPLUM_E_STATE, "neighbour graph missing: initialize first");
     tick(EV_HARD0);
-    if ((rc = make_clusters(A, n, nb, cl, scratch, st))) return rc;
+    if ((rc = make_clusters(A, n, nb, cl, scratch, p.dt_tree, st))) return rc;
     const int m = cl.m, ncl = cl.ncl;
     stats.n_clusters = ncl;
     stats.n_cluster_particles = m;
@@ -729,7 +729,12 @@ int System::merge_and_recompute() {
     hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.keep.p, M.kscan.p, M.perm.p);
     if ((rc = gather_all(M.perm.p, m))) return rc;
     if ((rc = set_radii())) return rc;
-    return soft_force(true, false);
+    // the recomputed force repeats the step's neighbour counts for all but the merged
+    // bodies: the forecaster plans with it but does not train on it again
+    skip_train = true;
+    rc = soft_force(true, false);
+    skip_train = false;
+    return rc;
 }
 
 // One timed run of the batched Hermite kernels on the current clusters (gathered from
@@ -738,7 +743,7 @@ int System::merge_and_recompute() {
 int System::hard_trial(int wg_per_cu, int lds_max, double *ms) {
     int rc;
     *ms = 0.;
-    if ((rc = make_clusters(A, n, nb, cl, scratch, st))) return rc;
+    if ((rc = make_clusters(A, n, nb, cl, scratch, p.dt_tree, st))) return rc;
     const int m = cl.m, ncl = cl.ncl;
     if (ncl == 0) return GPLUM_OK;
     if ((rc = hb.ensure(m))) return rc;

@@ -55,6 +55,7 @@ struct System {
     int hard_wg_per_cu = 8;             // persistent Hermite workgroups per CU
     int hard_lds_max = HERMITE_LDS;     // largest cluster whose j-data is staged in LDS
     bool tuning = false;                // inside autotune(): no training, no statistics
+    bool skip_train = false;            // the force recomputed after mergers: no training
     ParticleSoA A, B;
     RadiiScratch rad;
     MergeScratch mrg;
