@@ -4,7 +4,8 @@
 // the ones stated in oracle/orc_tree.c, implemented here level-synchronously:
 //   keys    : 63-bit Morton keys of the particle bbox cube, radix-sorted (stable, so
 //             ties keep the previous order);
-//   nodes   : breadth-first, one kernel pair per level (child count, child write);
+//   nodes   : breadth-first, one kernel triple per level (child count, scan, child
+//             write) with the level ranges kept on the device;
 //   moments : bottom-up per level, one thread per node, FP64, FDPS MomentQuadrupole
 //             accumulation order (src/particle.h:1649-1715);
 //   groups  : maximal nodes with <= n_group_limit particles, in particle order;
@@ -137,54 +138,168 @@ __global__ void k_root(int n, int32_t *start, int32_t *count, int32_t *level, in
     start[0] = 0; count[0] = n; level[0] = 0; parent[0] = -1;
 }
 
-__global__ void k_count_children(const uint64_t *__restrict__ key, int lb, int le, int n_leaf,
-                                 const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-                                 const int32_t *__restrict__ level, int32_t *__restrict__ nchild) {
-    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= le) return;
-    const int s = start[c], cnt = count[c], lv = level[c];
-    int nc = 0;
-    if (cnt > n_leaf && lv < MAXLEVEL) {
-        int b = s;
-        const int e = s + cnt;
-        while (b < e) {
-            const int d = digit_at(key[b], lv);
-            int lo = b, hi = e;              // first index with digit > d
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (digit_at(key[mid], lv) <= d) lo = mid + 1; else hi = mid;
+// Level kernels: child count per node, then child write (children in digit order,
+// appended after the level).  The level's node range is read from lvl[L], lvl[L+1]
+// (the previous level's k_level_scan writes it), so the levels run back to back
+// without a host round trip.  LEVEL_GRID workgroups each own one contiguous chunk of
+// the level: the count kernel sums its chunk's children, k_level_scan turns the
+// chunk sums into chunk bases, and the write kernel scans its chunk in tiles.
+constexpr int LEVEL_GRID = 256, LEVEL_BLOCK = 256;
+constexpr int LVL_ERR = MAXLEVEL + 4;       // lvl[]: ranges 0..MAXLEVEL+2, flag, totals
+
+// First index in [s, e) whose digit at level lv is >= d, for d = 1..7 (bnd[0] = s,
+// bnd[8] = e): seven binary searches advanced in lockstep, so their loads are in flight
+// together (a node of the top levels spans up to all N keys).
+__device__ inline void digit_bounds(const uint64_t *__restrict__ key, int s, int e, int lv, int bnd[9]) {
+    int lo[7], hi[7];
+#pragma unroll
+    for (int d = 0; d < 7; d++) { lo[d] = s; hi[d] = e; }
+    for (int span = e - s; span > 0; span >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 7; d++) {
+            if (lo[d] < hi[d]) {
+                const int mid = (lo[d] + hi[d]) >> 1;
+                if (digit_at(key[mid], lv) <= d) lo[d] = mid + 1; else hi[d] = mid;
             }
-            nc++;
-            b = lo;
         }
     }
-    nchild[c - lb] = nc;
+    bnd[0] = s;
+#pragma unroll
+    for (int d = 0; d < 7; d++) bnd[d + 1] = lo[d];
+    bnd[8] = e;
 }
 
-__global__ void k_write_children(const uint64_t *__restrict__ key, int lb, int le, int next_begin,
-                                 const int32_t *__restrict__ scan, int32_t *__restrict__ start,
-                                 int32_t *__restrict__ count, int32_t *__restrict__ level,
-                                 int32_t *__restrict__ parent, int32_t *__restrict__ child,
-                                 int32_t *__restrict__ nchild_out, const int32_t *__restrict__ nchild) {
-    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= le) return;
-    const int nc = nchild[c - lb];
-    nchild_out[c] = nc;
-    child[c] = nc ? next_begin + scan[c - lb] : -1;
-    if (!nc) return;
-    const int s = start[c], e = s + count[c], lv = level[c];
-    int b = s, w = next_begin + scan[c - lb];
-    while (b < e) {
-        const int d = digit_at(key[b], lv);
-        int lo = b, hi = e;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (digit_at(key[mid], lv) <= d) lo = mid + 1; else hi = mid;
+__device__ inline void level_chunk(const int32_t *__restrict__ lvl, int L, int &lb, int &m, int &c0, int &c1) {
+    lb = lvl[L];
+    m = lvl[L + 1] - lb;
+    const int chunk = (m + LEVEL_GRID - 1) / LEVEL_GRID;
+    c0 = min(m, (int)blockIdx.x * chunk);
+    c1 = min(m, c0 + chunk);
+}
+
+// block-wide sum (LEVEL_BLOCK threads), result in every thread
+__device__ inline int block_sum(int v, int *s_w) {
+    int tot = 0;
+    (void)wave_excl_scan(v, &tot);
+    if (__lane_id() == 0) s_w[threadIdx.x >> 6] = tot;
+    __syncthreads();
+    int all = 0;
+    for (int w = 0; w < LEVEL_BLOCK / 64; w++) all += s_w[w];
+    __syncthreads();
+    return all;
+}
+
+__global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
+    const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, int n_leaf,
+    const int32_t *__restrict__ start, const int32_t *__restrict__ count, const int32_t *__restrict__ level,
+    int32_t *__restrict__ nchild, int32_t *__restrict__ part) {
+    __shared__ int s_w[LEVEL_BLOCK / 64];
+    int lb, m, c0, c1;
+    level_chunk(lvl, L, lb, m, c0, c1);
+    if (m <= 0) return;
+    int sum = 0;
+    for (int r = c0 + (int)threadIdx.x; r < c1; r += LEVEL_BLOCK) {
+        const int c = lb + r;
+        const int s = start[c], cnt = count[c], lv = level[c];
+        int nc = 0;
+        if (cnt > n_leaf && lv < MAXLEVEL) {
+            int bnd[9];
+            digit_bounds(key, s, s + cnt, lv, bnd);
+#pragma unroll
+            for (int d = 0; d < 8; d++) nc += bnd[d + 1] > bnd[d] ? 1 : 0;
         }
-        start[w] = b; count[w] = lo - b; level[w] = lv + 1; parent[w] = c;
-        w++;
-        b = lo;
+        nchild[r] = nc;
+        sum += nc;
     }
+    const int all = block_sum(sum, s_w);
+    if (threadIdx.x == 0) part[blockIdx.x] = all;
+}
+
+// one workgroup: exclusive scan of the LEVEL_GRID chunk sums (in place) and the next
+// level's range lvl[L+2] = lvl[L+1] + total (flag lvl[LVL_ERR] past cap)
+__global__ __launch_bounds__(LEVEL_GRID) void k_level_scan(int32_t *__restrict__ lvl, int L,
+                                                           int32_t *__restrict__ part, int cap) {
+    __shared__ int s_w[LEVEL_GRID / 64];
+    const int m = lvl[L + 1] - lvl[L];
+    const int t = threadIdx.x;
+    const int v = (m > 0) ? part[t] : 0;
+    int tot = 0;
+    const int ex = wave_excl_scan(v, &tot);
+    if (__lane_id() == 0) s_w[t >> 6] = tot;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int w = 0; w < LEVEL_GRID / 64; w++) {
+        if (w < (t >> 6)) base += s_w[w];
+        all += s_w[w];
+    }
+    part[t] = base + ex;
+    if (t == 0) {
+        const int next = lvl[L + 1] + all;
+        if (next > cap) { lvl[LVL_ERR] = 1; lvl[L + 2] = lvl[L + 1]; }
+        else lvl[L + 2] = next;
+    }
+}
+
+__global__ __launch_bounds__(LEVEL_BLOCK) void k_write_children_lv(
+    const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, const int32_t *__restrict__ part,
+    int32_t *__restrict__ start, int32_t *__restrict__ count, int32_t *__restrict__ level,
+    int32_t *__restrict__ parent, int32_t *__restrict__ child, int32_t *__restrict__ nchild_out,
+    const int32_t *__restrict__ nchild) {
+    __shared__ int s_w[LEVEL_BLOCK / 64];
+    int lb, m, c0, c1;
+    level_chunk(lvl, L, lb, m, c0, c1);
+    if (m <= 0 || lvl[LVL_ERR]) return;
+    const int next_begin = lvl[L + 1];
+    int carry = part[blockIdx.x];
+    for (int r0 = c0; r0 < c1; r0 += LEVEL_BLOCK) {      // tiles of the chunk, in order
+        const int r = r0 + (int)threadIdx.x;
+        const int nc = (r < c1) ? nchild[r] : 0;
+        int tot = 0;
+        const int ex = wave_excl_scan(nc, &tot);
+        if (__lane_id() == 0) s_w[threadIdx.x >> 6] = tot;
+        __syncthreads();
+        int base = carry, all = 0;
+        for (int w = 0; w < LEVEL_BLOCK / 64; w++) {
+            if (w < (int)(threadIdx.x >> 6)) base += s_w[w];
+            all += s_w[w];
+        }
+        __syncthreads();
+        carry += all;
+        if (r >= c1) continue;
+        const int c = lb + r;
+        nchild_out[c] = nc;
+        const int first = next_begin + base + ex;
+        child[c] = nc ? first : -1;
+        if (!nc) continue;
+        const int s = start[c], e = s + count[c], lv = level[c];
+        int bnd[9];
+        digit_bounds(key, s, e, lv, bnd);
+        int w = first;
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            if (bnd[d + 1] > bnd[d]) {          // non-empty children in digit order
+                start[w] = bnd[d]; count[w] = bnd[d + 1] - bnd[d]; level[w] = lv + 1; parent[w] = c;
+                w++;
+            }
+        }
+    }
+}
+
+// totals of the four interaction-list scans (last count + last offset), one read-back
+__global__ void k_list_totals(int ng, const int32_t *__restrict__ c0, const int32_t *__restrict__ o0,
+                              const int32_t *__restrict__ c1, const int32_t *__restrict__ o1,
+                              const int32_t *__restrict__ c2, const int32_t *__restrict__ o2,
+                              const int32_t *__restrict__ c3, const int32_t *__restrict__ o3,
+                              int32_t *__restrict__ out) {
+    if (threadIdx.x != 0) return;
+    out[0] = ng > 0 ? c0[ng - 1] + o0[ng - 1] : 0;
+    out[1] = ng > 0 ? c1[ng - 1] + o1[ng - 1] : 0;
+    out[2] = ng > 0 ? c2[ng - 1] + o2[ng - 1] : 0;
+    out[3] = ng > 0 ? c3[ng - 1] + o3[ng - 1] : 0;
+}
+
+__global__ void k_lvl_init(int32_t *lvl) {
+    if (threadIdx.x < LVL_ERR + 1) lvl[threadIdx.x] = (threadIdx.x == 1) ? 1 : 0;
 }
 
 __global__ void k_leaf_tail(int lb, int le, int32_t *nchild_out, int32_t *child) {
@@ -476,27 +591,29 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
     GPLUM_ENSURE(t.grp_flag, cap);   // reused as per-level child counts
     hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, n, t.node_start.p, t.node_count.p,
                        t.node_level.p, t.node_parent.p);
-    t.level_begin.assign(1, 0);
-    int lb = 0, le = 1;
-    while (le > lb) {
-        const int m = le - lb;
-        hipLaunchKernelGGL(k_count_children, dim3(ceil_div(m, 256)), dim3(256), 0, st, t.key.p, lb, le,
-                           p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p,
-                           t.grp_flag.p);
-        int rc = excl_scan(t, t.grp_flag.p, t.nchild_scan.p, m, st);
-        if (rc) return rc;
-        int total = 0;
-        if ((rc = scan_total(t.grp_flag.p, t.nchild_scan.p, m, &total, st))) return rc;
-        if (le + total > cap) return fail(GPLUM_E_OVERFLOW, "tree node capacity exceeded");
-        hipLaunchKernelGGL(k_write_children, dim3(ceil_div(m, 256)), dim3(256), 0, st, t.key.p, lb, le,
-                           le, t.nchild_scan.p, t.node_start.p, t.node_count.p, t.node_level.p,
-                           t.node_parent.p, t.node_child.p, t.node_nchild.p, t.grp_flag.p);
-        t.level_begin.push_back(le);
-        lb = le;
-        le = le + total;
+    // levels 0..MAXLEVEL back to back, ranges kept on the device (a level past the last
+    // one is empty and its kernels return at once); one read-back at the end
+    GPLUM_ENSURE(t.lvl, LVL_ERR + 5);
+    hipLaunchKernelGGL(k_lvl_init, dim3(1), dim3(64), 0, st, t.lvl.p);
+    GPLUM_ENSURE(t.lvl_part, LEVEL_GRID);
+    for (int L = 0; L <= MAXLEVEL; L++) {
+        hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
+                           p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
+                           t.lvl_part.p);
+        hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(LEVEL_GRID), 0, st, t.lvl.p, L, t.lvl_part.p, cap);
+        hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
+                           t.lvl_part.p, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
+                           t.node_child.p, t.node_nchild.p, t.grp_flag.p);
     }
+    std::vector<int32_t> lv(LVL_ERR + 1);
+    GPLUM_HIP(hipMemcpyAsync(lv.data(), t.lvl.p, sizeof(int32_t) * (LVL_ERR + 1), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    GPLUM_HIP(hipGetLastError());
+    if (lv[LVL_ERR]) return fail(GPLUM_E_OVERFLOW, "tree node capacity exceeded");
+    t.level_begin.assign(1, 0);
+    for (int L = 1; L <= MAXLEVEL + 2 && lv[L] > lv[L - 1]; L++) t.level_begin.push_back(lv[L]);
     t.n_level = (int)t.level_begin.size() - 1;
-    t.n_node = le;
+    t.n_node = t.level_begin.back();
     GPLUM_ENSURE(t.node_mass, t.n_node);
     GPLUM_ENSURE(t.node_com, 3 * t.n_node);
     GPLUM_ENSURE(t.node_quad, 6 * t.n_node);
@@ -566,15 +683,17 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
         if (attempt == 3) return fail(GPLUM_E_OVERFLOW, "tree walk queue overflow");
     }
     int rc;
-    int tot_ep = 0, tot_epf = 0, tot_sp = 0, tot_epl = 0;
     if ((rc = excl_scan(t, t.ep_cnt.p, t.ep_off.p, ng, st))) return rc;
-    if ((rc = scan_total(t.ep_cnt.p, t.ep_off.p, ng, &tot_ep, st))) return rc;
     if ((rc = excl_scan(t, t.epf_cnt.p, t.epf_off.p, ng, st))) return rc;
-    if ((rc = scan_total(t.epf_cnt.p, t.epf_off.p, ng, &tot_epf, st))) return rc;
     if ((rc = excl_scan(t, t.sp_cnt.p, t.sp_off.p, ng, st))) return rc;
-    if ((rc = scan_total(t.sp_cnt.p, t.sp_off.p, ng, &tot_sp, st))) return rc;
     if ((rc = excl_scan(t, t.epl_cnt.p, t.epl_off.p, ng, st))) return rc;
-    if ((rc = scan_total(t.epl_cnt.p, t.epl_off.p, ng, &tot_epl, st))) return rc;
+    GPLUM_ENSURE(t.lvl, LVL_ERR + 5);
+    hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, ng, t.ep_cnt.p, t.ep_off.p, t.epf_cnt.p,
+                       t.epf_off.p, t.sp_cnt.p, t.sp_off.p, t.epl_cnt.p, t.epl_off.p, t.lvl.p + LVL_ERR + 1);
+    int32_t tot[4] = {0, 0, 0, 0};
+    GPLUM_HIP(hipMemcpyAsync(tot, t.lvl.p + LVL_ERR + 1, sizeof(tot), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    const int tot_ep = tot[0], tot_epf = tot[1], tot_sp = tot[2], tot_epl = tot[3];
     t.n_ep_total = tot_ep;
     t.n_epf_total = tot_epf;
     t.n_sp_total = tot_sp;
