@@ -201,6 +201,107 @@ def bench_copy(args):
                       "unit": "GB/s", "bytes_copied": nbytes, "steps": args.steps, "ms_per_copy": round(dt * 1e3, 4)}))
 
 
+def neighbour_pairs(s):
+    """Sorted (id_i << 32 | id_j) of a system's neighbour graph (order-free comparison)."""
+    pp = s.particles()
+    off, idx = s.neighbors()
+    ids = pp["id"].astype(np.int64)
+    rows = np.repeat(np.arange(len(pp)), np.diff(off))
+    return np.sort((ids[rows] << 32) | ids[idx])
+
+
+def bench_large(args):
+    """Configs 4 and 5: the full step on a larger disk.  config4: N = 4M over 0.5-3 AU,
+    forecaster on; the neighbour graph after the timed steps is recomputed with the
+    forecaster off and must be identical.  config5: N = 16M over 0.3-5 AU plus 10
+    embryos (m = 1e-6, circular, evenly spaced in a); energy drift of the FP32 path and
+    of the FP64-soft path over the same steps from the same state."""
+    from gplum_amd import gpu, ic
+    big = args.workload == "config5"
+    n = args.n if args.n != 1_000_000 else (16_000_000 if big else 4_000_000)
+    a_in, a_out = (0.3, 5.0) if big else (0.5, 3.0)
+    p = gpu.params(**params_dict())
+    t0 = time.perf_counter()
+    pp = ic.disk(n, 1e-12, a_in, a_out, seed=args.seed)
+    if big:
+        pp = ic.add_embryos(pp, 10, 1e-6, a_in, a_out)
+    n_tot = len(pp)
+    print("# %s: initial conditions for N=%d in %.1f s" % (args.workload, n_tot, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    s = gpu.System(p, pp)
+    del pp
+    s.set_forecast("network")
+    t0 = time.perf_counter()
+    s.initialize()
+    print("# initialised in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    tuning = None if args.no_tune else s.autotune(trials=1)
+    print("# %s: N=%d, init + tune %.1f s, tuning %s" % (args.workload, n_tot, time.perf_counter() - t0, tuning),
+          file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        s.step(1)
+    start_state = s.particles()
+    start_time = s.stats()["time_sys"]
+    s.reset_stats(timing=True)
+    e0 = s.energy()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.step(1)
+        print("# step %d done" % k, file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    e1 = s.energy()
+    st = s.stats()
+    drift32 = (e1[0] - e0[0] - (e1[5] - e0[5])) / e0[0]
+    calls = max(1, st["fc_calls"])
+    hit = 1.0 - st["fc_ovf_groups"] / float(max(1, st["n_groups"]) * calls)
+    out = {
+        "metric": "particle-steps/s (full P3T step, %s)" % args.workload, "value": round(n_tot * args.steps / elapsed, 1),
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 soft force, f64 hard/correction", "data": "synthetic",
+        "config": {"workload": "%s: disk N=%d, m=1e-12, a=%g-%g AU%s, theta=0.5 quadrupole, dt_tree=2^-4, "
+                               "collisions on" % (args.workload, n, a_in, a_out,
+                                                   " + 10 embryos m=1e-6" if big else ""),
+                   "n": n_tot, "seed": args.seed},
+        "autotune": tuning,
+        "breakdown_ms_per_step": {k: round(st[k] / args.steps, 3) for k in ("force_ms", "tree_ms", "finalize_ms",
+                                                                             "hard_ms")},
+        "hard": {"clusters": st["n_clusters"], "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
+                 "max_block_steps_one_cluster": st["max_block_steps"], "collisions": st["n_collisions"]},
+        "forecast": {"calls": st["fc_calls"], "groups_per_call": st["n_groups"],
+                     "overflowed_groups": st["fc_ovf_groups"], "hit_rate": round(hit, 5),
+                     "ms_per_step": round(st["fc_ms"] / args.steps, 3)},
+        "energy_error_fp32": drift32,
+    }
+    if not big:
+        # the exact search stays the judge: the same state with the forecaster off
+        s.soft_force(initial=False)
+        a = neighbour_pairs(s)
+        s_off = gpu.System(p, s.particles(), time_sys=st["time_sys"])
+        s_off.set_tuning(**{k: tuning[k] for k in ("n_group_limit", "soft_tile", "hard_wg_per_cu",
+                                                    "hard_lds_max")} if tuning else {})
+        s_off.soft_force(initial=False)
+        b = neighbour_pairs(s_off)
+        out["neighbours_identical_forecast_on_off"] = bool(np.array_equal(a, b))
+        out["neighbour_pairs"] = int(len(a))
+        s_off.close()
+    else:
+        # the same steps with the FP64 soft force (persistent hard part: identical
+        # results to per-cluster launches) from the same state
+        s64 = gpu.System(p, start_state, time_sys=start_time)
+        s64.set_variant(fp64_soft=True, per_cluster_hard=False)
+        s64.initialize()
+        f0 = s64.energy()
+        for k in range(args.steps):
+            s64.step(1)
+            print("# FP64-soft step %d done" % k, file=sys.stderr, flush=True)
+        f1 = s64.energy()
+        out["energy_error_fp64_soft"] = (f1[0] - f0[0] - (f1[5] - f0[5])) / f0[0]
+        s64.close()
+    s.close()
+    out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,9 +312,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--workload", choices=["step", "hard", "copy"], default="step",
+    ap.add_argument("--workload", choices=["step", "hard", "copy", "config4", "config5"], default="step",
                     help="step: full P3T step at N (default); hard: batched hard part alone; "
-                         "copy: plain device copy of --copy-mib MiB (calibrates the HBM byte counters)")
+                         "copy: plain device copy of --copy-mib MiB (calibrates the HBM byte counters); "
+                         "config4: N=4M disk over 0.5-3 AU; config5: N=16M over 0.3-5 AU + 10 embryos")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--clusters", type=int, default=5000)
     ap.add_argument("--cpu-clusters", type=int, default=20)
@@ -231,6 +333,9 @@ def main():
         return
     if args.workload == "copy":
         bench_copy(args)
+        return
+    if args.workload in ("config4", "config5"):
+        bench_large(args)
         return
 
     from gplum_amd import gpu

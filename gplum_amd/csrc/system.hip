@@ -529,7 +529,7 @@ int System::initialize() {
     }
     if ((rc = set_radii())) return rc;
     for (int r = 0; r < 2; r++)
-        if ((rc = soft_force(true, false))) return rc;
+        if ((rc = soft_force(true, plain_f64))) return rc;
     if ((rc = energy(e_init))) return rc;
     return GPLUM_OK;
 }

@@ -25,11 +25,16 @@ def density_code(rho_cgs):
 
 
 def solve_kepler(l, e, iters=30):
-    """Newton iteration for u - e sin u = l (vectorised)."""
+    """Newton iteration for u - e sin u = l (vectorised); stops once the largest
+    update is at the rounding level of u (|u| <= 2 pi)."""
     u = l + e * np.sin(l)
     for _ in range(iters):
         f = u - e * np.sin(u) - l
-        u = u - f / (1.0 - e * np.cos(u))
+        un = u - f / (1.0 - e * np.cos(u))
+        done = np.max(np.abs(un - u)) <= 4e-15
+        u = un
+        if done:
+            break
     return u
 
 
