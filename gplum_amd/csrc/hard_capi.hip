@@ -235,7 +235,10 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     if ((rc = hard_upload(c, p, pp, n, offsets, n_cluster, (mode & 2) != 0, pl))) return rc;
     GPLUM_HIP(hipEventRecord(c.e0, c.st));
     if (mode & 1) {
-        if ((rc = hard_launch(*p, c.hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st, c.st2, c.fork, c.join)))
+        // mode bit 2: every cluster above HARD_SEQ_MAX particles on the whole-GPU path
+        const int giant_min = (mode & 4) ? HARD_SEQ_MAX : GIANT_MIN_DEFAULT;
+        if ((rc = hard_launch(*p, c.hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st, c.st2, c.fork, c.join, 8,
+                              HERMITE_LDS, giant_min)))
             return rc;
     } else {
         for (int k = 0; k < n_cluster; k++)
@@ -286,7 +289,10 @@ extern "C" int gplum_hard_integrate_graph(const gplum_params *p, gplum_particle 
     for (const Pair &s : state)
         if (!rc && hipMemcpyAsync(s.dst, s.src, s.bytes, hipMemcpyDeviceToDevice, c.st) != hipSuccess)
             rc = fail(GPLUM_E_HIP, "hipMemcpyAsync failed during graph capture");
-    if (!rc) rc = hard_launch(*p, hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st, c.st2, c.fork, c.join);
+    // (the whole-GPU path for giant clusters is host-driven and cannot be captured)
+    if (!rc)
+        rc = hard_launch(*p, hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st, c.st2, c.fork, c.join, 8,
+                         HERMITE_LDS, GIANT_OFF);
     const hipError_t ec = hipStreamEndCapture(c.st, &graph);
     if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
     GPLUM_HIP(ec);

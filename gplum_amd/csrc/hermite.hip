@@ -558,9 +558,8 @@ __device__ inline void wave_sync() {
 // j-data, so the waves need not wait for each other.  Twins of a merged target are
 // integrated alongside with their own cut-off radius, as in the per-lane path.
 __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
-                                    int flag_col, long long &pairs) {
+                                    int flag_col, long long &pairs, int wave, int n_wave) {
     const int lane = (int)(threadIdx.x & 63);
-    const int wave = (int)(threadIdx.x >> 6), n_wave = (int)(blockDim.x >> 6);
     const bool lead = lane == 0;
     int rank = 0;                          // active particles seen so far (wave-uniform)
     for (int i0 = 0; i0 < n; i0 += 64) {
@@ -659,6 +658,40 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
     }
 }
 
+// After the collision test found the pair (ic, jc) in a block step and the active
+// particles were corrected (hermite.h:289-296, 449-458): perfect merging, the full hard
+// force of everybody, the merged targets' acc/jerk, the initial block step of the live
+// particles and the twins' steps.  Every thread of one workgroup calls it.
+__device__ void collision_tail(const HP &P, HardView &H, int b, int n, int ic, int jc, double &ed,
+                               long long &pairs) {
+    const int t = threadIdx.x;
+    const int B = blockDim.x;
+    if (t == 0) ed += perfect_merge(P, H, b, n, ic, jc);
+    __threadfence_block();
+    __syncthreads();
+    for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
+    __threadfence_block();
+    __syncthreads();
+    for (int i = t; i < n; i += B)
+        if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
+    __threadfence_block();
+    __syncthreads();
+    for (int i = t; i < n; i += B) {
+        const int g = b + i;
+        if (!(H.flags[g] & GPLUM_FLAG_DEAD)) {
+            double ad[3], jd[3], as[3], js[3];
+            ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js);
+            H.dt[g] = dt_initial(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, as, js);
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int i = t; i < n; i += B)
+        if (H.twin_of[b + i] >= 0) H.dt[b + i] = H.dt[b + H.twin_of[b + i]];
+    __threadfence_block();
+    __syncthreads();
+}
+
 // Block steps of one cluster from (sys_time, loop) to the end of the soft step on the
 // global-memory state H (j-data through J), then the final calcGravity.  Every
 // thread of the workgroup calls it; collisions and twins handled here.
@@ -746,7 +779,7 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                                (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
             __threadfence_block();
             __syncthreads();
-            coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs);
+            coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6);
         } else
         for (int i = t; i < n; i += B) {
             const int g = b + i;
@@ -789,31 +822,8 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         __threadfence_block();
         __syncthreads();
         if (flag_col) {
-            if (t == 0) ed += perfect_merge(P, H, b, n, ic, jc);
+            collision_tail(P, H, b, n, ic, jc, ed, pairs);
             n_col++;
-            __threadfence_block();
-            __syncthreads();
-            for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
-            __threadfence_block();
-            __syncthreads();
-            for (int i = t; i < n; i += B)
-                if (H.flags[b + i] & GPLUM_FLAG_MERGED) merge_acc_jerk(H, b, n, i);
-            __threadfence_block();
-            __syncthreads();
-            for (int i = t; i < n; i += B) {
-                const int g = b + i;
-                if (!(H.flags[g] & GPLUM_FLAG_DEAD)) {
-                    double ad[3], jd[3], as[3], js[3];
-                    ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js);
-                    H.dt[g] = dt_initial(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, as, js);
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
-            for (int i = t; i < n; i += B)
-                if (H.twin_of[b + i] >= 0) H.dt[b + i] = H.dt[b + H.twin_of[b + i]];
-            __threadfence_block();
-            __syncthreads();
         }
         // getSystemTime
         double tm = 1.e10;
@@ -978,7 +988,7 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
                                                            int *__restrict__ next, double *__restrict__ edisp,
                                                            int *__restrict__ ncol, long long *__restrict__ nblock,
                                                            long long *__restrict__ npair, int persistent,
-                                                           int lds_max) {
+                                                           int lds_max, int giant_min) {
     __shared__ double s_x[3 * HERMITE_LDS], s_v[3 * HERMITE_LDS], s_m[HERMITE_LDS], s_ri[HERMITE_LDS];
     __shared__ int64_t s_id[HERMITE_LDS];
     __shared__ double s_fr[HERMITE_LDS];
@@ -1011,8 +1021,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         const int b = cl_off[c];
         const int n = cl_off[c + 1] - b;
         if (REG && n > HARD_SEQ_MAX) continue;
-        if (LDSJ && !REG && (n <= HARD_SEQ_MAX || n > lds_max)) continue;
-        if (!LDSJ && n <= lds_max) continue;
+        if (LDSJ && !REG && (n <= HARD_SEQ_MAX || n > lds_max || n > giant_min)) continue;
+        if (!LDSJ && (n <= lds_max || n > giant_min)) continue;
         JData J;
         if (LDSJ) {
             J = JData{s_x, s_v, s_m, s_ri, s_id, 3};
@@ -1211,6 +1221,259 @@ static HardView hard_view(HardBuffers &hb) {
                     hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p, hb.act.p};
 }
 
+
+// ------------------------------------------------------------ giant clusters
+//
+// Clusters above giant_min particles (the reference integrates clusters of 64 and more
+// with OpenMP inside the cluster, hard.h:583-658) are spread over the whole GPU: each
+// block step is a short sequence of kernels (predict all; collision test over all pairs
+// with per-workgroup minima; decision and active flags; the active particles, one wave
+// each, as in the four-wave kernels; the collision tail in one workgroup; next block time
+// and system time), launched in batches by the host, which checks a done flag between
+// batches.  Same functions, same summation orders: results identical to the size-class
+// kernels.
+struct GiantState {
+    double t_s, sys_time, ed;
+    long long loop, pairs;
+    int done, flag_col, ic, jc, n_col;
+};
+
+constexpr int GIANT_T = 256;          // threads per workgroup of the grid kernels
+constexpr int GIANT_RED = 1024;       // threads of the one-workgroup kernels
+
+__device__ inline void block_min2(double &u, double &v, double *s_u, double *s_v) {
+    const int t = threadIdx.x;
+    s_u[t] = u;
+    s_v[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            if (s_u[t + w] < s_u[t]) s_u[t] = s_u[t + w];
+            if (s_v[t + w] < s_v[t]) s_v[t] = s_v[t + w];
+        }
+        __syncthreads();
+    }
+    u = s_u[0];
+    v = s_v[0];
+    __syncthreads();
+}
+
+// initial calcGravity of all members with flags cleared (hermite.h:259-270)
+__global__ __launch_bounds__(GIANT_T) void k_g_init(HP P, HardView H, int b, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gravity_full(P, H, b, n, i);
+    H.flags[b + i] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+    H.twin_of[b + i] = -1;
+}
+
+// loop state and the first block time: t_s = min(time + dt), sys_time = min(time) over
+// the live members; done once sys_time reaches the end of the soft step or the cap
+__global__ __launch_bounds__(GIANT_RED) void k_g_times(HP P, HardView H, int b, int n, GiantState *G, int first) {
+    __shared__ double s_u[GIANT_RED], s_v[GIANT_RED];
+    if (!first && G->done) return;
+    double tl = 1.e10, tm = 1.e10;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) {
+            tl = dmin(tl, H.time[b + i] + H.dt[b + i]);
+            tm = dmin(tm, H.time[b + i]);
+        }
+    block_min2(tl, tm, s_u, s_v);
+    if (threadIdx.x != 0) return;
+    if (first) {
+        G->loop = 0;
+        G->ed = 0.;
+        G->n_col = 0;
+        G->pairs = (long long)n * (long long)(n - 1);
+        G->sys_time = 0.;
+    } else {
+        G->loop++;
+        G->sys_time = tm;
+    }
+    G->t_s = tl;
+    G->flag_col = 0;
+    G->done = (G->sys_time < P.dt_tree && G->loop < BLOCK_STEP_CAP) ? 0 : 1;
+}
+
+__global__ __launch_bounds__(GIANT_T) void k_g_predict(HardView H, int b, int n, const GiantState *G) {
+    if (G->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    predict(H, b + i, G->t_s - H.time[b + i]);
+}
+
+// collisionDetermination over the pairs (i, j) of this workgroup: i from blockIdx.x,
+// j from the chunk blockIdx.y; per-workgroup minimum in (R, i, j) order, which picks
+// the pair the sequential search (ascending i, then j, strict <) finds
+__device__ inline bool rij_less(double r, int i, int j, double r2, int i2, int j2) {
+    return r < r2 || (r == r2 && (i < i2 || (i == i2 && j < j2)));
+}
+
+__global__ __launch_bounds__(GIANT_T) void k_g_collide(HardView H, int b, int n, const GiantState *G,
+                                                       double *__restrict__ part_r, int *__restrict__ part_i,
+                                                       int *__restrict__ part_j) {
+    __shared__ double s_r[GIANT_T];
+    __shared__ int s_i[GIANT_T], s_j[GIANT_T];
+    if (G->done) return;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + t;
+    const int jchunk = (n + gridDim.y - 1) / gridDim.y;
+    const int j0 = blockIdx.y * jchunk, j1 = min(n, j0 + jchunk);
+    double rmin = 1.;
+    int bi = 0x7fffffff, bj = 0x7fffffff;
+    if (i < n && !(H.flags[b + i] & GPLUM_FLAG_DEAD)) {
+        const double xi = H.xp[3 * (b + i)], yi = H.xp[3 * (b + i) + 1], zi = H.xp[3 * (b + i) + 2];
+        const double fri = H.f[b + i] * H.r_planet[b + i];
+        for (int j = j0; j < j1; j++) {
+            if (i == j || (H.flags[b + j] & GPLUM_FLAG_DEAD)) continue;
+            const double dx = xi - H.xp[3 * (b + j)];
+            const double dy = yi - H.xp[3 * (b + j) + 1];
+            const double dz = zi - H.xp[3 * (b + j) + 2];
+            const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+            const double r2 = fri + H.f[b + j] * H.r_planet[b + j];
+            if (!(d2 < COL_FILTER * r2 * r2)) continue;
+            const double Rr = sqrt(d2) / r2;
+            if (Rr < rmin) { rmin = Rr; bi = i; bj = j; }
+        }
+    }
+    s_r[t] = rmin; s_i[t] = bi; s_j[t] = bj;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w && rij_less(s_r[t + w], s_i[t + w], s_j[t + w], s_r[t], s_i[t], s_j[t])) {
+            s_r[t] = s_r[t + w]; s_i[t] = s_i[t + w]; s_j[t] = s_j[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int k = blockIdx.y * gridDim.x + blockIdx.x;
+        part_r[k] = s_r[0]; part_i[k] = s_i[0]; part_j[k] = s_j[0];
+    }
+}
+
+// the pair with the smallest R (< 1: a collision, the lighter body is the impactor; all
+// live members are corrected to t_s), then the active flags of the step
+__global__ __launch_bounds__(GIANT_RED) void k_g_decide(HP P, HardView H, int b, int n, GiantState *G,
+                                                        const double *__restrict__ part_r,
+                                                        const int *__restrict__ part_i,
+                                                        const int *__restrict__ part_j, int nparts) {
+    __shared__ double s_r[GIANT_RED];
+    __shared__ int s_i[GIANT_RED], s_j[GIANT_RED];
+    __shared__ int s_col;
+    if (G->done) return;
+    const int t = threadIdx.x;
+    double r = 1.;
+    int bi = 0x7fffffff, bj = 0x7fffffff;
+    if (P.collision)
+        for (int k = t; k < nparts; k += blockDim.x)
+            if (rij_less(part_r[k], part_i[k], part_j[k], r, bi, bj)) { r = part_r[k]; bi = part_i[k]; bj = part_j[k]; }
+    s_r[t] = r; s_i[t] = bi; s_j[t] = bj;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w && rij_less(s_r[t + w], s_i[t + w], s_j[t + w], s_r[t], s_i[t], s_j[t])) {
+            s_r[t] = s_r[t + w]; s_i[t] = s_i[t + w]; s_j[t] = s_j[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        s_col = 0;
+        if (s_r[0] < 1.) {
+            const int gi = s_i[0], gj = s_j[0];
+            s_col = 1;
+            if (H.mass[b + gi] < H.mass[b + gj]) { G->ic = gi; G->jc = gj; } else { G->ic = gj; G->jc = gi; }
+        }
+        G->flag_col = s_col;
+    }
+    __syncthreads();
+    const double t_s = G->t_s;
+    const int col = s_col;
+    for (int i = t; i < n; i += blockDim.x) {
+        if (col) H.dt[b + i] = t_s - H.time[b + i];
+        H.act[b + i] = !(H.flags[b + i] & GPLUM_FLAG_DEAD) && (col || H.time[b + i] + H.dt[b + i] == t_s);
+    }
+}
+
+// the active members, one wave each (coop_correct_active over the whole grid)
+__global__ __launch_bounds__(GIANT_T) void k_g_correct(HP P, HardView H, int b, int n, GiantState *G) {
+    if (G->done) return;
+    const JData J{H.xp + 3 * b, H.vp + 3 * b, H.mass + b, H.rinv + b, H.id + b, 3};
+    long long pairs = 0;
+    const int waves = (int)(blockDim.x >> 6);
+    coop_correct_active(P, H, J, b, n, G->t_s, G->flag_col, pairs, (int)blockIdx.x * waves + (int)(threadIdx.x >> 6),
+                        (int)gridDim.x * waves);
+    if ((threadIdx.x & 63) == 0 && pairs) atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)pairs);
+}
+
+// perfect merging and its aftermath in one workgroup (rare)
+__global__ __launch_bounds__(GIANT_RED) void k_g_tail(HP P, HardView H, int b, int n, GiantState *G) {
+    if (G->done || !G->flag_col) return;
+    double ed = 0.;
+    long long pairs = 0;
+    collision_tail(P, H, b, n, G->ic, G->jc, ed, pairs);
+    if (pairs) atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)pairs);
+    if (threadIdx.x == 0) {
+        G->ed += ed;
+        G->n_col++;
+    }
+}
+
+// final calcGravity with potentials (hermite.h:466-470), then the merged targets
+__global__ __launch_bounds__(GIANT_T) void k_g_final(HP P, HardView H, int b, int n, GiantState *G) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gravity_full(P, H, b, n, i);
+    atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)(n - 1));
+}
+__global__ __launch_bounds__(GIANT_T) void k_g_final_merged(HardView H, int b, int n, const GiantState *G, int c,
+                                                            double *edisp, int *ncol, long long *nblock,
+                                                            long long *npair, double time_end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && G->n_col > 0 && (H.flags[b + i] & GPLUM_FLAG_MERGED)) merge_acc_jerk(H, b, n, i);
+    if (i == 0) {
+        edisp[c] = G->ed;
+        ncol[c] = G->n_col;
+        nblock[c] = (G->sys_time < time_end) ? -1 : G->loop;
+        npair[c] = G->pairs;
+    }
+}
+
+// Integrates cluster c = [b, b + n) with the kernels above; the host checks the done
+// flag every GIANT_BATCH block steps.
+constexpr int GIANT_BATCH = 16;
+static int hard_giant(const HP &P, HardView &H, HardBuffers &hb, int c, int b, int n, hipStream_t st) {
+    const int g_n = ceil_div(n, GIANT_T);
+    const int g_jc = std::max(1, std::min(64, ceil_div(n, 64)));   // j chunks of the collision test
+    const int g_w = std::min(4096, ceil_div(n, GIANT_T / 64));
+    GPLUM_ENSURE(hb.g_state, sizeof(GiantState));
+    GPLUM_ENSURE(hb.g_r, g_n * g_jc);
+    GPLUM_ENSURE(hb.g_i, g_n * g_jc);
+    GPLUM_ENSURE(hb.g_j, g_n * g_jc);
+    GiantState *G = reinterpret_cast<GiantState *>(hb.g_state.p);
+    hipLaunchKernelGGL(k_g_init, dim3(g_n), dim3(GIANT_T), 0, st, P, H, b, n);
+    hipLaunchKernelGGL(k_g_times, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G, 1);
+    for (;;) {
+        for (int s = 0; s < GIANT_BATCH; s++) {
+            hipLaunchKernelGGL(k_g_predict, dim3(g_n), dim3(GIANT_T), 0, st, H, b, n, G);
+            hipLaunchKernelGGL(k_g_collide, dim3(g_n, g_jc), dim3(GIANT_T), 0, st, H, b, n, G, hb.g_r.p, hb.g_i.p,
+                               hb.g_j.p);
+            hipLaunchKernelGGL(k_g_decide, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G, hb.g_r.p, hb.g_i.p,
+                               hb.g_j.p, g_n * g_jc);
+            hipLaunchKernelGGL(k_g_correct, dim3(g_w), dim3(GIANT_T), 0, st, P, H, b, n, G);
+            hipLaunchKernelGGL(k_g_tail, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G);
+            hipLaunchKernelGGL(k_g_times, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G, 0);
+        }
+        GPLUM_HIP(hipGetLastError());
+        int done = 0;
+        GPLUM_HIP(hipMemcpyAsync(&done, &G->done, sizeof(int), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        if (done) break;
+    }
+    hipLaunchKernelGGL(k_g_final, dim3(g_n), dim3(GIANT_T), 0, st, P, H, b, n, G);
+    hipLaunchKernelGGL(k_g_final_merged, dim3(g_n), dim3(GIANT_T), 0, st, H, b, n, G, c, hb.edisp.p, hb.ncol.p,
+                       hb.nblock.p, hb.npair.p, P.dt_tree);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
 // zeroes the three cluster-queue counters (a kernel, not a memset, so that the launch
 // sequence can be captured into a graph and replayed)
 __global__ void k_reset_queue(int *next) {
@@ -1219,7 +1482,7 @@ __global__ void k_reset_queue(int *next) {
 
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2,
-                hipEvent_t ev_fork, hipEvent_t ev_join, int wg_per_cu, int lds_max) {
+                hipEvent_t ev_fork, hipEvent_t ev_join, int wg_per_cu, int lds_max, int giant_min) {
     if (ncl <= 0) return GPLUM_OK;
     HP P = make_hp(p);
     HardView H = hard_view(hb);
@@ -1240,14 +1503,30 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
         GPLUM_HIP(hipEventRecord(ev_fork, st));
         GPLUM_HIP(hipStreamWaitEvent(sb, ev_fork, 0));
     }
+    giant_min = std::max(HARD_SEQ_MAX, giant_min);
     if (hb.max_size > lds_max)
         hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
-                           ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max);
+                           ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max,
+                           giant_min);
     if (big)
         hipLaunchKernelGGL((k_hermite<true, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
-                           ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max);
+                           ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max,
+                           giant_min);
     hipLaunchKernelGGL((k_hermite<true, true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order, ncl,
-                       hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max);
+                       hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max, giant_min);
+    if (hb.max_size > giant_min) {
+        // giant clusters: each over the whole GPU, one after the other on st (the
+        // size-class kernels above run meanwhile)
+        std::vector<int32_t> off(ncl + 1);
+        GPLUM_HIP(hipMemcpyAsync(off.data(), d_off, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        for (int c = 0; c < ncl; c++) {
+            const int n_c = off[c + 1] - off[c];
+            if (n_c <= giant_min) continue;
+            int rc = hard_giant(P, H, hb, c, off[c], n_c, st);
+            if (rc) return rc;
+        }
+    }
     if (sb != st) {
         GPLUM_HIP(hipEventRecord(ev_join, sb));
         GPLUM_HIP(hipStreamWaitEvent(st, ev_join, 0));
@@ -1292,13 +1571,13 @@ int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, c
     HardView H = hard_view(hb);
     if (n_c <= HARD_SEQ_MAX)
         hipLaunchKernelGGL((k_hermite<true, true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS);
+                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS, 0x7fffffff);
     else if (n_c <= HERMITE_LDS)
         hipLaunchKernelGGL((k_hermite<true, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
-                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS);
+                           hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS, 0x7fffffff);
     else
         hipLaunchKernelGGL((k_hermite<false, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr,
-                           1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS);
+                           1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS, 0x7fffffff);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

@@ -22,6 +22,9 @@ struct HardBuffers {
     DArray<int> next, ncol;
     DArray<double> edisp;
     DArray<long long> nblock, npair;    // block steps, hard_pair evaluations per cluster
+    DArray<unsigned char> g_state;      // giant-cluster loop state
+    DArray<double> g_r;                 // giant-cluster collision partial minima
+    DArray<int> g_i, g_j;
     int max_size = 0;
     int ensure(int n) {
         GPLUM_ENSURE(pos, 3 * n); GPLUM_ENSURE(vel, 3 * n); GPLUM_ENSURE(acc_s, 3 * n);
@@ -51,11 +54,15 @@ struct IsoArgs {
 // st2/ev_fork/ev_join (optional): clusters above HARD_SEQ_MAX run on st2 concurrently
 // with the small-cluster kernel on st, and st waits for st2.  wg_per_cu: persistent
 // workgroups per CU; lds_max (<= HERMITE_LDS): the largest cluster whose j-data is
-// staged in LDS (larger ones read it from global memory).  Neither changes results.
+// staged in LDS (larger ones read it from global memory); clusters above giant_min run
+// one at a time over the whole GPU, host-driven (not capturable in a graph).  None of
+// these changes results.
+constexpr int GIANT_MIN_DEFAULT = 1024;
+constexpr int GIANT_OFF = 0x7fffffff;
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2 = nullptr,
                 hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr, int wg_per_cu = 8,
-                int lds_max = HERMITE_LDS);
+                int lds_max = HERMITE_LDS, int giant_min = GIANT_MIN_DEFAULT);
 // Sun + hard acc/jerk and the initial block step of every cluster particle (as the
 // initial force correction leaves them), for callers that bring bare clusters.
 int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off, hipStream_t st);

@@ -562,7 +562,7 @@ int System::hard_part(bool persistent) {
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
         if (persistent) {
             if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join,
-                                  hard_wg_per_cu, hard_lds_max)))
+                                  hard_wg_per_cu, hard_lds_max, hard_giant_min)))
                 return rc;
         } else {
             // the comparator: one launch per cluster
@@ -754,7 +754,7 @@ int System::hard_trial(int wg_per_cu, int lds_max, double *ms) {
     hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
     tick(EV_HARD0);
     if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join, wg_per_cu,
-                          lds_max)))
+                          lds_max, hard_giant_min)))
         return rc;
     tick(EV_HARD1);
     *ms = elapsed(EV_HARD0, EV_HARD1);
@@ -818,6 +818,7 @@ int System::autotune(int trials, gplum_tuning *out) {
         out->soft_tile = b_tile;
         out->hard_wg_per_cu = b_wg;
         out->hard_lds_max = b_lds;
+        out->hard_giant_min = hard_giant_min;
         out->soft_ms = best_soft;
         out->hard_ms = best_hard < 1e300 ? best_hard : 0.;
     }
@@ -968,7 +969,8 @@ int gplum_system_autotune(gplum_system *s, int trials, gplum_tuning *out) {
 int gplum_system_get_tuning(gplum_system *s, gplum_tuning *out) {
     if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
-    *out = gplum_tuning{S->p.n_group_limit, S->soft_tile, S->hard_wg_per_cu, S->hard_lds_max, 0., 0.};
+    *out = gplum_tuning{S->p.n_group_limit, S->soft_tile, S->hard_wg_per_cu, S->hard_lds_max, S->hard_giant_min,
+                        0., 0.};
     return GPLUM_OK;
 }
 
@@ -979,11 +981,13 @@ int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t) {
         return gplum::fail(GPLUM_E_ARG, "soft_tile must be 64, 128, 192 or 256");
     if (t->hard_wg_per_cu < 1 || t->hard_wg_per_cu > 64) return gplum::fail(GPLUM_E_ARG, "hard_wg_per_cu must be 1..64");
     if (t->hard_lds_max < 64 || t->hard_lds_max > 256) return gplum::fail(GPLUM_E_ARG, "hard_lds_max must be 64..256");
+    if (t->hard_giant_min < 64) return gplum::fail(GPLUM_E_ARG, "hard_giant_min must be >= 64");
     System *S = reinterpret_cast<System *>(s);
     S->p.n_group_limit = t->n_group_limit;
     S->soft_tile = t->soft_tile;
     S->hard_wg_per_cu = t->hard_wg_per_cu;
     S->hard_lds_max = t->hard_lds_max;
+    S->hard_giant_min = t->hard_giant_min;
     return GPLUM_OK;
 }
 

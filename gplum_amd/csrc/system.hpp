@@ -54,6 +54,7 @@ struct System {
     int soft_tile = SOFT_TILE;          // j-particles staged per LDS pass of the soft kernel
     int hard_wg_per_cu = 8;             // persistent Hermite workgroups per CU
     int hard_lds_max = HERMITE_LDS;     // largest cluster whose j-data is staged in LDS
+    int hard_giant_min = GIANT_MIN_DEFAULT;   // larger clusters run over the whole GPU
     bool tuning = false;                // inside autotune(): no training, no statistics
     bool skip_train = false;            // the force recomputed after mergers: no training
     ParticleSoA A, B;

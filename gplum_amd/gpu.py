@@ -83,16 +83,17 @@ def lib():
     return _lib
 
 
-def hard_integrate(p, pp, offsets, persistent=True, prepare=False, counts=False):
+def hard_integrate(p, pp, offsets, persistent=True, prepare=False, counts=False, giant=False):
     """Integrate clusters pp[offsets[c]:offsets[c+1]] over one soft step (in place).
     prepare: compute acc/jerk and the first block step on the device first.
+    giant: every cluster above 64 particles on the whole-GPU path (same results).
     Returns (edisp, block_steps, kernel_ms), plus pair evaluations when counts."""
     off = np.ascontiguousarray(offsets, dtype=np.int32)
     ed = ctypes.c_double(0.0)
     nb = ctypes.c_int64(0)
     ms = ctypes.c_double(0.0)
     npair = ctypes.c_int64(0)
-    mode = (1 if persistent else 0) | (2 if prepare else 0)
+    mode = (1 if persistent else 0) | (2 if prepare else 0) | (4 if giant else 0)
     check(lib().gplum_hard_integrate(ctypes.byref(p), abi.ptr(pp), len(pp), abi.ptr(off), len(off) - 1, mode,
                                      ctypes.byref(ed), ctypes.byref(nb), ctypes.byref(ms),
                                      ctypes.byref(npair)))
@@ -134,6 +135,7 @@ class Tuning(ctypes.Structure):
     """gplum_tuning (include/gplum_system.h)."""
     _fields_ = [("n_group_limit", ctypes.c_int32), ("soft_tile", ctypes.c_int32),
                 ("hard_wg_per_cu", ctypes.c_int32), ("hard_lds_max", ctypes.c_int32),
+                ("hard_giant_min", ctypes.c_int32),
                 ("soft_ms", ctypes.c_double), ("hard_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -234,8 +236,9 @@ class System:
         check(lib().gplum_system_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, n_group_limit=256, soft_tile=256, hard_wg_per_cu=8, hard_lds_max=256):
-        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, 0.0, 0.0)
+    def set_tuning(self, n_group_limit=256, soft_tile=256, hard_wg_per_cu=8, hard_lds_max=256,
+                   hard_giant_min=1024, **_):
+        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, hard_giant_min, 0.0, 0.0)
         check(lib().gplum_system_set_tuning(self.h, ctypes.byref(t)))
 
     FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}

@@ -127,6 +127,7 @@ typedef struct gplum_tuning {
     int32_t soft_tile;       /* j-particles per LDS pass of the FP32 soft kernel      */
     int32_t hard_wg_per_cu;  /* persistent Hermite workgroups per CU                  */
     int32_t hard_lds_max;    /* clusters up to this size stage their j-data in LDS    */
+    int32_t hard_giant_min;  /* larger clusters run over the whole GPU (not tuned)    */
     double soft_ms, hard_ms; /* best measured soft stage / Hermite kernels, ms        */
 } gplum_tuning;
 int gplum_system_autotune(gplum_system *s, int trials, gplum_tuning *out);
@@ -154,7 +155,9 @@ int gplum_system_put_forecast(gplum_system *s, const float *theta, const float *
  * pos/vel/dt/acc0/r_out_inv (FPHard(FPGrav) local time, src/particle.h:1096-1116),
  * replacing the cluster loops of HardSystem::timeIntegrate (src/hard.h:583-788).
  * mode bit 0 set: one persistent batched launch (largest cluster first); clear: one
- * launch per cluster (plain comparator).  mode bit 1 set: the clusters are bare
+ * launch per cluster (plain comparator).  mode bit 2 (with bit 0): every cluster above
+ * 64 particles runs on the whole-GPU path meant for giant clusters (normally above 1024;
+ * same results).  mode bit 1 set: the clusters are bare
  * (pos, vel, mass, r_out_inv, acc0, dt only): Sun and hard acc/jerk and the first
  * block step are computed on the device first, as the initial force correction
  * would leave them (not timed).  Outputs (each may be null): edisp (collision

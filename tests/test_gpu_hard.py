@@ -129,3 +129,27 @@ def test_graph_replay_with_merging(oracle_lib):
     assert bad == 0
     assert nb_g == nb_e and ed_g == ed_e
     assert _same_state(got, eager) == []
+
+
+def test_giant_path_matches_oracle(no_collision_case):
+    """The whole-GPU path for giant clusters (here forced for every cluster above 64
+    particles): same operations, same orders, so bit-identical to the oracle."""
+    pg, pp, off, ref, nb_ref = no_collision_case
+    got = pp.copy()
+    ed_g, nb_g, ms = gpu.hard_integrate(pg, got, off, persistent=True, giant=True)
+    print("giant path: %d block steps in %.1f ms" % (nb_g, ms))
+    assert nb_g == nb_ref
+    for f in ("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_s", "dt", "time"):
+        assert np.array_equal(got[f], ref[f]), f
+    assert np.allclose(got["phi_d"], ref["phi_d"], rtol=1e-13, atol=0.0)
+
+
+def test_giant_path_with_merging(oracle_lib):
+    po, pg, pp, off = make_clusters(oracle_lib, 6, seed=11, size_max=120, collision=1, f=100.0, m_hi=1e-8)
+    a = pp.copy()
+    ed_a, nb_a, _ = gpu.hard_integrate(pg, a, off, persistent=True)
+    b = pp.copy()
+    ed_b, nb_b, _ = gpu.hard_integrate(pg, b, off, persistent=True, giant=True)
+    print("merging: %d collisions-worth of dissipation %.3e, block steps %d" % (((a["flags"] & 1) != 0).sum(), ed_a, nb_a))
+    assert nb_a == nb_b and ed_a == ed_b
+    assert _same_state(a, b) == []
