@@ -451,6 +451,16 @@ def main():
         "hard": {"clusters": st["n_clusters"], "cluster_particles": st["n_cluster_particles"],
                  "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
                  "max_block_steps_one_cluster": st["max_block_steps"], "collisions": st["n_collisions"]},
+        # the kernel with the most time in the step: FP64 Hermite, bound by the chain of
+        # block steps of the longest cluster (one wave), not by the chip's FP64 rate
+        "roofline_hermite": {"bound": "latency (longest cluster's block-step chain)",
+                             "achieved": round(FLOP_HARD_PAIR * st["hard_pairs"] / (st["hard_ms"] * 1e-3) / 1e12, 4),
+                             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(FLOP_HARD_PAIR * st["hard_pairs"] / (st["hard_ms"] * 1e-3) / 1e12
+                                           / PEAK_FP64_TFLOPS, 6),
+                             "kernel": "k_hermite size classes (hard stage time, HIP events)",
+                             "us_per_block_step_longest_cluster": round(1e3 * st["hard_ms"] / args.steps
+                                                                        / max(1, st["max_block_steps"]), 3)},
         "energy_error_over_timed_steps": de,
         "plain_fp64_per_cluster": plain,
         "forecast": {"calls": st["fc_calls"], "slots": st["fc_slots"], "overflowed_groups": st["fc_ovf_groups"],
