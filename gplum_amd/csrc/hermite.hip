@@ -174,19 +174,20 @@ __device__ inline void pair_term_j(const HP &P, const JData &J, int j, const dou
               J.v[3 * j + 2], J.m[j], J.ri[j], ta, tj, tphi);
 }
 
-// Sum over j = j0, j0 + s, j0 + 2s, ... < n (j != i by id) in ascending j, HARD_ILP
-// pairs evaluated at a time.  The self pair is evaluated too (its summands may be
-// NaN) and not added.
+// Sum over j = j0, j0 + s, j0 + 2s, ... < n (j != i by id) in ascending j, ILP pairs
+// evaluated at a time.  The self pair is evaluated too (its summands may be NaN) and
+// not added.
+template <int ILP = HARD_ILP>
 __device__ inline void pair_sum_strided(const HP &P, const JData &J, int j0, int s, int n, int64_t idi,
                                         const double xi[3], const double vi[3], double rinv_i, double acc[3],
                                         double jerk[3], double *phi) {
     int j = j0;
-    for (; j + (HARD_ILP - 1) * s < n; j += HARD_ILP * s) {
-        double ta[HARD_ILP][3], tj[HARD_ILP][3], tp[HARD_ILP];
+    for (; j + (ILP - 1) * s < n; j += ILP * s) {
+        double ta[ILP][3], tj[ILP][3], tp[ILP];
 #pragma unroll
-        for (int u = 0; u < HARD_ILP; u++) pair_term_j(P, J, j + u * s, xi, vi, rinv_i, ta[u], tj[u], phi ? &tp[u] : nullptr);
+        for (int u = 0; u < ILP; u++) pair_term_j(P, J, j + u * s, xi, vi, rinv_i, ta[u], tj[u], phi ? &tp[u] : nullptr);
 #pragma unroll
-        for (int u = 0; u < HARD_ILP; u++) {
+        for (int u = 0; u < ILP; u++) {
             if (J.id[j + u * s] == idi) continue;
             for (int k = 0; k < 3; k++) { acc[k] += ta[u][k]; jerk[k] += tj[u][k]; }
             if (phi) *phi -= tp[u];
@@ -340,7 +341,8 @@ __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int 
 __device__ inline void coop_pair_sum(const HP &P, const JData &J, int n, int64_t idi, const double xi[3],
                                      const double vi[3], double rinv_i, double acc[3], double jerk[3]) {
     double a[3] = {0., 0., 0.}, jk[3] = {0., 0., 0.};
-    pair_sum_strided(P, J, (int)(threadIdx.x & 63), 64, n, idi, xi, vi, rinv_i, a, jk, nullptr);
+    // two pairs at a time: the four-wave kernels hold more state per lane
+    pair_sum_strided<2>(P, J, (int)(threadIdx.x & 63), 64, n, idi, xi, vi, rinv_i, a, jk, nullptr);
     for (int d = 32; d > 0; d >>= 1)
         for (int k = 0; k < 3; k++) {
             a[k] += __shfl_xor(a[k], d);
@@ -694,8 +696,10 @@ __device__ void collision_tail(const HP &P, HardView &H, int b, int n, int ic, i
 
 // Block steps of one cluster from (sys_time, loop) to the end of the soft step on the
 // global-memory state H (j-data through J), then the final calcGravity.  Every
-// thread of the workgroup calls it; collisions and twins handled here.
-template <bool LDSJ>
+// thread of the workgroup calls it; collisions and twins handled here.  COOP: the
+// cluster has more than HARD_SEQ_MAX particles (the other correction path is left out
+// of the instantiation, which keeps its register use down).
+template <bool LDSJ, bool COOP>
 __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
                                int64_t *s_id, double *s_fr, int b, int n, double &sys_time, long long &loop,
                                double &ed, int &n_col, long long &pairs) {
@@ -773,7 +777,7 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
             __syncthreads();
         }
         // correct the active particles (hermite.h:300-375)
-        if (n > HARD_SEQ_MAX) {
+        if (COOP) {
             for (int i = t; i < n; i += B)
                 H.act[b + i] = !(H.flags[b + i] & GPLUM_FLAG_DEAD) &&
                                (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
@@ -945,7 +949,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         }
         __threadfence_block();
         __syncthreads();
-        integrate_from<true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
+        integrate_from<true, false>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
         return;
     }
     // final calcGravity with potentials (hermite.h:466-470), current positions
@@ -1047,7 +1051,7 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
                 H.twin_of[b + i] = -1;
             }
             __syncthreads();
-            integrate_from<LDSJ>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
+            integrate_from<LDSJ, true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
         }
         atomicAdd(&s_pairs, (unsigned long long)pairs);
         __syncthreads();
