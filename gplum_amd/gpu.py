@@ -117,6 +117,27 @@ def hard_integrate_graph(p, pp, offsets, replays=4, prepare=False):
     return ed.value, nb.value, ms.value, bad.value
 
 
+class SnapshotHeader(ctypes.Structure):
+    """gplum_snapshot_header (include/gplum_system.h)."""
+    _fields_ = [("time", ctypes.c_double), ("n_body", ctypes.c_int32), ("id_next", ctypes.c_int32),
+                ("e_init", ctypes.c_double * 5), ("e_now", ctypes.c_double * 5)]
+
+
+def snapshot_read(path):
+    """(header, particles) of an ASCII snapshot in the reference's format."""
+    h = SnapshotHeader()
+    n = ctypes.c_int(0)
+    check(lib().gplum_snapshot_read(path.encode(), ctypes.byref(h), None, 0, ctypes.byref(n)))
+    pp = np.zeros(n.value, dtype=abi.PARTICLE)
+    check(lib().gplum_snapshot_read(path.encode(), ctypes.byref(h), abi.ptr(pp), len(pp), ctypes.byref(n)))
+    return h, pp
+
+
+def snapshot_write(path, header, pp):
+    pp = np.ascontiguousarray(pp)
+    check(lib().gplum_snapshot_write(path.encode(), ctypes.byref(header), abi.ptr(pp), len(pp)))
+
+
 class Stats(ctypes.Structure):
     """gplum_stats (include/gplum_system.h)."""
     _fields_ = [(n, ctypes.c_int64) for n in (
@@ -223,6 +244,15 @@ class System:
 
     def set_variant(self, fp64_soft=False, per_cluster_hard=False):
         check(lib().gplum_system_set_variant(self.h, int(fp64_soft), int(per_cluster_hard)))
+
+    def snapshot(self, path, id_next=0):
+        """makeSnap: the system's particles and energies as a reference-format snapshot."""
+        check(lib().gplum_system_snapshot(self.h, path.encode(), int(id_next)))
+
+    def energy_line(self):
+        buf = ctypes.create_string_buffer(256)
+        check(lib().gplum_system_energy_line(self.h, buf, len(buf)))
+        return buf.value.decode()
 
     def autotune(self, trials=2):
         """Pick the launch shapes (i-group size, soft j-tile, Hermite grid and LDS

@@ -167,6 +167,25 @@ int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, int n, const
                          int n_cluster, int mode, double *edisp, int64_t *block_steps,
                          double *kernel_ms, int64_t *pair_evals);
 
+/* Snapshots in the reference's ASCII format: FileHeader::readAscii / writeAscii
+ * (src/energy.h:89-116) and FPGrav::readAscii / writeAscii (src/particle.h:829-855),
+ * same printf/scanf formats.  Energies: etot, ekin, ephi_sun, ephi_planet, edisp.
+ * gplum_snapshot_read with pp == NULL reads the header only (n = n_body); fields not in
+ * the file are zero, id_cluster = id.  gplum_system_snapshot writes the system's
+ * particles with its initial and current energies (makeSnap, src/func.h:53-66).
+ * gplum_system_energy_line formats the energy-log line of outputStep (src/func.h:87-130,
+ * without the optional columns): time, n, etot, relative error, largest cluster, number
+ * of clusters, isolated particles. */
+typedef struct gplum_snapshot_header {
+    double time;
+    int32_t n_body, id_next;
+    double e_init[5], e_now[5];
+} gplum_snapshot_header;
+int gplum_snapshot_read(const char *path, gplum_snapshot_header *h, gplum_particle *pp, int cap, int *n);
+int gplum_snapshot_write(const char *path, const gplum_snapshot_header *h, const gplum_particle *pp, int n);
+int gplum_system_snapshot(gplum_system *s, const char *path, int32_t id_next);
+int gplum_system_energy_line(gplum_system *s, char *buf, int cap);
+
 /* The persistent form of gplum_hard_integrate captured into a hipGraph and replayed
  * `replays` times (no counterpart in the reference; same clusters, results and
  * outputs as mode 1, for callers that replay a fixed batch).  The graph restores the

@@ -147,3 +147,27 @@ def test_config1_energy_drift(oracle_lib, init3000):
     assert wg < 10 * wo + 1e-12
     run.close()
     s.close()
+
+
+def test_system_snapshot_and_energy_line(tmp_path):
+    """makeSnap / outputStep equivalents: the snapshot holds the system's particles (to
+    the 16 significant digits of the format) and energies; the energy line parses."""
+    p = gpu.params(collision=1, **CFG1)
+    s = gpu.System(p, ring(3000, seed=8))
+    s.initialize()
+    s.step(1)
+    path = str(tmp_path / "snap000001.dat")
+    s.snapshot(path, id_next=3000)
+    h, pp = gpu.snapshot_read(path)
+    q = s.particles()
+    e = s.energy()
+    assert h.n_body == len(q) and h.id_next == 3000
+    assert abs(h.time - 2.0 ** -4) == 0.0
+    assert np.array_equal(pp["id"], q["id"])
+    for f in ("pos", "vel", "mass", "r_planet"):
+        assert np.allclose(pp[f], q[f], rtol=2e-15, atol=0.0), f
+    assert abs(h.e_now[0] - e[0]) <= 1e-15 * abs(e[0])
+    line = s.energy_line().split("\t")
+    assert len(line) == 7 and int(line[1]) == len(q)
+    assert abs(float(line[2]) - e[0]) <= 1e-14 * abs(e[0])
+    s.close()
