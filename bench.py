@@ -231,6 +231,8 @@ def bench_large(args):
     s = gpu.System(p, pp)
     del pp
     s.set_forecast("network")
+    if args.giant_min:
+        s.set_tuning(hard_giant_min=args.giant_min)
     t0 = time.perf_counter()
     s.initialize()
     print("# initialised in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -322,6 +324,8 @@ def main():
     ap.add_argument("--compare-clusters", type=int, default=100)
     ap.add_argument("--replays", type=int, default=3, help="graph replays (hard workload)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes)")
+    ap.add_argument("--giant-min", type=int, default=0,
+                    help="clusters above this size on the whole-GPU path (config4/5; 0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -11,7 +11,10 @@
 // mass, 1/r_out, id) lives in LDS for clusters of up to HERMITE_LDS particles.
 // The kernel is persistent: workgroups pull clusters (largest first) from an atomic
 // queue until it is empty.
+#include <algorithm>
 #include <cfloat>
+#include <mutex>
+#include <vector>
 
 #include "hermite.hpp"
 
@@ -1232,9 +1235,9 @@ static HardView hard_view(HardBuffers &hb) {
 // with OpenMP inside the cluster, hard.h:583-658) are spread over the whole GPU: each
 // block step is a short sequence of kernels (predict all; collision test over all pairs
 // with per-workgroup minima; decision and active flags; the active particles, one wave
-// each, as in the four-wave kernels; the collision tail in one workgroup; next block time
-// and system time), launched in batches by the host, which checks a done flag between
-// batches.  Same functions, same summation orders: results identical to the size-class
+// each, as in the eight-wave kernels; the collision aftermath phase by phase; next
+// block time and system time), launched in batches by the host, which checks a done
+// flag between batches.  Same functions, same summation orders: results identical to the size-class
 // kernels.
 struct GiantState {
     double t_s, sys_time, ed;
@@ -1407,17 +1410,40 @@ __global__ __launch_bounds__(GIANT_T) void k_g_correct(HP P, HardView H, int b, 
     if ((threadIdx.x & 63) == 0 && pairs) atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)pairs);
 }
 
-// perfect merging and its aftermath in one workgroup (rare)
-__global__ __launch_bounds__(GIANT_RED) void k_g_tail(HP P, HardView H, int b, int n, GiantState *G) {
+// perfect merging and its aftermath (the phases of collision_tail, each over the grid;
+// all return at once in the block steps without a collision)
+__global__ void k_g_merge(HP P, HardView H, int b, int n, GiantState *G) {
+    if (G->done || !G->flag_col || threadIdx.x != 0) return;
+    G->ed += perfect_merge(P, H, b, n, G->ic, G->jc);
+    G->n_col++;
+}
+__global__ __launch_bounds__(GIANT_T) void k_g_tail_gravity(HP P, HardView H, int b, int n, GiantState *G) {
     if (G->done || !G->flag_col) return;
-    double ed = 0.;
-    long long pairs = 0;
-    collision_tail(P, H, b, n, G->ic, G->jc, ed, pairs);
-    if (pairs) atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)pairs);
-    if (threadIdx.x == 0) {
-        G->ed += ed;
-        G->n_col++;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gravity_full(P, H, b, n, i);
+    atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)(n - 1));
+}
+__global__ __launch_bounds__(GIANT_T) void k_g_tail_merged(HardView H, int b, int n, const GiantState *G) {
+    if (G->done || !G->flag_col) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (H.flags[b + i] & GPLUM_FLAG_MERGED)) merge_acc_jerk(H, b, n, i);
+}
+__global__ __launch_bounds__(GIANT_T) void k_g_tail_dt(HP P, HardView H, int b, int n, const GiantState *G) {
+    if (G->done || !G->flag_col) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = b + i;
+    if (!(H.flags[g] & GPLUM_FLAG_DEAD)) {
+        double ad[3], jd[3], as[3], js[3];
+        ld3(H.acc_d, g, ad); ld3(H.jerk_d, g, jd); ld3(H.acc_s, g, as); ld3(H.jerk_s, g, js);
+        H.dt[g] = dt_initial(P, H.time[g], H.dt[g], H.acc0[g], ad, jd, as, js);
     }
+}
+__global__ __launch_bounds__(GIANT_T) void k_g_tail_twins(HardView H, int b, int n, const GiantState *G) {
+    if (G->done || !G->flag_col) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && H.twin_of[b + i] >= 0) H.dt[b + i] = H.dt[b + H.twin_of[b + i]];
 }
 
 // final calcGravity with potentials (hermite.h:466-470), then the merged targets
@@ -1440,40 +1466,152 @@ __global__ __launch_bounds__(GIANT_T) void k_g_final_merged(HardView H, int b, i
     }
 }
 
-// Integrates cluster c = [b, b + n) with the kernels above; the host checks the done
-// flag every GIANT_BATCH block steps.
-constexpr int GIANT_BATCH = 16;
-static int hard_giant(const HP &P, HardView &H, HardBuffers &hb, int c, int b, int n, hipStream_t st) {
-    const int g_n = ceil_div(n, GIANT_T);
-    const int g_jc = std::max(1, std::min(64, ceil_div(n, 64)));   // j chunks of the collision test
-    const int g_w = std::min(4096, ceil_div(n, GIANT_T / 64));
-    GPLUM_ENSURE(hb.g_state, sizeof(GiantState));
-    GPLUM_ENSURE(hb.g_r, g_n * g_jc);
-    GPLUM_ENSURE(hb.g_i, g_n * g_jc);
-    GPLUM_ENSURE(hb.g_j, g_n * g_jc);
-    GiantState *G = reinterpret_cast<GiantState *>(hb.g_state.p);
-    hipLaunchKernelGGL(k_g_init, dim3(g_n), dim3(GIANT_T), 0, st, P, H, b, n);
-    hipLaunchKernelGGL(k_g_times, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G, 1);
-    for (;;) {
-        for (int s = 0; s < GIANT_BATCH; s++) {
-            hipLaunchKernelGGL(k_g_predict, dim3(g_n), dim3(GIANT_T), 0, st, H, b, n, G);
-            hipLaunchKernelGGL(k_g_collide, dim3(g_n, g_jc), dim3(GIANT_T), 0, st, H, b, n, G, hb.g_r.p, hb.g_i.p,
-                               hb.g_j.p);
-            hipLaunchKernelGGL(k_g_decide, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G, hb.g_r.p, hb.g_i.p,
-                               hb.g_j.p, g_n * g_jc);
-            hipLaunchKernelGGL(k_g_correct, dim3(g_w), dim3(GIANT_T), 0, st, P, H, b, n, G);
-            hipLaunchKernelGGL(k_g_tail, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G);
-            hipLaunchKernelGGL(k_g_times, dim3(1), dim3(GIANT_RED), 0, st, P, H, b, n, G, 0);
-        }
-        GPLUM_HIP(hipGetLastError());
-        int done = 0;
-        GPLUM_HIP(hipMemcpyAsync(&done, &G->done, sizeof(int), hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipStreamSynchronize(st));
-        if (done) break;
+// Giant clusters run concurrently, one per stream of a small per-device pool (streams
+// at the default priority, created once and kept for the process).  Each slot captures
+// GIANT_BATCH block steps of its cluster into a hipGraph once (10 kernels per step, no
+// memset nodes; every kernel returns at once after the done flag is set) and replays it
+// per round, then reads its done flag; a finished slot takes the next giant cluster.
+// The graph keeps the host from spending its time on kernel launches.
+constexpr int GIANT_BATCH = 32;
+constexpr int GIANT_STREAMS = 8;
+
+struct GiantSlot {
+    int c = -1, b = 0, n = 0, g_n = 0, g_jc = 0, g_w = 0;
+    GiantState *G = nullptr;
+    double *r = nullptr;
+    int *pi = nullptr, *pj = nullptr;
+    hipGraphExec_t exec = nullptr;      // GIANT_BATCH block steps of this cluster
+};
+
+struct GiantPool {
+    hipStream_t st[GIANT_STREAMS];
+    hipEvent_t join[GIANT_STREAMS];
+    hipEvent_t fork;
+    int *done_h;          // pinned host copies of the slots' done flags (asynchronous reads)
+};
+
+// the calling thread's device's pool (created on first use)
+static int giant_pool(GiantPool *out) {
+    static std::mutex mu;
+    static GiantPool pool[64] = {};
+    int dev = 0;
+    GPLUM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(GPLUM_E_NODEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    GiantPool &p = pool[dev];
+    for (int k = 0; k < GIANT_STREAMS; k++) {
+        if (!p.st[k]) GPLUM_HIP(hipStreamCreateWithFlags(&p.st[k], hipStreamNonBlocking));
+        if (!p.join[k]) GPLUM_HIP(hipEventCreateWithFlags(&p.join[k], hipEventDisableTiming));
     }
-    hipLaunchKernelGGL(k_g_final, dim3(g_n), dim3(GIANT_T), 0, st, P, H, b, n, G);
-    hipLaunchKernelGGL(k_g_final_merged, dim3(g_n), dim3(GIANT_T), 0, st, H, b, n, G, c, hb.edisp.p, hb.ncol.p,
-                       hb.nblock.p, hb.npair.p, P.dt_tree);
+    if (!p.fork) GPLUM_HIP(hipEventCreateWithFlags(&p.fork, hipEventDisableTiming));
+    if (!p.done_h) GPLUM_HIP(hipHostMalloc((void **)&p.done_h, sizeof(int) * GIANT_STREAMS, hipHostMallocDefault));
+    *out = p;
+    return GPLUM_OK;
+}
+
+static void giant_start(const HP &P, HardView &H, GiantSlot &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_g_init, dim3(g.g_n), dim3(GIANT_T), 0, st, P, H, g.b, g.n);
+    hipLaunchKernelGGL(k_g_times, dim3(1), dim3(GIANT_RED), 0, st, P, H, g.b, g.n, g.G, 1);
+}
+
+static void giant_steps(const HP &P, HardView &H, GiantSlot &g, hipStream_t st) {
+    for (int s = 0; s < GIANT_BATCH; s++) {
+        hipLaunchKernelGGL(k_g_predict, dim3(g.g_n), dim3(GIANT_T), 0, st, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_collide, dim3(g.g_n, g.g_jc), dim3(GIANT_T), 0, st, H, g.b, g.n, g.G, g.r, g.pi, g.pj);
+        hipLaunchKernelGGL(k_g_decide, dim3(1), dim3(GIANT_RED), 0, st, P, H, g.b, g.n, g.G, g.r, g.pi, g.pj,
+                           g.g_n * g.g_jc);
+        hipLaunchKernelGGL(k_g_correct, dim3(g.g_w), dim3(GIANT_T), 0, st, P, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_merge, dim3(1), dim3(64), 0, st, P, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_tail_gravity, dim3(g.g_n), dim3(GIANT_T), 0, st, P, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_tail_merged, dim3(g.g_n), dim3(GIANT_T), 0, st, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_tail_dt, dim3(g.g_n), dim3(GIANT_T), 0, st, P, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_tail_twins, dim3(g.g_n), dim3(GIANT_T), 0, st, H, g.b, g.n, g.G);
+        hipLaunchKernelGGL(k_g_times, dim3(1), dim3(GIANT_RED), 0, st, P, H, g.b, g.n, g.G, 0);
+    }
+}
+
+static void giant_finish(const HP &P, HardView &H, HardBuffers &hb, GiantSlot &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_g_final, dim3(g.g_n), dim3(GIANT_T), 0, st, P, H, g.b, g.n, g.G);
+    hipLaunchKernelGGL(k_g_final_merged, dim3(g.g_n), dim3(GIANT_T), 0, st, H, g.b, g.n, g.G, g.c, hb.edisp.p,
+                       hb.ncol.p, hb.nblock.p, hb.npair.p, P.dt_tree);
+}
+
+// Integrates the clusters listed in giants (host offsets off) on the giant path; the
+// slots start after event fork (recorded on st once the cluster data were gathered) and
+// st waits for all of them.
+static int hard_giants(const HP &P, HardView &H, HardBuffers &hb, const std::vector<int> &giants,
+                       const std::vector<int32_t> &off, hipStream_t st, const GiantPool &gp) {
+    const hipStream_t *se = gp.st;
+    int rc = GPLUM_OK;
+    const int K = std::min<int>(GIANT_STREAMS, (int)giants.size());
+    int max_n = 0;
+    for (int c : giants) max_n = std::max(max_n, off[c + 1] - off[c]);
+    const int max_parts = ceil_div(max_n, GIANT_T) * std::max(1, std::min(64, ceil_div(max_n, 64)));
+    GPLUM_ENSURE(hb.g_state, sizeof(GiantState) * (size_t)K);
+    GPLUM_ENSURE(hb.g_r, (size_t)max_parts * K);
+    GPLUM_ENSURE(hb.g_i, (size_t)max_parts * K);
+    GPLUM_ENSURE(hb.g_j, (size_t)max_parts * K);
+    for (int k = 0; k < K; k++) GPLUM_HIP(hipStreamWaitEvent(se[k], gp.fork, 0));
+    std::vector<GiantSlot> slot(K);
+    size_t next = 0;
+    auto release = [&](GiantSlot &g) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    };
+    auto assign = [&](int k) -> int {
+        GiantSlot &g = slot[k];
+        release(g);
+        if (next >= giants.size()) { g.c = -1; return GPLUM_OK; }
+        g.c = giants[next++];
+        g.b = off[g.c];
+        g.n = off[g.c + 1] - g.b;
+        g.g_n = ceil_div(g.n, GIANT_T);
+        g.g_jc = std::max(1, std::min(64, ceil_div(g.n, 64)));
+        g.g_w = std::min(4096, ceil_div(g.n, GIANT_T / 64));
+        g.G = reinterpret_cast<GiantState *>(hb.g_state.p) + k;
+        g.r = hb.g_r.p + (size_t)k * max_parts;
+        g.pi = hb.g_i.p + (size_t)k * max_parts;
+        g.pj = hb.g_j.p + (size_t)k * max_parts;
+        giant_start(P, H, g, se[k]);
+        hipGraph_t graph = nullptr;
+        GPLUM_HIP(hipStreamBeginCapture(se[k], hipStreamCaptureModeThreadLocal));
+        giant_steps(P, H, g, se[k]);
+        const hipError_t ec = hipStreamEndCapture(se[k], &graph);
+        if (ec != hipSuccess) return fail(GPLUM_E_HIP, "graph capture of the giant-cluster steps failed");
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(GPLUM_E_HIP, "graph instantiation of the giant-cluster steps failed");
+        return GPLUM_OK;
+    };
+    for (int k = 0; k < K; k++)
+        if ((rc = assign(k))) break;
+    for (; !rc;) {
+        bool any = false;
+        for (int k = 0; k < K; k++)
+            if (slot[k].c >= 0) {
+                GPLUM_HIP(hipGraphLaunch(slot[k].exec, se[k]));
+                GPLUM_HIP(hipMemcpyAsync(&gp.done_h[k], &slot[k].G->done, sizeof(int), hipMemcpyDeviceToHost,
+                                         se[k]));
+                any = true;
+            }
+        if (!any) break;
+        GPLUM_HIP(hipGetLastError());
+        for (int k = 0; k < K; k++)
+            if (slot[k].c >= 0) {
+                GPLUM_HIP(hipStreamSynchronize(se[k]));
+                if (gp.done_h[k]) {
+                    giant_finish(P, H, hb, slot[k], se[k]);
+                    if ((rc = assign(k))) break;
+                }
+            }
+    }
+    for (int k = 0; k < K; k++) release(slot[k]);
+    if (rc) return rc;
+    // st waits for every slot
+    for (int k = 0; k < K; k++) {
+        GPLUM_HIP(hipEventRecord(gp.join[k], se[k]));
+        GPLUM_HIP(hipStreamWaitEvent(st, gp.join[k], 0));
+    }
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }
@@ -1508,6 +1646,13 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
         GPLUM_HIP(hipStreamWaitEvent(sb, ev_fork, 0));
     }
     giant_min = std::max(HARD_SEQ_MAX, giant_min);
+    // giant clusters start right after the gather: their stream pool waits on this event
+    GiantPool gpool{};
+    if (hb.max_size > giant_min) {
+        int rc = giant_pool(&gpool);
+        if (rc) return rc;
+        GPLUM_HIP(hipEventRecord(gpool.fork, st));
+    }
     if (hb.max_size > lds_max)
         hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
                            ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max,
@@ -1519,17 +1664,19 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
     hipLaunchKernelGGL((k_hermite<true, true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order, ncl,
                        hb.next.p, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max, giant_min);
     if (hb.max_size > giant_min) {
-        // giant clusters: each over the whole GPU, one after the other on st (the
+        // giant clusters: several at a time over the whole GPU on a stream pool (the
         // size-class kernels above run meanwhile)
         std::vector<int32_t> off(ncl + 1);
         GPLUM_HIP(hipMemcpyAsync(off.data(), d_off, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
-        for (int c = 0; c < ncl; c++) {
-            const int n_c = off[c + 1] - off[c];
-            if (n_c <= giant_min) continue;
-            int rc = hard_giant(P, H, hb, c, off[c], n_c, st);
-            if (rc) return rc;
-        }
+        std::vector<int> giants;
+        for (int c = 0; c < ncl; c++)
+            if (off[c + 1] - off[c] > giant_min) giants.push_back(c);
+        // largest first
+        std::stable_sort(giants.begin(), giants.end(),
+                         [&](int x, int y) { return off[x + 1] - off[x] > off[y + 1] - off[y]; });
+        int rc = hard_giants(P, H, hb, giants, off, st, gpool);
+        if (rc) return rc;
     }
     if (sb != st) {
         GPLUM_HIP(hipEventRecord(ev_join, sb));

@@ -55,8 +55,8 @@ struct IsoArgs {
 // with the small-cluster kernel on st, and st waits for st2.  wg_per_cu: persistent
 // workgroups per CU; lds_max (<= HERMITE_LDS): the largest cluster whose j-data is
 // staged in LDS (larger ones read it from global memory); clusters above giant_min run
-// one at a time over the whole GPU, host-driven (not capturable in a graph).  None of
-// these changes results.
+// over the whole GPU, several at a time on a stream pool, host-driven (not capturable in
+// a graph).  None of these changes results.
 constexpr int GIANT_MIN_DEFAULT = 1024;
 constexpr int GIANT_OFF = 0x7fffffff;
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
