@@ -213,6 +213,49 @@ __device__ inline void gravity_loop(const HP &P, const JData &J, int n, int64_t 
     pair_sum_strided(P, J, 0, 1, n, idi, xi, vi, rinv_i, acc, jerk, phi);
 }
 
+// gravity_loop for the one-particle-per-lane path of lane i: the lane's partners are
+// j = k + (k >= i) for k = 0 .. n-2, ascending j as in gravity_loop, so every lane runs
+// the same n-1 iterations (the loop over all j with a self test would run n iterations
+// with one lane masked off in each).  A partner with the same id (never, before any
+// merger) is evaluated and not added, as gravity_loop skips it.
+__device__ inline void gravity_loop_lane(const HP &P, const JData &J, int n, int i, int64_t idi,
+                                         const double xi[3], const double vi[3], double rinv_i,
+                                         double acc[3], double jerk[3], double *phi) {
+    acc[0] = acc[1] = acc[2] = 0.;
+    jerk[0] = jerk[1] = jerk[2] = 0.;
+    const int m = n - 1;
+    int k = 0;
+    for (; k + HARD_ILP <= m; k += HARD_ILP) {
+        double ta[HARD_ILP][3], tj[HARD_ILP][3], tp[HARD_ILP];
+        int jj[HARD_ILP];
+#pragma unroll
+        for (int u = 0; u < HARD_ILP; u++) {
+            jj[u] = k + u + ((k + u) >= i ? 1 : 0);
+            pair_term_j(P, J, jj[u], xi, vi, rinv_i, ta[u], tj[u], phi ? &tp[u] : nullptr);
+        }
+#pragma unroll
+        for (int u = 0; u < HARD_ILP; u++) {
+            const bool use = J.id[jj[u]] != idi;
+            for (int d = 0; d < 3; d++) {
+                acc[d] = use ? acc[d] + ta[u][d] : acc[d];
+                jerk[d] = use ? jerk[d] + tj[u][d] : jerk[d];
+            }
+            if (phi) *phi = use ? *phi - tp[u] : *phi;
+        }
+    }
+    for (; k < m; k++) {
+        const int j = k + (k >= i ? 1 : 0);
+        double ta[3], tj[3], tp;
+        pair_term_j(P, J, j, xi, vi, rinv_i, ta, tj, phi ? &tp : nullptr);
+        const bool use = J.id[j] != idi;
+        for (int d = 0; d < 3; d++) {
+            acc[d] = use ? acc[d] + ta[d] : acc[d];
+            jerk[d] = use ? jerk[d] + tj[d] : jerk[d];
+        }
+        if (phi) *phi = use ? *phi - tp : *phi;
+    }
+}
+
 // calcGravity (full, with phi_d and Sun phi), gravity_hard.h:203-256, current pos/vel.
 __device__ void gravity_full(const HP &P, HardView &H, int b, int n, int i) {
     double x[3], v[3], as[3], js[3], ad[3], jd[3], phs, phd = 0.;
@@ -884,7 +927,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     // initial calcGravity (hermite.h:259-270)
     if (own) {
         star_acc_jerk(P, x, v, as, js, nullptr);
-        gravity_loop(P, J, n, id, x, v, rinv, ad, jd, nullptr);
+        gravity_loop_lane(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
     }
     const double time_end = P.dt_tree;
     double a0s[3], j0s[3], a0d[3], j0d[3], xp[3], vp[3], a2s[3], a3s[3], a2d[3], a3d[3];
@@ -904,8 +947,8 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             double rmin = 1.;
             int bi = 0x7fffffff, bj = -1;
             if (own) {
-                for (int j = 0; j < n; j++) {
-                    if (j == t) continue;
+                for (int k = 0; k < n - 1; k++) {
+                    const int j = k + (k >= t ? 1 : 0);   // the partners, ascending
                     const double dx = s_x[3 * t] - s_x[3 * j];
                     const double dy = s_x[3 * t + 1] - s_x[3 * j + 1];
                     const double dz = s_x[3 * t + 2] - s_x[3 * j + 2];
@@ -923,12 +966,12 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         if (own && time + dt == t_s) {
             pairs += 3 * (long long)(n - 1);
             star_acc_jerk(P, xp, vp, as, js, nullptr);
-            gravity_loop(P, J, n, id, xp, vp, rinv, ad, jd, nullptr);
+            gravity_loop_lane(P, J, n, t, id, xp, vp, rinv, ad, jd, nullptr);
             const double t_c = t_s - time;
             for (int ite = 0; ite < 2; ite++) {
                 correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
                 star_acc_jerk(P, x, v, as, js, nullptr);
-                gravity_loop(P, J, n, id, x, v, rinv, ad, jd, nullptr);
+                gravity_loop_lane(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
             }
             time += t_c;
             dt = dt_block(P, time, dt, acc0, ad, jd, a2d, a3d, as, js, a2s, a3s);
@@ -965,7 +1008,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     if (own) {
         double phs = 0., phd = 0.;
         star_acc_jerk(P, x, v, as, js, &phs);
-        gravity_loop(P, J, n, id, x, v, rinv, ad, jd, &phd);
+        gravity_loop_lane(P, J, n, t, id, x, v, rinv, ad, jd, &phd);
         pairs += n - 1;
         st3(H.pos, g, x); st3(H.vel, g, v);
         st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
