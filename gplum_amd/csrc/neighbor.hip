@@ -107,7 +107,7 @@ __global__ void k_cand_keys(const int2 *__restrict__ cand, int nc, uint64_t *__r
     key[c] = ((uint64_t)(uint32_t)q.x << 32) | (uint64_t)(uint32_t)q.y;
 }
 
-__global__ void k_finalize(NbParams P, PView V, int n, int initial, const int32_t *__restrict__ row_off,
+__global__ void k_finalize(NbParams P, PView V, int n, int ncand, int initial, const int32_t *__restrict__ row_off,
                            const int32_t *__restrict__ number_fp32, const uint64_t *__restrict__ keys,
                            unsigned char *__restrict__ accepted, double *__restrict__ acc,
                            double *__restrict__ phi, double *__restrict__ acc0, int32_t *__restrict__ number,
@@ -124,8 +124,10 @@ __global__ void k_finalize(NbParams P, PView V, int n, int initial, const int32_
     a.number = 0;
     a.id_cluster = V.id[i];
     a.phi += V.mass[i] * V.r_out_inv[i];     // self term (gravity_soft.h:280, 292)
-    const int b = row_off[i], m = number_fp32[i];
-    for (int c = b; c < b + m; c++) {
+    // rows end at ncand even if the FP32 counts and the candidates found disagree (an
+    // error the host reports after the stage)
+    const int b = row_off[i], e = min(b + number_fp32[i], ncand);
+    for (int c = b; c < e; c++) {
         const int j = (int)(uint32_t)(keys[c] & 0xffffffffULL);
         bool acc_flag = false;
         if (V.id[j] != V.id[i]) acc_flag = correct_pair(P, V, i, j, initial != 0, a);
@@ -178,14 +180,15 @@ __global__ void k_finalize(NbParams P, PView V, int n, int initial, const int32_
     }
 }
 
-__global__ void k_compact_rows(int n, const int32_t *__restrict__ row_off, const int32_t *__restrict__ number_fp32,
-                               const uint64_t *__restrict__ keys, const unsigned char *__restrict__ accepted,
-                               const int32_t *__restrict__ nbr_off, int32_t *__restrict__ nbr_idx) {
+__global__ void k_compact_rows(int n, int ncand, const int32_t *__restrict__ row_off,
+                               const int32_t *__restrict__ number_fp32, const uint64_t *__restrict__ keys,
+                               const unsigned char *__restrict__ accepted, const int32_t *__restrict__ nbr_off,
+                               int32_t *__restrict__ nbr_idx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int w = nbr_off[i];
-    const int b = row_off[i], m = number_fp32[i];
-    for (int c = b; c < b + m; c++)
+    const int b = row_off[i], e = min(b + number_fp32[i], ncand);
+    for (int c = b; c < e; c++)
         if (accepted[c]) nbr_idx[w++] = (int)(uint32_t)(keys[c] & 0xffffffffULL);
 }
 
@@ -313,6 +316,22 @@ static int nb_scan(Scratch &s, const T *in, T *out, int n, hipStream_t st) {
 }
 
 template <class T>
+static int last_sum(const T *in, const T *scan, int n, T *total, hipStream_t st);
+
+// info[0] = 1 if the FP32 counts do not add up to the candidates found; nbr_off[n] and
+// info[1] = the accepted total (the last offset plus the last count)
+__global__ void k_finalize_totals(int n, int ncand, const int32_t *__restrict__ number_fp32,
+                                  const int32_t *__restrict__ row_off, const int32_t *__restrict__ number,
+                                  int32_t *__restrict__ nbr_off, int32_t *__restrict__ info) {
+    if (threadIdx.x != 0) return;
+    const int total = n > 0 ? row_off[n - 1] + number_fp32[n - 1] : 0;
+    info[0] = (total != ncand) ? 1 : 0;
+    const int nn = n > 0 ? nbr_off[n - 1] + number[n - 1] : 0;
+    nbr_off[n] = nn;
+    info[1] = nn;
+}
+
+template <class T>
 static int last_sum(const T *in, const T *scan, int n, T *total, hipStream_t st) {
     T a = 0, b = 0;
     if (n <= 0) { *total = 0; return GPLUM_OK; }
@@ -349,34 +368,33 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
     }
     int rc = nb_scan(s, ps.number.p, nb.row_off.p, n, st);
     if (rc) return rc;
-    int total = 0;
-    if ((rc = last_sum(ps.number.p, nb.row_off.p, n, &total, st))) return rc;
-    if (total != ncand)
-        return fail(GPLUM_E_STATE, "candidate count mismatch: " + std::to_string(total) + " vs " +
-                                       std::to_string(ncand));
     // number (FP32 count) is overwritten by the FP64 count: keep a copy
     GPLUM_ENSURE(nb.number_fp32, n);
     GPLUM_HIP(hipMemcpyAsync(nb.number_fp32.p, ps.number.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
     NbParams P = make_nb_params(p);
     PView V{ps.pos.p, ps.vel.p, ps.acc_d.p, ps.mass.p, ps.r_out.p, ps.r_out_inv.p, ps.r_search.p, ps.id.p};
-    hipLaunchKernelGGL(k_finalize, dim3(ceil_div(n, 128)), dim3(128), 0, st, P, V, n, initial ? 1 : 0,
+    hipLaunchKernelGGL(k_finalize, dim3(ceil_div(n, 128)), dim3(128), 0, st, P, V, n, ncand, initial ? 1 : 0,
                        nb.row_off.p, nb.number_fp32.p, nb.keys.p, nb.accepted.p, ps.acc.p, ps.phi.p,
                        ps.acc0.p, ps.number.p, ps.id_cluster.p, nb.acc_d_new.p, ps.jerk_d.p, ps.phi_d.p,
                        ps.acc_s.p, ps.jerk_s.p, ps.phi_s.p, ps.dt.p, ps.time.p);
     if (initial)
         GPLUM_HIP(hipMemcpyAsync(ps.acc_d.p, nb.acc_d_new.p, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
-    // accepted-neighbour CSR
+    // accepted-neighbour CSR (accepted <= candidates: the index array is sized by the
+    // candidates, so nothing waits for the accepted total before the compaction)
     if ((rc = nb_scan(s, ps.number.p, nb.nbr_off.p, n, st))) return rc;
-    int nn = 0;
-    if ((rc = last_sum(ps.number.p, nb.nbr_off.p, n, &nn, st))) return rc;
-    GPLUM_HIP(hipMemcpyAsync(nb.nbr_off.p + n, &nn, sizeof(int), hipMemcpyHostToDevice, st));
-    GPLUM_ENSURE(nb.nbr_idx, nn + 1);
-    hipLaunchKernelGGL(k_compact_rows, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, nb.row_off.p,
+    GPLUM_ENSURE(nb.info, 2);
+    hipLaunchKernelGGL(k_finalize_totals, dim3(1), dim3(64), 0, st, n, ncand, nb.number_fp32.p, nb.row_off.p,
+                       ps.number.p, nb.nbr_off.p, nb.info.p);
+    GPLUM_ENSURE(nb.nbr_idx, (size_t)ncand + 1);
+    hipLaunchKernelGGL(k_compact_rows, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ncand, nb.row_off.p,
                        nb.number_fp32.p, nb.keys.p, nb.accepted.p, nb.nbr_off.p, nb.nbr_idx.p);
-    nb.n_nbr = nn;
-    nb.n = n;
     GPLUM_HIP(hipGetLastError());
+    int32_t info[2] = {0, 0};
+    GPLUM_HIP(hipMemcpyAsync(info, nb.info.p, sizeof(info), hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
+    if (info[0]) return fail(GPLUM_E_STATE, "candidate count mismatch (FP32 counts vs candidates found)");
+    nb.n_nbr = info[1];
+    nb.n = n;
     return GPLUM_OK;
 }
 

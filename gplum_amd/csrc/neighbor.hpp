@@ -19,6 +19,7 @@ struct NbState {
     DArray<uint64_t> keys_in, keys;
     DArray<unsigned char> accepted;
     DArray<int32_t> row_off, number_fp32, nbr_off, nbr_idx;
+    DArray<int32_t> info;               // device: candidate-total check, accepted total
     DArray<double> acc_d_new;
     int n = 0, n_nbr = 0;
 };

@@ -41,22 +41,36 @@ int ParticleSoA::ensure(int n) {
 
 // ----------------------------------------------------------------- kernels
 
-template <class T, int K>
-__global__ void k_gather(int n, const int32_t *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
+// Every particle array at once (one launch instead of one per array): dst[i] = src[perm[i]].
+struct SoAPtrs {
+    double *v3[7];      // pos, vel, acc, acc_d, jerk_d, acc_s, jerk_s
+    double *v1[13];     // mass, r_planet, f, phi, phi_d, phi_s, r_out, r_out_inv, r_search, v_disp, time, dt, acc0
+    int64_t *l[2];      // id, id_cluster
+    int32_t *w[6];      // number, rank, id_max, id_min, n_cluster, flags
+};
+
+static SoAPtrs soa_ptrs(ParticleSoA &A) {
+    return SoAPtrs{{A.pos.p, A.vel.p, A.acc.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p},
+                   {A.mass.p, A.r_planet.p, A.f.p, A.phi.p, A.phi_d.p, A.phi_s.p, A.r_out.p, A.r_out_inv.p,
+                    A.r_search.p, A.v_disp.p, A.time.p, A.dt.p, A.acc0.p},
+                   {A.id.p, A.id_cluster.p},
+                   {A.number.p, A.rank.p, A.id_max.p, A.id_min.p, A.n_cluster.p, A.flags.p}};
+}
+
+__global__ void k_gather_all(int n, const int32_t *__restrict__ perm, SoAPtrs src, SoAPtrs dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = perm[i];
 #pragma unroll
-    for (int k = 0; k < K; k++) dst[K * i + k] = src[K * s + k];
-}
-
-template <class T, int K>
-__global__ void k_scatter(int n, const int32_t *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int d = perm[i];
+    for (int k = 0; k < 7; k++)
 #pragma unroll
-    for (int k = 0; k < K; k++) dst[K * d + k] = src[K * i + k];
+        for (int c = 0; c < 3; c++) dst.v3[k][3 * i + c] = src.v3[k][3 * s + c];
+#pragma unroll
+    for (int k = 0; k < 13; k++) dst.v1[k][i] = src.v1[k][s];
+#pragma unroll
+    for (int k = 0; k < 2; k++) dst.l[k][i] = src.l[k][s];
+#pragma unroll
+    for (int k = 0; k < 6; k++) dst.w[k][i] = src.w[k][s];
 }
 
 __global__ void k_jrec(int n, const double *__restrict__ pos, const double *__restrict__ mass,
@@ -313,21 +327,8 @@ int System::gather_all(const int32_t *perm, int m) {
     int rc = B.ensure(std::max(m, 1));
     if (rc) return rc;
     const int gb = ceil_div(m, 256);
-    if (m > 0) {
-#define GA3(f) hipLaunchKernelGGL((k_gather<double, 3>), dim3(gb), dim3(256), 0, st, m, perm, A.f.p, B.f.p)
-#define GA1(f) hipLaunchKernelGGL((k_gather<double, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.f.p, B.f.p)
-        GA3(pos); GA3(vel); GA3(acc); GA3(acc_d); GA3(jerk_d); GA3(acc_s); GA3(jerk_s);
-        GA1(mass); GA1(r_planet); GA1(f); GA1(phi); GA1(phi_d); GA1(phi_s); GA1(r_out); GA1(r_out_inv);
-        GA1(r_search); GA1(v_disp); GA1(time); GA1(dt); GA1(acc0);
-#undef GA3
-#undef GA1
-        hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.id.p, B.id.p);
-        hipLaunchKernelGGL((k_gather<int64_t, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.id_cluster.p,
-                           B.id_cluster.p);
-#define GI(f) hipLaunchKernelGGL((k_gather<int32_t, 1>), dim3(gb), dim3(256), 0, st, m, perm, A.f.p, B.f.p)
-        GI(number); GI(rank); GI(id_max); GI(id_min); GI(n_cluster); GI(flags);
-#undef GI
-    }
+    if (m > 0)
+        hipLaunchKernelGGL(k_gather_all, dim3(gb), dim3(256), 0, st, m, perm, soa_ptrs(A), soa_ptrs(B));
     GPLUM_HIP(hipGetLastError());
     A.swap(B);
     n = m;
