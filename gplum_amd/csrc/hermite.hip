@@ -387,7 +387,7 @@ __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int 
 __device__ inline void coop_pair_sum(const HP &P, const JData &J, int n, int64_t idi, const double xi[3],
                                      const double vi[3], double rinv_i, double acc[3], double jerk[3]) {
     double a[3] = {0., 0., 0.}, jk[3] = {0., 0., 0.};
-    // two pairs at a time: the four-wave kernels hold more state per lane
+    // two pairs at a time: the eight-wave kernels hold more state per lane
     pair_sum_strided<2>(P, J, (int)(threadIdx.x & 63), 64, n, idi, xi, vi, rinv_i, a, jk, nullptr);
     for (int d = 32; d > 0; d >>= 1)
         for (int k = 0; k < 3; k++) {
