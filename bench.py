@@ -268,7 +268,7 @@ def bench_large(args):
         "breakdown_ms_per_step": {k: round(st[k] / args.steps, 3) for k in ("force_ms", "tree_ms", "finalize_ms",
                                                                              "hard_ms")},
         "hard": {"clusters": st["n_clusters"], "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
-                 "max_block_steps_one_cluster": st["max_block_steps"], "collisions": st["n_collisions"]},
+                 "max_block_steps_one_cluster": st["max_block_steps"], "members_of_that_cluster": st["max_block_cluster"], "collisions": st["n_collisions"]},
         "forecast": {"calls": st["fc_calls"], "groups_per_call": st["n_groups"],
                      "overflowed_groups": st["fc_ovf_groups"], "hit_rate": round(hit, 5),
                      "ms_per_step": round(st["fc_ms"] / args.steps, 3)},
@@ -454,7 +454,7 @@ def main():
                                                                              "finalize_ms", "hard_ms")},
         "hard": {"clusters": st["n_clusters"], "cluster_particles": st["n_cluster_particles"],
                  "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
-                 "max_block_steps_one_cluster": st["max_block_steps"], "collisions": st["n_collisions"]},
+                 "max_block_steps_one_cluster": st["max_block_steps"], "members_of_that_cluster": st["max_block_cluster"], "collisions": st["n_collisions"]},
         # the kernel with the most time in the step: FP64 Hermite, bound by the chain of
         # block steps of the longest cluster (one wave), not by the chip's FP64 rate
         "roofline_hermite": {"bound": "latency (longest cluster's block-step chain)",

@@ -50,11 +50,24 @@ __host__ __device__ inline double smax(double a, double b) { return (a < b) ? b 
 // fmod(x, d) for x >= 0.  When d is a power of two (every block step is: dt_tree is
 // one and the steps are its halvings), x / d is an exact scaling, so
 // x - floor(x / d) * d is exact and equals fmod bit for bit at a fraction of its cost;
-// other divisors take fmod itself.
+// other divisors take fmod itself.  The scaling is a multiplication by 2^(1-e)
+// (d = 2^(e-1)): x * 2^(1-e) and x / d round the same real number, so they are equal,
+// and the multiplication skips the division sequence.
 __host__ __device__ inline double fmod_block(double x, double d) {
     int e = 0;
-    if (x >= 0. && d > 0. && frexp(d, &e) == 0.5) return x - floor(x / d) * d;
+    if (x >= 0. && d > 0. && frexp(d, &e) == 0.5) {
+        const double q = (e > -1000) ? x * ldexp(1.0, 1 - e) : x / d;
+        return x - floor(q) * d;
+    }
     return fmod(x, d);
+}
+
+// 1 / d, by a scaling when d is a power of two (the same correctly rounded value as the
+// division, as in fmod_block), else by the division.
+__host__ __device__ inline double recip_block(double d) {
+    int e = 0;
+    if (d > 0. && frexp(d, &e) == 0.5 && e > -1000) return ldexp(1.0, 1 - e);
+    return 1. / d;
 }
 
 // x^(1/3) for finite x > 0 from IEEE-exact operations only (frexp, ldexp, +, -, *, /):

@@ -94,17 +94,55 @@ __device__ inline double dt_initial(const HP &P, double time, double dt, double 
     return dt_next;
 }
 
-// FPHard::calcDeltat, src/particle.h:1276-1314.
+// calc_dt4 with the hardware square root and reciprocal (v_sqrt_f64, v_rcp_f64, not
+// correctly rounded; Acc * Snap as sqrt(Acc2 * Snap2), Jerk * Crac likewise): within a
+// few 1e-7 of calc_dt4 relative, DBL_MAX exactly where calc_dt4 gives it (Jerk > 0 iff
+// Jerk2 > 0).  A product that underflows (0 from non-zero factors, or subnormal) gives
+// NaN, which no grid test passes.
+__device__ inline double calc_dt4_fast(double eta, double alpha2, double acc0, const double a[3],
+                                       const double j[3], const double s[3], const double c[3]) {
+    const double Acc2 = dot3(a[0], a[1], a[2], a[0], a[1], a[2]) + alpha2 * acc0 * acc0;
+    const double Jerk2 = dot3(j[0], j[1], j[2], j[0], j[1], j[2]);
+    const double Snap2 = dot3(s[0], s[1], s[2], s[0], s[1], s[2]);
+    const double Crac2 = dot3(c[0], c[1], c[2], c[0], c[1], c[2]);
+    if (!(Jerk2 > 0.)) return DBL_MAX;
+    const double as2 = Acc2 * Snap2, jc2 = Jerk2 * Crac2;
+    const bool ok = (as2 >= DBL_MIN || (Acc2 == 0. || Snap2 == 0.)) && (jc2 >= DBL_MIN || Crac2 == 0.);
+    const double num = __builtin_amdgcn_sqrt(as2) + Jerk2;
+    const double den = __builtin_amdgcn_sqrt(jc2) + Snap2;
+    return ok ? eta * __builtin_amdgcn_sqrt(num * __builtin_amdgcn_rcp(den)) : __builtin_nan("");
+}
+
+// True when t is farther than 2^-10 relative from every d * 2^k (k integer), for
+// t > 2^-1000 and finite: then t and any value within 2^-11 of it relative lie on the
+// same side of each of those grid points.  t = m_t 2^e_t, d = m_d 2^e_d with mantissas
+// in [0.5, 1): the grid points nearest t are at m_t / m_d in {1/2, 1, 2}.
+__device__ inline bool clear_of_grid(double t, double d) {
+    if (!(t > 0x1p-1000 && t < DBL_MAX)) return false;
+    int et = 0, ed = 0;
+    const double r = frexp(t, &et) * __builtin_amdgcn_rcp(frexp(d, &ed));
+    const double tol = 0x1p-10;
+    return r > 0.5 * (1. + tol) && (r < 1. - tol || r > 1. + tol) && r < 2. * (1. - tol);
+}
+
+// FPHard::calcDeltat, src/particle.h:1276-1314.  dt_1 enters the result only through
+// dt_1 < dt_next for dt_next = D, D/2, D/4, ... (D the step left by the fmod loop,
+// halvings exact above 2^-1000), so a dt_1 from calc_dt4_fast that is clear of that
+// grid decides every comparison as the correctly rounded dt_1 does; near the grid
+// calc_dt4 itself is evaluated.  Same result, bit for bit.
 __device__ inline double dt_block(const HP &P, double time, double dt, double acc0,
                                   const double acc_d[3], const double jerk_d[3], const double a2_d[3],
                                   const double a3_d[3], const double acc_s[3], const double jerk_s[3],
                                   const double a2_s[3], const double a3_s[3]) {
     double dt_next = dmin(2. * dt, 0.5 * P.dt_tree);
-    const double dt_1 = dmin(calc_dt4(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
-                             calc_dt4(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
     double rem = fmod_block(time, dt_next);
     for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(time, dt_next); }
     if (rem != 0.0) return P.dt_min;
+    double dt_1 = dmin(calc_dt4_fast(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
+                       calc_dt4_fast(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
+    if (!clear_of_grid(dt_1, dt_next))
+        dt_1 = dmin(calc_dt4(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
+                    calc_dt4(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
     for (int it = 0; dt_1 < dt_next && 0.5 * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
     if (dt_next < 2. * P.dt_min) dt_next = P.dt_min;
     return dt_next;
@@ -217,7 +255,9 @@ __device__ inline void gravity_loop(const HP &P, const JData &J, int n, int64_t 
 // j = k + (k >= i) for k = 0 .. n-2, ascending j as in gravity_loop, so every lane runs
 // the same n-1 iterations (the loop over all j with a self test would run n iterations
 // with one lane masked off in each).  A partner with the same id (never, before any
-// merger) is evaluated and not added, as gravity_loop skips it.
+// merger) is evaluated and not added, as gravity_loop skips it; CHECK = false leaves
+// the test out, for clusters whose ids are known to be distinct.
+template <bool CHECK = true>
 __device__ inline void gravity_loop_lane(const HP &P, const JData &J, int n, int i, int64_t idi,
                                          const double xi[3], const double vi[3], double rinv_i,
                                          double acc[3], double jerk[3], double *phi) {
@@ -235,7 +275,7 @@ __device__ inline void gravity_loop_lane(const HP &P, const JData &J, int n, int
         }
 #pragma unroll
         for (int u = 0; u < HARD_ILP; u++) {
-            const bool use = J.id[jj[u]] != idi;
+            const bool use = !CHECK || J.id[jj[u]] != idi;
             for (int d = 0; d < 3; d++) {
                 acc[d] = use ? acc[d] + ta[u][d] : acc[d];
                 jerk[d] = use ? jerk[d] + tj[u][d] : jerk[d];
@@ -247,12 +287,73 @@ __device__ inline void gravity_loop_lane(const HP &P, const JData &J, int n, int
         const int j = k + (k >= i ? 1 : 0);
         double ta[3], tj[3], tp;
         pair_term_j(P, J, j, xi, vi, rinv_i, ta, tj, phi ? &tp : nullptr);
-        const bool use = J.id[j] != idi;
+        const bool use = !CHECK || J.id[j] != idi;
         for (int d = 0; d < 3; d++) {
             acc[d] = use ? acc[d] + ta[d] : acc[d];
             jerk[d] = use ? jerk[d] + tj[d] : jerk[d];
         }
         if (phi) *phi = use ? *phi - tp : *phi;
+    }
+}
+
+// star_acc_jerk and gravity_loop_lane<false> of lane i in a cluster of M + 1 particles,
+// as straight-line code: the Sun term's and the pairs' dependency chains (square root
+// and division sequences) sit in one basic block, where the compiler interleaves them;
+// a lone wave has nothing else to hide their latency behind.  Same operations, and the
+// pair sums in the same order (from zero, ascending j).
+template <int M>
+__device__ inline void force_lane_fixed(const HP &P, const JData &J, int i, const double xi[3],
+                                        const double vi[3], double rinv_i, double as[3], double js[3],
+                                        double ad[3], double jd[3]) {
+    double ta[M][3], tj[M][3];
+#pragma unroll
+    for (int k = 0; k < M; k++) pair_term_j(P, J, k + (k >= i ? 1 : 0), xi, vi, rinv_i, ta[k], tj[k], nullptr);
+    star_acc_jerk(P, xi, vi, as, js, nullptr);
+    for (int d = 0; d < 3; d++) { ad[d] = 0.; jd[d] = 0.; }
+#pragma unroll
+    for (int k = 0; k < M; k++)
+        for (int d = 0; d < 3; d++) { ad[d] += ta[k][d]; jd[d] += tj[k][d]; }
+}
+
+// Partner j's predicted state (position, velocity, mass, 1/r_out), held in registers
+// for the three force evaluations of a block step.
+struct JReg {
+    double x[3], v[3], m, ri;
+};
+__device__ inline void load_jreg(const JData &J, int j, JReg &r) {
+    for (int d = 0; d < 3; d++) { r.x[d] = J.x[3 * j + d]; r.v[d] = J.v[3 * j + d]; }
+    r.m = J.m[j];
+    r.ri = J.ri[j];
+}
+
+// force_lane_fixed<M> on partner data in registers.
+template <int M>
+__device__ inline void force_jreg(const HP &P, const JReg (&pj)[M], const double xi[3], const double vi[3],
+                                  double rinv_i, double as[3], double js[3], double ad[3], double jd[3]) {
+    double ta[M][3], tj[M][3];
+#pragma unroll
+    for (int k = 0; k < M; k++)
+        pair_term(P, xi, vi, rinv_i, pj[k].x[0], pj[k].x[1], pj[k].x[2], pj[k].v[0], pj[k].v[1], pj[k].v[2],
+                  pj[k].m, pj[k].ri, ta[k], tj[k], nullptr);
+    star_acc_jerk(P, xi, vi, as, js, nullptr);
+    for (int d = 0; d < 3; d++) { ad[d] = 0.; jd[d] = 0.; }
+#pragma unroll
+    for (int k = 0; k < M; k++)
+        for (int d = 0; d < 3; d++) { ad[d] += ta[k][d]; jd[d] += tj[k][d]; }
+}
+
+// Sun and hard force of lane i (ids distinct): straight-line for clusters of up to five.
+__device__ inline void force_lane(const HP &P, const JData &J, int n, int i, const double xi[3],
+                                  const double vi[3], double rinv_i, double as[3], double js[3],
+                                  double ad[3], double jd[3]) {
+    switch (n - 1) {
+    case 1: force_lane_fixed<1>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
+    case 2: force_lane_fixed<2>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
+    case 3: force_lane_fixed<3>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
+    case 4: force_lane_fixed<4>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
+    default:
+        star_acc_jerk(P, xi, vi, as, js, nullptr);
+        gravity_loop_lane<false>(P, J, n, i, 0, xi, vi, rinv_i, ad, jd, nullptr);
     }
 }
 
@@ -292,7 +393,7 @@ __device__ inline void predict(HardView &H, int g, double Dt) {
 // FPHard::correct, src/particle.h:1208-1274 (quirk Q1).
 __device__ inline void correct(HardView &H, int g, double Dt) {
     const double Dt2 = Dt * Dt;
-    const double Dt_inv = 1. / Dt;
+    const double Dt_inv = recip_block(Dt);
     const double Dt_inv2 = Dt_inv * Dt_inv;
     const double Dt_inv3 = Dt_inv2 * Dt_inv;
     for (int k = 0; k < 3; k++) {
@@ -340,7 +441,7 @@ __device__ inline void correct_regs(double Dt, const double xp[3], const double 
                                     const double a0d[3], const double j0d[3], const double jd[3], double pos[3],
                                     double vel[3], double a2s[3], double a3s[3], double a2d[3], double a3d[3]) {
     const double Dt2 = Dt * Dt;
-    const double Dt_inv = 1. / Dt;
+    const double Dt_inv = recip_block(Dt);
     const double Dt_inv2 = Dt_inv * Dt_inv;
     const double Dt_inv3 = Dt_inv2 * Dt_inv;
     for (int k = 0; k < 3; k++) {
@@ -581,6 +682,19 @@ __device__ inline void wave_min2(double &u, double &v) {
     u = dmin(u, dpp_f64<DPP_ROW_MIRROR>(u)); v = dmin(v, dpp_f64<DPP_ROW_MIRROR>(v));
     u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, 16)), dmin(readlane_f64(u, 32), readlane_f64(u, 48)));
     v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, 16)), dmin(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+// wave_min2 where only lanes below n can hold values under the others' 1e10 (n is
+// wave-uniform): clusters of up to four read their lanes directly.
+__device__ inline void wave_min2_n(double &u, double &v, int n) {
+    if (n <= 2) {
+        u = dmin(readlane_f64(u, 0), readlane_f64(u, 1));
+        v = dmin(readlane_f64(v, 0), readlane_f64(v, 1));
+    } else if (n <= 4) {
+        u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, 1)), dmin(readlane_f64(u, 2), readlane_f64(u, 3)));
+        v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, 1)), dmin(readlane_f64(v, 2), readlane_f64(v, 3)));
+    } else {
+        wave_min2(u, v);
+    }
 }
 __device__ inline void wave_argmin(double &v, int &i, int &j) {
     for (int d = 32; d > 0; d >>= 1) {
@@ -902,11 +1016,12 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
 __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
                                double *s_m, double *s_ri, int64_t *s_id, double *s_fr, int b, int n,
                                double &sys_time, long long &loop, double &ed, int &n_col, long long &pairs) {
+    n = __builtin_amdgcn_readfirstlane(n);   // wave-uniform: scalar branches on the size
     const int t = threadIdx.x;
     const bool own = t < n;
     const int g = b + t;
     double x[3] = {0., 0., 0.}, v[3] = {0., 0., 0.}, as[3], js[3], ad[3], jd[3];
-    double time = 0., dt = 0., acc0 = 0., rinv = 0.;
+    double time = 0., dt = 0., acc0 = 0., rinv = 0., fr = 0.;
     int64_t id = -1;
     if (own) {
         ld3(H.pos, g, x);
@@ -919,15 +1034,31 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         s_m[t] = H.mass[g];
         s_ri[t] = rinv;
         s_id[t] = id;
-        s_fr[t] = H.f[g] * H.r_planet[g];
+        fr = H.f[g] * H.r_planet[g];
+        s_fr[t] = fr;
         st3(s_x, t, x);
         st3(s_v, t, v);
     }
     __syncthreads();
-    // initial calcGravity (hermite.h:259-270)
-    if (own) {
+    // The clusters the reference forms have distinct ids, and the partner loops below
+    // leave the id test out; a cluster with a repeated id takes the global-memory path
+    // from the start (the non-register kernels' preamble, then integrate_from).
+    // (One call site of integrate_from serves both hand-overs: a second inlined copy
+    // would double the kernel's register allocation.)
+    bool rep = false;
+    if (own)
+        for (int j = 0; j < n; j++) rep = rep || (j != t && s_id[j] == id);
+    const bool from_start = __ballot(rep) != 0;
+    if (from_start) {
+        if (own) {
+            gravity_full(P, H, b, n, t);
+            H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+            H.twin_of[g] = -1;
+        }
+    } else if (own) {
+        // initial calcGravity (hermite.h:259-270)
         star_acc_jerk(P, x, v, as, js, nullptr);
-        gravity_loop_lane(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
+        gravity_loop_lane<false>(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
     }
     const double time_end = P.dt_tree;
     double a0s[3], j0s[3], a0d[3], j0d[3], xp[3], vp[3], a2s[3], a3s[3], a2d[3], a3d[3];
@@ -935,7 +1066,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     // the system time at the end of each step
     double t_s = wave_min(own ? time + dt : 1.e10);
     bool handover = false;
-    while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
+    while (!from_start && sys_time < time_end && loop < BLOCK_STEP_CAP) {
         __syncthreads();   // the previous step's j-data reads are done
         if (own) {
             predict_regs(t_s - time, x, v, as, js, ad, jd, a0s, j0s, a0d, j0d, xp, vp);
@@ -949,11 +1080,11 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             if (own) {
                 for (int k = 0; k < n - 1; k++) {
                     const int j = k + (k >= t ? 1 : 0);   // the partners, ascending
-                    const double dx = s_x[3 * t] - s_x[3 * j];
-                    const double dy = s_x[3 * t + 1] - s_x[3 * j + 1];
-                    const double dz = s_x[3 * t + 2] - s_x[3 * j + 2];
+                    const double dx = xp[0] - s_x[3 * j];
+                    const double dy = xp[1] - s_x[3 * j + 1];
+                    const double dz = xp[2] - s_x[3 * j + 2];
                     const double d2 = dot3(dx, dy, dz, dx, dy, dz);
-                    const double r2 = s_fr[t] + s_fr[j];
+                    const double r2 = fr + s_fr[j];
                     if (!(d2 < COL_FILTER * r2 * r2)) continue;   // R >= 1: never the minimum
                     const double Rr = sqrt(d2) / r2;
                     if (Rr < rmin) { rmin = Rr; bi = t; bj = j; }
@@ -965,26 +1096,44 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         }
         if (own && time + dt == t_s) {
             pairs += 3 * (long long)(n - 1);
-            star_acc_jerk(P, xp, vp, as, js, nullptr);
-            gravity_loop_lane(P, J, n, t, id, xp, vp, rinv, ad, jd, nullptr);
             const double t_c = t_s - time;
-            for (int ite = 0; ite < 2; ite++) {
-                correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
-                star_acc_jerk(P, x, v, as, js, nullptr);
-                gravity_loop_lane(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
+            if (n == 2) {
+                // the partner's predicted state is read once for the three evaluations
+                JReg pj[1];
+                load_jreg(J, t == 0 ? 1 : 0, pj[0]);
+                force_jreg<1>(P, pj, xp, vp, rinv, as, js, ad, jd);
+                for (int ite = 0; ite < 2; ite++) {
+                    correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                    force_jreg<1>(P, pj, x, v, rinv, as, js, ad, jd);
+                }
+            } else if (n == 3) {
+                JReg pj[2];
+                load_jreg(J, t == 0 ? 1 : 0, pj[0]);
+                load_jreg(J, t == 2 ? 1 : 2, pj[1]);
+                force_jreg<2>(P, pj, xp, vp, rinv, as, js, ad, jd);
+                for (int ite = 0; ite < 2; ite++) {
+                    correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                    force_jreg<2>(P, pj, x, v, rinv, as, js, ad, jd);
+                }
+            } else {
+                force_lane(P, J, n, t, xp, vp, rinv, as, js, ad, jd);
+                for (int ite = 0; ite < 2; ite++) {
+                    correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                    force_lane(P, J, n, t, x, v, rinv, as, js, ad, jd);
+                }
             }
             time += t_c;
             dt = dt_block(P, time, dt, acc0, ad, jd, a2d, a3d, as, js, a2s, a3s);
         }
         sys_time = own ? time : 1.e10;
         t_s = own ? time + dt : 1.e10;
-        wave_min2(sys_time, t_s);
+        wave_min2_n(sys_time, t_s, n);
         loop++;
     }
-    if (handover) {
+    if (handover || from_start) {
         // a collision in this block step: hand the cluster to the global-memory path
         // at the start of the step (kept out of the loop, which then holds less state)
-        if (own) {
+        if (handover && own) {
             st3(H.pos, g, x); st3(H.vel, g, v);
             st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
             st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
@@ -1008,7 +1157,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     if (own) {
         double phs = 0., phd = 0.;
         star_acc_jerk(P, x, v, as, js, &phs);
-        gravity_loop_lane(P, J, n, t, id, x, v, rinv, ad, jd, &phd);
+        gravity_loop_lane<false>(P, J, n, t, id, x, v, rinv, ad, jd, &phd);
         pairs += n - 1;
         st3(H.pos, g, x); st3(H.vel, g, v);
         st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
@@ -1067,6 +1216,12 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         const int q = s_cl;
         __syncthreads();
         if (q >= ncl) break;
+        // The queue hands out the costliest clusters first (estimated block steps times
+        // size).  Their waves take their SIMD's issue slots first, so that the longest
+        // chains of block steps, which bound the stage, run close to lone-wave speed
+        // while the waves beside them fill the remaining cycles.
+        if (q < HARD_PRIO_CLUSTERS) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
         const int c = order ? order[q] : q;
         const int b = cl_off[c];
         const int n = cl_off[c + 1] - b;
@@ -1220,7 +1375,7 @@ __global__ void k_isolated(HP P, IsoView V, int n, double time_start, double tim
             }
             star_acc_jerk(P, xp, vp, as, js, nullptr);
             for (int ite = 0; ite < 2; ite++) {
-                const double Dt2 = Dt * Dt, Dt_inv = 1. / Dt, Dt_inv2 = Dt_inv * Dt_inv, Dt_inv3 = Dt_inv2 * Dt_inv;
+                const double Dt2 = Dt * Dt, Dt_inv = recip_block(Dt), Dt_inv2 = Dt_inv * Dt_inv, Dt_inv3 = Dt_inv2 * Dt_inv;
                 for (int k = 0; k < 3; k++) {
                     const double Am_s = as[k] - a0s[k], J0_s = Dt * j0s[k], J1_s = Dt * js[k];
                     const double A2_s = 3. * Am_s - (J1_s + 2. * J0_s), A3_s = -2. * Am_s + (J1_s + J0_s);

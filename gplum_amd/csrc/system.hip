@@ -588,13 +588,18 @@ int System::hard_part(bool persistent) {
         GPLUM_HIP(hipMemcpyAsync(nbk.data(), hb.nblock.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
         std::vector<long long> npr(ncl);
         GPLUM_HIP(hipMemcpyAsync(npr.data(), hb.npair.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
+        std::vector<int32_t> off(ncl + 1);
+        GPLUM_HIP(hipMemcpyAsync(off.data(), cl.off.p, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
         for (int c = 0; c < ncl; c++) {
             if (nbk[c] < 0) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
             step_ncol += nc[c];
             step_edisp += ed[c];
             stats.block_steps += nbk[c];
-            stats.max_block_steps = std::max<int64_t>(stats.max_block_steps, nbk[c]);
+            if (nbk[c] > stats.max_block_steps) {
+                stats.max_block_steps = nbk[c];
+                stats.max_block_cluster = off[c + 1] - off[c];
+            }
             stats.hard_pairs += npr[c];
         }
     }

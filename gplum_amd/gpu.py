@@ -145,7 +145,7 @@ class Stats(ctypes.Structure):
         "n_neighbors",
         "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
         "cand_retries", "n_force_calls", "fc_calls", "fc_slots", "fc_ovf_groups",
-        "fc_ovf_candidates", "hard_pairs", "max_block_steps")] + [(n, ctypes.c_double) for n in (
+        "fc_ovf_candidates", "hard_pairs", "max_block_steps", "max_block_cluster")] + [(n, ctypes.c_double) for n in (
             "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms")]
 
     def as_dict(self):

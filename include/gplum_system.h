@@ -53,6 +53,7 @@ typedef struct gplum_stats {
     int64_t fc_ovf_candidates;                /* candidates redone by the fallback pass    */
     int64_t hard_pairs;                       /* Hermite pair evaluations (hard_pair calls) */
     int64_t max_block_steps;                  /* most block steps of one cluster (max over calls) */
+    int64_t max_block_cluster;                /* members of that cluster                   */
     double force_ms, tree_ms, finalize_ms, hard_ms;
     double time_sys;
     double fc_loss;                           /* last training loss (Poisson NLL)          */

@@ -97,6 +97,27 @@ def test_hermite_perfect_merging_matches_oracle(oracle_lib):
         assert np.array_equal(got[f], ref[f]), f
 
 
+def test_repeated_ids_match_oracle(oracle_lib):
+    """A cluster member that repeats another member's id is skipped as a partner (the
+    oracle's same-id test); such clusters leave the register path for the global-memory
+    path, which keeps the test."""
+    po, pg, pp, off = make_clusters(oracle_lib, 30, seed=17, size_max=40, collision=0)
+    rep = 0
+    for c in range(len(off) - 1):
+        if off[c + 1] - off[c] >= 3 and c % 2 == 0:
+            pp["id"][off[c] + 2] = pp["id"][off[c]]
+            rep += 1
+    assert rep > 3
+    ref = pp.copy()
+    nc, ed, nb_ref = oracle_lib.hermite_clusters(po, ref, off)
+    got = pp.copy()
+    ed_g, nb_g, ms = gpu.hard_integrate(pg, got, off, persistent=True)
+    print("clusters with a repeated id %d, block steps %d (oracle %d)" % (rep, nb_g, nb_ref))
+    assert nb_g == nb_ref
+    for f in ("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_s", "dt", "time"):
+        assert np.array_equal(got[f], ref[f]), f
+
+
 def _same_state(a, b, fields=("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_d", "phi_s", "dt", "time",
                               "mass", "r_planet", "id", "flags")):
     return [f for f in fields if not np.array_equal(a[f], b[f])]
