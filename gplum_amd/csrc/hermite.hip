@@ -296,25 +296,6 @@ __device__ inline void gravity_loop_lane(const HP &P, const JData &J, int n, int
     }
 }
 
-// star_acc_jerk and gravity_loop_lane<false> of lane i in a cluster of M + 1 particles,
-// as straight-line code: the Sun term's and the pairs' dependency chains (square root
-// and division sequences) sit in one basic block, where the compiler interleaves them;
-// a lone wave has nothing else to hide their latency behind.  Same operations, and the
-// pair sums in the same order (from zero, ascending j).
-template <int M>
-__device__ inline void force_lane_fixed(const HP &P, const JData &J, int i, const double xi[3],
-                                        const double vi[3], double rinv_i, double as[3], double js[3],
-                                        double ad[3], double jd[3]) {
-    double ta[M][3], tj[M][3];
-#pragma unroll
-    for (int k = 0; k < M; k++) pair_term_j(P, J, k + (k >= i ? 1 : 0), xi, vi, rinv_i, ta[k], tj[k], nullptr);
-    star_acc_jerk(P, xi, vi, as, js, nullptr);
-    for (int d = 0; d < 3; d++) { ad[d] = 0.; jd[d] = 0.; }
-#pragma unroll
-    for (int k = 0; k < M; k++)
-        for (int d = 0; d < 3; d++) { ad[d] += ta[k][d]; jd[d] += tj[k][d]; }
-}
-
 // Partner j's predicted state (position, velocity, mass, 1/r_out), held in registers
 // for the three force evaluations of a block step.
 struct JReg {
@@ -340,21 +321,6 @@ __device__ inline void force_jreg(const HP &P, const JReg (&pj)[M], const double
 #pragma unroll
     for (int k = 0; k < M; k++)
         for (int d = 0; d < 3; d++) { ad[d] += ta[k][d]; jd[d] += tj[k][d]; }
-}
-
-// Sun and hard force of lane i (ids distinct): straight-line for clusters of up to five.
-__device__ inline void force_lane(const HP &P, const JData &J, int n, int i, const double xi[3],
-                                  const double vi[3], double rinv_i, double as[3], double js[3],
-                                  double ad[3], double jd[3]) {
-    switch (n - 1) {
-    case 1: force_lane_fixed<1>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
-    case 2: force_lane_fixed<2>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
-    case 3: force_lane_fixed<3>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
-    case 4: force_lane_fixed<4>(P, J, i, xi, vi, rinv_i, as, js, ad, jd); break;
-    default:
-        star_acc_jerk(P, xi, vi, as, js, nullptr);
-        gravity_loop_lane<false>(P, J, n, i, 0, xi, vi, rinv_i, ad, jd, nullptr);
-    }
 }
 
 // calcGravity (full, with phi_d and Sun phi), gravity_hard.h:203-256, current pos/vel.
@@ -1006,6 +972,60 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
     __syncthreads();
 }
 
+// Pairs side by side for clusters of 3 to 8 particles (n (n - 1) <= 64): lane
+// L = i (n - 1) + k evaluates the pair of particle i with its k-th partner
+// j = k + (k >= i), so one pair_term latency covers all pairs of the block step; the
+// owner lane i adds its n - 1 summands from the LDS from zero in ascending j, the sum
+// of gravity_loop_lane.  Lanes of inactive particles' pairs stay idle.
+struct PairLane {
+    bool on;          // lane L < n (n - 1)
+    int i, j;
+    double m_j, ri_i, ri_j;
+};
+__device__ inline PairLane pair_lane(const double *s_m, const double *s_ri, int n, int L) {
+    PairLane q{false, 0, 0, 0., 0., 0.};
+    const int m = n - 1;
+    if (L < n * m) {
+        q.on = true;
+        q.i = L / m;
+        const int k = L - q.i * m;
+        q.j = k + (k >= q.i ? 1 : 0);
+        q.m_j = s_m[q.j];
+        q.ri_i = s_ri[q.i];
+        q.ri_j = s_ri[q.j];
+    }
+    return q;
+}
+
+// Sun and hard force of the active owners (oact) at (xo, vo); pact: this lane's pair
+// belongs to an active particle; xj, vj: its partner's predicted state.  s_io holds
+// 6 doubles per particle, s_tm 6 per lane.
+__device__ inline void force_pairs(const HP &P, const PairLane &q, bool oact, bool pact, int t, int n,
+                                   const double xo[3], const double vo[3], const double xj[3], const double vj[3],
+                                   double *s_io, double *s_tm, double as[3], double js[3], double ad[3],
+                                   double jd[3]) {
+    if (oact)
+        for (int d = 0; d < 3; d++) { s_io[6 * t + d] = xo[d]; s_io[6 * t + 3 + d] = vo[d]; }
+    wave_sync();
+    if (pact) {
+        double xi[3], vi[3], ta[3], tj[3];
+        for (int d = 0; d < 3; d++) { xi[d] = s_io[6 * q.i + d]; vi[d] = s_io[6 * q.i + 3 + d]; }
+        pair_term(P, xi, vi, q.ri_i, xj[0], xj[1], xj[2], vj[0], vj[1], vj[2], q.m_j, q.ri_j, ta, tj, nullptr);
+        for (int d = 0; d < 3; d++) { s_tm[6 * t + d] = ta[d]; s_tm[6 * t + 3 + d] = tj[d]; }
+    }
+    if (oact) star_acc_jerk(P, xo, vo, as, js, nullptr);
+    wave_sync();
+    if (oact) {
+        const int m = n - 1;
+        for (int d = 0; d < 3; d++) { ad[d] = 0.; jd[d] = 0.; }
+        for (int k = 0; k < m; k++) {
+            const int L = t * m + k;
+            for (int d = 0; d < 3; d++) { ad[d] += s_tm[6 * L + d]; jd[d] += s_tm[6 * L + 3 + d]; }
+        }
+    }
+    wave_sync();   // summands read before the next evaluation writes them
+}
+
 // Clusters that fit one wave: lane i keeps particle i's whole state in registers
 // and only the predicted j-data (and f r_planet) goes through LDS, so a block step
 // costs a few LDS round trips and wave shuffles instead of global-memory round
@@ -1014,8 +1034,9 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
 // over from the start of that block step (it predicts the same positions and finds
 // the same pair), so merging and twins need no register path.
 __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
-                               double *s_m, double *s_ri, int64_t *s_id, double *s_fr, int b, int n,
-                               double &sys_time, long long &loop, double &ed, int &n_col, long long &pairs) {
+                               double *s_m, double *s_ri, int64_t *s_id, double *s_fr, double *s_io,
+                               double *s_tm, int b, int n, double &sys_time, long long &loop, double &ed,
+                               int &n_col, long long &pairs) {
     n = __builtin_amdgcn_readfirstlane(n);   // wave-uniform: scalar branches on the size
     const int t = threadIdx.x;
     const bool own = t < n;
@@ -1049,6 +1070,9 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     if (own)
         for (int j = 0; j < n; j++) rep = rep || (j != t && s_id[j] == id);
     const bool from_start = __ballot(rep) != 0;
+    // pairs side by side (force_pairs) for 3 to 8 particles
+    const bool side = n >= 3 && n * (n - 1) <= 64;
+    const PairLane q = side ? pair_lane(s_m, s_ri, n, t) : PairLane{false, 0, 0, 0., 0., 0.};
     if (from_start) {
         if (own) {
             gravity_full(P, H, b, n, t);
@@ -1094,7 +1118,25 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             if (__ballot(rmin < 1.)) wave_argmin(rmin, bi, bj);
             if (__ballot(rmin < 1.)) { handover = true; break; }
         }
-        if (own && time + dt == t_s) {
+        if (side) {
+            const bool oact = own && time + dt == t_s;
+            const bool pact = q.on && ((__ballot(oact) >> q.i) & 1ull);
+            // the partner's predicted state, once for the three evaluations
+            double xj[3] = {0., 0., 0.}, vj[3] = {0., 0., 0.};
+            if (pact)
+                for (int d = 0; d < 3; d++) { xj[d] = s_x[3 * q.j + d]; vj[d] = s_v[3 * q.j + d]; }
+            const double t_c = t_s - time;
+            if (oact) pairs += 3 * (long long)(n - 1);
+            force_pairs(P, q, oact, pact, t, n, xp, vp, xj, vj, s_io, s_tm, as, js, ad, jd);
+            for (int ite = 0; ite < 2; ite++) {
+                if (oact) correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                force_pairs(P, q, oact, pact, t, n, x, v, xj, vj, s_io, s_tm, as, js, ad, jd);
+            }
+            if (oact) {
+                time += t_c;
+                dt = dt_block(P, time, dt, acc0, ad, jd, a2d, a3d, as, js, a2s, a3s);
+            }
+        } else if (own && time + dt == t_s) {
             pairs += 3 * (long long)(n - 1);
             const double t_c = t_s - time;
             if (n == 2) {
@@ -1106,20 +1148,13 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
                     correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
                     force_jreg<1>(P, pj, x, v, rinv, as, js, ad, jd);
                 }
-            } else if (n == 3) {
-                JReg pj[2];
-                load_jreg(J, t == 0 ? 1 : 0, pj[0]);
-                load_jreg(J, t == 2 ? 1 : 2, pj[1]);
-                force_jreg<2>(P, pj, xp, vp, rinv, as, js, ad, jd);
-                for (int ite = 0; ite < 2; ite++) {
-                    correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
-                    force_jreg<2>(P, pj, x, v, rinv, as, js, ad, jd);
-                }
             } else {
-                force_lane(P, J, n, t, xp, vp, rinv, as, js, ad, jd);
+                star_acc_jerk(P, xp, vp, as, js, nullptr);
+                gravity_loop_lane<false>(P, J, n, t, id, xp, vp, rinv, ad, jd, nullptr);
                 for (int ite = 0; ite < 2; ite++) {
                     correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
-                    force_lane(P, J, n, t, x, v, rinv, as, js, ad, jd);
+                    star_acc_jerk(P, x, v, as, js, nullptr);
+                    gravity_loop_lane<false>(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
                 }
             }
             time += t_c;
@@ -1198,6 +1233,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
     // would occupy scalar registers for the whole kernel (and be spilled and reloaded
     // in its loop); the other paths keep them as arguments
     __shared__ HP s_P;
+    // register path, 3 to 8 particles: particle states and pair summands (force_pairs)
+    __shared__ double s_io[REG ? 6 * 8 : 1], s_tm[REG ? 6 * 64 : 1];
     const int t = threadIdx.x;
     const int B = blockDim.x;
     if (REG) {
@@ -1216,12 +1253,6 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         const int q = s_cl;
         __syncthreads();
         if (q >= ncl) break;
-        // The queue hands out the costliest clusters first (estimated block steps times
-        // size).  Their waves take their SIMD's issue slots first, so that the longest
-        // chains of block steps, which bound the stage, run close to lone-wave speed
-        // while the waves beside them fill the remaining cycles.
-        if (q < HARD_PRIO_CLUSTERS) __builtin_amdgcn_s_setprio(2);
-        else __builtin_amdgcn_s_setprio(0);
         const int c = order ? order[q] : q;
         const int b = cl_off[c];
         const int n = cl_off[c + 1] - b;
@@ -1243,7 +1274,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         long long loop = 0;
         double sys_time = 0.;
         if (REG) {
-            integrate_regs(P, H, J, R, s_x, s_v, s_m, s_ri, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
+            integrate_regs(P, H, J, R, s_x, s_v, s_m, s_ri, s_id, s_fr, s_io, s_tm, b, n, sys_time, loop, ed, n_col,
+                           pairs);
         } else {
             // initial calcGravity for all, flags cleared (hermite.h:259-270)
             for (int i = t; i < n; i += B) {

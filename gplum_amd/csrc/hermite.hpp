@@ -58,9 +58,6 @@ struct IsoArgs {
 // over the whole GPU, several at a time on a stream pool, host-driven (not capturable in
 // a graph).  None of these changes results.
 constexpr int GIANT_MIN_DEFAULT = 1024;
-// Clusters at the head of the cost-ordered queue whose waves run at raised issue
-// priority (s_setprio) in the persistent Hermite kernels.
-constexpr int HARD_PRIO_CLUSTERS = 64;
 constexpr int GIANT_OFF = 0x7fffffff;
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2 = nullptr,
