@@ -972,75 +972,60 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
     __syncthreads();
 }
 
-// Pairs side by side for clusters of 3 to 8 particles (n (n - 1) <= 64): lane
-// L = i (n - 1) + k evaluates the pair of particle i with its k-th partner
-// j = k + (k >= i), so one pair_term latency covers all pairs of the block step; the
-// owner lane i adds its n - 1 summands from the LDS from zero in ascending j, the sum
-// of gravity_loop_lane.  Lanes of inactive particles' pairs stay idle.
-struct PairLane {
-    bool on;          // lane L < n (n - 1)
-    int i, j;
-    double m_j, ri_i, ri_j;
-};
-__device__ inline PairLane pair_lane(const double *s_m, const double *s_ri, int n, int L) {
-    PairLane q{false, 0, 0, 0., 0., 0.};
-    const int m = n - 1;
-    if (L < n * m) {
-        q.on = true;
-        q.i = L / m;
-        const int k = L - q.i * m;
-        q.j = k + (k >= q.i ? 1 : 0);
-        q.m_j = s_m[q.j];
-        q.ri_i = s_ri[q.i];
-        q.ri_j = s_ri[q.j];
-    }
-    return q;
-}
-
-// Sun and hard force of the active owners (oact) at (xo, vo); pact: this lane's pair
-// belongs to an active particle; xj, vj: its partner's predicted state.  s_io holds
-// 6 doubles per particle, s_tm 6 per lane.
-__device__ inline void force_pairs(const HP &P, const PairLane &q, bool oact, bool pact, int t, int n,
-                                   const double xo[3], const double vo[3], const double xj[3], const double vj[3],
-                                   double *s_io, double *s_tm, double as[3], double js[3], double ad[3],
-                                   double jd[3]) {
-    if (oact)
-        for (int d = 0; d < 3; d++) { s_io[6 * t + d] = xo[d]; s_io[6 * t + 3 + d] = vo[d]; }
+// Pairs side by side for clusters of 3 to 8 particles (n (n - 1) <= 64): particle p
+// is held by the m = n - 1 lanes p m .. p m + m - 1, each keeping the particle's whole
+// state (computed alike), and lane p m + k evaluates p's pair with its k-th partner
+// j = k + (k >= p).  One pair_term latency covers all pairs of a block step; every lane
+// of the group then adds the group's m summands from the LDS, from zero in ascending j
+// (the sum of gravity_loop_lane).  Called by the lanes of active particles only.
+__device__ inline void side_force(const HP &P, int t, int base, int m, const double xo[3], const double vo[3],
+                                  double rinv, const double xj[3], const double vj[3], double mj, double rinv_j,
+                                  double *s_tm, double as[3], double js[3], double ad[3], double jd[3]) {
+    double ta[3], tj[3];
+    pair_term(P, xo, vo, rinv, xj[0], xj[1], xj[2], vj[0], vj[1], vj[2], mj, rinv_j, ta, tj, nullptr);
+    for (int d = 0; d < 3; d++) { s_tm[6 * t + d] = ta[d]; s_tm[6 * t + 3 + d] = tj[d]; }
+    star_acc_jerk(P, xo, vo, as, js, nullptr);
     wave_sync();
-    if (pact) {
-        double xi[3], vi[3], ta[3], tj[3];
-        for (int d = 0; d < 3; d++) { xi[d] = s_io[6 * q.i + d]; vi[d] = s_io[6 * q.i + 3 + d]; }
-        pair_term(P, xi, vi, q.ri_i, xj[0], xj[1], xj[2], vj[0], vj[1], vj[2], q.m_j, q.ri_j, ta, tj, nullptr);
-        for (int d = 0; d < 3; d++) { s_tm[6 * t + d] = ta[d]; s_tm[6 * t + 3 + d] = tj[d]; }
-    }
-    if (oact) star_acc_jerk(P, xo, vo, as, js, nullptr);
-    wave_sync();
-    if (oact) {
-        const int m = n - 1;
-        for (int d = 0; d < 3; d++) { ad[d] = 0.; jd[d] = 0.; }
-        for (int k = 0; k < m; k++) {
-            const int L = t * m + k;
-            for (int d = 0; d < 3; d++) { ad[d] += s_tm[6 * L + d]; jd[d] += s_tm[6 * L + 3 + d]; }
-        }
+    for (int d = 0; d < 3; d++) { ad[d] = 0.; jd[d] = 0.; }
+    for (int k = 0; k < m; k++) {
+        const int L = base + k;
+        for (int d = 0; d < 3; d++) { ad[d] += s_tm[6 * L + d]; jd[d] += s_tm[6 * L + 3 + d]; }
     }
     wave_sync();   // summands read before the next evaluation writes them
 }
 
-// Clusters that fit one wave: lane i keeps particle i's whole state in registers
-// and only the predicted j-data (and f r_planet) goes through LDS, so a block step
-// costs a few LDS round trips and wave shuffles instead of global-memory round
-// trips.  Same operations in the same order as integrate_from.  At the first block
-// step with a collision the state is written back to H and integrate_from takes
-// over from the start of that block step (it predicts the same positions and finds
-// the same pair), so merging and twins need no register path.
+// wave_min2 over the particles' values when the lanes p m (p < n) hold them and the
+// other lanes hold copies or 1e10 (m, n wave-uniform): up to four particles read
+// their lanes directly.
+__device__ inline void wave_min2_lead(double &u, double &v, int n, int m) {
+    if (n <= 4) {
+        u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, m)), dmin(readlane_f64(u, 2 * m), readlane_f64(u, 3 * m)));
+        v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, m)), dmin(readlane_f64(v, 2 * m), readlane_f64(v, 3 * m)));
+    } else {
+        wave_min2(u, v);
+    }
+}
+
+// Clusters that fit one wave: each particle's whole state stays in registers (one
+// lane per particle, or a group of lanes for 3 to 8 particles, see side_force) and
+// only the predicted j-data (and f r_planet) goes through LDS, so a block step costs a
+// few LDS round trips and wave shuffles instead of global-memory round trips.  Same
+// operations in the same order as integrate_from.  At the first block step with a
+// collision the state is written back to H and integrate_from takes over from the
+// start of that block step (it predicts the same positions and finds the same pair),
+// so merging and twins need no register path.
 __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
-                               double *s_m, double *s_ri, int64_t *s_id, double *s_fr, double *s_io,
-                               double *s_tm, int b, int n, double &sys_time, long long &loop, double &ed,
-                               int &n_col, long long &pairs) {
+                               double *s_m, double *s_ri, int64_t *s_id, double *s_fr, double *s_tm, int b,
+                               int n, double &sys_time, long long &loop, double &ed, int &n_col,
+                               long long &pairs) {
     n = __builtin_amdgcn_readfirstlane(n);   // wave-uniform: scalar branches on the size
     const int t = threadIdx.x;
-    const bool own = t < n;
-    const int g = b + t;
+    const bool side = n >= 3 && n * (n - 1) <= 64;
+    const int m = side ? n - 1 : 1;            // lanes per particle
+    const int p = t / m;                       // this lane's particle
+    const bool own = p < n;
+    const bool lead = own && t == p * m;       // the lane that writes p's shared state
+    const int g = b + p;
     double x[3] = {0., 0., 0.}, v[3] = {0., 0., 0.}, as[3], js[3], ad[3], jd[3];
     double time = 0., dt = 0., acc0 = 0., rinv = 0., fr = 0.;
     int64_t id = -1;
@@ -1052,13 +1037,15 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         acc0 = H.acc0[g];
         rinv = H.rinv[g];
         id = H.id[g];
-        s_m[t] = H.mass[g];
-        s_ri[t] = rinv;
-        s_id[t] = id;
         fr = H.f[g] * H.r_planet[g];
-        s_fr[t] = fr;
-        st3(s_x, t, x);
-        st3(s_v, t, v);
+        if (lead) {
+            s_m[p] = H.mass[g];
+            s_ri[p] = rinv;
+            s_id[p] = id;
+            s_fr[p] = fr;
+            st3(s_x, p, x);
+            st3(s_v, p, v);
+        }
     }
     __syncthreads();
     // The clusters the reference forms have distinct ids, and the partner loops below
@@ -1068,21 +1055,27 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     // would double the kernel's register allocation.)
     bool rep = false;
     if (own)
-        for (int j = 0; j < n; j++) rep = rep || (j != t && s_id[j] == id);
+        for (int j = 0; j < n; j++) rep = rep || (j != p && s_id[j] == id);
     const bool from_start = __ballot(rep) != 0;
-    // pairs side by side (force_pairs) for 3 to 8 particles
-    const bool side = n >= 3 && n * (n - 1) <= 64;
-    const PairLane q = side ? pair_lane(s_m, s_ri, n, t) : PairLane{false, 0, 0, 0., 0., 0.};
+    // side_force: this lane's partner and its constants
+    int jp = 0;
+    double m_jp = 0., ri_jp = 0.;
+    if (side && own) {
+        const int k = t - p * m;
+        jp = k + (k >= p ? 1 : 0);
+        m_jp = s_m[jp];
+        ri_jp = s_ri[jp];
+    }
     if (from_start) {
-        if (own) {
-            gravity_full(P, H, b, n, t);
+        if (lead) {
+            gravity_full(P, H, b, n, p);
             H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
             H.twin_of[g] = -1;
         }
     } else if (own) {
         // initial calcGravity (hermite.h:259-270)
         star_acc_jerk(P, x, v, as, js, nullptr);
-        gravity_loop_lane<false>(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
+        gravity_loop_lane<false>(P, J, n, p, id, x, v, rinv, ad, jd, nullptr);
     }
     const double time_end = P.dt_tree;
     double a0s[3], j0s[3], a0d[3], j0d[3], xp[3], vp[3], a2s[3], a3s[3], a2d[3], a3d[3];
@@ -1094,8 +1087,10 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         __syncthreads();   // the previous step's j-data reads are done
         if (own) {
             predict_regs(t_s - time, x, v, as, js, ad, jd, a0s, j0s, a0d, j0d, xp, vp);
-            st3(s_x, t, xp);
-            st3(s_v, t, vp);
+            if (lead) {
+                st3(s_x, p, xp);
+                st3(s_v, p, vp);
+            }
         }
         __syncthreads();
         if (P.collision) {
@@ -1103,7 +1098,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             int bi = 0x7fffffff, bj = -1;
             if (own) {
                 for (int k = 0; k < n - 1; k++) {
-                    const int j = k + (k >= t ? 1 : 0);   // the partners, ascending
+                    const int j = k + (k >= p ? 1 : 0);   // the partners, ascending
                     const double dx = xp[0] - s_x[3 * j];
                     const double dy = xp[1] - s_x[3 * j + 1];
                     const double dz = xp[2] - s_x[3 * j + 2];
@@ -1111,38 +1106,30 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
                     const double r2 = fr + s_fr[j];
                     if (!(d2 < COL_FILTER * r2 * r2)) continue;   // R >= 1: never the minimum
                     const double Rr = sqrt(d2) / r2;
-                    if (Rr < rmin) { rmin = Rr; bi = t; bj = j; }
+                    if (Rr < rmin) { rmin = Rr; bi = p; bj = j; }
                 }
             }
             // the argmin only when some pair is below 1 (rare)
             if (__ballot(rmin < 1.)) wave_argmin(rmin, bi, bj);
             if (__ballot(rmin < 1.)) { handover = true; break; }
         }
-        if (side) {
-            const bool oact = own && time + dt == t_s;
-            const bool pact = q.on && ((__ballot(oact) >> q.i) & 1ull);
-            // the partner's predicted state, once for the three evaluations
-            double xj[3] = {0., 0., 0.}, vj[3] = {0., 0., 0.};
-            if (pact)
-                for (int d = 0; d < 3; d++) { xj[d] = s_x[3 * q.j + d]; vj[d] = s_v[3 * q.j + d]; }
+        if (own && time + dt == t_s) {
+            if (lead) pairs += 3 * (long long)(n - 1);
             const double t_c = t_s - time;
-            if (oact) pairs += 3 * (long long)(n - 1);
-            force_pairs(P, q, oact, pact, t, n, xp, vp, xj, vj, s_io, s_tm, as, js, ad, jd);
-            for (int ite = 0; ite < 2; ite++) {
-                if (oact) correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
-                force_pairs(P, q, oact, pact, t, n, x, v, xj, vj, s_io, s_tm, as, js, ad, jd);
-            }
-            if (oact) {
-                time += t_c;
-                dt = dt_block(P, time, dt, acc0, ad, jd, a2d, a3d, as, js, a2s, a3s);
-            }
-        } else if (own && time + dt == t_s) {
-            pairs += 3 * (long long)(n - 1);
-            const double t_c = t_s - time;
-            if (n == 2) {
+            if (side) {
+                // the partner's predicted state, once for the three evaluations
+                double xj[3], vj[3];
+                ld3(s_x, jp, xj);
+                ld3(s_v, jp, vj);
+                side_force(P, t, p * m, m, xp, vp, rinv, xj, vj, m_jp, ri_jp, s_tm, as, js, ad, jd);
+                for (int ite = 0; ite < 2; ite++) {
+                    correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                    side_force(P, t, p * m, m, x, v, rinv, xj, vj, m_jp, ri_jp, s_tm, as, js, ad, jd);
+                }
+            } else if (n == 2) {
                 // the partner's predicted state is read once for the three evaluations
                 JReg pj[1];
-                load_jreg(J, t == 0 ? 1 : 0, pj[0]);
+                load_jreg(J, p == 0 ? 1 : 0, pj[0]);
                 force_jreg<1>(P, pj, xp, vp, rinv, as, js, ad, jd);
                 for (int ite = 0; ite < 2; ite++) {
                     correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
@@ -1150,11 +1137,11 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
                 }
             } else {
                 star_acc_jerk(P, xp, vp, as, js, nullptr);
-                gravity_loop_lane<false>(P, J, n, t, id, xp, vp, rinv, ad, jd, nullptr);
+                gravity_loop_lane<false>(P, J, n, p, id, xp, vp, rinv, ad, jd, nullptr);
                 for (int ite = 0; ite < 2; ite++) {
                     correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
                     star_acc_jerk(P, x, v, as, js, nullptr);
-                    gravity_loop_lane<false>(P, J, n, t, id, x, v, rinv, ad, jd, nullptr);
+                    gravity_loop_lane<false>(P, J, n, p, id, x, v, rinv, ad, jd, nullptr);
                 }
             }
             time += t_c;
@@ -1162,13 +1149,13 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         }
         sys_time = own ? time : 1.e10;
         t_s = own ? time + dt : 1.e10;
-        wave_min2_n(sys_time, t_s, n);
+        wave_min2_lead(sys_time, t_s, n, m);
         loop++;
     }
     if (handover || from_start) {
         // a collision in this block step: hand the cluster to the global-memory path
         // at the start of the step (kept out of the loop, which then holds less state)
-        if (handover && own) {
+        if (handover && lead) {
             st3(H.pos, g, x); st3(H.vel, g, v);
             st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
             st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
@@ -1184,25 +1171,27 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     }
     // final calcGravity with potentials (hermite.h:466-470), current positions
     __syncthreads();
-    if (own) {
-        st3(s_x, t, x);
-        st3(s_v, t, v);
+    if (lead) {
+        st3(s_x, p, x);
+        st3(s_v, p, v);
     }
     __syncthreads();
     if (own) {
         double phs = 0., phd = 0.;
         star_acc_jerk(P, x, v, as, js, &phs);
-        gravity_loop_lane<false>(P, J, n, t, id, x, v, rinv, ad, jd, &phd);
-        pairs += n - 1;
-        st3(H.pos, g, x); st3(H.vel, g, v);
-        st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
-        st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
-        H.time[g] = time;
-        H.dt[g] = dt;
-        H.phi_s[g] = phs;
-        H.phi_d[g] = phd;
-        H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
-        H.twin_of[g] = -1;
+        gravity_loop_lane<false>(P, J, n, p, id, x, v, rinv, ad, jd, &phd);
+        if (lead) {
+            pairs += n - 1;
+            st3(H.pos, g, x); st3(H.vel, g, v);
+            st3(H.acc_s, g, as); st3(H.jerk_s, g, js);
+            st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
+            H.time[g] = time;
+            H.dt[g] = dt;
+            H.phi_s[g] = phs;
+            H.phi_d[g] = phd;
+            H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+            H.twin_of[g] = -1;
+        }
     }
     __syncthreads();
 }
@@ -1233,8 +1222,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
     // would occupy scalar registers for the whole kernel (and be spilled and reloaded
     // in its loop); the other paths keep them as arguments
     __shared__ HP s_P;
-    // register path, 3 to 8 particles: particle states and pair summands (force_pairs)
-    __shared__ double s_io[REG ? 6 * 8 : 1], s_tm[REG ? 6 * 64 : 1];
+    // register path, 3 to 8 particles: pair summands (side_force)
+    __shared__ double s_tm[REG ? 6 * 64 : 1];
     const int t = threadIdx.x;
     const int B = blockDim.x;
     if (REG) {
@@ -1274,7 +1263,7 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         long long loop = 0;
         double sys_time = 0.;
         if (REG) {
-            integrate_regs(P, H, J, R, s_x, s_v, s_m, s_ri, s_id, s_fr, s_io, s_tm, b, n, sys_time, loop, ed, n_col,
+            integrate_regs(P, H, J, R, s_x, s_v, s_m, s_ri, s_id, s_fr, s_tm, b, n, sys_time, loop, ed, n_col,
                            pairs);
         } else {
             // initial calcGravity for all, flags cleared (hermite.h:259-270)
