@@ -81,6 +81,7 @@ struct Tree {
     DArray<int32_t> walk_err;
     DArray<int32_t> lvl;                       // device level ranges + overflow flag
     DArray<int32_t> lvl_part;                  // per-chunk child sums of one level
+    DArray<int32_t> lvl_bnd;                   // digit bounds of one level's nodes (8 per node)
     DArray<unsigned char> scan_tmp;            // hipcub temporary storage
     int64_t n_ep_total = 0, n_epf_total = 0, n_sp_total = 0, n_epl_total = 0;
     int64_t n_epep_pairs = 0, n_epsp_pairs = 0;

@@ -649,19 +649,6 @@ __device__ inline void wave_min2(double &u, double &v) {
     u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, 16)), dmin(readlane_f64(u, 32), readlane_f64(u, 48)));
     v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, 16)), dmin(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
-// wave_min2 where only lanes below n can hold values under the others' 1e10 (n is
-// wave-uniform): clusters of up to four read their lanes directly.
-__device__ inline void wave_min2_n(double &u, double &v, int n) {
-    if (n <= 2) {
-        u = dmin(readlane_f64(u, 0), readlane_f64(u, 1));
-        v = dmin(readlane_f64(v, 0), readlane_f64(v, 1));
-    } else if (n <= 4) {
-        u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, 1)), dmin(readlane_f64(u, 2), readlane_f64(u, 3)));
-        v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, 1)), dmin(readlane_f64(v, 2), readlane_f64(v, 3)));
-    } else {
-        wave_min2(u, v);
-    }
-}
 __device__ inline void wave_argmin(double &v, int &i, int &j) {
     for (int d = 32; d > 0; d >>= 1) {
         const double ov = __shfl_xor(v, d);
@@ -998,7 +985,10 @@ __device__ inline void side_force(const HP &P, int t, int base, int m, const dou
 // other lanes hold copies or 1e10 (m, n wave-uniform): up to four particles read
 // their lanes directly.
 __device__ inline void wave_min2_lead(double &u, double &v, int n, int m) {
-    if (n <= 4) {
+    if (n <= 2) {
+        u = dmin(readlane_f64(u, 0), readlane_f64(u, m));
+        v = dmin(readlane_f64(v, 0), readlane_f64(v, m));
+    } else if (n <= 4) {
         u = dmin(dmin(readlane_f64(u, 0), readlane_f64(u, m)), dmin(readlane_f64(u, 2 * m), readlane_f64(u, 3 * m)));
         v = dmin(dmin(readlane_f64(v, 0), readlane_f64(v, m)), dmin(readlane_f64(v, 2 * m), readlane_f64(v, 3 * m)));
     } else {

@@ -189,27 +189,114 @@ __device__ inline int block_sum(int v, int *s_w) {
     return all;
 }
 
+// Nodes of at least COOP_MIN keys (the top levels, where a workgroup's chunk holds one
+// node) find their digit bounds with the whole workgroup: each round samples 255 cut
+// points of each of the seven search intervals at once and keeps the piece holding the
+// first key of a larger digit, so a node of N keys takes log_256 N rounds of parallel
+// loads instead of log_2 N dependent ones.  Smaller nodes keep one thread each (many
+// nodes per chunk: their searches run side by side).
+constexpr int COOP_MIN = 1 << 16;
+
+// bnd8[d] = first index in [s, e) whose digit at level lv is > d (d = 0..6), bnd8[7] = e:
+// the same bounds as digit_bounds (bnd[d + 1]).  Every thread of the workgroup calls it.
+__device__ void digit_bounds_wg(const uint64_t *__restrict__ key, int s, int e, int lv, int *s_lo, int *s_hi,
+                                int *s_first, int bnd8[8]) {
+    const int t = (int)threadIdx.x;
+    if (t < 7) { s_lo[t] = s; s_hi[t] = e; }
+    __syncthreads();
+    for (;;) {
+        bool open = false;
+        for (int d = 0; d < 7; d++) open = open || (s_hi[d] > s_lo[d]);
+        if (!open) break;            // the same in every thread (read after a barrier)
+        if (t < 7) s_first[t] = LEVEL_BLOCK - 1;
+        __syncthreads();
+        if (t < LEVEL_BLOCK - 1) {
+            for (int d = 0; d < 7; d++) {
+                const int lo = s_lo[d], hi = s_hi[d];
+                if (hi <= lo) continue;
+                const int q = lo + (int)(((long long)(hi - lo) * (t + 1)) >> 8);   // cut point t, in [lo, hi)
+                if (digit_at(key[q], lv) > d) atomicMin(&s_first[d], t);
+            }
+        }
+        __syncthreads();
+        if (t < 7) {
+            const int lo = s_lo[t], hi = s_hi[t];
+            if (hi > lo) {
+                const int f = s_first[t];
+                const long long len = hi - lo;
+                // cut points q_k = lo + (len (k + 1)) >> 8; the first index with a larger
+                // digit lies in (q_{f-1}, q_f] (f = 255: none among the cut points)
+                const int nlo = (f == 0) ? lo : lo + (int)((len * f) >> 8) + 1;
+                const int nhi = (f == LEVEL_BLOCK - 1) ? hi : lo + (int)((len * (f + 1)) >> 8);
+                s_lo[t] = nlo;
+                s_hi[t] = nhi;
+            }
+        }
+        __syncthreads();
+    }
+    for (int d = 0; d < 7; d++) bnd8[d] = s_lo[d];
+    bnd8[7] = e;
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
     const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, int n_leaf,
     const int32_t *__restrict__ start, const int32_t *__restrict__ count, const int32_t *__restrict__ level,
-    int32_t *__restrict__ nchild, int32_t *__restrict__ part) {
+    int32_t *__restrict__ nchild, int32_t *__restrict__ part, int32_t *__restrict__ bnd_out) {
     __shared__ int s_w[LEVEL_BLOCK / 64];
+    __shared__ int s_big[LEVEL_BLOCK], s_nbig, s_lo[7], s_hi[7], s_first[7];
     int lb, m, c0, c1;
     level_chunk(lvl, L, lb, m, c0, c1);
     if (m <= 0) return;
     int sum = 0;
-    for (int r = c0 + (int)threadIdx.x; r < c1; r += LEVEL_BLOCK) {
-        const int c = lb + r;
-        const int s = start[c], cnt = count[c], lv = level[c];
-        int nc = 0;
-        if (cnt > n_leaf && lv < MAXLEVEL) {
-            int bnd[9];
-            digit_bounds(key, s, s + cnt, lv, bnd);
+    for (int r0 = c0; r0 < c1; r0 += LEVEL_BLOCK) {
+        if (threadIdx.x == 0) s_nbig = 0;
+        __syncthreads();
+        const int r = r0 + (int)threadIdx.x;
+        if (r < c1) {
+            const int c = lb + r;
+            const int s = start[c], cnt = count[c], lv = level[c];
+            int nc = 0;
+            if (cnt > n_leaf && lv < MAXLEVEL) {
+                if (cnt >= COOP_MIN) {
+                    s_big[atomicAdd(&s_nbig, 1)] = r;   // the workgroup searches it below
+                    nc = -1;
+                } else {
+                    int bnd[9];
+                    digit_bounds(key, s, s + cnt, lv, bnd);
 #pragma unroll
-            for (int d = 0; d < 8; d++) nc += bnd[d + 1] > bnd[d] ? 1 : 0;
+                    for (int d = 0; d < 8; d++) {
+                        nc += bnd[d + 1] > bnd[d] ? 1 : 0;
+                        bnd_out[8 * (size_t)r + d] = bnd[d + 1];
+                    }
+                }
+            }
+            if (nc >= 0) {
+                nchild[r] = nc;
+                sum += nc;
+            }
         }
-        nchild[r] = nc;
-        sum += nc;
+        __syncthreads();
+        const int nbig = s_nbig;
+        for (int k = 0; k < nbig; k++) {
+            const int rb = s_big[k];
+            const int c = lb + rb;
+            const int s = start[c];
+            int bnd8[8];
+            digit_bounds_wg(key, s, s + count[c], level[c], s_lo, s_hi, s_first, bnd8);
+            if (threadIdx.x == 0) {
+                int nc = 0, prev = s;
+#pragma unroll
+                for (int d = 0; d < 8; d++) {
+                    nc += bnd8[d] > prev ? 1 : 0;
+                    prev = bnd8[d];
+                    bnd_out[8 * (size_t)rb + d] = bnd8[d];
+                }
+                nchild[rb] = nc;
+                sum += nc;
+            }
+        }
+        __syncthreads();   // s_nbig read by every thread before the next tile resets it
     }
     const int all = block_sum(sum, s_w);
     if (threadIdx.x == 0) part[blockIdx.x] = all;
@@ -244,7 +331,7 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_write_children_lv(
     const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, const int32_t *__restrict__ part,
     int32_t *__restrict__ start, int32_t *__restrict__ count, int32_t *__restrict__ level,
     int32_t *__restrict__ parent, int32_t *__restrict__ child, int32_t *__restrict__ nchild_out,
-    const int32_t *__restrict__ nchild) {
+    const int32_t *__restrict__ nchild, const int32_t *__restrict__ bnd_in) {
     __shared__ int s_w[LEVEL_BLOCK / 64];
     int lb, m, c0, c1;
     level_chunk(lvl, L, lb, m, c0, c1);
@@ -271,9 +358,12 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_write_children_lv(
         const int first = next_begin + base + ex;
         child[c] = nc ? first : -1;
         if (!nc) continue;
-        const int s = start[c], e = s + count[c], lv = level[c];
+        // the digit bounds found by the count kernel
+        const int s = start[c], lv = level[c];
         int bnd[9];
-        digit_bounds(key, s, e, lv, bnd);
+        bnd[0] = s;
+#pragma unroll
+        for (int d = 0; d < 8; d++) bnd[d + 1] = bnd_in[8 * (size_t)r + d];
         int w = first;
 #pragma unroll
         for (int d = 0; d < 8; d++) {
@@ -596,14 +686,15 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
     GPLUM_ENSURE(t.lvl, LVL_ERR + 5);
     hipLaunchKernelGGL(k_lvl_init, dim3(1), dim3(64), 0, st, t.lvl.p);
     GPLUM_ENSURE(t.lvl_part, LEVEL_GRID);
+    GPLUM_ENSURE(t.lvl_bnd, 8 * ((size_t)n + 64));   // a level holds at most n nodes
     for (int L = 0; L <= MAXLEVEL; L++) {
         hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
-                           t.lvl_part.p);
+                           t.lvl_part.p, t.lvl_bnd.p);
         hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(LEVEL_GRID), 0, st, t.lvl.p, L, t.lvl_part.p, cap);
         hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            t.lvl_part.p, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
-                           t.node_child.p, t.node_nchild.p, t.grp_flag.p);
+                           t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
     }
     std::vector<int32_t> lv(LVL_ERR + 1);
     GPLUM_HIP(hipMemcpyAsync(lv.data(), t.lvl.p, sizeof(int32_t) * (LVL_ERR + 1), hipMemcpyDeviceToHost, st));
