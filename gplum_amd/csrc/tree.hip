@@ -140,10 +140,10 @@ __global__ void k_root(int n, int32_t *start, int32_t *count, int32_t *level, in
 
 // Level kernels: child count per node, then child write (children in digit order,
 // appended after the level).  The level's node range is read from lvl[L], lvl[L+1]
-// (the previous level's k_level_scan writes it), so the levels run back to back
+// (the previous level's write kernel writes it), so the levels run back to back
 // without a host round trip.  LEVEL_GRID workgroups each own one contiguous chunk of
-// the level: the count kernel sums its chunk's children, k_level_scan turns the
-// chunk sums into chunk bases, and the write kernel scans its chunk in tiles.
+// the level: the count kernel sums its chunk's children, and the write kernel turns
+// the chunk sums into its chunk's base and scans its chunk in tiles.
 constexpr int LEVEL_GRID = 256, LEVEL_BLOCK = 256;
 constexpr int LVL_ERR = MAXLEVEL + 4;       // lvl[]: ranges 0..MAXLEVEL+2, flag, totals
 
@@ -302,42 +302,34 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
     if (threadIdx.x == 0) part[blockIdx.x] = all;
 }
 
-// one workgroup: exclusive scan of the LEVEL_GRID chunk sums (in place) and the next
-// level's range lvl[L+2] = lvl[L+1] + total (flag lvl[LVL_ERR] past cap)
-__global__ __launch_bounds__(LEVEL_GRID) void k_level_scan(int32_t *__restrict__ lvl, int L,
-                                                           int32_t *__restrict__ part, int cap) {
-    __shared__ int s_w[LEVEL_GRID / 64];
-    const int m = lvl[L + 1] - lvl[L];
-    const int t = threadIdx.x;
-    const int v = (m > 0) ? part[t] : 0;
-    int tot = 0;
-    const int ex = wave_excl_scan(v, &tot);
-    if (__lane_id() == 0) s_w[t >> 6] = tot;
-    __syncthreads();
-    int base = 0, all = 0;
-    for (int w = 0; w < LEVEL_GRID / 64; w++) {
-        if (w < (t >> 6)) base += s_w[w];
-        all += s_w[w];
-    }
-    part[t] = base + ex;
-    if (t == 0) {
-        const int next = lvl[L + 1] + all;
-        if (next > cap) { lvl[LVL_ERR] = 1; lvl[L + 2] = lvl[L + 1]; }
-        else lvl[L + 2] = next;
-    }
-}
-
+// The chunk sums of the count kernel give each workgroup its base (the sum over the
+// chunks before it) and the level's total; workgroup 0 publishes the next level's
+// range lvl[L+2] = lvl[L+1] + total, or the overflow flag lvl[LVL_ERR] past cap (then
+// nothing is written).
 __global__ __launch_bounds__(LEVEL_BLOCK) void k_write_children_lv(
-    const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, const int32_t *__restrict__ part,
-    int32_t *__restrict__ start, int32_t *__restrict__ count, int32_t *__restrict__ level,
+    const uint64_t *__restrict__ key, int32_t *__restrict__ lvl, int L, const int32_t *__restrict__ part,
+    int cap, int32_t *__restrict__ start, int32_t *__restrict__ count, int32_t *__restrict__ level,
     int32_t *__restrict__ parent, int32_t *__restrict__ child, int32_t *__restrict__ nchild_out,
     const int32_t *__restrict__ nchild, const int32_t *__restrict__ bnd_in) {
+    static_assert(LEVEL_GRID == LEVEL_BLOCK, "one chunk sum per thread");
     __shared__ int s_w[LEVEL_BLOCK / 64];
     int lb, m, c0, c1;
     level_chunk(lvl, L, lb, m, c0, c1);
-    if (m <= 0 || lvl[LVL_ERR]) return;
+    if (m <= 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) lvl[L + 2] = lvl[L + 1];   // empty: so is the next
+        return;
+    }
     const int next_begin = lvl[L + 1];
-    int carry = part[blockIdx.x];
+    const int ps = part[threadIdx.x];
+    const int before = block_sum((int)threadIdx.x < (int)blockIdx.x ? ps : 0, s_w);
+    const int total = block_sum(ps, s_w);
+    const bool over = next_begin + total > cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (over) { lvl[LVL_ERR] = 1; lvl[L + 2] = next_begin; }
+        else lvl[L + 2] = next_begin + total;
+    }
+    if (over) return;
+    int carry = before;
     for (int r0 = c0; r0 < c1; r0 += LEVEL_BLOCK) {      // tiles of the chunk, in order
         const int r = r0 + (int)threadIdx.x;
         const int nc = (r < c1) ? nchild[r] : 0;
@@ -691,9 +683,8 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
         hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
                            t.lvl_part.p, t.lvl_bnd.p);
-        hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(LEVEL_GRID), 0, st, t.lvl.p, L, t.lvl_part.p, cap);
         hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
-                           t.lvl_part.p, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
+                           t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
                            t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
     }
     std::vector<int32_t> lv(LVL_ERR + 1);
