@@ -133,14 +133,18 @@ def bench_hard(args):
           file=sys.stderr, flush=True)
     # per-cluster launches run the clusters one after another: compare on a sample
     k = min(args.compare_clusters, n_cl)
-    s_off = off[: k + 1]
-    sub = pp[: s_off[-1]]
-    sa, sb = sub.copy(), sub.copy()
-    _, nb_a, ms_sp = gpu.hard_integrate(p, sa, s_off, persistent=True, prepare=True)
-    _, nb_b, ms_c = gpu.hard_integrate(p, sb, s_off, persistent=False, prepare=True)
-    print("# sample of %d clusters: persistent %.2f ms, per-cluster launches %.2f ms" % (k, ms_sp, ms_c),
-          file=sys.stderr, flush=True)
-    same = bool(np.array_equal(sa["pos"], sb["pos"]) and np.array_equal(sa["vel"], sb["vel"]) and nb_a == nb_b)
+    per_cluster = None
+    if k > 0:
+        s_off = off[: k + 1]
+        sub = pp[: s_off[-1]]
+        sa, sb = sub.copy(), sub.copy()
+        _, nb_a, ms_sp = gpu.hard_integrate(p, sa, s_off, persistent=True, prepare=True)
+        _, nb_b, ms_c = gpu.hard_integrate(p, sb, s_off, persistent=False, prepare=True)
+        print("# sample of %d clusters: persistent %.2f ms, per-cluster launches %.2f ms" % (k, ms_sp, ms_c),
+              file=sys.stderr, flush=True)
+        same = bool(np.array_equal(sa["pos"], sb["pos"]) and np.array_equal(sa["vel"], sb["vel"]) and nb_a == nb_b)
+        per_cluster = {"sample_clusters": k, "ms": round(ms_c, 3), "persistent_ms": round(ms_sp, 3),
+                       "speedup_persistent": round(ms_c / max(ms_sp, 1e-9), 3), "identical_results": same}
     value = n / (ms_p * 1e-3)
     achieved = FLOP_HARD_PAIR * npair / (ms_p * 1e-3) / 1e12
     out = {
@@ -155,8 +159,7 @@ def bench_hard(args):
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
                      "compute": "fp64 VALU, no MFMA", "kernel": "k_hermite size classes (persistent, graph replay)",
                      "flops_per_launch": FLOP_HARD_PAIR * npair, "avg_launch_ms": round(ms_p, 3)},
-        "per_cluster_launches": {"sample_clusters": k, "ms": round(ms_c, 3), "persistent_ms": round(ms_sp, 3),
-                                 "speedup_persistent": round(ms_c / ms_sp, 3), "identical_results": same},
+        "per_cluster_launches": per_cluster,
         "graph": {"replays": args.replays, "ms_per_replay": round(ms_p, 3), "eager_launch_ms": round(ms_e, 3),
                   "identical_to_eager_and_across_replays": graph_same},
         "block_steps": nb, "pair_evaluations": npair,
