@@ -9,8 +9,8 @@
 // predicted positions of the others), so they are corrected in parallel with one
 // barrier per block step.  The j-data of the cluster (predicted position/velocity,
 // mass, 1/r_out, id) lives in LDS for clusters of up to HERMITE_LDS particles.
-// The kernel is persistent: workgroups pull clusters (largest first) from an atomic
-// queue until it is empty.
+// The kernel is persistent: workgroups pull clusters (costliest first: estimated block
+// steps times size) from an atomic queue until it is empty.
 #include <algorithm>
 #include <cfloat>
 #include <mutex>
@@ -22,7 +22,7 @@ namespace gplum {
 
 #define SECONDORDER 5.e-1
 #define THIRDORDER 1.6666666666666667e-1
-#define ALPHA_Q1 1.1666666666666667e-3   // quirk Q1, src/particle.h:339
+#define ALPHA_POS 1.1666666666666667e-3  // ALPHA = 7/6000 of the corrector's position term, src/particle.h:339 (kept as written)
 #define ONE_TWELFTH 8.3333333333333333e-2
 #define ONE_SIXTIETH 1.6666666666666667e-2
 
@@ -356,7 +356,7 @@ __device__ inline void predict(HardView &H, int g, double Dt) {
     }
 }
 
-// FPHard::correct, src/particle.h:1208-1274 (quirk Q1).
+// FPHard::correct, src/particle.h:1208-1274 (position term with ALPHA = 7/6000, as written).
 __device__ inline void correct(HardView &H, int g, double Dt) {
     const double Dt2 = Dt * Dt;
     const double Dt_inv = recip_block(Dt);
@@ -380,7 +380,7 @@ __device__ inline void correct(HardView &H, int g, double Dt) {
         H.a3_d[q] = 6. * A3_d * Dt_inv3;
         const double A2_sd = A2_s + A2_d;
         const double A3_sd = A3_s + A3_d;
-        H.pos[q] = H.xp[q] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+        H.pos[q] = H.xp[q] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_POS * A3_sd);
         H.vel[q] = H.vp[q] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
     }
 }
@@ -427,7 +427,7 @@ __device__ inline void correct_regs(double Dt, const double xp[3], const double 
         a3d[k] = 6. * A3_d * Dt_inv3;
         const double A2_sd = A2_s + A2_d;
         const double A3_sd = A3_s + A3_d;
-        pos[k] = xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+        pos[k] = xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_POS * A3_sd);
         vel[k] = vp[k] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
     }
 }
@@ -1397,7 +1397,7 @@ __global__ void k_isolated(HP P, IsoView V, int n, double time_start, double tim
                     a2d[k] = 2. * (A2_d + 3. * A3_d) * Dt_inv2;
                     a3d[k] = 6. * A3_d * Dt_inv3;
                     const double A2_sd = A2_s + A2_d, A3_sd = A3_s + A3_d;
-                    x[k] = xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+                    x[k] = xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_POS * A3_sd);
                     v[k] = vp[k] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
                 }
                 star_acc_jerk(P, x, v, as, js, nullptr);

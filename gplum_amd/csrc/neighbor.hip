@@ -49,7 +49,8 @@ __device__ inline bool correct_pair(const NbParams &P, const PView &V, int i, in
     const double drdv = drx * dvx + dry * dvy + drz * dvz;
     const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
     const double da2 = dax * dax + day * day + daz * daz;
-    // quirk Q2: std::min/std::max order (NaN when dv2 == 0 falls back to dr2)
+    // the reference's t_min (units of length) with its std::min/std::max order: NaN when
+    // dv2 == 0, and dr2_min then falls back to dr2
     const double t_min = smin(smax(-drdv / sqrt(dv2), 0.), P.dt_tree);
     const double dr2_min = smin(dr2, dr2 + 2. * drdv * t_min + dv2 * t_min * t_min);
     const double r_crit = P.R_search2 * r_out;

@@ -14,7 +14,7 @@
 
 #define SECONDORDER  5.e-1
 #define THIRDORDER   1.6666666666666667e-1
-#define ALPHA_Q1     1.1666666666666667e-3   /* quirk Q1, particle.h:339 */
+#define ALPHA_POS    1.1666666666666667e-3   /* ALPHA = 7/6000 of the position term, particle.h:339 (as written) */
 #define ONE_TWELFTH  8.3333333333333333e-2
 #define ONE_SIXTIETH 1.6666666666666667e-2
 #define HALVE_CAP    160
@@ -141,7 +141,7 @@ static void predict(orc_hp *h, double Dt)
     }
 }
 
-/* FPHard::correct, particle.h:1208-1274 (quirk Q1: 3*ALPHA with ALPHA = 7/6000). */
+/* FPHard::correct, particle.h:1208-1274 (position term 3*ALPHA with ALPHA = 7/6000, as written). */
 static void correct(orc_hp *h, double Dt)
 {
     gplum_particle *b = &h->b;
@@ -166,7 +166,7 @@ static void correct(orc_hp *h, double Dt)
         h->a3_d[k] = 6. * A3_d * Dt_inv3;
         double A2_sd = A2_s + A2_d;
         double A3_sd = A3_s + A3_d;
-        b->pos[k] = h->xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_Q1 * A3_sd);
+        b->pos[k] = h->xp[k] + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_POS * A3_sd);
         b->vel[k] = h->vp[k] + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
     }
 }
@@ -533,7 +533,7 @@ void orc_cluster_prepare(const gplum_params *p, gplum_particle *part, int n)
 
 /* ------------------------------------------------------------- Kepler */
 
-/* solveKeplerEq, kepler.h:5-38 (Q8: fabs). */
+/* solveKeplerEq, kepler.h:5-38 (fabs where the reference writes an unqualified abs). */
 double orc_solve_kepler(double l, double ecc)
 {
     double ecc2 = ecc * ecc, ecc3 = ecc2 * ecc, ecc4 = ecc2 * ecc2, ecc5 = ecc3 * ecc2, ecc6 = ecc3 * ecc3;

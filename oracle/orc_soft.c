@@ -143,7 +143,7 @@ void orc_force_clear(gplum_force *f, int n)
 /*
  * EP-EP FP32 kernel with neighbour counting: gravity_kernel_epep.pikg:53-97
  * (accumulation convention of the PIKG functor: af += m r/r^3, pf -= m/r,
- * counts accumulated, id_max/id_min max/min-merged; Q4: rank += drank^2).
+ * counts accumulated, id_max/id_min max/min-merged; rank += drank^2, the PIKG form).
  *
  * Fixed FP32 operation order (the HIP kernel uses the same):
  *   xiloc = (float)(xi - xi0)            (difference in double, rounded once)
@@ -223,7 +223,8 @@ void orc_epep_f32(const gplum_epi *epi, int ni, const gplum_epj *epj, int nj,
 }
 
 /* EP-SP quadrupole FP32, gravity_kernel_epsp.pikg:47-100 with the correct trace
- * tr = xx+yy+zz (:75; quirk Q3 of the non-PIKG functor is NOT reproduced). */
+ * tr = xx+yy+zz (:75; the non-PIKG functor's xx+yy+xx trace, gravity_kernel.hpp:177, is NOT
+ * reproduced). */
 void orc_epsp_quad_f32(const gplum_epi *epi, int ni, const gplum_spj_quad *spj, int nj,
                        gplum_force *force, float eps2)
 {
@@ -437,7 +438,7 @@ void orc_epsp_mono_f64(const gplum_epi *epi, int ni, const gplum_spj_mono *spj, 
  * correctForceBetween2Particles (src/gravity_soft.h:75-153) and its Initial
  * variant (:155-242), with USE_RE_SEARCH_NEIGHBOR (main_p3t.cpp:15) and
  * USE_INDIVIDUAL_CUTOFF.  Operation order follows the reference; the FDPS vector
- * dot product is x*x'+y*y'+z*z' left to right.  Quirk Q2: t_min uses
+ * dot product is x*x'+y*y'+z*z' left to right.  t_min (units of length) uses
  * std::min/std::max, so dv2 == 0 gives NaN and dr2_min falls back to dr2.
  * The self pair (j_id == i_id) only adds m_j * r_out_inv to phi (:104-107).
  */
