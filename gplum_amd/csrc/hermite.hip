@@ -451,16 +451,49 @@ __device__ inline void gravity_pc(const HP &P, HardView &H, const JData &J, int 
 // the 64 partial sums (every lane ends with the same total).  This fixed order is the
 // hard-force summation order of clusters above HARD_SEQ_MAX particles (the oracle sums
 // the same way); smaller clusters sum sequentially in j.
+// Lane permutations within rows of 16 lanes (DPP): quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror.  Applied in this order with a min, they
+// leave every lane of a row holding the row's minimum.
+constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+// quad_perm [3,2,1,0] and row_ror:8 (rotate a row of 16 lanes by 8: lane ^ 8)
+constexpr int DPP_QUAD_3210 = 0x1B, DPP_ROW_ROR8 = 0x128;
+
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 __device__ inline void coop_pair_sum(const HP &P, const JData &J, int n, int64_t idi, const double xi[3],
                                      const double vi[3], double rinv_i, double acc[3], double jerk[3]) {
     double a[3] = {0., 0., 0.}, jk[3] = {0., 0., 0.};
     // two pairs at a time: the eight-wave kernels hold more state per lane
     pair_sum_strided<2>(P, J, (int)(threadIdx.x & 63), 64, n, idi, xi, vi, rinv_i, a, jk, nullptr);
-    for (int d = 32; d > 0; d >>= 1)
+    // butterfly x += x[lane ^ d] for d = 32, 16, 8, 4, 2, 1 (every lane active): the two
+    // levels across rows of 16 lanes by shuffles, the four inside a row by DPP moves
+    // (row_ror:8 is lane ^ 8; quad_perm [3,2,1,0] after row_half_mirror is lane ^ 4;
+    // quad_perm [2,3,0,1] and [1,0,3,2] are lane ^ 2 and lane ^ 1), which skip the LDS
+    // crossbar.  The same partner values in the same order as the shuffle butterfly.
+    for (int d = 32; d > 8; d >>= 1)
         for (int k = 0; k < 3; k++) {
             a[k] += __shfl_xor(a[k], d);
             jk[k] += __shfl_xor(jk[k], d);
         }
+    for (int k = 0; k < 3; k++) { a[k] += dpp_f64<DPP_ROW_ROR8>(a[k]); jk[k] += dpp_f64<DPP_ROW_ROR8>(jk[k]); }
+    for (int k = 0; k < 3; k++) {
+        a[k] += dpp_f64<DPP_QUAD_3210>(dpp_f64<DPP_ROW_HALF_MIRROR>(a[k]));
+        jk[k] += dpp_f64<DPP_QUAD_3210>(dpp_f64<DPP_ROW_HALF_MIRROR>(jk[k]));
+    }
+    for (int k = 0; k < 3; k++) { a[k] += dpp_f64<DPP_QUAD_2301>(a[k]); jk[k] += dpp_f64<DPP_QUAD_2301>(jk[k]); }
+    for (int k = 0; k < 3; k++) { a[k] += dpp_f64<DPP_QUAD_1032>(a[k]); jk[k] += dpp_f64<DPP_QUAD_1032>(jk[k]); }
     for (int k = 0; k < 3; k++) { acc[k] = a[k]; jerk[k] = jk[k]; }
 }
 
@@ -609,25 +642,6 @@ __device__ inline double block_argmin(Reduce &R, double v, int i, int j, int *bi
     *bj = R.j[0];
     __syncthreads();
     return r;
-}
-
-// Lane permutations within rows of 16 lanes (DPP): quad_perm [1,0,3,2], quad_perm
-// [2,3,0,1], row_half_mirror, row_mirror.  Applied in this order with a min, they
-// leave every lane of a row holding the row's minimum.
-constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
-
-template <int CTRL>
-__device__ inline double dpp_f64(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ inline double readlane_f64(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // Minimum over the 64 lanes of the wave (every lane active; values not NaN): four DPP
