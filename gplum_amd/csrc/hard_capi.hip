@@ -320,3 +320,13 @@ extern "C" int gplum_hard_integrate_graph(const gplum_params *p, gplum_particle 
     if (mismatches) *mismatches = bad;
     return hard_download(c, pp, offsets, pl, edisp, block_steps, nullptr);
 }
+
+extern "C" int gplum_hermite_dt_selfcheck(const gplum_params *p, int64_t n, uint64_t seed, int64_t *mismatches,
+                                          int64_t *fallbacks) {
+    if (!p) return gplum::fail(GPLUM_E_ARG, "null params");
+    long long m = 0, f = 0;
+    const int rc = gplum::hermite_dt_selfcheck(*p, (long long)n, seed, &m, &f);
+    if (mismatches) *mismatches = m;
+    if (fallbacks) *fallbacks = f;
+    return rc;
+}

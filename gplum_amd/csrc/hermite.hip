@@ -130,17 +130,25 @@ __device__ inline bool clear_of_grid(double t, double d) {
 // halvings exact above 2^-1000), so a dt_1 from calc_dt4_fast that is clear of that
 // grid decides every comparison as the correctly rounded dt_1 does; near the grid
 // calc_dt4 itself is evaluated.  Same result, bit for bit.
+// FAST = false: calc_dt4 always (the self-check's reference).
+template <bool FAST = true>
 __device__ inline double dt_block(const HP &P, double time, double dt, double acc0,
                                   const double acc_d[3], const double jerk_d[3], const double a2_d[3],
                                   const double a3_d[3], const double acc_s[3], const double jerk_s[3],
-                                  const double a2_s[3], const double a3_s[3]) {
+                                  const double a2_s[3], const double a3_s[3], int *fell_back = nullptr) {
     double dt_next = dmin(2. * dt, 0.5 * P.dt_tree);
     double rem = fmod_block(time, dt_next);
     for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(time, dt_next); }
     if (rem != 0.0) return P.dt_min;
-    double dt_1 = dmin(calc_dt4_fast(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
-                       calc_dt4_fast(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
-    if (!clear_of_grid(dt_1, dt_next))
+    double dt_1 = 0.;
+    bool exact = !FAST;
+    if (FAST) {
+        dt_1 = dmin(calc_dt4_fast(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
+                    calc_dt4_fast(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
+        exact = !clear_of_grid(dt_1, dt_next);
+        if (fell_back) *fell_back = exact ? 1 : 0;
+    }
+    if (exact)
         dt_1 = dmin(calc_dt4(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
                     calc_dt4(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
     for (int it = 0; dt_1 < dt_next && 0.5 * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
@@ -1963,6 +1971,67 @@ int isolated_launch(const gplum_params &p, const IsoArgs &a, int n, double t0, d
               a.number, a.n_cluster};
     hipLaunchKernelGGL(k_isolated, dim3(ceil_div(n, 128)), dim3(128), 0, st, P, V, n, t0, t1);
     GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+
+// ------------------------------------------------------- time-step self-check
+
+__device__ inline uint64_t mix64(uint64_t z) {     // splitmix64 finaliser
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+// sign * [0.5, 1) * 2^e with e uniform in [e0, e1]
+__device__ inline double rnd_mag(uint64_t &st, int e0, int e1) {
+    st += 0x9E3779B97F4A7C15ULL;
+    const uint64_t r = mix64(st);
+    const double m = 0.5 + 0.5 * (double)(r >> 11) * 0x1p-53;
+    const int e = e0 + (int)((r & 0x3FF) % (uint64_t)(e1 - e0 + 1));
+    return ((r >> 10) & 1) ? -ldexp(m, e) : ldexp(m, e);
+}
+
+// dt_block with the fast criterion against dt_block with calc_dt4 on random states:
+// accelerations and their derivatives over many decades, steps dt_tree 2^-k and times on
+// the step's grid.  Counts results that differ in any bit and fast values sent to the
+// exact evaluation.
+__global__ void k_dt_selfcheck(HP P, long long n, uint64_t seed, unsigned long long *out) {
+    unsigned long long mism = 0, fb = 0;
+    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
+        uint64_t st = seed ^ mix64((uint64_t)s * 0x2545F4914F6CDD1DULL);
+        double v[8][3];
+        const int e_lo[8] = {-20, -24, -28, -32, -12, -16, -20, -24};
+        const int e_hi[8] = {8, 14, 20, 26, 4, 8, 12, 16};
+        for (int q = 0; q < 8; q++)
+            for (int k = 0; k < 3; k++) v[q][k] = rnd_mag(st, e_lo[q], e_hi[q]);
+        const double acc0 = fabs(rnd_mag(st, -12, 4));
+        st += 0x9E3779B97F4A7C15ULL;
+        const uint64_t r = mix64(st);
+        const int kk = 1 + (int)(r % 30);                       // dt = dt_tree 2^-kk
+        const double dt = ldexp(P.dt_tree, -kk);
+        const double time = dt * (double)((r >> 8) % (1ULL << kk));   // on the grid of dt
+        int fell = 0;
+        const double a = dt_block<true>(P, time + dt, dt, acc0, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], &fell);
+        const double b = dt_block<false>(P, time + dt, dt, acc0, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+        mism += (__double_as_longlong(a) != __double_as_longlong(b)) ? 1 : 0;
+        fb += fell;
+    }
+    atomicAdd(&out[0], mism);
+    atomicAdd(&out[1], fb);
+}
+
+int hermite_dt_selfcheck(const gplum_params &p, long long n, uint64_t seed, long long *mismatches,
+                         long long *fallbacks) {
+    if (n < 0) return fail(GPLUM_E_ARG, "negative sample count");
+    DArray<unsigned long long> cnt;
+    GPLUM_ENSURE(cnt, 2);
+    GPLUM_HIP(hipMemset(cnt.p, 0, 2 * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_dt_selfcheck, dim3(1024), dim3(256), 0, 0, make_hp(p), n, seed, cnt.p);
+    GPLUM_HIP(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    GPLUM_HIP(hipMemcpy(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost));
+    if (mismatches) *mismatches = (long long)h[0];
+    if (fallbacks) *fallbacks = (long long)h[1];
     return GPLUM_OK;
 }
 

@@ -73,4 +73,8 @@ int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, c
 int isolated_launch(const gplum_params &p, const IsoArgs &a, int n, double t0, double t1,
                     hipStream_t st);
 
+// dt_block's fast criterion against the correctly rounded one on n random states.
+int hermite_dt_selfcheck(const gplum_params &p, long long n, uint64_t seed, long long *mismatches,
+                         long long *fallbacks);
+
 }  // namespace gplum

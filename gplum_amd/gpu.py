@@ -117,6 +117,16 @@ def hard_integrate_graph(p, pp, offsets, replays=4, prepare=False):
     return ed.value, nb.value, ms.value, bad.value
 
 
+def hermite_dt_selfcheck(p, n, seed=1):
+    """gplum_hermite_dt_selfcheck: (mismatches, fallbacks) of the fast time-step criterion
+    against the correctly rounded one on n random states."""
+    m = ctypes.c_int64(-1)
+    f = ctypes.c_int64(-1)
+    check(lib().gplum_hermite_dt_selfcheck(ctypes.byref(p), ctypes.c_int64(n), ctypes.c_uint64(seed),
+                                           ctypes.byref(m), ctypes.byref(f)))
+    return m.value, f.value
+
+
 class SnapshotHeader(ctypes.Structure):
     """gplum_snapshot_header (include/gplum_system.h)."""
     _fields_ = [("time", ctypes.c_double), ("n_body", ctypes.c_int32), ("id_next", ctypes.c_int32),

@@ -199,6 +199,17 @@ int gplum_hard_integrate_graph(const gplum_params *p, gplum_particle *pp, int n,
                                int n_cluster, int prepare, int replays, double *edisp, int64_t *block_steps,
                                double *ms_per_replay, int *mismatches);
 
+/* Self-check of the Hermite time-step rule (no counterpart in the reference; for tests
+ * and for users who port the library to other hardware).  FPHard::calcDeltat
+ * (particle.h:1276-1314) compares the criterion dt_1 with the candidate steps only;
+ * the kernels evaluate dt_1 with the hardware square root and reciprocal and fall back
+ * to the correctly rounded evaluation when the fast value lies within 2^-10 of a
+ * candidate.  This entry evaluates both forms on n random states (seeded) on the
+ * device and reports the results that differ in any bit (0 expected) and the fast
+ * values that took the fallback. */
+int gplum_hermite_dt_selfcheck(const gplum_params *p, int64_t n, uint64_t seed, int64_t *mismatches,
+                               int64_t *fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

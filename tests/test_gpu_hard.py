@@ -118,6 +118,17 @@ def test_repeated_ids_match_oracle(oracle_lib):
         assert np.array_equal(got[f], ref[f]), f
 
 
+def test_fast_time_step_criterion_matches_exact():
+    """The Hermite time-step rule with the hardware-sqrt criterion and the exact fallback
+    near the step grid gives the correctly rounded rule's step, bit for bit, on 4M random
+    states (accelerations and derivatives over many decades, steps dt_tree 2^-k)."""
+    pg = gpu.params(collision=0, **HARD)
+    mism, fb = gpu.hermite_dt_selfcheck(pg, 4_000_000, seed=7)
+    print("mismatches %d, fallbacks to the exact criterion %d of 4M" % (mism, fb))
+    assert mism == 0
+    assert fb > 0      # the exact branch ran
+
+
 def _same_state(a, b, fields=("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_d", "phi_s", "dt", "time",
                               "mass", "r_planet", "id", "flags")):
     return [f for f in fields if not np.array_equal(a[f], b[f])]
