@@ -142,8 +142,19 @@ __global__ void k_radii_binfinal(const int *__restrict__ range, const int *__res
                                  const double *__restrict__ csum, double *__restrict__ vd, int *__restrict__ np) {
     const int b = threadIdx.x;
     if (b >= RADII_NB) return;
+    // chunk sums added in ascending order; 32 loads in flight per batch (the adds
+    // form one dependent chain, the loads need not)
     double s = 0.;
-    for (int c = cbase[b]; c < cbase[b + 1]; c++) s += csum[c];
+    int c = cbase[b];
+    const int ce = cbase[b + 1];
+    for (; c + 32 <= ce; c += 32) {
+        double v[32];
+#pragma unroll
+        for (int k = 0; k < 32; k++) v[k] = csum[c + k];
+#pragma unroll
+        for (int k = 0; k < 32; k++) s += v[k];
+    }
+    for (; c < ce; c++) s += csum[c];
     const int cnt = range[2 * b + 1] - range[2 * b];
     np[b] = cnt;
     vd[b] = (cnt > 0) ? sqrt(s / cnt) : 0.;
