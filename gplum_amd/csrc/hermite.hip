@@ -105,12 +105,13 @@ __device__ inline double calc_dt4_fast(double eta, double alpha2, double acc0, c
     const double Jerk2 = dot3(j[0], j[1], j[2], j[0], j[1], j[2]);
     const double Snap2 = dot3(s[0], s[1], s[2], s[0], s[1], s[2]);
     const double Crac2 = dot3(c[0], c[1], c[2], c[0], c[1], c[2]);
-    if (!(Jerk2 > 0.)) return DBL_MAX;
+    // selects, not branches: the Sun and planet criteria of dt_block stay one basic block
     const double as2 = Acc2 * Snap2, jc2 = Jerk2 * Crac2;
     const bool ok = (as2 >= DBL_MIN || (Acc2 == 0. || Snap2 == 0.)) && (jc2 >= DBL_MIN || Crac2 == 0.);
     const double num = __builtin_amdgcn_sqrt(as2) + Jerk2;
     const double den = __builtin_amdgcn_sqrt(jc2) + Snap2;
-    return ok ? eta * __builtin_amdgcn_sqrt(num * __builtin_amdgcn_rcp(den)) : __builtin_nan("");
+    const double q = ok ? eta * __builtin_amdgcn_sqrt(num * __builtin_amdgcn_rcp(den)) : __builtin_nan("");
+    return (Jerk2 > 0.) ? q : DBL_MAX;
 }
 
 // True when t is farther than 2^-10 relative from every d * 2^k (k integer), for
@@ -118,11 +119,11 @@ __device__ inline double calc_dt4_fast(double eta, double alpha2, double acc0, c
 // same side of each of those grid points.  t = m_t 2^e_t, d = m_d 2^e_d with mantissas
 // in [0.5, 1): the grid points nearest t are at m_t / m_d in {1/2, 1, 2}.
 __device__ inline bool clear_of_grid(double t, double d) {
-    if (!(t > 0x1p-1000 && t < DBL_MAX)) return false;
     int et = 0, ed = 0;
     const double r = frexp(t, &et) * __builtin_amdgcn_rcp(frexp(d, &ed));
     const double tol = 0x1p-10;
-    return r > 0.5 * (1. + tol) && (r < 1. - tol || r > 1. + tol) && r < 2. * (1. - tol);
+    return (t > 0x1p-1000 && t < DBL_MAX) && r > 0.5 * (1. + tol) && (r < 1. - tol || r > 1. + tol) &&
+           r < 2. * (1. - tol);
 }
 
 // FPHard::calcDeltat, src/particle.h:1276-1314.  dt_1 enters the result only through
