@@ -47,14 +47,6 @@ def build(jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
     os.replace(tmp, LIB)
-    # C++ host program that drives the reference functor interface (include/gplum_fdps.hpp)
-    demo = os.path.join(HERE, "lib", "fdps_functor_demo")
-    cmd = ["g++", "-std=c++14", "-O2", "-I", os.path.join(ROOT, "include"),
-           os.path.join(HERE, "host", "fdps_functor_demo.cpp"), "-L", os.path.dirname(LIB),
-           "-lgplum_gpu", "-Wl,-rpath,$ORIGIN", "-o", demo]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("g++ failed for the functor demo:\n%s" % r.stderr[-4000:])
     return LIB
 
 

@@ -86,6 +86,7 @@ struct Tree {
     int64_t n_ep_total = 0, n_epf_total = 0, n_sp_total = 0, n_epl_total = 0;
     int64_t n_epep_pairs = 0, n_epsp_pairs = 0;
     int queue_cap = 1 << 14;
+    int coop_min = 1 << 16;                    // nodes of at least this many keys: workgroup-wide digit search
 };
 
 }  // namespace gplum

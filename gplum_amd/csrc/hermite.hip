@@ -98,7 +98,7 @@ __device__ inline double dt_initial(const HP &P, double time, double dt, double 
 // correctly rounded; Acc * Snap as sqrt(Acc2 * Snap2), Jerk * Crac likewise): within a
 // few 1e-7 of calc_dt4 relative, DBL_MAX exactly where calc_dt4 gives it (Jerk > 0 iff
 // Jerk2 > 0).  A product that underflows (0 from non-zero factors, or subnormal) gives
-// NaN, which no grid test passes.
+// NaN and one that overflows gives inf or NaN; dt_block sends either to calc_dt4.
 __device__ inline double calc_dt4_fast(double eta, double alpha2, double acc0, const double a[3],
                                        const double j[3], const double s[3], const double c[3]) {
     const double Acc2 = dot3(a[0], a[1], a[2], a[0], a[1], a[2]) + alpha2 * acc0 * acc0;
@@ -129,8 +129,10 @@ __device__ inline bool clear_of_grid(double t, double d) {
 // FPHard::calcDeltat, src/particle.h:1276-1314.  dt_1 enters the result only through
 // dt_1 < dt_next for dt_next = D, D/2, D/4, ... (D the step left by the fmod loop,
 // halvings exact above 2^-1000), so a dt_1 from calc_dt4_fast that is clear of that
-// grid decides every comparison as the correctly rounded dt_1 does; near the grid
-// calc_dt4 itself is evaluated.  Same result, bit for bit.
+// grid decides every comparison as the correctly rounded dt_1 does; near the grid, or
+// when either fast criterion is NaN or infinite (under- or overflow of its products,
+// which the minimum would otherwise drop), calc_dt4 itself is evaluated.  Same result,
+// bit for bit.
 // FAST = false: calc_dt4 always (the self-check's reference).
 template <bool FAST = true>
 __device__ inline double dt_block(const HP &P, double time, double dt, double acc0,
@@ -144,9 +146,11 @@ __device__ inline double dt_block(const HP &P, double time, double dt, double ac
     double dt_1 = 0.;
     bool exact = !FAST;
     if (FAST) {
-        dt_1 = dmin(calc_dt4_fast(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d),
-                    calc_dt4_fast(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s));
-        exact = !clear_of_grid(dt_1, dt_next);
+        const double f_planet = calc_dt4_fast(P.eta, P.alpha2, acc0, acc_d, jerk_d, a2_d, a3_d);
+        const double f_sun = calc_dt4_fast(P.eta_sun, P.alpha2, 0., acc_s, jerk_s, a2_s, a3_s);
+        dt_1 = dmin(f_planet, f_sun);
+        // DBL_MAX is a valid value (no jerk); NaN and inf are not
+        exact = !(f_planet <= DBL_MAX && f_sun <= DBL_MAX) || !clear_of_grid(dt_1, dt_next);
         if (fell_back) *fell_back = exact ? 1 : 0;
     }
     if (exact)
@@ -1988,14 +1992,16 @@ __device__ inline double rnd_mag(uint64_t &st, int e0, int e1) {
     st += 0x9E3779B97F4A7C15ULL;
     const uint64_t r = mix64(st);
     const double m = 0.5 + 0.5 * (double)(r >> 11) * 0x1p-53;
-    const int e = e0 + (int)((r & 0x3FF) % (uint64_t)(e1 - e0 + 1));
+    const int e = e0 + (int)(mix64(r ^ 0x5851F42D4C957F2DULL) % (uint64_t)(e1 - e0 + 1));
     return ((r >> 10) & 1) ? -ldexp(m, e) : ldexp(m, e);
 }
 
 // dt_block with the fast criterion against dt_block with calc_dt4 on random states:
 // accelerations and their derivatives over many decades, steps dt_tree 2^-k and times on
-// the step's grid.  Counts results that differ in any bit and fast values sent to the
-// exact evaluation.
+// the step's grid.  One sample in four widens every magnitude by 2^+-540, so that the
+// products of squares in the fast criterion underflow below DBL_MIN or overflow past
+// DBL_MAX.  Counts results that differ in any bit and fast values sent to the exact
+// evaluation.
 __global__ void k_dt_selfcheck(HP P, long long n, uint64_t seed, unsigned long long *out) {
     unsigned long long mism = 0, fb = 0;
     for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
@@ -2003,8 +2009,9 @@ __global__ void k_dt_selfcheck(HP P, long long n, uint64_t seed, unsigned long l
         double v[8][3];
         const int e_lo[8] = {-20, -24, -28, -32, -12, -16, -20, -24};
         const int e_hi[8] = {8, 14, 20, 26, 4, 8, 12, 16};
+        const int wide = ((s & 3) == 3) ? 540 : 0;
         for (int q = 0; q < 8; q++)
-            for (int k = 0; k < 3; k++) v[q][k] = rnd_mag(st, e_lo[q], e_hi[q]);
+            for (int k = 0; k < 3; k++) v[q][k] = rnd_mag(st, e_lo[q] - wide, e_hi[q] + wide);
         const double acc0 = fabs(rnd_mag(st, -12, 4));
         st += 0x9E3779B97F4A7C15ULL;
         const uint64_t r = mix64(st);

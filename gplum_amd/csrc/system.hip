@@ -460,6 +460,7 @@ int System::soft_force(bool initial, bool use_f64) {
     stats.force_ms += elapsed(EV_FORCE0, EV_FORCE1);
     stats.tree_ms += elapsed(EV_TREE0, EV_TREE1);
     stats.n_force_calls++;
+    if (use_f64) stats.n_force_f64_calls++;
     last_ncand = ncand;
     tick(EV_NB0);
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
@@ -738,7 +739,7 @@ int System::merge_and_recompute() {
     // the recomputed force repeats the step's neighbour counts for all but the merged
     // bodies: the forecaster plans with it but does not train on it again
     skip_train = true;
-    rc = soft_force(true, false);
+    rc = soft_force(true, plain_f64);
     skip_train = false;
     return rc;
 }
@@ -825,6 +826,7 @@ int System::autotune(int trials, gplum_tuning *out) {
         out->hard_wg_per_cu = b_wg;
         out->hard_lds_max = b_lds;
         out->hard_giant_min = hard_giant_min;
+        out->tree_coop_min = tree.coop_min;
         out->soft_ms = best_soft;
         out->hard_ms = best_hard < 1e300 ? best_hard : 0.;
     }
@@ -976,7 +978,7 @@ int gplum_system_get_tuning(gplum_system *s, gplum_tuning *out) {
     if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
     *out = gplum_tuning{S->p.n_group_limit, S->soft_tile, S->hard_wg_per_cu, S->hard_lds_max, S->hard_giant_min,
-                        0., 0.};
+                        S->tree.coop_min, 0., 0.};
     return GPLUM_OK;
 }
 
@@ -988,7 +990,9 @@ int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t) {
     if (t->hard_wg_per_cu < 1 || t->hard_wg_per_cu > 64) return gplum::fail(GPLUM_E_ARG, "hard_wg_per_cu must be 1..64");
     if (t->hard_lds_max < 64 || t->hard_lds_max > 256) return gplum::fail(GPLUM_E_ARG, "hard_lds_max must be 64..256");
     if (t->hard_giant_min < 64) return gplum::fail(GPLUM_E_ARG, "hard_giant_min must be >= 64");
+    if (t->tree_coop_min < 0) return gplum::fail(GPLUM_E_ARG, "tree_coop_min must be >= 0");
     System *S = reinterpret_cast<System *>(s);
+    S->tree.coop_min = t->tree_coop_min > 0 ? t->tree_coop_min : (1 << 16);
     S->p.n_group_limit = t->n_group_limit;
     S->soft_tile = t->soft_tile;
     S->hard_wg_per_cu = t->hard_wg_per_cu;

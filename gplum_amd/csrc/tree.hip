@@ -189,13 +189,12 @@ __device__ inline int block_sum(int v, int *s_w) {
     return all;
 }
 
-// Nodes of at least COOP_MIN keys (the top levels, where a workgroup's chunk holds one
-// node) find their digit bounds with the whole workgroup: each round samples 255 cut
-// points of each of the seven search intervals at once and keeps the piece holding the
-// first key of a larger digit, so a node of N keys takes log_256 N rounds of parallel
-// loads instead of log_2 N dependent ones.  Smaller nodes keep one thread each (many
-// nodes per chunk: their searches run side by side).
-constexpr int COOP_MIN = 1 << 16;
+// Nodes of at least coop_min keys (Tree::coop_min, 2^16 by default: the top levels,
+// where a workgroup's chunk holds one node) find their digit bounds with the whole
+// workgroup: each round samples 255 cut points of each of the seven search intervals at
+// once and keeps the piece holding the first key of a larger digit, so a node of N keys
+// takes log_256 N rounds of parallel loads instead of log_2 N dependent ones.  Smaller
+// nodes keep one thread each (many nodes per chunk: their searches run side by side).
 
 // bnd8[d] = first index in [s, e) whose digit at level lv is > d (d = 0..6), bnd8[7] = e:
 // the same bounds as digit_bounds (bnd[d + 1]).  Every thread of the workgroup calls it.
@@ -242,7 +241,7 @@ __device__ void digit_bounds_wg(const uint64_t *__restrict__ key, int s, int e, 
 __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
     const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, int n_leaf,
     const int32_t *__restrict__ start, const int32_t *__restrict__ count, const int32_t *__restrict__ level,
-    int32_t *__restrict__ nchild, int32_t *__restrict__ part, int32_t *__restrict__ bnd_out) {
+    int32_t *__restrict__ nchild, int32_t *__restrict__ part, int32_t *__restrict__ bnd_out, int coop_min) {
     __shared__ int s_w[LEVEL_BLOCK / 64];
     __shared__ int s_big[LEVEL_BLOCK], s_nbig, s_lo[7], s_hi[7], s_first[7];
     int lb, m, c0, c1;
@@ -258,7 +257,7 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
             const int s = start[c], cnt = count[c], lv = level[c];
             int nc = 0;
             if (cnt > n_leaf && lv < MAXLEVEL) {
-                if (cnt >= COOP_MIN) {
+                if (cnt >= coop_min) {
                     s_big[atomicAdd(&s_nbig, 1)] = r;   // the workgroup searches it below
                     nc = -1;
                 } else {
@@ -682,7 +681,7 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
     for (int L = 0; L <= MAXLEVEL; L++) {
         hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
-                           t.lvl_part.p, t.lvl_bnd.p);
+                           t.lvl_part.p, t.lvl_bnd.p, t.coop_min);
         hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
                            t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);

@@ -155,7 +155,7 @@ class Stats(ctypes.Structure):
         "n_neighbors",
         "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
         "cand_retries", "n_force_calls", "fc_calls", "fc_slots", "fc_ovf_groups",
-        "fc_ovf_candidates", "hard_pairs", "max_block_steps", "max_block_cluster")] + [(n, ctypes.c_double) for n in (
+        "fc_ovf_candidates", "hard_pairs", "max_block_steps", "max_block_cluster", "n_force_f64_calls")] + [(n, ctypes.c_double) for n in (
             "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms")]
 
     def as_dict(self):
@@ -166,7 +166,7 @@ class Tuning(ctypes.Structure):
     """gplum_tuning (include/gplum_system.h)."""
     _fields_ = [("n_group_limit", ctypes.c_int32), ("soft_tile", ctypes.c_int32),
                 ("hard_wg_per_cu", ctypes.c_int32), ("hard_lds_max", ctypes.c_int32),
-                ("hard_giant_min", ctypes.c_int32),
+                ("hard_giant_min", ctypes.c_int32), ("tree_coop_min", ctypes.c_int32),
                 ("soft_ms", ctypes.c_double), ("hard_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -277,8 +277,8 @@ class System:
         return t.as_dict()
 
     def set_tuning(self, n_group_limit=256, soft_tile=256, hard_wg_per_cu=8, hard_lds_max=256,
-                   hard_giant_min=1024, **_):
-        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, hard_giant_min, 0.0, 0.0)
+                   hard_giant_min=1024, tree_coop_min=0, **_):
+        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, hard_giant_min, tree_coop_min, 0.0, 0.0)
         check(lib().gplum_system_set_tuning(self.h, ctypes.byref(t)))
 
     FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}

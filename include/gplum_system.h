@@ -54,6 +54,7 @@ typedef struct gplum_stats {
     int64_t hard_pairs;                       /* Hermite pair evaluations (hard_pair calls) */
     int64_t max_block_steps;                  /* most block steps of one cluster (max over calls) */
     int64_t max_block_cluster;                /* members of that cluster                   */
+    int64_t n_force_f64_calls;                /* soft forces run with the FP64 kernels     */
     double force_ms, tree_ms, finalize_ms, hard_ms;
     double time_sys;
     double fc_loss;                           /* last training loss (Poisson NLL)          */
@@ -129,6 +130,8 @@ typedef struct gplum_tuning {
     int32_t hard_wg_per_cu;  /* persistent Hermite workgroups per CU                  */
     int32_t hard_lds_max;    /* clusters up to this size stage their j-data in LDS    */
     int32_t hard_giant_min;  /* larger clusters run over the whole GPU (not tuned)    */
+    int32_t tree_coop_min;   /* tree nodes of at least this many particles split with a
+                                workgroup-wide search (not tuned; 0 = 65536)          */
     double soft_ms, hard_ms; /* best measured soft stage / Hermite kernels, ms        */
 } gplum_tuning;
 int gplum_system_autotune(gplum_system *s, int trials, gplum_tuning *out);

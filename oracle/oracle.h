@@ -110,13 +110,20 @@ void orc_cluster_prepare(const gplum_params *p, gplum_particle *pp, int n);
 /* timeIntegrateKepler_isolated (src/hermite.h:787-815), kepler.h:3-93. */
 void orc_kepler_isolated(const gplum_params *p, gplum_particle *pi,
                          double time_start, double time_end);
+/* The same with solveKeplerEq's int-abs outcome (orc_solve_kepler_int_abs). */
+void orc_kepler_isolated_int_abs(const gplum_params *p, gplum_particle *pi,
+                                 double time_start, double time_end);
 /* timeIntegrate_isolated (src/hermite.h:734-785): Sun-only Hermite, used when
  * e >= 0.8 or eps2_sun != 0 (src/hard.h:803-811). */
 void orc_hermite_isolated(const gplum_params *p, gplum_particle *pi);
+/* The dispatch between the two (hard.h:803-811), n_cluster = 1. */
+void orc_isolated(const gplum_params *p, gplum_particle *pi, double time_start, double time_end);
 /* Kepler-equation pieces, src/kepler.h:3-93 (exposed for tests). */
 double orc_cbrt_exact_ops(double x);
 double orc_exp_exact_ops(double x);
 double orc_solve_kepler(double l, double ecc);
+/* solveKeplerEq with int abs(int) bound at kepler.h:23 (Newton only if |residual| >= 1). */
+double orc_solve_kepler_int_abs(double l, double ecc);
 void orc_posvel_to_elements(const double pos[3], const double vel[3], double mu,
                             double *ax, double *ecc, double *n, double *u,
                             double P[3], double Q[3]);

@@ -63,6 +63,12 @@ def lib():
         L.orc_cluster_prepare.argtypes = [P, vp, i32]
         L.orc_solve_kepler.argtypes = [f64, f64]
         L.orc_solve_kepler.restype = f64
+        L.orc_solve_kepler_int_abs.argtypes = [f64, f64]
+        L.orc_solve_kepler_int_abs.restype = f64
+        L.orc_isolated.argtypes = [P, vp, f64, f64]
+        L.orc_kepler_isolated_int_abs.argtypes = [P, vp, f64, f64]
+        L.orc_eccentricity.argtypes = [P, vp, vp]
+        L.orc_eccentricity.restype = f64
         L.orc_set_cutoff_radii.argtypes = [P, vp, i32]
         L.orc_energy.argtypes = [vp, i32, vp]
         L.orc_tree_build.argtypes = [P, vp, i32, vp]
@@ -233,10 +239,29 @@ def hermite_clusters(p, pp, offsets):
     return nc, ed, nb
 
 
-def kepler_isolated(p, pp, t0, t1):
+def kepler_isolated(p, pp, t0, t1, int_abs=False):
+    """timeIntegrateKepler_isolated on every particle; int_abs: solveKeplerEq with the
+    int abs(int) outcome of kepler.h:23 (Newton only for residuals of 1 or more)."""
+    f = lib().orc_kepler_isolated_int_abs if int_abs else lib().orc_kepler_isolated
     for i in range(len(pp)):
-        lib().orc_kepler_isolated(ctypes.byref(p), ctypes.c_void_p(pp.ctypes.data + i * pp.itemsize),
-                                  t0, t1)
+        f(ctypes.byref(p), ctypes.c_void_p(pp.ctypes.data + i * pp.itemsize), t0, t1)
+
+
+def hermite_isolated(p, pp):
+    """timeIntegrate_isolated (Sun-only Hermite over one soft step) on every particle."""
+    for i in range(len(pp)):
+        lib().orc_hermite_isolated(ctypes.byref(p), ctypes.c_void_p(pp.ctypes.data + i * pp.itemsize))
+
+
+def isolated(p, pp, t0, t1):
+    """The isolated-particle branch of timeIntegrate (Kepler or Sun-only Hermite)."""
+    for i in range(len(pp)):
+        lib().orc_isolated(ctypes.byref(p), ctypes.c_void_p(pp.ctypes.data + i * pp.itemsize), t0, t1)
+
+
+def eccentricity(p, pp):
+    return np.array([lib().orc_eccentricity(ctypes.byref(p), ctypes.c_void_p(pp.ctypes.data + i * pp.itemsize),
+                                            None) for i in range(len(pp))])
 
 
 class P3T:

@@ -557,6 +557,34 @@ double orc_solve_kepler(double l, double ecc)
     return u;
 }
 
+/* solveKeplerEq (kepler.h:5-38) when the unqualified `abs` of kepler.h:23 binds to
+ * int abs(int): the residual is truncated to an int first, so the Newton refinement
+ * runs only when |u - e sin u - l| >= 1.  Which overload binds depends on the headers
+ * the reference is compiled with; the kernels follow orc_solve_kepler (fabs).  This
+ * variant exists so that tests can show how far the two outcomes lie apart. */
+double orc_solve_kepler_int_abs(double l, double ecc)
+{
+    double ecc2 = ecc * ecc, ecc3 = ecc2 * ecc, ecc4 = ecc2 * ecc2, ecc5 = ecc3 * ecc2, ecc6 = ecc3 * ecc3;
+    double u = l
+        + (ecc - ecc3 / 8. + ecc5 / 192.) * sin(l)
+        + (ecc2 / 2. - ecc4 / 6. + ecc6 / 48.) * sin(2. * l)
+        + (3. * ecc3 / 8. - 27. * ecc5 / 128.) * sin(3. * l)
+        + (ecc4 / 3. - 4. * ecc6 / 15.) * sin(4. * l)
+        + 125. * ecc5 / 384. * sin(5. * l)
+        + 27. * ecc6 / 80. * sin(6. * l);
+    if (abs((int)((u - ecc * sin(u)) - l)) > 1.e-15) {
+        double u0;
+        int loop = 0;
+        do {
+            u0 = u;
+            double sinu0 = sin(u0), cosu0 = cos(u0);
+            u = u0 - ((u0 - ecc * sinu0 - l) / (1. - ecc * cosu0));
+            loop++;
+        } while (fabs(u - u0) > 1.e-15 && loop < 10);
+    }
+    return u;
+}
+
 /* posVel2OrbitalElement, kepler.h:39-76. */
 void orc_posvel_to_elements(const double pos[3], const double vel[3], double mu,
                             double *ax, double *ecc, double *n, double *u,
@@ -603,20 +631,31 @@ void orc_elements_to_posvel(double pos[3], double vel[3], double mu, double ax,
     for (int k = 0; k < 3; k++) vel[k] = ax * ax * n * rinv * (-sinu * P[k] + ecc_sq * cosu * Q[k]);
 }
 
-/* timeIntegrateKepler_isolated, hermite.h:787-815. */
-void orc_kepler_isolated(const gplum_params *p, gplum_particle *pi, double time_start, double time_end)
+/* timeIntegrateKepler_isolated, hermite.h:787-815, with the Kepler solver given. */
+static void kepler_isolated(const gplum_params *p, gplum_particle *pi, double time_start, double time_end,
+                            double (*solve)(double, double))
 {
     double ax, ecc, n, u, P[3], Q[3];
     orc_posvel_to_elements(pi->pos, pi->vel, p->m_sun, &ax, &ecc, &n, &u, P, Q);
     double l = u - ecc * sin(u);
     l += n * (time_end - time_start);
-    u = orc_solve_kepler(l, ecc);
+    u = solve(l, ecc);
     orc_elements_to_posvel(pi->pos, pi->vel, p->m_sun, ax, ecc, n, u, P, Q);
     pi->time += (time_end - time_start);
     pi->phi_d = 0.;
     for (int k = 0; k < 3; k++) { pi->acc_d[k] = 0.; pi->jerk_d[k] = 0.; }
     orc_star_gravity(p, pi);
     orc_calc_dt_initial(p, pi);
+}
+
+void orc_kepler_isolated(const gplum_params *p, gplum_particle *pi, double time_start, double time_end)
+{
+    kepler_isolated(p, pi, time_start, time_end, orc_solve_kepler);
+}
+
+void orc_kepler_isolated_int_abs(const gplum_params *p, gplum_particle *pi, double time_start, double time_end)
+{
+    kepler_isolated(p, pi, time_start, time_end, orc_solve_kepler_int_abs);
 }
 
 /* timeIntegrate_isolated, hermite.h:734-785, through FPHard (local time). */
@@ -644,4 +683,16 @@ void orc_hermite_isolated(const gplum_params *p, gplum_particle *part)
     orc_star_gravity(p, &h.b);
     h.b.time = h.time_c + h.b.time;
     *part = h.b;
+}
+
+/* The isolated-particle branch of HardSystem::timeIntegrate (hard.h:793-818): the
+ * Kepler drift when e < 0.8 and the Sun is unsoftened, else the Sun-only Hermite;
+ * n_cluster = 1 either way. */
+void orc_isolated(const gplum_params *p, gplum_particle *pi, double time_start, double time_end)
+{
+    if (orc_eccentricity(p, pi, NULL) < 0.8 && p->eps2_sun == 0.)
+        orc_kepler_isolated(p, pi, time_start, time_end);
+    else
+        orc_hermite_isolated(p, pi);
+    pi->n_cluster = 1;
 }

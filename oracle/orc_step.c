@@ -96,11 +96,7 @@ int orc_p3t_step(const gplum_params *p, gplum_particle *pp, int *n_io, orc_nlist
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) {
         if (pp[i].neighbor.number) continue;
-        if (orc_eccentricity(p, &pp[i], NULL) < 0.8 && p->eps2_sun == 0.)
-            orc_kepler_isolated(p, &pp[i], time_sys, time_sys + p->dt_tree);
-        else
-            orc_hermite_isolated(p, &pp[i]);
-        pp[i].n_cluster = 1;
+        orc_isolated(p, &pp[i], time_sys, time_sys + p->dt_tree);
     }
     *edisp += ed;
     /* soft force + correction */

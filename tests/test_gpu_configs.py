@@ -54,6 +54,10 @@ def test_config5_embryos_fp32_and_fp64_paths():
         s.step(2)
         e1 = s.energy()
         st = s.stats()
+        # the FP64 variant runs every soft force, the recomputes after merging included,
+        # with the FP64 kernels; the production path none
+        assert st["n_force_f64_calls"] == (st["n_force_calls"] if fp64 else 0), st
+        assert st["n_collisions"] > 0
         errs.append((e1[0] - e0[0] - (e1[5] - e0[5])) / e0[0])
         print("config5 (N=1M + 10 embryos) fp64_soft=%d: energy error %.3e, largest cluster %d, collisions %d"
               % (fp64, errs[-1], st["max_cluster"], st["n_collisions"]))

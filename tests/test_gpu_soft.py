@@ -111,3 +111,31 @@ def test_ragged_sizes(oracle_lib, ni, nj):
             print("max rel |acc| err %.3e" % rel.max())
             assert rel.max() < 1e-5
         assert np.abs(f_gpu["phi"] - ph).max() < 1e-5 * np.abs(ph).max()
+
+
+def test_epsp_quadrupole_scaling_on_gpu(oracle_lib):
+    """The GPU EP-SP kernels against the direct sum of a compact group's constituents
+    (tests/test_oracle.py): the FP64 twin's quadrupole error falls as (s/d)^3, and the
+    FP32 monopole/quadrupole kernels carry the same expansion error as the FP64 oracle."""
+    from tests.test_oracle import epsp_errors
+    dists = np.array([10.0, 20.0, 40.0, 80.0])
+
+    def gpu_quad64(epi, sp):
+        return gpu.calc_force_epsp_quad_f64(epi, sp, abi.cleared_force(len(epi)))
+
+    def gpu32(fn):
+        def call(epi, sp):
+            f = abi.cleared_force(len(epi))
+            fn(epi, sp, f)
+            return f["acc"].astype(np.float64), f["phi"].astype(np.float64)
+        return call
+
+    e = epsp_errors(oracle_lib.epsp_mono_f64, gpu_quad64, np.random.default_rng(5), 1.0, dists)
+    for col in range(2):
+        slope_q = np.polyfit(np.log(dists), np.log(e[:, 1, col]), 1)[0]
+        assert -3.3 < slope_q < -2.7, (col, slope_q)
+    ref = epsp_errors(oracle_lib.epsp_mono_f64, oracle_lib.epsp_quad_f64, np.random.default_rng(5), 1.0, dists)
+    np.testing.assert_allclose(e[:, 1, :], ref[:, 1, :], rtol=1e-6)
+    e32 = epsp_errors(gpu32(gpu.calc_force_epsp_mono), gpu32(gpu.calc_force_epsp_quad),
+                      np.random.default_rng(5), 1.0, dists[:3])
+    np.testing.assert_allclose(e32, ref[:3], rtol=0.05)

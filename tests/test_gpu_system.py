@@ -45,14 +45,18 @@ def test_init3000_golden(oracle_lib, init3000):
     assert flagged == [624, 1728, 1742, 2165]
 
 
-@pytest.mark.parametrize("quad", [1, 0])
-def test_tree_force_and_neighbours_match_oracle(oracle_lib, quad):
+@pytest.mark.parametrize("quad,coop_min", [(1, 0), (0, 0), (1, 64)])
+def test_tree_force_and_neighbours_match_oracle(oracle_lib, quad, coop_min):
+    """coop_min 64: every tree node above 64 particles is split by the workgroup-wide
+    digit search (normally only nodes of 65,536 particles and more)."""
     po, pg = both_params(oracle_lib, quadrupole=quad, **CFG1)
     pp = ring(6000, seed=21)
     # radii: host arithmetic must agree bit for bit
     ref = pp.copy()
     oracle_lib.set_cutoff_radii(po, ref)
     s = gpu.System(pg, pp)
+    if coop_min:
+        s.set_tuning(n_group_limit=pg.n_group_limit, tree_coop_min=coop_min)
     s.set_radii()
     got = by_id(s.particles())
     for f in ("r_out", "r_out_inv", "r_search", "v_disp"):

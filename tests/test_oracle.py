@@ -129,12 +129,85 @@ def test_fp32_kernels_against_fp64(oracle_lib):
     tree.free()
 
 
+def blob_moments(rng, k, s, centre):
+    """k point masses within about s of centre, and their FDPS-style moments (mass, centre
+    of mass, quad = sum m (x - com)(x - com)^T as {xx, yy, zz, xy, xz, yz})."""
+    m = rng.uniform(0.5, 1.5, k) * 1e-9
+    x = centre + rng.normal(0.0, s, (k, 3))
+    com = (m[:, None] * x).sum(0) / m.sum()
+    d = x - com
+    q = np.array([(m * d[:, a] * d[:, b]).sum() for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))])
+    spq = np.zeros(1, dtype=abi.SPJ_QUAD)
+    spq["mass"], spq["pos"], spq["quad"] = m.sum(), com, q
+    spm = np.zeros(1, dtype=abi.SPJ_MONO)
+    spm["mass"], spm["pos"] = m.sum(), com
+    return m, x, spq, spm
+
+
+def epsp_errors(orc_mono, orc_quad, rng, s, dists):
+    """rms relative errors of the EP-SP monopole / quadrupole acc and phi against the
+    direct sum over the blob's constituents, for i-particles at each distance d."""
+    out = []
+    m, x, spq, spm = blob_moments(rng, 24, s, np.zeros(3))
+    for d in dists:
+        u = rng.normal(size=(64, 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        epi = np.zeros(64, dtype=abi.EPI)
+        epi["pos"] = d * u
+        epi["r_out"] = 1e-6
+        epi["r_search"] = 1e-6
+        dr = x[None, :, :] - epi["pos"][:, None, :]
+        r = np.linalg.norm(dr, axis=2)
+        acc = (m[None, :, None] * dr / r[:, :, None] ** 3).sum(1)
+        phi = -(m[None, :] / r).sum(1)
+        row = []
+        for fn, sp in ((orc_mono, spm), (orc_quad, spq)):
+            a, ph = fn(epi, sp)
+            row.append((helpers.rel_err_rms(a, acc), helpers.rel_err_rms(ph, phi)))
+        out.append(row)
+    return np.array(out)       # [distance, mono/quad, acc/phi]
+
+
+def test_epsp_quadrupole_error_scales_as_cube(oracle_lib):
+    """The EP-SP expansion (gravity_kernel_epsp.pikg:63-94) against the direct sum of the
+    constituents of a compact group of size s seen from distance d: the monopole's error
+    falls as (s/d)^2, the quadrupole's as (s/d)^3 (the dipole vanishes about the centre of
+    mass).  A quadrupole with a wrong trace, sign or factor keeps an (s/d)^2 term."""
+    rng = np.random.default_rng(5)
+    dists = np.array([10.0, 20.0, 40.0, 80.0])
+    e = epsp_errors(oracle_lib.epsp_mono_f64, oracle_lib.epsp_quad_f64, rng, 1.0, dists)
+    for col in range(2):
+        slope_m = np.polyfit(np.log(dists), np.log(e[:, 0, col]), 1)[0]
+        slope_q = np.polyfit(np.log(dists), np.log(e[:, 1, col]), 1)[0]
+        assert -2.3 < slope_m < -1.7, (col, slope_m)
+        assert -3.3 < slope_q < -2.7, (col, slope_q)
+    assert np.all(e[:, 1, :] < 0.35 * e[:, 0, :])
+    # the FP32 kernels carry the same expansion to within FP32 rounding
+    rng = np.random.default_rng(5)
+
+    def f32(fn):
+        def call(epi, sp):
+            f = abi.cleared_force(len(epi))
+            fn(epi, sp, f)
+            return f["acc"].astype(np.float64), f["phi"].astype(np.float64)
+        return call
+    e32 = epsp_errors(f32(oracle_lib.epsp_mono_f32), f32(oracle_lib.epsp_quad_f32), rng, 1.0, dists[:2])
+    np.testing.assert_allclose(e32, e[:2], rtol=0.05)
+
+
 def test_tree_force_against_direct_sum(oracle_lib):
-    """Tree soft force (theta 0.5, quadrupole) + FP64 correction vs the direct sum of the
-    changeover-split force  sum_j m_j K(r)/r^3 dr  (K=1 beyond r_out)."""
+    """Tree soft force (theta 0.5) + FP64 correction vs the direct sum of the
+    changeover-split force  sum_j m_j K(r)/r^3 dr  (K=1 beyond r_out): 2.0e-4 rms with the
+    quadrupole, 3.9e-3 with the monopole only, so the bar of 4e-4 needs the quadrupole."""
+    for quad, lo, hi in ((1, 0.0, 4e-4), (0, 2e-3, 8e-3)):
+        err = _tree_force_error(oracle_lib, quad)
+        assert lo < err < hi, (quad, err)
+
+
+def _tree_force_error(oracle_lib, quadrupole):
     p, pp = helpers.disk_with_radii(oracle_lib, 3000, seed=7, a_in=0.9, a_out=1.1, mass=1e-9,
                                     R_cut0=3.0 * 2 ** (1 / 3), R_search0=1.1, R_search1=6.0,
-                                    gamma=0.1, dt_tree=2.0 ** -6)
+                                    gamma=0.1, dt_tree=2.0 ** -6, quadrupole=quadrupole)
     q = pp.copy()
     tree = oracle_lib.tree_build(p, q)
     oracle_lib.soft_force(p, q, tree, use_f64=True)
@@ -153,8 +226,7 @@ def test_tree_force_against_direct_sum(oracle_lib):
     w = pp["mass"][None, :] * K / r ** 3
     np.fill_diagonal(w, 0.0)
     acc = (w[:, :, None] * d).sum(1)
-    err = helpers.rel_err_rms(q["acc"], acc)
-    assert err < 2e-3, err
+    return helpers.rel_err_rms(q["acc"], acc)
 
 
 def _bruteforce_neighbours(oracle_lib, p, pp, initial):
