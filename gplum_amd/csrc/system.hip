@@ -12,7 +12,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -194,6 +196,8 @@ __global__ void k_hard_scatter(int m, const int32_t *__restrict__ idx, const int
 System::System(const gplum_params &prm) : p(prm) {}
 
 System::~System() {
+    if (log_rem) std::fclose(log_rem);
+    if (log_col) std::fclose(log_col);
     if (st) (void)hipStreamDestroy(st);
     if (st2) (void)hipStreamDestroy(st2);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -267,6 +271,7 @@ int System::set_particles(const gplum_particle *pp, int count, double time) {
     nb.n = 0;
     nb.n_nbr = 0;
     have_nbr = false;
+    istep = std::llround(time / p.dt_tree);
     return GPLUM_OK;
 }
 
@@ -673,75 +678,185 @@ static int excl_sum(DArray<unsigned char> &tmp, const T *in, T *out, int n, hipS
     return GPLUM_OK;
 }
 
-int System::merge_and_recompute() {
+// Flags after the step's force: sel = DEAD or MERGED (the twins and targets of this
+// step's mergers), rem = live and outside [r_min, r_max] (removeParticlesOutOfBoundary,
+// func.h:237-247: |x|^2 > r_max^2 or |x|^2 < r_min^2), keep = live and inside.  Merging
+// moves no live particle, so the bounds can be tested before it.  cnt[0] counts sel,
+// cnt[1] rem.
+__global__ void k_post_flags(int n, const double *__restrict__ pos, const int32_t *__restrict__ flags, double rmax2,
+                             double rmin2, int32_t *__restrict__ sel, int32_t *__restrict__ keep,
+                             int32_t *__restrict__ rem, int32_t *__restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int f = flags[i];
+    const double pos2 = pos[3 * i] * pos[3 * i] + pos[3 * i + 1] * pos[3 * i + 1] + pos[3 * i + 2] * pos[3 * i + 2];
+    const bool live = !(f & GPLUM_FLAG_DEAD);
+    const bool out = live && (pos2 > rmax2 || pos2 < rmin2);
+    const int s = (f & (GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED)) ? 1 : 0;
+    sel[i] = s;
+    rem[i] = out ? 1 : 0;
+    keep[i] = (live && !out) ? 1 : 0;
+    if (s) atomicAdd(&cnt[0], 1);
+    if (out) atomicAdd(&cnt[1], 1);
+}
+
+// the removed particles' fields for the energy bookkeeping and the remove log
+__global__ void k_gather_removed(int k, const int32_t *__restrict__ idx, PartPtrs P, double *__restrict__ out,
+                                 int64_t *__restrict__ ids) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= k) return;
+    const int i = idx[a];
+    double *o = out + 11 * (size_t)a;
+    o[0] = P.time[i];
+    o[1] = P.mass[i];
+    for (int c = 0; c < 3; c++) { o[2 + c] = P.pos[3 * i + c]; o[5 + c] = P.vel[3 * i + c]; }
+    o[8] = P.phi[i];
+    o[9] = P.phi_d[i];
+    o[10] = P.phi_s[i];
+    ids[a] = P.id[i];
+}
+
+// MergeParticle (func.h:160-207) and removeParticlesOutOfBoundary (func.h:209-380) on the
+// device, one compaction for both.  After the hard part a merged body is its target
+// (MERGED) plus dead twins carrying the target's id; only those few particles are listed
+// on the host to pair them up.  The removed particles (normally none, a handful at
+// most) are copied to the host, which books their energy as the reference's rank 0 does
+// (func.h:305-333, taken in ascending id) and writes the remove log (func.h:347-352).
+int System::merge_and_remove(int *n_removed) {
+    *n_removed = 0;
     MergeScratch &M = mrg;
     GPLUM_ENSURE(M.flag, n + 1);
     GPLUM_ENSURE(M.scan, n + 1);
     GPLUM_ENSURE(M.keep, n + 1);
     GPLUM_ENSURE(M.kscan, n + 1);
-    hipLaunchKernelGGL(k_merge_flag, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.flags.p, M.flag.p, M.keep.p);
-    int rc = excl_sum(rad.tmp, M.flag.p, M.scan.p, n, st);
-    if (rc) return rc;
-    if ((rc = excl_sum(rad.tmp, M.keep.p, M.kscan.p, n, st))) return rc;
-    int32_t tail[4] = {0, 0, 0, 0};
-    GPLUM_HIP(hipMemcpyAsync(&tail[0], M.flag.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipMemcpyAsync(&tail[1], M.scan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipMemcpyAsync(&tail[2], M.keep.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipMemcpyAsync(&tail[3], M.kscan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_ENSURE(M.rem, n + 1);
+    GPLUM_ENSURE(M.rscan, n + 1);
+    GPLUM_ENSURE(M.count, 2);
+    GPLUM_HIP(hipMemsetAsync(M.count.p, 0, 2 * sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_post_flags, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.flags.p,
+                       r_max * r_max, r_min * r_min, M.flag.p, M.keep.p, M.rem.p, M.count.p);
+    int32_t cnt[2] = {0, 0};
+    GPLUM_HIP(hipMemcpyAsync(cnt, M.count.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
-    const int k = tail[0] + tail[1], m = tail[2] + tail[3];
-    if (k == 0) return GPLUM_OK;
-    GPLUM_ENSURE(M.sel, k);
-    hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.flag.p, M.scan.p, M.sel.p);
-    std::vector<int32_t> sel(k), fl(k);
-    std::vector<int64_t> ids(k);
-    GPLUM_HIP(hipMemcpyAsync(sel.data(), M.sel.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    for (int a = 0; a < k; a++) {
-        GPLUM_HIP(hipMemcpyAsync(&fl[a], A.flags.p + sel[a], sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipMemcpyAsync(&ids[a], A.id.p + sel[a], sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    }
-    GPLUM_HIP(hipStreamSynchronize(st));
-    // pair twins with their target (same id), both in ascending index
-    std::vector<int32_t> tgt, off{0}, twin;
-    for (int a = 0; a < k; a++) {
-        if ((fl[a] & GPLUM_FLAG_DEAD) || !(fl[a] & GPLUM_FLAG_MERGED)) continue;
-        tgt.push_back(sel[a]);
-        for (int b = 0; b < k; b++)
-            if ((fl[b] & GPLUM_FLAG_DEAD) && ids[b] == ids[a]) twin.push_back(sel[b]);
-        off.push_back((int32_t)twin.size());
-    }
-    int n_dead = 0;
-    for (int a = 0; a < k; a++) n_dead += (fl[a] & GPLUM_FLAG_DEAD) ? 1 : 0;
-    if (n_dead != (int)twin.size()) return fail(GPLUM_E_STATE, "a merged particle has no live target");
-    const int ntg = (int)tgt.size();
-    if (ntg > 0) {
-        GPLUM_ENSURE(M.tgt, ntg);
-        GPLUM_ENSURE(M.twin_off, ntg + 1);
-        GPLUM_ENSURE(M.twin, twin.size() + 1);
-        GPLUM_ENSURE(M.ed, ntg);
-        GPLUM_HIP(hipMemcpyAsync(M.tgt.p, tgt.data(), sizeof(int32_t) * ntg, hipMemcpyHostToDevice, st));
-        GPLUM_HIP(hipMemcpyAsync(M.twin_off.p, off.data(), sizeof(int32_t) * (ntg + 1), hipMemcpyHostToDevice, st));
-        if (!twin.empty())
-            GPLUM_HIP(hipMemcpyAsync(M.twin.p, twin.data(), sizeof(int32_t) * twin.size(), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_merge_apply, dim3(ceil_div(ntg, 64)), dim3(64), 0, st, ntg, M.tgt.p, M.twin_off.p,
-                           M.twin.p, A.mass.p, A.vel.p, A.phi.p, A.phi_d.p, A.flags.p, M.ed.p);
-        std::vector<double> ed(ntg);
-        GPLUM_HIP(hipMemcpyAsync(ed.data(), M.ed.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+    const int k = cnt[0], n_rem = cnt[1];
+    if (k == 0 && n_rem == 0) return GPLUM_OK;
+    int rc;
+    if (k > 0) {
+        if ((rc = excl_sum(rad.tmp, M.flag.p, M.scan.p, n, st))) return rc;
+        GPLUM_ENSURE(M.sel, k);
+        hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.flag.p, M.scan.p, M.sel.p);
+        std::vector<int32_t> sel(k), fl(k);
+        std::vector<int64_t> ids(k);
+        GPLUM_HIP(hipMemcpyAsync(sel.data(), M.sel.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
-        for (int a = 0; a < ntg; a++) edisp += ed[a];
+        for (int a = 0; a < k; a++) {
+            GPLUM_HIP(hipMemcpyAsync(&fl[a], A.flags.p + sel[a], sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            GPLUM_HIP(hipMemcpyAsync(&ids[a], A.id.p + sel[a], sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        }
+        GPLUM_HIP(hipStreamSynchronize(st));
+        // pair twins with their target (same id), both in ascending index
+        std::vector<int32_t> tgt, off{0}, twin;
+        for (int a = 0; a < k; a++) {
+            if ((fl[a] & GPLUM_FLAG_DEAD) || !(fl[a] & GPLUM_FLAG_MERGED)) continue;
+            tgt.push_back(sel[a]);
+            for (int b = 0; b < k; b++)
+                if ((fl[b] & GPLUM_FLAG_DEAD) && ids[b] == ids[a]) twin.push_back(sel[b]);
+            off.push_back((int32_t)twin.size());
+        }
+        int n_dead = 0;
+        for (int a = 0; a < k; a++) n_dead += (fl[a] & GPLUM_FLAG_DEAD) ? 1 : 0;
+        if (n_dead != (int)twin.size()) return fail(GPLUM_E_STATE, "a merged particle has no live target");
+        const int ntg = (int)tgt.size();
+        if (ntg > 0) {
+            GPLUM_ENSURE(M.tgt, ntg);
+            GPLUM_ENSURE(M.twin_off, ntg + 1);
+            GPLUM_ENSURE(M.twin, twin.size() + 1);
+            GPLUM_ENSURE(M.ed, ntg);
+            GPLUM_HIP(hipMemcpyAsync(M.tgt.p, tgt.data(), sizeof(int32_t) * ntg, hipMemcpyHostToDevice, st));
+            GPLUM_HIP(hipMemcpyAsync(M.twin_off.p, off.data(), sizeof(int32_t) * (ntg + 1), hipMemcpyHostToDevice, st));
+            if (!twin.empty())
+                GPLUM_HIP(hipMemcpyAsync(M.twin.p, twin.data(), sizeof(int32_t) * twin.size(), hipMemcpyHostToDevice,
+                                         st));
+            hipLaunchKernelGGL(k_merge_apply, dim3(ceil_div(ntg, 64)), dim3(64), 0, st, ntg, M.tgt.p, M.twin_off.p,
+                               M.twin.p, A.mass.p, A.vel.p, A.phi.p, A.phi_d.p, A.flags.p, M.ed.p);
+            std::vector<double> ed(ntg);
+            GPLUM_HIP(hipMemcpyAsync(ed.data(), M.ed.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+            GPLUM_HIP(hipStreamSynchronize(st));
+            for (int a = 0; a < ntg; a++) edisp += ed[a];
+        }
     }
-    // removeParticle: keep the live particles in their order
+    if (n_rem > 0) {
+        // the removed particles after the merge (a target's mass, velocity and potentials
+        // are its merged ones), ascending index, then ascending id
+        if ((rc = excl_sum(rad.tmp, M.rem.p, M.rscan.p, n, st))) return rc;
+        GPLUM_ENSURE(M.ridx, n_rem);
+        GPLUM_ENSURE(M.rdata, 11 * (size_t)n_rem);
+        GPLUM_ENSURE(M.rid, n_rem);
+        hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.rem.p, M.rscan.p, M.ridx.p);
+        hipLaunchKernelGGL(k_gather_removed, dim3(ceil_div(n_rem, 64)), dim3(64), 0, st, n_rem, M.ridx.p, ptrs(A),
+                           M.rdata.p, M.rid.p);
+        std::vector<double> rd(11 * (size_t)n_rem);
+        std::vector<int64_t> rid(n_rem);
+        GPLUM_HIP(hipMemcpyAsync(rd.data(), M.rdata.p, sizeof(double) * rd.size(), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(rid.data(), M.rid.p, sizeof(int64_t) * n_rem, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        std::vector<Removed> R(n_rem);
+        for (int a = 0; a < n_rem; a++) {
+            const double *o = &rd[11 * (size_t)a];
+            Removed &q = R[a];
+            q.id = rid[a];
+            q.time = o[0];
+            q.mass = o[1];
+            for (int c = 0; c < 3; c++) { q.pos[c] = o[2 + c]; q.vel[c] = o[5 + c]; }
+            q.phi = o[8];
+            q.phi_d = o[9];
+            q.phi_s = o[10];
+        }
+        std::stable_sort(R.begin(), R.end(), [](const Removed &x, const Removed &y) { return x.id < y.id; });
+        double loc = 0.;
+        for (int a = 0; a < n_rem; a++) {
+            const Removed &q = R[a];
+            const double mi = q.mass, s = 0.5 * mi;
+            loc -= (s * q.vel[0]) * q.vel[0] + (s * q.vel[1]) * q.vel[1] + (s * q.vel[2]) * q.vel[2];
+            loc -= mi * q.phi_s;
+            loc -= mi * q.phi_d;
+            loc -= mi * q.phi;
+            for (int b = 0; b < a; b++) {
+                const Removed &r = R[b];
+                if (q.id == r.id) continue;
+                const double dr[3] = {q.pos[0] - r.pos[0], q.pos[1] - r.pos[1], q.pos[2] - r.pos[2]};
+                const double rinv = 1. / std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] + p.eps2);
+                loc += -mi * r.mass * rinv;
+            }
+            if (log_rem)
+                std::fprintf(log_rem, "%.8f\t%lld\t%.15e\t%.15e\t%.15e\t%.15e\t%.15e\t%.15e\t%.15e\n", q.time,
+                             (long long)q.id, q.mass, q.pos[0], q.pos[1], q.pos[2], q.vel[0], q.vel[1], q.vel[2]);
+        }
+        if (log_rem) std::fflush(log_rem);
+        edisp += loc;
+        n_remove_tot += n_rem;
+        *n_removed = n_rem;
+    }
+    // removeParticle: keep the live particles inside the boundary in their order
+    if ((rc = excl_sum(rad.tmp, M.keep.p, M.kscan.p, n, st))) return rc;
+    int32_t tail[2] = {0, 0};
+    GPLUM_HIP(hipMemcpyAsync(&tail[0], M.keep.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipMemcpyAsync(&tail[1], M.kscan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    const int m = tail[0] + tail[1];
     GPLUM_ENSURE(M.perm, m + 1);
     hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.keep.p, M.kscan.p, M.perm.p);
-    if ((rc = gather_all(M.perm.p, m))) return rc;
-    if ((rc = set_radii())) return rc;
-    // the recomputed force repeats the step's neighbour counts for all but the merged
-    // bodies: the forecaster plans with it but does not train on it again
-    skip_train = true;
-    rc = soft_force(true, plain_f64);
-    skip_train = false;
-    return rc;
+    return gather_all(M.perm.p, m);
+}
+
+// Energy::calcEnergy of the step (main_p3t.cpp:629, 692): the partial sums stay on the
+// device; gplum_system_energy reduces the state it is asked about.
+int System::energy_launch() {
+    GPLUM_ENSURE(eparts_step, 4 * 256);
+    hipLaunchKernelGGL(k_energy, dim3(256), dim3(256), 0, st, n, A.mass.p, A.vel.p, A.phi.p, A.phi_d.p, A.phi_s.p,
+                       eparts_step.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
 }
 
 // One timed run of the batched Hermite kernels on the current clusters (gathered from
@@ -833,6 +948,11 @@ int System::autotune(int trials, gplum_tuning *out) {
     return GPLUM_OK;
 }
 
+// One soft step in the order of main_p3t.cpp:423-699: kick, hard part, soft force +
+// correctForceLong, kick, energy, MergeParticle, removeParticlesOutOfBoundary and, after
+// a collision, a removal or every reset_step-th step, setCutoffRadii + soft force +
+// correctForceLongInitial + energy (the domain decomposition of that branch is a no-op on
+// one GPU).
 int System::step(bool persistent) {
     int rc;
     if ((rc = kick())) return rc;
@@ -841,10 +961,23 @@ int System::step(bool persistent) {
     edisp += step_edisp;
     if ((rc = soft_force(false, plain_f64))) return rc;
     if ((rc = kick())) return rc;
-    if (step_ncol) {
-        if ((rc = merge_and_recompute())) return rc;
+    if ((rc = energy_launch())) return rc;
+    int n_removed = 0;
+    if ((rc = merge_and_remove(&n_removed))) return rc;
+    const bool reset = reset_step > 0 && istep % reset_step == reset_step - 1;
+    if (step_ncol || n_removed || reset) {
+        if ((rc = set_radii())) return rc;
+        // the recomputed force repeats the step's neighbour counts for all but the merged
+        // or removed bodies: the forecaster plans with it but does not train on it again
+        skip_train = true;
+        rc = soft_force(true, plain_f64);
+        skip_train = false;
+        if (rc) return rc;
+        if ((rc = energy_launch())) return rc;
+        n_recompute++;
     }
     time_sys += p.dt_tree;
+    istep++;
     stats.steps++;
     return GPLUM_OK;
 }
@@ -957,7 +1090,37 @@ int gplum_system_stats(gplum_system *s, gplum_stats *out) {
     out->n_candidates = S->last_ncand;
     out->n_neighbors = S->nb.n_nbr;
     out->n_collisions = S->n_col_tot;
+    out->n_removed = S->n_remove_tot;
+    out->n_recomputes = S->n_recompute;
     out->time_sys = S->time_sys;
+    return GPLUM_OK;
+}
+
+int gplum_system_set_boundary(gplum_system *s, double r_max, double r_min, int reset_step) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    if (!(r_max >= r_min) || !(r_min >= 0.)) return gplum::fail(GPLUM_E_ARG, "need 0 <= r_min <= r_max");
+    if (reset_step < 0) return gplum::fail(GPLUM_E_ARG, "reset_step must be >= 0");
+    System *S = reinterpret_cast<System *>(s);
+    S->r_max = r_max;
+    S->r_min = r_min;
+    S->reset_step = reset_step;
+    return GPLUM_OK;
+}
+
+int gplum_system_set_event_logs(gplum_system *s, const char *collision_path, const char *remove_path) {
+    if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
+    System *S = reinterpret_cast<System *>(s);
+    FILE *fc = nullptr, *fr = nullptr;
+    if (collision_path && *collision_path && !(fc = std::fopen(collision_path, "a")))
+        return gplum::fail(GPLUM_E_ARG, std::string("cannot open ") + collision_path);
+    if (remove_path && *remove_path && !(fr = std::fopen(remove_path, "a"))) {
+        if (fc) std::fclose(fc);
+        return gplum::fail(GPLUM_E_ARG, std::string("cannot open ") + remove_path);
+    }
+    if (S->log_col) std::fclose(S->log_col);
+    if (S->log_rem) std::fclose(S->log_rem);
+    S->log_col = fc;
+    S->log_rem = fr;
     return GPLUM_OK;
 }
 

@@ -33,10 +33,18 @@ struct RadiiScratch {                // setCutoffRadii on the device (radii.hip)
 };
 int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScratch &R, hipStream_t st);
 
-struct MergeScratch {                 // MergeParticle on the device (system.hip)
-    DArray<int32_t> flag, scan, sel, keep, kscan, perm;
+struct MergeScratch {                 // MergeParticle + boundary removal on the device (system.hip)
+    DArray<int32_t> flag, scan, sel, keep, kscan, perm, rem, rscan, ridx, count;
     DArray<int32_t> tgt, twin_off, twin;
-    DArray<double> ed;
+    DArray<double> ed, rdata;
+    DArray<int64_t> rid;
+};
+
+// A particle that left the system through removeParticlesOutOfBoundary (func.h:209-380),
+// as the remove log and the energy bookkeeping need it.
+struct Removed {
+    int64_t id;
+    double time, mass, pos[3], vel[3], phi, phi_d, phi_s;
 };
 
 struct System {
@@ -71,8 +79,15 @@ struct System {
     ClusterList cl;
     Scratch scratch;
     HardBuffers hb;
-    DArray<double> time_c, eparts;
+    DArray<double> time_c, eparts, eparts_step;
     double e_init[5] = {0., 0., 0., 0., 0.};
+    // main loop (main_p3t.cpp:423-699): boundary of removeParticlesOutOfBoundary, the
+    // periodic recompute interval and the step counter (read.h:111-115 defaults)
+    double r_max = 40., r_min = 0.1;
+    int reset_step = 1024;
+    long long istep = 0;
+    int n_remove_tot = 0, n_recompute = 0;
+    FILE *log_rem = nullptr, *log_col = nullptr;   // remove / collision logs (append)
     long long last_epep_near = 0, last_epep_far = 0, last_epsp = 0, last_groups = 0;
     gplum_stats stats{};
 
@@ -89,7 +104,8 @@ struct System {
     int initialize();
     int kick();
     int hard_part(bool persistent);
-    int merge_and_recompute();
+    int merge_and_remove(int *n_removed);
+    int energy_launch();
     int step(bool persistent);
     int hard_trial(int wg_per_cu, int lds_max, double *ms);
     int autotune(int trials, gplum_tuning *out);

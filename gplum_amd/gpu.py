@@ -64,6 +64,8 @@ def lib():
         L.gplum_system_soft_force.argtypes = [vp, i32, i32]
         L.gplum_system_hard_part.argtypes = [vp, i32]
         L.gplum_system_step.argtypes = [vp, i32]
+        L.gplum_system_set_boundary.argtypes = [vp, f64, f64, i32]
+        L.gplum_system_set_event_logs.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
         L.gplum_system_energy.argtypes = [vp, vp]
         L.gplum_system_neighbors.argtypes = [vp, vp, vp, i32, ctypes.POINTER(i32)]
         L.gplum_system_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -148,6 +150,36 @@ def snapshot_write(path, header, pp):
     check(lib().gplum_snapshot_write(path.encode(), ctypes.byref(header), abi.ptr(pp), len(pp)))
 
 
+ENERGY_FIELDS = ("etot", "ekin", "ephi_sun", "ephi_planet", "ephi", "ephi_d", "edisp")
+
+
+class Energy(ctypes.Structure):
+    """gplum_energy (Energy, src/energy.h:3-11)."""
+    _fields_ = [(n, ctypes.c_double) for n in ENERGY_FIELDS]
+
+
+class FileHeader(ctypes.Structure):
+    """gplum_file_header (FileHeader, src/energy.h:70-79), the binary restart header."""
+    _fields_ = [("n_body", ctypes.c_int32), ("id_next", ctypes.c_int32), ("time", ctypes.c_double),
+                ("e_init", Energy), ("e_now", Energy)]
+
+
+def restart_write(path, header, pp):
+    """The binary restart file snap_tmp.dat (makeSnapTmp) of particles pp."""
+    pp = np.ascontiguousarray(pp, dtype=abi.PARTICLE)
+    check(lib().gplum_restart_write(path.encode(), ctypes.byref(header), abi.ptr(pp), len(pp)))
+
+
+def restart_read(path):
+    """(header, particles) of a binary restart file."""
+    h = FileHeader()
+    n = ctypes.c_int(0)
+    check(lib().gplum_restart_read(path.encode(), ctypes.byref(h), None, 0, ctypes.byref(n)))
+    pp = np.zeros(n.value, dtype=abi.PARTICLE)
+    check(lib().gplum_restart_read(path.encode(), ctypes.byref(h), abi.ptr(pp), len(pp), ctypes.byref(n)))
+    return h, pp
+
+
 class Stats(ctypes.Structure):
     """gplum_stats (include/gplum_system.h)."""
     _fields_ = [(n, ctypes.c_int64) for n in (
@@ -155,7 +187,8 @@ class Stats(ctypes.Structure):
         "n_neighbors",
         "n_clusters", "n_cluster_particles", "max_cluster", "block_steps", "n_collisions",
         "cand_retries", "n_force_calls", "fc_calls", "fc_slots", "fc_ovf_groups",
-        "fc_ovf_candidates", "hard_pairs", "max_block_steps", "max_block_cluster", "n_force_f64_calls")] + [(n, ctypes.c_double) for n in (
+        "fc_ovf_candidates", "hard_pairs", "max_block_steps", "max_block_cluster", "n_force_f64_calls",
+        "n_removed", "n_recomputes")] + [(n, ctypes.c_double) for n in (
             "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms")]
 
     def as_dict(self):
@@ -228,6 +261,13 @@ class System:
     def step(self, nsteps=1):
         check(lib().gplum_system_step(self.h, nsteps))
 
+    def set_boundary(self, r_max=40.0, r_min=0.1, reset_step=1024):
+        check(lib().gplum_system_set_boundary(self.h, r_max, r_min, reset_step))
+
+    def set_event_logs(self, collision_path=None, remove_path=None):
+        enc = lambda x: x.encode() if x else None
+        check(lib().gplum_system_set_event_logs(self.h, enc(collision_path), enc(remove_path)))
+
     def energy(self):
         out = np.zeros(6)
         check(lib().gplum_system_energy(self.h, abi.ptr(out)))
@@ -254,6 +294,10 @@ class System:
 
     def set_variant(self, fp64_soft=False, per_cluster_hard=False):
         check(lib().gplum_system_set_variant(self.h, int(fp64_soft), int(per_cluster_hard)))
+
+    def restart_write(self, path, id_next=0):
+        """makeSnapTmp: the system's particles and energies as a binary restart file."""
+        check(lib().gplum_system_restart_write(self.h, path.encode(), int(id_next)))
 
     def snapshot(self, path, id_next=0):
         """makeSnap: the system's particles and energies as a reference-format snapshot."""

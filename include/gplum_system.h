@@ -55,6 +55,8 @@ typedef struct gplum_stats {
     int64_t max_block_steps;                  /* most block steps of one cluster (max over calls) */
     int64_t max_block_cluster;                /* members of that cluster                   */
     int64_t n_force_f64_calls;                /* soft forces run with the FP64 kernels     */
+    int64_t n_removed;                        /* particles removed out of the boundary     */
+    int64_t n_recomputes;                     /* steps ending with the Initial recompute   */
     double force_ms, tree_ms, finalize_ms, hard_ms;
     double time_sys;
     double fc_loss;                           /* last training loss (Poisson NLL)          */
@@ -94,8 +96,24 @@ int gplum_system_soft_force(gplum_system *s, int initial, int use_f64);
  * per cluster). */
 int gplum_system_hard_part(gplum_system *s, int persistent);
 
-/* nsteps soft steps of the loop at main_p3t.cpp:423-699. */
+/* nsteps soft steps of the loop at main_p3t.cpp:423-699: kick, hard part, tree force +
+ * correctForceLong, kick, calcEnergy, MergeParticle (func.h:160-207),
+ * removeParticlesOutOfBoundary (func.h:209-380) and, after a collision, a removal or
+ * every reset_step-th step (istep % reset_step == reset_step - 1, istep starting at
+ * round(time_sys / dt_tree)), setCutoffRadii + tree force + correctForceLongInitial. */
 int gplum_system_step(gplum_system *s, int nsteps);
+
+/* The main loop's run parameters param.r_max, param.r_min and param.reset_step
+ * (read.h:80-84, defaults 40, 0.1, 1024 at read.h:111-115): particles with |x| > r_max
+ * or |x| < r_min leave the system at the end of a step, their energy booked as
+ * dissipated (func.h:305-333); reset_step 0 disables the periodic recompute. */
+int gplum_system_set_boundary(gplum_system *s, double r_max, double r_min, int reset_step);
+
+/* The collision and remove logs (fout_col, fout_rem, main_p3t.cpp:379-395): files the
+ * system appends a line to per merger (Collision0::write2File, collisionA.h:122-148) and
+ * per removed particle (func.h:347-352), in the reference's formats.  NULL or "" closes
+ * a log. */
+int gplum_system_set_event_logs(gplum_system *s, const char *collision_path, const char *remove_path);
 
 /* Energy::calcEnergy (energy.h:29-63): out = {etot, ekin, ephi_sun, ephi, ephi_d, edisp}. */
 int gplum_system_energy(gplum_system *s, double out[6]);
@@ -189,6 +207,47 @@ int gplum_snapshot_read(const char *path, gplum_snapshot_header *h, gplum_partic
 int gplum_snapshot_write(const char *path, const gplum_snapshot_header *h, const gplum_particle *pp, int n);
 int gplum_system_snapshot(gplum_system *s, const char *path, int32_t id_next);
 int gplum_system_energy_line(gplum_system *s, char *buf, int cap);
+
+/* The binary restart file snap_tmp.dat (makeSnapTmp, src/func.h:68-80; read back at
+ * main_p3t.cpp:208, 273-290): FDPS writeParticleBinary(filename, header) stores the
+ * header's FileHeader::writeBinary (an fwrite of the object, src/energy.h:119-126) and
+ * then every particle's FPGrav::writeBinary (an fwrite of the object, src/particle.h:
+ * 868-876), in array order.  The records below are those objects' bytes in the
+ * reference's default build (src/Makefile:29-36: USE_INDIVIDUAL_CUTOFF, USE_QUAD,
+ * COLLISION; without polar coordinates, the 6th-order Sun, binary merging and a constant
+ * random velocity), on x86-64 with natural alignment. */
+typedef struct gplum_energy {          /* Energy, src/energy.h:3-11 (56 B) */
+    double etot, ekin, ephi_sun, ephi_planet, ephi, ephi_d, edisp;
+} gplum_energy;
+
+typedef struct gplum_file_header {     /* FileHeader, src/energy.h:70-79 (128 B) */
+    int32_t n_body, id_next;
+    double time;
+    gplum_energy e_init, e_now;
+} gplum_file_header;
+
+typedef struct gplum_fpgrav {          /* FPGrav, src/particle.h:93-170, 404-495 (344 B) */
+    int32_t id_local, myrank;          /* EPIGrav                                        */
+    double pos[3], r_out, r_search;
+    int64_t id;                        /* EPJGrav                                        */
+    double mass, vel[3], acc_d[3];
+    double acc[3], acc_s[3], jerk_d[3], jerk_s[3], acc_gd[3];   /* FPGrav               */
+    double phi, phi_d, phi_s, v_disp, r_out_inv, time, dt, acc0, r_planet, f;
+    gplum_neighbor_info neighbor;
+    int64_t id_cluster;
+    int32_t n_cluster;
+    uint8_t inDomain, isSent, isDead, isMerged;
+} gplum_fpgrav;
+
+/* gplum_restart_write converts pp to FPGrav records (id_local = array index, myrank 0,
+ * acc_gd 0, inDomain 1, isSent 0, isDead/isMerged from flags); gplum_restart_read does
+ * the reverse (pp == NULL: header only, *n = n_body).  A run restarts as the reference
+ * does (main_p3t.cpp:273-290): time_sys = round(time / dt_tree) dt_tree, then
+ * gplum_system_set_particles + gplum_system_initialize.  gplum_system_restart_write
+ * writes the system's particles with its initial and current energies (makeSnapTmp). */
+int gplum_restart_write(const char *path, const gplum_file_header *h, const gplum_particle *pp, int n);
+int gplum_restart_read(const char *path, gplum_file_header *h, gplum_particle *pp, int cap, int *n);
+int gplum_system_restart_write(gplum_system *s, const char *path, int32_t id_next);
 
 /* The persistent form of gplum_hard_integrate captured into a hipGraph and replayed
  * `replays` times (no counterpart in the reference; same clusters, results and

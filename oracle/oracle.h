@@ -167,14 +167,42 @@ int64_t orc_soft_force(const gplum_params *p, gplum_particle *pp, int n,
                        const orc_tree *t, int use_f64, int64_t *n_epsp);
 
 /* ------------------------------------------------------------- full P3T step */
+/* Main-loop state of one rank (main_p3t.cpp:241-250, read.h:111-115). */
+typedef struct orc_loop {
+    double r_max, r_min;        /* removeParticlesOutOfBoundary bounds (40, 0.1)           */
+    int32_t reset_step;         /* periodic recompute interval (1024)                      */
+    int32_t istep;              /* step counter: in before the step, incremented by it     */
+    int32_t n_remove;           /* out: particles removed by the step                      */
+    int32_t recomputed;         /* out: 1 if the step ended with the Initial recompute     */
+    double e_now[5];            /* out: calcEnergy at the end of the step (orc_energy)     */
+    gplum_particle *removed;    /* out (optional): removed particles, ascending id         */
+    int32_t removed_cap;
+} orc_loop;
+/* Defaults of read.h:111-115 and istep = round(time_sys / dt_tree). */
+void orc_loop_default(const gplum_params *p, double time_sys, orc_loop *lp);
+
 /* Initialisation of main_p3t.cpp:289-364: radii, two rounds of tree force +
  * correctForceLongInitial; nl receives the neighbour graph. */
 void orc_p3t_init(const gplum_params *p, gplum_particle *pp, int n, orc_nlist *nl,
                   double time_sys);
-/* One soft step of main_p3t.cpp:423-699 (no boundary removal). Returns the number
- * of collisions; n may shrink when merged twins are removed. */
+/* One soft step of main_p3t.cpp:423-699: kick, hard part, soft force, kick, energy,
+ * MergeParticle, removeParticlesOutOfBoundary, and the recompute (setCutoffRadii + tree
+ * force + correctForceLongInitial + energy) after a collision, a removal or every
+ * reset_step-th step.  lp may be NULL (defaults).  Returns the number of collisions; n
+ * shrinks when merged twins or out-of-bounds particles leave. */
 int orc_p3t_step(const gplum_params *p, gplum_particle *pp, int *n, orc_nlist *nl,
-                 double *edisp, double time_sys);
+                 double *edisp, double time_sys, orc_loop *lp);
+
+/* ------------------------------------------------------ binary restart file */
+/* makeSnapTmp / readParticleBinary (func.h:68-80; energy.h:119-138; particle.h:860-876):
+ * FileHeader then one FPGrav per particle, each an fwrite of the reference's object,
+ * laid out by the compiler from declarations that follow the reference's classes. */
+int orc_fpgrav_size(void);
+int orc_file_header_size(void);
+int orc_restart_write(const char *path, int32_t id_next, double time, const double e_init[7],
+                      const double e_now[7], const gplum_particle *pp, int n);
+int orc_restart_read(const char *path, int32_t *n_body, int32_t *id_next, double *time, double e_init[7],
+                     double e_now[7], gplum_particle *pp, int cap);
 
 #ifdef __cplusplus
 }

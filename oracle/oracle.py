@@ -77,8 +77,13 @@ def lib():
         L.orc_soft_force.restype = ctypes.c_int64
         L.orc_nlist_free.argtypes = [vp]
         L.orc_p3t_init.argtypes = [P, vp, i32, vp, f64]
-        L.orc_p3t_step.argtypes = [P, vp, vp, vp, vp, f64]
+        L.orc_p3t_step.argtypes = [P, vp, vp, vp, vp, f64, vp]
         L.orc_p3t_step.restype = i32
+        L.orc_loop_default.argtypes = [P, f64, vp]
+        L.orc_restart_write.argtypes = [ctypes.c_char_p, ctypes.c_int32, f64, vp, vp, vp, i32]
+        L.orc_restart_write.restype = i32
+        L.orc_restart_read.argtypes = [ctypes.c_char_p, vp, vp, vp, vp, vp, vp, i32]
+        L.orc_restart_read.restype = i32
         _lib = L
     return _lib
 
@@ -264,23 +269,41 @@ def eccentricity(p, pp):
                                             None) for i in range(len(pp))])
 
 
+class Loop(ctypes.Structure):
+    """orc_loop: main-loop state (boundary, reset interval, step counter, outputs)."""
+    _fields_ = [("r_max", ctypes.c_double), ("r_min", ctypes.c_double), ("reset_step", ctypes.c_int32),
+                ("istep", ctypes.c_int32), ("n_remove", ctypes.c_int32), ("recomputed", ctypes.c_int32),
+                ("e_now", ctypes.c_double * 5), ("removed", ctypes.c_void_p), ("removed_cap", ctypes.c_int32)]
+
+
 class P3T:
     """Whole-system oracle run (initialisation + soft steps)."""
 
-    def __init__(self, p, pp, time_sys=0.0):
+    def __init__(self, p, pp, time_sys=0.0, r_max=None, r_min=None, reset_step=None):
         self.p = p
         self.pp = pp.copy()
         self.n = ctypes.c_int(len(pp))
         self.nl = NList()
         self.edisp = ctypes.c_double(0.0)
         self.time = time_sys
+        self.loop = Loop()
+        lib().orc_loop_default(ctypes.byref(p), time_sys, ctypes.byref(self.loop))
+        for k, v in (("r_max", r_max), ("r_min", r_min), ("reset_step", reset_step)):
+            if v is not None:
+                setattr(self.loop, k, v)
+        self.removed_buf = np.zeros(4096, dtype=abi.PARTICLE)
+        self.loop.removed = self.removed_buf.ctypes.data
+        self.loop.removed_cap = len(self.removed_buf)
+        self.removed = []          # per step: the removed particles (ascending id)
         lib().orc_p3t_init(ctypes.byref(p), abi.ptr(self.pp), len(self.pp), ctypes.byref(self.nl),
                            time_sys)
 
     def step(self):
         ncol = lib().orc_p3t_step(ctypes.byref(self.p), abi.ptr(self.pp), ctypes.byref(self.n),
-                                  ctypes.byref(self.nl), ctypes.byref(self.edisp), self.time)
+                                  ctypes.byref(self.nl), ctypes.byref(self.edisp), self.time,
+                                  ctypes.byref(self.loop))
         self.time += self.p.dt_tree
+        self.removed.append(self.removed_buf[: min(self.loop.n_remove, len(self.removed_buf))].copy())
         if self.n.value != len(self.pp):
             self.pp = self.pp[: self.n.value].copy()
         return ncol

@@ -63,10 +63,86 @@ static int merge_particles(gplum_particle *pp, int n, double *edisp)
     return m;
 }
 
+static int cmp_removed(const void *a, const void *b)
+{
+    const gplum_particle *x = *(const gplum_particle *const *)a, *y = *(const gplum_particle *const *)b;
+    return (x->id < y->id) ? -1 : (x->id > y->id);
+}
+
+/* removeParticlesOutOfBoundary (func.h:209-380) on one rank: particles with
+ * |x|^2 > r_max^2 or |x|^2 < r_min^2 leave the system.  The energy they carry is booked
+ * as dissipated (func.h:305-333): per removed particle i, edisp -= 1/2 m v.v, m phi_s,
+ * m phi_d, m phi, and for every earlier removed j of another id edisp += -m_i m_j /
+ * sqrt(|x_i - x_j|^2 + eps2).  The removed particles are taken in ascending id (the
+ * reference's order is that of its OpenMP push_back into remove_list, one thread:
+ * array order).  They are copied to removed[0..cap) in that order for the remove log
+ * (func.h:347-352).  Returns the number removed; the survivors keep their order. */
+static int remove_out_of_boundary(const gplum_params *p, gplum_particle *pp, int *n_io, double r_max, double r_min,
+                                  double *edisp, gplum_particle *removed, int cap)
+{
+    const int n = *n_io;
+    const double rmax2 = r_max * r_max, rmin2 = r_min * r_min;
+    int k = 0;
+    for (int i = 0; i < n; i++) {
+        const double *x = pp[i].pos;
+        const double pos2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        if (pos2 > rmax2 || pos2 < rmin2) k++;
+    }
+    if (k == 0) return 0;
+    const gplum_particle **rl = (const gplum_particle **)malloc(sizeof(*rl) * (size_t)k);
+    gplum_particle *copy = (gplum_particle *)malloc(sizeof(gplum_particle) * (size_t)k);
+    int a = 0, m = 0;
+    for (int i = 0; i < n; i++) {
+        const double *x = pp[i].pos;
+        const double pos2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        if (pos2 > rmax2 || pos2 < rmin2) copy[a++] = pp[i];
+    }
+    for (a = 0; a < k; a++) rl[a] = &copy[a];
+    qsort(rl, (size_t)k, sizeof(*rl), cmp_removed);
+    double loc = 0.;
+    for (a = 0; a < k; a++) {
+        const gplum_particle *q = rl[a];
+        const double mi = q->mass, s = 0.5 * mi;
+        loc -= (s * q->vel[0]) * q->vel[0] + (s * q->vel[1]) * q->vel[1] + (s * q->vel[2]) * q->vel[2];
+        loc -= mi * q->phi_s;
+        loc -= mi * q->phi_d;
+        loc -= mi * q->phi;
+        for (int b = 0; b < a; b++) {
+            const gplum_particle *r = rl[b];
+            if (q->id == r->id) continue;
+            const double dr[3] = {q->pos[0] - r->pos[0], q->pos[1] - r->pos[1], q->pos[2] - r->pos[2]};
+            const double rinv = 1. / sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] + p->eps2);
+            loc += -mi * r->mass * rinv;
+        }
+        if (removed && a < cap) removed[a] = *q;
+    }
+    for (int i = 0; i < n; i++) {
+        const double *x = pp[i].pos;
+        const double pos2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        if (!(pos2 > rmax2 || pos2 < rmin2)) pp[m++] = pp[i];
+    }
+    *edisp += loc;
+    *n_io = m;
+    free(rl);
+    free(copy);
+    return k;
+}
+
+void orc_loop_default(const gplum_params *p, double time_sys, orc_loop *lp)
+{
+    memset(lp, 0, sizeof(*lp));
+    lp->r_max = 40.;
+    lp->r_min = 0.1;
+    lp->reset_step = 1024;
+    lp->istep = (int32_t)llround(time_sys / p->dt_tree);
+}
+
 /* One soft step, main_p3t.cpp:423-699. */
 int orc_p3t_step(const gplum_params *p, gplum_particle *pp, int *n_io, orc_nlist *nl,
-                 double *edisp, double time_sys)
+                 double *edisp, double time_sys, orc_loop *lp)
 {
+    orc_loop dflt;
+    if (!lp) { orc_loop_default(p, time_sys, &dflt); lp = &dflt; }
     int n = *n_io;
     const double hdt = 0.5 * p->dt_tree;
     /* 1st kick (gravity_soft.h:5-12, particle.h:878-884) */
@@ -104,11 +180,21 @@ int orc_p3t_step(const gplum_params *p, gplum_particle *pp, int *n_io, orc_nlist
     /* 2nd kick */
     for (int i = 0; i < n; i++)
         for (int k = 0; k < 3; k++) pp[i].vel[k] += hdt * pp[i].acc[k];
-    if (n_col) {
-        n = merge_particles(pp, n, edisp);
+    double e[5];
+    orc_energy(pp, n, e);
+    if (n_col) n = merge_particles(pp, n, edisp);
+    const int n_remove = remove_out_of_boundary(p, pp, &n, lp->r_max, lp->r_min, edisp, lp->removed,
+                                                lp->removed_cap);
+    const int reset = lp->reset_step > 0 && lp->istep % lp->reset_step == lp->reset_step - 1;
+    if (n_col || n_remove || reset) {
         orc_set_cutoff_radii(p, pp, n);
         soft_and_correct(p, pp, n, 1, nl);
+        orc_energy(pp, n, e);
     }
+    memcpy(lp->e_now, e, sizeof(e));
+    lp->n_remove = n_remove;
+    lp->recomputed = (n_col || n_remove || reset) ? 1 : 0;
+    lp->istep++;
     *n_io = n;
     return n_col;
 }
