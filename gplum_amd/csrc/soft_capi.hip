@@ -5,6 +5,8 @@
 // epi[0], as the PIKG kernel is) and downloads the accumulated Force array.  Calls
 // from different host threads use different streams and buffers (FDPS calls the
 // functors from many OpenMP threads, src/gravity_kernel.hpp:8-23).
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "soft_kernels.hpp"
@@ -125,6 +127,56 @@ int run_call(const gplum_epi *epi, int ni, const void *jp, size_t jsize, int nj,
     return GPLUM_OK;
 }
 
+// ------------------------------------------------ no-allocation EP-EP variant
+//
+// The i-chunk table of the kernel (one SoftGroup per SOFT_BLOCK i-particles) is the
+// only device memory the call needs beyond its inputs and outputs.  Here it lives in a
+// fixed file-scope device array: each device holds its own copy until the process
+// ends, every call on that device reuses it (calls that overlap on one device race for
+// it), and the call allocates nothing.  Host code never passes the array's name to a
+// kernel (in host code it is a stand-in, not a device address): the address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+constexpr int NOALLOC_MAX_GROUPS = 4096;              // ni <= 4096 * 256 = 1,048,576
+static __device__ __align__(256) SoftGroup scratch[NOALLOC_MAX_GROUPS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+SoftGroup *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_of_current_device(SoftGroup **out) {
+    int dev = 0;
+    GPLUM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(GPLUM_E_NODEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        GPLUM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GPLUM_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = static_cast<SoftGroup *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return GPLUM_OK;
+}
+
+// the i-chunk table of make_groups, written on the device (every entry is rewritten by
+// each call, so nothing of an earlier call is read)
+__global__ void k_noalloc_groups(SoftGroup *g, int ng, int ni, int nj) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ng) return;
+    SoftGroup s{};
+    s.i_begin = k * SOFT_BLOCK;
+    s.ni = min(SOFT_BLOCK, ni - s.i_begin);
+    s.j_begin = 0;
+    s.nj = nj;
+    s.jf_begin = 0;
+    s.njf = 0;
+    s.s_begin = 0;
+    s.ns = 0;
+    s.i_ref = 0;
+    s.pad = 0;
+    g[k] = s;
+}
+
 }  // namespace
 }  // namespace gplum
 
@@ -160,6 +212,27 @@ int gplum_calc_force_epsp_quad_f64(const gplum_epi *epi, int ni, const gplum_spj
                                    gplum_force *force, float eps2, double *acc64, double *phi64) {
     return run_call<gplum_spj_quad, SrcSPQ, SPQ, true>(epi, ni, spj, sizeof(gplum_spj_quad), nj,
                                                        force, eps2, acc64, phi64);
+}
+
+int gplum_calc_force_epep_noalloc(const gplum_epi *d_epi, int ni, const gplum_epj *d_epj, int nj,
+                                  gplum_force *d_force, float eps2, void *stream) {
+    if (ni < 0 || nj < 0 || (ni > 0 && (!d_epi || !d_force)) || (nj > 0 && !d_epj))
+        return fail(GPLUM_E_ARG, "null pointer or negative size");
+    if (ni > NOALLOC_MAX_GROUPS * SOFT_BLOCK)
+        return fail(GPLUM_E_ARG, "ni = " + std::to_string(ni) + " exceeds the no-allocation limit of " +
+                                     std::to_string(NOALLOC_MAX_GROUPS * SOFT_BLOCK) + " i-particles");
+    if (ni == 0) return GPLUM_OK;
+    SoftGroup *groups = nullptr;
+    int rc = scratch_of_current_device(&groups);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int ng = ceil_div(ni, SOFT_BLOCK);
+    hipLaunchKernelGGL(k_noalloc_groups, dim3(ceil_div(ng, 256)), dim3(256), 0, st, groups, ng, ni, nj);
+    launch_soft_f32<SrcEPI, SrcEPJ, SrcNone, SinkForce, EP>(groups, ng, SrcEPI{d_epi}, SrcEPJ{d_epj},
+                                                             SrcEPJ{d_epj}, SrcNone{}, SinkForce{d_force}, eps2,
+                                                             CandBuf{nullptr, nullptr, 0u}, st);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
 }
 
 }  // extern "C"

@@ -51,6 +51,7 @@ def lib():
                                         ctypes.POINTER(f64)]
         for n in ("gplum_calc_force_epep", "gplum_calc_force_epsp_quad", "gplum_calc_force_epsp_mono"):
             getattr(L, n).argtypes = [vp, i32, vp, i32, vp, f32]
+        L.gplum_calc_force_epep_noalloc.argtypes = [vp, i32, vp, i32, vp, f32, vp]
         for n in ("gplum_calc_force_epep_f64", "gplum_calc_force_epsp_quad_f64"):
             getattr(L, n).argtypes = [vp, i32, vp, i32, vp, f32, vp, vp]
         L.gplum_system_create.argtypes = [P, ctypes.POINTER(vp)]

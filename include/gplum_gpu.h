@@ -144,6 +144,18 @@ int gplum_calc_force_epsp_quad_f64(const gplum_epi *epi, int ni,
                                    gplum_force *force, float eps2,
                                    double *acc64, double *phi64);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): gplum_calc_force_epep on device-resident arrays (d_epi[ni], d_epj[nj],
+ * d_force[ni], accumulated as there), launched asynchronously on `stream` (a
+ * hipStream_t; NULL = the default stream).  It allocates nothing: its i-chunk table
+ * lives in a fixed per-device array of 4096 entries, so ni <= 1,048,576 (larger ni
+ * returns GPLUM_E_ARG).  Every call on a device shares that array: calls that overlap
+ * on one device (different streams or host threads) race. */
+int gplum_calc_force_epep_noalloc(const gplum_epi *d_epi, int ni,
+                                  const gplum_epj *d_epj, int nj,
+                                  gplum_force *d_force, float eps2, void *stream);
+
 /* ------------------------------------------------- FDPS MultiWalk interface */
 
 /* eps2 used by the MultiWalk kernels (the PIKG functors take it as constructor
