@@ -235,10 +235,12 @@ extern "C" int gplum_hard_integrate(const gplum_params *p, gplum_particle *pp, i
     if ((rc = hard_upload(c, p, pp, n, offsets, n_cluster, (mode & 2) != 0, pl))) return rc;
     GPLUM_HIP(hipEventRecord(c.e0, c.st));
     if (mode & 1) {
-        // mode bit 2: every cluster above HARD_SEQ_MAX particles on the whole-GPU path
+        // mode bit 2: every cluster above HARD_SEQ_MAX particles on the whole-GPU path;
+        // mode bit 3: clusters of 2 to 4 bodies on the one-lane-per-particle path too
         const int giant_min = (mode & 4) ? HARD_SEQ_MAX : GIANT_MIN_DEFAULT;
+        const int terms_max = (mode & 8) ? 0 : TERMS_MAX;
         if ((rc = hard_launch(*p, c.hb, n_cluster, c.off.p, c.order.p, true, c.n_cu, c.st, c.st2, c.fork, c.join, 8,
-                              HERMITE_LDS, giant_min)))
+                              HERMITE_LDS, giant_min, terms_max)))
             return rc;
     } else {
         for (int k = 0; k < n_cluster; k++)

@@ -30,6 +30,7 @@ struct HP {                                  // physics constants for the kernel
     double m_sun, eps2, eps2_sun, dt_tree, dt_min, eta, eta_0, eta_sun, eta_sun0, alpha2;
     Cutoff cut;
     int collision;
+    int terms_max;     // clusters of 2 .. terms_max (<= 4) bodies run integrate_terms
 };
 
 __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
@@ -1023,6 +1024,318 @@ __device__ inline void wave_min2_lead(double &u, double &v, int n, int m) {
     }
 }
 
+// ------------------------------------------- clusters of 2 to 4 bodies: lane per term
+//
+// Lane L = 16 T + 4 p + c works on particle p (the quads of a row), term T (the rows:
+// T = 0 the Sun, T = k + 1 the pair with p's k-th partner in ascending index,
+// j = k + (k >= p)) and component c (0, 1, 2 = x, y, z; c = 3 pads the quad and keeps
+// zeros).  The Sun term and every pair term of every particle are evaluated by the same
+// instructions, a vector operation acts on x, y and z at once, and a dot product is the
+// quad's sum of its lanes' products in two DPP steps.  Every lane of particle p keeps
+// p's state for its component (the same in all rows), so predictor and corrector need
+// no exchange; after each evaluation the rows are gathered through the LDS (the Sun
+// row, then the partner rows in ascending j added from zero: the sums of
+// star_acc_jerk and gravity_loop_lane), and the Sun and planet time-step criteria
+// likewise.  Same operations in the same order as the one-lane-per-particle path.
+
+constexpr int DPP_ROW_ROR4 = 0x124;
+
+// (s0 + s1) + s2 on every lane of a quad of lane values s0..s3, which is dot3's sum,
+// when the padding value s3 is +0 and s2 is not -0 (squares), or s3 is -0 (any s2):
+// s0 + s1 = s1 + s0 and the last addition commute, and s2 + s3 = s2.  All four lanes of
+// the quad must be active.
+__device__ inline double quad_sum(double s) {
+    s = s + dpp_f64<DPP_QUAD_1032>(s);
+    return s + dpp_f64<DPP_QUAD_2301>(s);
+}
+__device__ inline double quad_sq(double a) { return quad_sum(a * a); }
+__device__ inline double quad_dot(double a, double b, bool pad) { return quad_sum(pad ? -0. : a * b); }
+
+// The lane's term of a hard force evaluation, in one instruction stream for the Sun row
+// (star_acc_jerk: xj = vj = -0, so that xj - x = -x exactly; mT = m_sun, eps2T =
+// eps2_sun) and the partner rows (pair_term with r_out_inv = min of the two 1/r_out):
+// the Sun's (1/s, then sqrt) and the pair's (sqrt, then 1/r) sequences are both
+// evaluated and the row's own one selected; the Sun row takes K = 1 and dK/dt = -0,
+// exact identities (1 K = K, alpha 1 = alpha, 3 alpha + (-0) = 3 alpha) that leave
+// star_acc_jerk's acc = mj dr and jerk = mj (dv - 3 alpha dr).
+__device__ inline void term_eval(const Cutoff &cut, bool sun, bool pad, double xi, double vi, double xj, double vj,
+                                 double mT, double eps2T, double r_out_inv, double &ta, double &tj) {
+    const double dr = xj - xi;
+    const double dv = vj - vi;
+    const double s = quad_sq(dr) + eps2T;
+    const double a = 1. / s, b = sqrt(s);
+    const double y1 = sun ? a : b;                 // Sun: 1/r^2; pair: r
+    const double c = sqrt(y1), d = 1. / y1;
+    const double rinv = sun ? c : d;
+    const double r2inv = sun ? y1 : d * d;
+    const double r3inv = r2inv * rinv;
+    const double mj_r3 = mT * r3inv;
+    const double alpha = quad_dot(dr, dv, pad) * r2inv;
+    const double K = sun ? 1. : 1. - cutoff_K(cut, y1, r_out_inv);
+    const double dKdt = sun ? -0. : cutoff_dKdt(cut, y1, r_out_inv, alpha);
+    const double alpha_c = alpha * K;
+    const double c2 = 3. * alpha_c + dKdt;
+    ta = mj_r3 * K * dr;
+    tj = mj_r3 * (K * dv - c2 * dr);
+}
+
+// Each lane of particle p gets the Sun term and the sum of the partner terms of its
+// component (rows 1..m at quad position q = 4 p + c, added in ascending j from zero).
+__device__ inline void gather_terms(double *s_tm, int L, int q, int m, double ta, double tj, double &as,
+                                    double &js, double &ad, double &jd) {
+    s_tm[2 * L] = ta;
+    s_tm[2 * L + 1] = tj;
+    wave_sync();
+    as = s_tm[2 * q];
+    js = s_tm[2 * q + 1];
+    ad = 0.;
+    jd = 0.;
+    for (int k = 1; k <= m; k++) {
+        ad += s_tm[2 * (16 * k + q)];
+        jd += s_tm[2 * (16 * k + q) + 1];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// predict_regs and correct_regs for one component (same expressions).
+__device__ inline void predict_one(double Dt, double pos, double vel, double as, double js, double ad, double jd,
+                                   double &xp, double &vp) {
+    const double acc_sd = as + ad;
+    const double jerk_sd = js + jd;
+    xp = pos + Dt * (vel + Dt * (SECONDORDER * acc_sd + THIRDORDER * Dt * jerk_sd));
+    vp = vel + Dt * (acc_sd + Dt * SECONDORDER * jerk_sd);
+}
+__device__ inline void correct_one(double Dt, double Dt2, double Dt_inv2, double Dt_inv3, double xp, double vp,
+                                   double as, double a0s, double j0s, double js, double ad, double a0d, double j0d,
+                                   double jd, double &pos, double &vel, double &a2s, double &a3s, double &a2d,
+                                   double &a3d) {
+    const double Am_s = as - a0s;
+    const double J0_s = Dt * j0s;
+    const double J1_s = Dt * js;
+    const double A2_s = 3. * Am_s - (J1_s + 2. * J0_s);
+    const double A3_s = -2. * Am_s + (J1_s + J0_s);
+    a2s = 2. * Dt_inv2 * (A2_s + 3. * A3_s);
+    a3s = 6. * Dt_inv3 * A3_s;
+    const double Am_d = ad - a0d;
+    const double J0_d = j0d * Dt;
+    const double J1_d = jd * Dt;
+    const double A2_d = 3. * Am_d - (J1_d + 2. * J0_d);
+    const double A3_d = -2. * Am_d + (J1_d + J0_d);
+    a2d = 2. * (A2_d + 3. * A3_d) * Dt_inv2;
+    a3d = 6. * A3_d * Dt_inv3;
+    const double A2_sd = A2_s + A2_d;
+    const double A3_sd = A3_s + A3_d;
+    pos = xp + ONE_SIXTIETH * Dt2 * (5. * A2_sd + 3. * ALPHA_POS * A3_sd);
+    vel = vp + ONE_TWELFTH * Dt * (4. * A2_sd + 3. * A3_sd);
+}
+
+// calc_dt4_fast and calc_dt4 of the lane's term (the row's vectors, dots over the quad).
+__device__ inline double dt4_fast_quad(double eta, double alpha2, double acc0, double a, double j, double s,
+                                       double c) {
+    const double Acc2 = quad_sq(a) + alpha2 * acc0 * acc0;
+    const double Jerk2 = quad_sq(j);
+    const double Snap2 = quad_sq(s);
+    const double Crac2 = quad_sq(c);
+    const double as2 = Acc2 * Snap2, jc2 = Jerk2 * Crac2;
+    const bool ok = (as2 >= DBL_MIN || (Acc2 == 0. || Snap2 == 0.)) && (jc2 >= DBL_MIN || Crac2 == 0.);
+    const double num = __builtin_amdgcn_sqrt(as2) + Jerk2;
+    const double den = __builtin_amdgcn_sqrt(jc2) + Snap2;
+    const double q = ok ? eta * __builtin_amdgcn_sqrt(num * __builtin_amdgcn_rcp(den)) : __builtin_nan("");
+    return (Jerk2 > 0.) ? q : DBL_MAX;
+}
+__device__ inline double dt4_quad(double eta, double alpha2, double acc0, double a, double j, double s, double c) {
+    const double Acc = sqrt(quad_sq(a) + alpha2 * acc0 * acc0);
+    const double Jerk2 = quad_sq(j);
+    const double Jerk = sqrt(Jerk2);
+    const double Snap2 = quad_sq(s);
+    const double Snap = sqrt(Snap2);
+    const double Crac = sqrt(quad_sq(c));
+    return (Jerk > 0.) ? eta * sqrt((Acc * Snap + Jerk2) / (Jerk * Crac + Snap2)) : DBL_MAX;
+}
+
+// Physics constants of the lane-per-term path, in registers (the kernel keeps HP in LDS).
+struct TermConsts {
+    Cutoff cut;
+    double dt_tree, dt_min, alpha2;
+};
+
+// dt_block with the Sun criterion from the Sun row and the planet criterion from the
+// partner rows (each lane evaluates its own row's, reads the other from the LDS); the
+// exact criteria when either fast one is not finite or dt_1 lies near the step grid.
+__device__ inline double dt_block_rows(const TermConsts &C, bool sun, double *s_tm, int L, int q, double time,
+                                       double dt, double etaT, double acc0T, double a, double j, double s, double c) {
+    double dt_next = dmin(2. * dt, 0.5 * C.dt_tree);
+    double rem = fmod_block(time, dt_next);
+    for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(time, dt_next); }
+    const int other = 2 * (sun ? 16 + q : q);     // a partner row's lane, or the Sun row's
+    const double f = dt4_fast_quad(etaT, C.alpha2, acc0T, a, j, s, c);
+    s_tm[2 * L] = f;
+    wave_sync();
+    const double fo = s_tm[other];
+    const double f_planet = sun ? fo : f, f_sun = sun ? f : fo;
+    double dt_1 = dmin(f_planet, f_sun);
+    const bool exact = !(f_planet <= DBL_MAX && f_sun <= DBL_MAX) || !clear_of_grid(dt_1, dt_next);
+    if (__ballot(exact)) {
+        const double e = dt4_quad(etaT, C.alpha2, acc0T, a, j, s, c);
+        s_tm[2 * L + 1] = e;
+        wave_sync();
+        const double eo = s_tm[other + 1];
+        if (exact) dt_1 = dmin(sun ? eo : e, sun ? e : eo);
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int it = 0; dt_1 < dt_next && 0.5 * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
+    if (dt_next < 2. * C.dt_min) dt_next = C.dt_min;
+    return (rem != 0.0) ? C.dt_min : dt_next;
+}
+
+// Block steps of a cluster of 2 to 4 particles with distinct ids from the start of the
+// soft step (the layout above; one wave).  Returns false when a block step has a
+// collision: the state of the start of that block step is then in H (flags and twins
+// cleared) for integrate_from.  Else the final calcGravity with potentials is done and
+// written to H.  s_m, s_ri, s_id, s_fr hold the cluster's masses, 1/r_out, ids and
+// f r_planet; s_x, s_v and s_tm are free.
+__device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double *s_x, double *s_v, double *s_m,
+                                double *s_ri, int64_t *s_id, double *s_fr, double *s_tm, int b, int n,
+                                double &sys_time, long long &loop, long long &pairs) {
+    const int L = threadIdx.x;
+    const int T = L >> 4, p = (L >> 2) & 3, c = L & 3;
+    const int q = L & 15;                           // 4 p + c
+    const int m = n - 1;
+    const bool own = p < n;                         // lanes of a particle (any row)
+    const bool row = own && T < n;                  // lanes of a term of a particle
+    const bool sun = T == 0, pad = c == 3;
+    const int j = (T >= 1) ? (T - 1) + ((T - 1) >= p ? 1 : 0) : p;   // the row's partner
+    const int g = b + (own ? p : 0);
+    const TermConsts C{P.cut, P.dt_tree, P.dt_min, P.alpha2};
+    const double mT = sun ? P.m_sun : s_m[j < n ? j : 0];
+    const double eps2T = sun ? P.eps2_sun : P.eps2;
+    const double r_out_inv = dmin(s_ri[own ? p : 0], s_ri[j < n ? j : 0]);
+    const double etaT = sun ? P.eta_sun : P.eta;
+    const double r2c = s_fr[own ? p : 0] + s_fr[j < n ? j : 0];
+    const double col_thr = COL_FILTER * r2c * r2c;
+    const bool col_row = P.collision && row && !sun;
+    const int src = 4 * (j < n ? j : 0) + c;      // row 0 lane of the partner's component
+    const double time_end = P.dt_tree;
+    double x = 0., v = 0., time = 0., dt = 0., acc0 = 0.;
+    if (own && !pad) {
+        x = H.pos[3 * g + c];
+        v = H.vel[3 * g + c];
+    }
+    if (own) {
+        time = H.time[g];
+        dt = H.dt[g];
+        acc0 = H.acc0[g];
+    }
+    const double acc0T = sun ? 0. : acc0;
+    const bool lead = row && sun && c == 0;
+    double as = 0., js = 0., ad = 0., jd = 0., a2s = 0., a3s = 0., a2d = 0., a3d = 0.;
+    {
+        // initial calcGravity (hermite.h:259-270) at the current state
+        double xj = __shfl(x, src), vj = __shfl(v, src);
+        if (sun) { xj = -0.; vj = -0.; }
+        double ta = 0., tj = 0.;
+        if (row) term_eval(C.cut, sun, pad, x, v, xj, vj, mT, eps2T, r_out_inv, ta, tj);
+        if (row) gather_terms(s_tm, L, q, m, ta, tj, as, js, ad, jd);
+    }
+    double tn = own ? time + dt : 1.e10;
+    tn = dmin(tn, dpp_f64<DPP_ROW_ROR4>(tn));
+    tn = dmin(tn, dpp_f64<DPP_ROW_ROR8>(tn));
+    double t_s = readlane_f64(tn, 0);
+    bool handover = false;
+    while (sys_time < time_end && loop < BLOCK_STEP_CAP) {
+        const double Dt = t_s - time;
+        const double a0s = as, j0s = js, a0d = ad, j0d = jd;
+        double xp = 0., vp = 0.;
+        predict_one(Dt, x, v, as, js, ad, jd, xp, vp);
+        double xj = __shfl(xp, src), vj = __shfl(vp, src);
+        if (sun) { xj = -0.; vj = -0.; }
+        if (P.collision) {
+            // collisionDetermination: a pair with R = |dx| / (f_i R_i + f_j R_j) < 1
+            const double dx = xp - xj;
+            const double d2 = quad_sq(dx);
+            const bool near = col_row && d2 < col_thr;
+            if (__ballot(near)) {
+                const bool hit = near && sqrt(d2) / r2c < 1.;
+                if (__ballot(hit)) { handover = true; break; }
+            }
+        }
+        if (row && time + dt == t_s) {
+            if (lead) pairs += 3 * (long long)m;
+            const double Dt2 = Dt * Dt;
+            const double Dt_inv = recip_block(Dt);
+            const double Dt_inv2 = Dt_inv * Dt_inv;
+            const double Dt_inv3 = Dt_inv2 * Dt_inv;
+            double ta, tj;
+            term_eval(C.cut, sun, pad, xp, vp, xj, vj, mT, eps2T, r_out_inv, ta, tj);
+            gather_terms(s_tm, L, q, m, ta, tj, as, js, ad, jd);
+            for (int ite = 0; ite < 2; ite++) {
+                correct_one(Dt, Dt2, Dt_inv2, Dt_inv3, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s,
+                            a2d, a3d);
+                term_eval(C.cut, sun, pad, x, v, xj, vj, mT, eps2T, r_out_inv, ta, tj);
+                gather_terms(s_tm, L, q, m, ta, tj, as, js, ad, jd);
+            }
+            time += Dt;
+            dt = dt_block_rows(C, sun, s_tm, L, q, time, dt, etaT, acc0T, sun ? as : ad, sun ? js : jd,
+                               sun ? a2s : a2d, sun ? a3s : a3d);
+        }
+        double tm = own ? time : 1.e10;
+        tn = own ? time + dt : 1.e10;
+        tm = dmin(tm, dpp_f64<DPP_ROW_ROR4>(tm));
+        tn = dmin(tn, dpp_f64<DPP_ROW_ROR4>(tn));
+        tm = dmin(tm, dpp_f64<DPP_ROW_ROR8>(tm));
+        tn = dmin(tn, dpp_f64<DPP_ROW_ROR8>(tn));
+        sys_time = readlane_f64(tm, 0);
+        t_s = readlane_f64(tn, 0);
+        loop++;
+    }
+    if (handover) {
+        // the state at the start of this block step, for integrate_from
+        if (row && sun && !pad) {
+            H.pos[3 * g + c] = x; H.vel[3 * g + c] = v;
+            H.acc_s[3 * g + c] = as; H.jerk_s[3 * g + c] = js;
+            H.acc_d[3 * g + c] = ad; H.jerk_d[3 * g + c] = jd;
+        }
+        if (lead) {
+            H.time[g] = time;
+            H.dt[g] = dt;
+            H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+            H.twin_of[g] = -1;
+        }
+        return false;
+    }
+    // final calcGravity with potentials (hermite.h:466-470), one lane per particle
+    __syncthreads();
+    if (row && sun && !pad) {
+        s_x[3 * p + c] = x;
+        s_v[3 * p + c] = v;
+    }
+    if (lead) {
+        s_tm[2 * p] = time;
+        s_tm[2 * p + 1] = dt;
+    }
+    __syncthreads();
+    if (L < n) {
+        const int gi = b + L;
+        double xi[3], vi[3], fas[3], fjs[3], fad[3], fjd[3], phs = 0., phd = 0.;
+        ld3(s_x, L, xi);
+        ld3(s_v, L, vi);
+        star_acc_jerk(P, xi, vi, fas, fjs, &phs);
+        gravity_loop_lane<false>(P, J, n, L, s_id[L], xi, vi, s_ri[L], fad, fjd, &phd);
+        pairs += n - 1;
+        st3(H.pos, gi, xi); st3(H.vel, gi, vi);
+        st3(H.acc_s, gi, fas); st3(H.jerk_s, gi, fjs);
+        st3(H.acc_d, gi, fad); st3(H.jerk_d, gi, fjd);
+        H.time[gi] = s_tm[2 * L];
+        H.dt[gi] = s_tm[2 * L + 1];
+        H.phi_s[gi] = phs;
+        H.phi_d[gi] = phd;
+        H.flags[gi] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
+        H.twin_of[gi] = -1;
+    }
+    __syncthreads();
+    return true;
+}
+
 // Clusters that fit one wave: each particle's whole state stays in registers (one
 // lane per particle, or a group of lanes for 3 to 8 particles, see side_force) and
 // only the predicted j-data (and f r_planet) goes through LDS, so a block step costs a
@@ -1074,6 +1387,13 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     if (own)
         for (int j = 0; j < n; j++) rep = rep || (j != p && s_id[j] == id);
     const bool from_start = __ballot(rep) != 0;
+    // 2 to 4 bodies: the lane-per-term path (a collision hands over to integrate_from below)
+    const bool terms = !from_start && n <= P.terms_max;
+    bool terms_handover = false;
+    if (terms) {
+        if (integrate_terms(P, H, J, s_x, s_v, s_m, s_ri, s_id, s_fr, s_tm, b, n, sys_time, loop, pairs)) return;
+        terms_handover = true;
+    }
     // side_force: this lane's partner and its constants
     int jp = 0;
     double m_jp = 0., ri_jp = 0.;
@@ -1089,7 +1409,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
             H.flags[g] &= ~(GPLUM_FLAG_DEAD | GPLUM_FLAG_MERGED);
             H.twin_of[g] = -1;
         }
-    } else if (own) {
+    } else if (own && !terms) {
         // initial calcGravity (hermite.h:259-270)
         star_acc_jerk(P, x, v, as, js, nullptr);
         gravity_loop_lane<false>(P, J, n, p, id, x, v, rinv, ad, jd, nullptr);
@@ -1100,7 +1420,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     // the system time at the end of each step
     double t_s = wave_min(own ? time + dt : 1.e10);
     bool handover = false;
-    while (!from_start && sys_time < time_end && loop < BLOCK_STEP_CAP) {
+    while (!from_start && !terms && sys_time < time_end && loop < BLOCK_STEP_CAP) {
         __syncthreads();   // the previous step's j-data reads are done
         if (own) {
             predict_regs(t_s - time, x, v, as, js, ad, jd, a0s, j0s, a0d, j0d, xp, vp);
@@ -1169,7 +1489,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
         wave_min2_lead(sys_time, t_s, n, m);
         loop++;
     }
-    if (handover || from_start) {
+    if (handover || from_start || terms_handover) {
         // a collision in this block step: hand the cluster to the global-memory path
         // at the start of the step (kept out of the loop, which then holds less state)
         if (handover && lead) {
@@ -1454,6 +1774,7 @@ HP make_hp(const gplum_params &p) {
     h.m_sun = p.m_sun; h.eps2 = p.eps2; h.eps2_sun = p.eps2_sun; h.dt_tree = p.dt_tree;
     h.dt_min = p.dt_min; h.eta = p.eta; h.eta_0 = p.eta_0; h.eta_sun = p.eta_sun;
     h.eta_sun0 = p.eta_sun0; h.alpha2 = p.alpha2; h.cut = cutoff_of(p); h.collision = p.collision;
+    h.terms_max = TERMS_MAX;
     return h;
 }
 
@@ -1860,9 +2181,10 @@ __global__ void k_reset_queue(int *next) {
 
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2,
-                hipEvent_t ev_fork, hipEvent_t ev_join, int wg_per_cu, int lds_max, int giant_min) {
+                hipEvent_t ev_fork, hipEvent_t ev_join, int wg_per_cu, int lds_max, int giant_min, int terms_max) {
     if (ncl <= 0) return GPLUM_OK;
     HP P = make_hp(p);
+    P.terms_max = std::max(0, std::min(TERMS_MAX, terms_max));
     HardView H = hard_view(hb);
     GPLUM_ENSURE(hb.next, 3);
     GPLUM_ENSURE(hb.edisp, ncl);

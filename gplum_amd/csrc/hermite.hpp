@@ -11,6 +11,9 @@ constexpr int HERMITE_LDS = 256;     // clusters up to this size keep j-data in 
 // j (one lane per particle); larger ones with 64 lane-strided partial sums and a
 // butterfly (all lanes on one particle), an order the oracle reproduces.
 constexpr int HARD_SEQ_MAX = 64;
+// Clusters of 2 .. TERMS_MAX bodies spread each block step over one lane per particle,
+// term (Sun or partner) and component (4 n^2 <= 64 lanes).
+constexpr int TERMS_MAX = 4;
 
 // Gathered cluster particles (SoA, vectors interleaved x,y,z).
 struct HardBuffers {
@@ -56,13 +59,15 @@ struct IsoArgs {
 // workgroups per CU; lds_max (<= HERMITE_LDS): the largest cluster whose j-data is
 // staged in LDS (larger ones read it from global memory); clusters above giant_min run
 // over the whole GPU, several at a time on a stream pool, host-driven (not capturable in
-// a graph).  None of these changes results.
+// a graph).  terms_max: clusters of 2 .. terms_max bodies take the lane-per-term path
+// (0: every cluster up to HARD_SEQ_MAX takes the one-lane-per-particle path).  None of
+// these changes results.
 constexpr int GIANT_MIN_DEFAULT = 1024;
 constexpr int GIANT_OFF = 0x7fffffff;
 int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off,
                 const int32_t *d_order, bool persistent, int n_cu, hipStream_t st, hipStream_t st2 = nullptr,
                 hipEvent_t ev_fork = nullptr, hipEvent_t ev_join = nullptr, int wg_per_cu = 8,
-                int lds_max = HERMITE_LDS, int giant_min = GIANT_MIN_DEFAULT);
+                int lds_max = HERMITE_LDS, int giant_min = GIANT_MIN_DEFAULT, int terms_max = TERMS_MAX);
 // Sun + hard acc/jerk and the initial block step of every cluster particle (as the
 // initial force correction leaves them), for callers that bring bare clusters.
 int hard_prepare(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *d_off, hipStream_t st);

@@ -89,16 +89,33 @@ __global__ void k_soft_groups(int ng, const int32_t *__restrict__ gstart, const 
                               const int32_t *__restrict__ ep_off, const int32_t *__restrict__ ep_cnt,
                               const int32_t *__restrict__ epf_off, const int32_t *__restrict__ epf_cnt,
                               const int32_t *__restrict__ sp_off, const int32_t *__restrict__ sp_cnt,
-                              SoftGroup *__restrict__ out) {
+                              SoftGroup *__restrict__ out, unsigned long long *__restrict__ pairs) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    SoftGroup s;
-    s.i_begin = gstart[g]; s.ni = gcount[g];
-    s.j_begin = ep_off[g]; s.nj = ep_cnt[g];
-    s.jf_begin = epf_off[g]; s.njf = epf_cnt[g];
-    s.s_begin = sp_off[g]; s.ns = sp_cnt[g];
-    s.i_ref = gstart[g]; s.pad = 0;
-    out[g] = s;
+    unsigned long long ep = 0, epf = 0, sp = 0;
+    if (g < ng) {
+        SoftGroup s;
+        s.i_begin = gstart[g]; s.ni = gcount[g];
+        s.j_begin = ep_off[g]; s.nj = ep_cnt[g];
+        s.jf_begin = epf_off[g]; s.njf = epf_cnt[g];
+        s.s_begin = sp_off[g]; s.ns = sp_cnt[g];
+        s.i_ref = gstart[g]; s.pad = 0;
+        out[g] = s;
+        ep = (unsigned long long)s.ni * (unsigned long long)s.nj;
+        epf = (unsigned long long)s.ni * (unsigned long long)s.njf;
+        sp = (unsigned long long)s.ni * (unsigned long long)s.ns;
+    }
+    // interaction counts of the force call (near EP, far EP, SP pairs): a wave sum, then
+    // one atomic per wave
+    for (int d = 32; d > 0; d >>= 1) {
+        ep += __shfl_xor(ep, d);
+        epf += __shfl_xor(epf, d);
+        sp += __shfl_xor(sp, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&pairs[0], ep);
+        atomicAdd(&pairs[1], epf);
+        atomicAdd(&pairs[2], sp);
+    }
 }
 
 __global__ void k_kick(int n3, double hdt, const double *__restrict__ acc, double *__restrict__ vel) {
@@ -365,9 +382,11 @@ int System::soft_force(bool initial, bool use_f64) {
                        A.r_search.p, jrec.p);
     const int ng = tree.n_group;
     GPLUM_ENSURE(groups, ng);
+    GPLUM_ENSURE(pair_cnt, 3);
+    GPLUM_HIP(hipMemsetAsync(pair_cnt.p, 0, 3 * sizeof(unsigned long long), st));
     hipLaunchKernelGGL(k_soft_groups, dim3(ceil_div(ng, 256)), dim3(256), 0, st, ng, tree.group_start.p,
                        tree.group_count.p, tree.ep_off.p, tree.ep_cnt.p, tree.epf_off.p, tree.epf_cnt.p,
-                       tree.sp_off.p, tree.sp_cnt.p, groups.p);
+                       tree.sp_off.p, tree.sp_cnt.p, groups.p, pair_cnt.p);
     SrcRecI is{jrec.p};
     SrcRecJ jn{jrec.p, tree.ep_list.p};
     SrcRecJ jf{jrec.p, tree.epf_list.p};
@@ -481,18 +500,7 @@ int System::soft_force(bool initial, bool use_f64) {
         stats.fc_loss = fc.last_loss;
     }
     have_nbr = true;
-    // interaction counts (sum over groups of ni*nj, ni*ns) for the roofline
-    std::vector<SoftGroup> hg(ng);
-    GPLUM_HIP(hipMemcpy(hg.data(), groups.p, sizeof(SoftGroup) * ng, hipMemcpyDeviceToHost));
-    long long ep = 0, epf = 0, sp_ = 0;
-    for (auto &g : hg) {
-        ep += (long long)g.ni * g.nj;
-        epf += (long long)g.ni * g.njf;
-        sp_ += (long long)g.ni * g.ns;
-    }
-    last_epep_near = ep;
-    last_epep_far = epf;
-    last_epsp = sp_;
+    // the interaction counts stay on the device (pair_cnt) until the statistics are read
     last_groups = ng;
     return GPLUM_OK;
 }
@@ -569,7 +577,7 @@ int System::hard_part(bool persistent) {
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
         if (persistent) {
             if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join,
-                                  hard_wg_per_cu, hard_lds_max, hard_giant_min)))
+                                  hard_wg_per_cu, hard_lds_max, hard_giant_min, hard_terms_max)))
                 return rc;
         } else {
             // the comparator: one launch per cluster
@@ -876,7 +884,7 @@ int System::hard_trial(int wg_per_cu, int lds_max, double *ms) {
     hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
     tick(EV_HARD0);
     if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join, wg_per_cu,
-                          lds_max, hard_giant_min)))
+                          lds_max, hard_giant_min, hard_terms_max)))
         return rc;
     tick(EV_HARD1);
     *ms = elapsed(EV_HARD0, EV_HARD1);
@@ -942,6 +950,7 @@ int System::autotune(int trials, gplum_tuning *out) {
         out->hard_lds_max = b_lds;
         out->hard_giant_min = hard_giant_min;
         out->tree_coop_min = tree.coop_min;
+        out->hard_terms_max = hard_terms_max;
         out->soft_ms = best_soft;
         out->hard_ms = best_hard < 1e300 ? best_hard : 0.;
     }
@@ -1048,6 +1057,8 @@ int gplum_system_step(gplum_system *s, int nsteps) {
         int rc = S->step(true);
         if (rc) return rc;
     }
+    // the steps' last kernels (the energy sums) are complete when the call returns
+    GPLUM_HIP(hipStreamSynchronize(S->st));
     return GPLUM_OK;
 }
 
@@ -1082,9 +1093,14 @@ int gplum_system_stats(gplum_system *s, gplum_stats *out) {
     if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
     *out = S->stats;
-    out->n_epep_near = S->last_epep_near;
-    out->n_epep_far = S->last_epep_far;
-    out->n_epsp = S->last_epsp;
+    unsigned long long pc[3] = {0ull, 0ull, 0ull};
+    if (S->pair_cnt.p) {
+        GPLUM_HIP(hipStreamSynchronize(S->st));
+        GPLUM_HIP(hipMemcpy(pc, S->pair_cnt.p, sizeof(pc), hipMemcpyDeviceToHost));
+    }
+    out->n_epep_near = (int64_t)pc[0];
+    out->n_epep_far = (int64_t)pc[1];
+    out->n_epsp = (int64_t)pc[2];
     out->n_sp_entries = S->tree.n_sp_total;
     out->n_groups = S->last_groups;
     out->n_candidates = S->last_ncand;
@@ -1141,7 +1157,7 @@ int gplum_system_get_tuning(gplum_system *s, gplum_tuning *out) {
     if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
     *out = gplum_tuning{S->p.n_group_limit, S->soft_tile, S->hard_wg_per_cu, S->hard_lds_max, S->hard_giant_min,
-                        S->tree.coop_min, 0., 0.};
+                        S->tree.coop_min, S->hard_terms_max, 0, 0., 0.};
     return GPLUM_OK;
 }
 
@@ -1154,6 +1170,8 @@ int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t) {
     if (t->hard_lds_max < 64 || t->hard_lds_max > 256) return gplum::fail(GPLUM_E_ARG, "hard_lds_max must be 64..256");
     if (t->hard_giant_min < 64) return gplum::fail(GPLUM_E_ARG, "hard_giant_min must be >= 64");
     if (t->tree_coop_min < 0) return gplum::fail(GPLUM_E_ARG, "tree_coop_min must be >= 0");
+    if (t->hard_terms_max < 0 || t->hard_terms_max > gplum::TERMS_MAX)
+        return gplum::fail(GPLUM_E_ARG, "hard_terms_max must be 0.." + std::to_string(gplum::TERMS_MAX));
     System *S = reinterpret_cast<System *>(s);
     S->tree.coop_min = t->tree_coop_min > 0 ? t->tree_coop_min : (1 << 16);
     S->p.n_group_limit = t->n_group_limit;
@@ -1161,6 +1179,7 @@ int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t) {
     S->hard_wg_per_cu = t->hard_wg_per_cu;
     S->hard_lds_max = t->hard_lds_max;
     S->hard_giant_min = t->hard_giant_min;
+    S->hard_terms_max = t->hard_terms_max;
     return GPLUM_OK;
 }
 

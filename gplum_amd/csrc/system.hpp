@@ -63,6 +63,7 @@ struct System {
     int hard_wg_per_cu = 8;             // persistent Hermite workgroups per CU
     int hard_lds_max = HERMITE_LDS;     // largest cluster whose j-data is staged in LDS
     int hard_giant_min = GIANT_MIN_DEFAULT;   // larger clusters run over the whole GPU
+    int hard_terms_max = TERMS_MAX;     // clusters of 2 .. this many bodies: lane per term
     bool tuning = false;                // inside autotune(): no training, no statistics
     bool skip_train = false;            // the force recomputed after mergers: no training
     ParticleSoA A, B;
@@ -88,7 +89,8 @@ struct System {
     long long istep = 0;
     int n_remove_tot = 0, n_recompute = 0;
     FILE *log_rem = nullptr, *log_col = nullptr;   // remove / collision logs (append)
-    long long last_epep_near = 0, last_epep_far = 0, last_epsp = 0, last_groups = 0;
+    long long last_groups = 0;
+    DArray<unsigned long long> pair_cnt;   // near EP, far EP, SP pairs of the last force call
     gplum_stats stats{};
 
     explicit System(const gplum_params &prm);

@@ -86,17 +86,19 @@ def lib():
     return _lib
 
 
-def hard_integrate(p, pp, offsets, persistent=True, prepare=False, counts=False, giant=False):
+def hard_integrate(p, pp, offsets, persistent=True, prepare=False, counts=False, giant=False, lane_per_particle=False):
     """Integrate clusters pp[offsets[c]:offsets[c+1]] over one soft step (in place).
     prepare: compute acc/jerk and the first block step on the device first.
     giant: every cluster above 64 particles on the whole-GPU path (same results).
+    lane_per_particle: clusters of 2 to 4 bodies on the one-lane-per-particle path instead
+    of the lane-per-term path (same results).
     Returns (edisp, block_steps, kernel_ms), plus pair evaluations when counts."""
     off = np.ascontiguousarray(offsets, dtype=np.int32)
     ed = ctypes.c_double(0.0)
     nb = ctypes.c_int64(0)
     ms = ctypes.c_double(0.0)
     npair = ctypes.c_int64(0)
-    mode = (1 if persistent else 0) | (2 if prepare else 0) | (4 if giant else 0)
+    mode = (1 if persistent else 0) | (2 if prepare else 0) | (4 if giant else 0) | (8 if lane_per_particle else 0)
     check(lib().gplum_hard_integrate(ctypes.byref(p), abi.ptr(pp), len(pp), abi.ptr(off), len(off) - 1, mode,
                                      ctypes.byref(ed), ctypes.byref(nb), ctypes.byref(ms),
                                      ctypes.byref(npair)))
@@ -201,6 +203,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("n_group_limit", ctypes.c_int32), ("soft_tile", ctypes.c_int32),
                 ("hard_wg_per_cu", ctypes.c_int32), ("hard_lds_max", ctypes.c_int32),
                 ("hard_giant_min", ctypes.c_int32), ("tree_coop_min", ctypes.c_int32),
+                ("hard_terms_max", ctypes.c_int32), ("pad_", ctypes.c_int32),
                 ("soft_ms", ctypes.c_double), ("hard_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -322,8 +325,9 @@ class System:
         return t.as_dict()
 
     def set_tuning(self, n_group_limit=256, soft_tile=256, hard_wg_per_cu=8, hard_lds_max=256,
-                   hard_giant_min=1024, tree_coop_min=0, **_):
-        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, hard_giant_min, tree_coop_min, 0.0, 0.0)
+                   hard_giant_min=1024, tree_coop_min=0, hard_terms_max=4, **_):
+        t = Tuning(n_group_limit, soft_tile, hard_wg_per_cu, hard_lds_max, hard_giant_min, tree_coop_min,
+                   hard_terms_max, 0, 0.0, 0.0)
         check(lib().gplum_system_set_tuning(self.h, ctypes.byref(t)))
 
     FORECAST_MODES = {"off": 0, "network": 1, "uniform": 2}

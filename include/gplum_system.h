@@ -150,6 +150,10 @@ typedef struct gplum_tuning {
     int32_t hard_giant_min;  /* larger clusters run over the whole GPU (not tuned)    */
     int32_t tree_coop_min;   /* tree nodes of at least this many particles split with a
                                 workgroup-wide search (not tuned; 0 = 65536)          */
+    int32_t hard_terms_max;  /* clusters of 2 .. this many (<= 4) bodies spread a block
+                                step over one lane per particle, term and component;
+                                0: one lane per particle (not tuned; same results)   */
+    int32_t pad_;
     double soft_ms, hard_ms; /* best measured soft stage / Hermite kernels, ms        */
 } gplum_tuning;
 int gplum_system_autotune(gplum_system *s, int trials, gplum_tuning *out);
@@ -179,7 +183,9 @@ int gplum_system_put_forecast(gplum_system *s, const float *theta, const float *
  * mode bit 0 set: one persistent batched launch (largest cluster first); clear: one
  * launch per cluster (plain comparator).  mode bit 2 (with bit 0): every cluster above
  * 64 particles runs on the whole-GPU path meant for giant clusters (normally above 1024;
- * same results).  mode bit 1 set: the clusters are bare
+ * same results).  mode bit 3 (with bit 0): clusters of 2 to 4 bodies run one lane per
+ * particle instead of one lane per particle, term and component (same results).
+ * mode bit 1 set: the clusters are bare
  * (pos, vel, mass, r_out_inv, acc0, dt only): Sun and hard acc/jerk and the first
  * block step are computed on the device first, as the initial force correction
  * would leave them (not timed).  Outputs (each may be null): edisp (collision

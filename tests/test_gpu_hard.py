@@ -18,11 +18,13 @@ HARD = dict(eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002, dt_tree=2.0 ** 
             dt_min=2.0 ** -20, gamma=0.1)
 
 
-def make_clusters(oracle_lib, n_clusters, seed, size_max, collision, f=1.0, m_hi=1e-9):
+def make_clusters(oracle_lib, n_clusters, seed, size_max, collision, f=1.0, m_hi=1e-9, planar=False):
     po = oracle_lib.params(collision=collision, **HARD)
     pg = gpu.params(collision=collision, **HARD)
     pp, off = ic.clusters(n_clusters, seed=seed, size_max=size_max, m_hi=m_hi)
     pp["f"] = f
+    if planar:
+        pp["pos"][:, 2] = 0.0     # z and v_z exactly zero: signed zeros reach the results
     # acc0 as correctForceLong forms it: number / sum r_out^2 / m_j over the cluster
     for c in range(n_clusters):
         a, b = off[c], off[c + 1]
@@ -161,6 +163,47 @@ def test_graph_replay_with_merging(oracle_lib):
     assert bad == 0
     assert nb_g == nb_e and ed_g == ed_e
     assert _same_state(got, eager) == []
+
+
+@pytest.mark.parametrize("planar", [False, True])
+def test_small_clusters_lane_per_term(oracle_lib, planar):
+    """Clusters of 2 to 4 bodies (one lane per particle, term and component): bit-identical
+    to the oracle and to the one-lane-per-particle path; in the planar case z and v_z stay
+    exactly zero and the Sun's z-terms are -0, so the sign handling of the quad dot
+    products shows in the compared bits."""
+    po, pg, pp, off = make_clusters(oracle_lib, 300, seed=23, size_max=4, collision=0, planar=planar)
+    assert np.diff(off).min() == 2 and np.diff(off).max() == 4
+    ref = pp.copy()
+    nc, ed, nb_ref = oracle_lib.hermite_clusters(po, ref, off)
+    got = pp.copy()
+    ed_g, nb_g, ms = gpu.hard_integrate(pg, got, off, persistent=True)
+    lpp = pp.copy()
+    ed_l, nb_l, ms_l = gpu.hard_integrate(pg, lpp, off, persistent=True, lane_per_particle=True)
+    print("planar %s: %d clusters, block steps %d (oracle %d), kernel %.2f ms (one lane per particle %.2f ms)"
+          % (planar, len(off) - 1, nb_g, nb_ref, ms, ms_l))
+    assert nb_g == nb_ref == nb_l
+    assert _same_state(got, ref, fields=("pos", "vel", "acc_d", "jerk_d", "acc_s", "jerk_s", "phi_s", "dt",
+                                         "time")) == []
+    assert np.allclose(got["phi_d"], ref["phi_d"], rtol=1e-13, atol=0.0)
+    assert _same_state(got, lpp) == []
+    if planar:
+        assert (got["pos"][:, 2] == 0.0).all() and np.signbit(got["acc_s"][:, 2]).any()
+
+
+def test_small_clusters_lane_per_term_collisions(oracle_lib):
+    """Clusters of 2 to 4 bodies with inflated radii: the block step of a collision hands
+    the cluster to the global-memory path; the same merges as the oracle."""
+    po, pg, pp, off = make_clusters(oracle_lib, 200, seed=29, size_max=4, collision=1, f=300.0, m_hi=1e-8)
+    ref = pp.copy()
+    nc, ed, nb_ref = oracle_lib.hermite_clusters(po, ref, off)
+    got = pp.copy()
+    ed_g, nb_g, ms = gpu.hard_integrate(pg, got, off, persistent=True)
+    print("collisions %d, block steps %d (oracle %d)" % (nc, nb_g, nb_ref))
+    assert nc > 10
+    assert nb_g == nb_ref
+    assert ed_g == ed
+    assert _same_state(got, ref, fields=("pos", "vel", "acc_d", "jerk_d", "mass", "r_planet", "dt", "time", "id",
+                                         "flags")) == []
 
 
 def test_giant_path_matches_oracle(no_collision_case):
