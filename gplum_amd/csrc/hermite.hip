@@ -1026,17 +1026,19 @@ __device__ inline void wave_min2_lead(double &u, double &v, int n, int m) {
 
 // ------------------------------------------- clusters of 2 to 4 bodies: lane per term
 //
-// Lane L = 16 T + 4 p + c works on particle p (the quads of a row), term T (the rows:
-// T = 0 the Sun, T = k + 1 the pair with p's k-th partner in ascending index,
-// j = k + (k >= p)) and component c (0, 1, 2 = x, y, z; c = 3 pads the quad and keeps
+// Lane L = 16 T + 4 q + c works on term T (the rows: T = 0 the Sun, T = k + 1 the pair
+// with particle p's k-th partner in ascending index, j = k + (k >= p)), particle p = q
+// (the quads of a row; for 2 bodies p = q & 1 and T = (L >> 4) & 1, so that every lane
+// holds a copy) and component c (0, 1, 2 = x, y, z; c = 3 pads the quad and keeps
 // zeros).  The Sun term and every pair term of every particle are evaluated by the same
 // instructions, a vector operation acts on x, y and z at once, and a dot product is the
 // quad's sum of its lanes' products in two DPP steps.  Every lane of particle p keeps
 // p's state for its component (the same in all rows), so predictor and corrector need
-// no exchange; after each evaluation the rows are gathered through the LDS (the Sun
-// row, then the partner rows in ascending j added from zero: the sums of
-// star_acc_jerk and gravity_loop_lane), and the Sun and planet time-step criteria
-// likewise.  Same operations in the same order as the one-lane-per-particle path.
+// no exchange; after each evaluation every row gets each row's term of its (p, c) by
+// cross-row lane swaps (v_permlane16/32_swap, no LDS) and adds them as star_acc_jerk
+// and gravity_loop_lane do (the Sun row; the partner rows in ascending j from zero), and
+// the Sun and planet time-step criteria likewise.  Same operations in the same order
+// as the one-lane-per-particle path.
 
 constexpr int DPP_ROW_ROR4 = 0x124;
 
@@ -1050,6 +1052,56 @@ __device__ inline double quad_sum(double s) {
 }
 __device__ inline double quad_sq(double a) { return quad_sum(a * a); }
 __device__ inline double quad_dot(double a, double b, bool pad) { return quad_sum(pad ? -0. : a * b); }
+
+// Cross-row exchanges.  swap16: each lane gets the values of the even and the odd row
+// of its pair of rows (0, 1) or (2, 3), same lane within the row; swap32: the values of
+// its half's row at the same position in the lower and the upper half.  Combined, each
+// row reads the four rows' values of its lane position.
+__device__ inline double dbl_of(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline void swap16(double x, double &even, double &odd) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    even = dbl_of(lo[0], hi[0]);
+    odd = dbl_of(lo[1], hi[1]);
+}
+__device__ inline void swap32(double x, double &low, double &high) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    low = dbl_of(lo[0], hi[0]);
+    high = dbl_of(lo[1], hi[1]);
+}
+// the row values r[0..N-1] of the lane's position (N = 2: rows 0 and 1 of its pair)
+template <int N>
+__device__ inline void rows_of(double x, double (&r)[4]) {
+    if (N == 2) {
+        swap16(x, r[0], r[1]);
+        r[2] = r[3] = 0.;
+    } else {
+        double lo, hi;
+        swap32(x, lo, hi);
+        swap16(lo, r[0], r[1]);
+        swap16(hi, r[2], r[3]);
+    }
+}
+
+// K(x) and dK/dr of cutfunc.h with selects (the polynomial evaluated everywhere).
+__device__ inline double cutoff_K_sel(const Cutoff &c, double rij, double r_out_inv) {
+    const double x = (c.gamma - rij * r_out_inv) * c.g_1_inv;
+    const double x2 = x * x;
+    const double p = (((-20. * x + 70.) * x - 84.) * x + 35.) * x2 * x2;
+    return (x < 0.) ? 0. : ((x >= 1.) ? 1. : p);
+}
+__device__ inline double cutoff_dKdt_sel(const Cutoff &c, double rij, double r_out_inv, double alpha) {
+    const double x = (c.gamma - rij * r_out_inv) * c.g_1_inv;
+    const double x_1 = x - 1.;
+    const double d = 140. * x * x * x * x_1 * x_1 * x_1 * r_out_inv * c.g_1_inv;
+    const double dKdr = (x < 0. || x >= 1.) ? 0. : d;
+    return alpha * rij * dKdr;
+}
 
 // The lane's term of a hard force evaluation, in one instruction stream for the Sun row
 // (star_acc_jerk: xj = vj = -0, so that xj - x = -x exactly; mT = m_sun, eps2T =
@@ -1071,30 +1123,30 @@ __device__ inline void term_eval(const Cutoff &cut, bool sun, bool pad, double x
     const double r3inv = r2inv * rinv;
     const double mj_r3 = mT * r3inv;
     const double alpha = quad_dot(dr, dv, pad) * r2inv;
-    const double K = sun ? 1. : 1. - cutoff_K(cut, y1, r_out_inv);
-    const double dKdt = sun ? -0. : cutoff_dKdt(cut, y1, r_out_inv, alpha);
+    const double K = sun ? 1. : 1. - cutoff_K_sel(cut, y1, r_out_inv);
+    const double dKdt = sun ? -0. : cutoff_dKdt_sel(cut, y1, r_out_inv, alpha);
     const double alpha_c = alpha * K;
     const double c2 = 3. * alpha_c + dKdt;
     ta = mj_r3 * K * dr;
     tj = mj_r3 * (K * dv - c2 * dr);
 }
 
-// Each lane of particle p gets the Sun term and the sum of the partner terms of its
-// component (rows 1..m at quad position q = 4 p + c, added in ascending j from zero).
-__device__ inline void gather_terms(double *s_tm, int L, int q, int m, double ta, double tj, double &as,
-                                    double &js, double &ad, double &jd) {
-    s_tm[2 * L] = ta;
-    s_tm[2 * L + 1] = tj;
-    wave_sync();
-    as = s_tm[2 * q];
-    js = s_tm[2 * q + 1];
+// Every row of particle p gets the Sun term and the sum of the partner terms of its
+// component (rows 1..N-1, added in ascending j from zero).
+template <int N>
+__device__ inline void gather_terms(double ta, double tj, double &as, double &js, double &ad, double &jd) {
+    double ra[4], rj[4];
+    rows_of<N>(ta, ra);
+    rows_of<N>(tj, rj);
+    as = ra[0];
+    js = rj[0];
     ad = 0.;
     jd = 0.;
-    for (int k = 1; k <= m; k++) {
-        ad += s_tm[2 * (16 * k + q)];
-        jd += s_tm[2 * (16 * k + q) + 1];
+#pragma unroll
+    for (int k = 1; k < N; k++) {
+        ad += ra[k];
+        jd += rj[k];
     }
-    __builtin_amdgcn_wave_barrier();
 }
 
 // predict_regs and correct_regs for one component (same expressions).
@@ -1159,62 +1211,74 @@ struct TermConsts {
     double dt_tree, dt_min, alpha2;
 };
 
+// the Sun row's and the first partner row's values of the lane's position
+template <int N>
+__device__ inline void sun_planet_of(double x, double &f_sun, double &f_planet) {
+    if (N == 2) {
+        swap16(x, f_sun, f_planet);
+    } else {
+        double lo, hi;
+        swap32(x, lo, hi);
+        swap16(lo, f_sun, f_planet);
+    }
+}
+
 // dt_block with the Sun criterion from the Sun row and the planet criterion from the
-// partner rows (each lane evaluates its own row's, reads the other from the LDS); the
+// partner rows (each lane evaluates its own row's, reads the other by a lane swap); the
 // exact criteria when either fast one is not finite or dt_1 lies near the step grid.
-__device__ inline double dt_block_rows(const TermConsts &C, bool sun, double *s_tm, int L, int q, double time,
-                                       double dt, double etaT, double acc0T, double a, double j, double s, double c) {
+template <int N>
+__device__ inline double dt_block_rows(const TermConsts &C, double time, double dt, double etaT, double acc0T,
+                                       double a, double j, double s, double c) {
     double dt_next = dmin(2. * dt, 0.5 * C.dt_tree);
     double rem = fmod_block(time, dt_next);
     for (int it = 0; rem != 0.0 && it < HALVE_CAP; it++) { dt_next *= 0.5; rem = fmod_block(time, dt_next); }
-    const int other = 2 * (sun ? 16 + q : q);     // a partner row's lane, or the Sun row's
-    const double f = dt4_fast_quad(etaT, C.alpha2, acc0T, a, j, s, c);
-    s_tm[2 * L] = f;
-    wave_sync();
-    const double fo = s_tm[other];
-    const double f_planet = sun ? fo : f, f_sun = sun ? f : fo;
+    double f_sun, f_planet;
+    sun_planet_of<N>(dt4_fast_quad(etaT, C.alpha2, acc0T, a, j, s, c), f_sun, f_planet);
     double dt_1 = dmin(f_planet, f_sun);
     const bool exact = !(f_planet <= DBL_MAX && f_sun <= DBL_MAX) || !clear_of_grid(dt_1, dt_next);
     if (__ballot(exact)) {
-        const double e = dt4_quad(etaT, C.alpha2, acc0T, a, j, s, c);
-        s_tm[2 * L + 1] = e;
-        wave_sync();
-        const double eo = s_tm[other + 1];
-        if (exact) dt_1 = dmin(sun ? eo : e, sun ? e : eo);
+        double e_sun, e_planet;
+        sun_planet_of<N>(dt4_quad(etaT, C.alpha2, acc0T, a, j, s, c), e_sun, e_planet);
+        if (exact) dt_1 = dmin(e_planet, e_sun);
     }
-    __builtin_amdgcn_wave_barrier();
     for (int it = 0; dt_1 < dt_next && 0.5 * dt < dt_next && it < HALVE_CAP; it++) dt_next *= 0.5;
     if (dt_next < 2. * C.dt_min) dt_next = C.dt_min;
     return (rem != 0.0) ? C.dt_min : dt_next;
 }
 
-// Block steps of a cluster of 2 to 4 particles with distinct ids from the start of the
-// soft step (the layout above; one wave).  Returns false when a block step has a
+// Block steps of a cluster of N (2 to 4) particles with distinct ids from the start of
+// the soft step (the layout above; one wave).  Returns false when a block step has a
 // collision: the state of the start of that block step is then in H (flags and twins
 // cleared) for integrate_from.  Else the final calcGravity with potentials is done and
 // written to H.  s_m, s_ri, s_id, s_fr hold the cluster's masses, 1/r_out, ids and
 // f r_planet; s_x, s_v and s_tm are free.
+template <int N>
 __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double *s_x, double *s_v, double *s_m,
-                                double *s_ri, int64_t *s_id, double *s_fr, double *s_tm, int b, int n,
-                                double &sys_time, long long &loop, long long &pairs) {
+                                double *s_ri, int64_t *s_id, double *s_fr, double *s_tm, int b, double &sys_time,
+                                long long &loop, long long &pairs) {
+    constexpr int m = N - 1;
     const int L = threadIdx.x;
-    const int T = L >> 4, p = (L >> 2) & 3, c = L & 3;
-    const int q = L & 15;                           // 4 p + c
-    const int m = n - 1;
-    const bool own = p < n;                         // lanes of a particle (any row)
-    const bool row = own && T < n;                  // lanes of a term of a particle
+    const int T = (N == 2) ? ((L >> 4) & 1) : (L >> 4);
+    const int p = (N == 2) ? ((L >> 2) & 1) : ((L >> 2) & 3);
+    const int c = L & 3;
+    // lanes of a particle: every row takes part in the evaluations, so that the cross-row
+    // swaps read computed values (rows T >= N, 3 bodies, evaluate a don't-care term)
+    const bool own = p < N;
+    const bool row = own && T < N;                  // lanes of a term of a particle
+    const bool first = (N == 2) ? L < 32 && ((L >> 2) & 3) < 2 : true;   // not a copy
     const bool sun = T == 0, pad = c == 3;
     const int j = (T >= 1) ? (T - 1) + ((T - 1) >= p ? 1 : 0) : p;   // the row's partner
-    const int g = b + (own ? p : 0);
+    const int jc = (j < N) ? j : 0, pc = own ? p : 0;
+    const int g = b + pc;
     const TermConsts C{P.cut, P.dt_tree, P.dt_min, P.alpha2};
-    const double mT = sun ? P.m_sun : s_m[j < n ? j : 0];
+    const double mT = sun ? P.m_sun : s_m[jc];
     const double eps2T = sun ? P.eps2_sun : P.eps2;
-    const double r_out_inv = dmin(s_ri[own ? p : 0], s_ri[j < n ? j : 0]);
+    const double r_out_inv = dmin(s_ri[pc], s_ri[jc]);
     const double etaT = sun ? P.eta_sun : P.eta;
-    const double r2c = s_fr[own ? p : 0] + s_fr[j < n ? j : 0];
+    const double r2c = s_fr[pc] + s_fr[jc];
     const double col_thr = COL_FILTER * r2c * r2c;
     const bool col_row = P.collision && row && !sun;
-    const int src = 4 * (j < n ? j : 0) + c;      // row 0 lane of the partner's component
+    const int src = 4 * jc + c;                     // a lane of the partner's component
     const double time_end = P.dt_tree;
     double x = 0., v = 0., time = 0., dt = 0., acc0 = 0.;
     if (own && !pad) {
@@ -1227,15 +1291,17 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
         acc0 = H.acc0[g];
     }
     const double acc0T = sun ? 0. : acc0;
-    const bool lead = row && sun && c == 0;
+    const bool lead = row && sun && c == 0 && first;
     double as = 0., js = 0., ad = 0., jd = 0., a2s = 0., a3s = 0., a2d = 0., a3d = 0.;
     {
         // initial calcGravity (hermite.h:259-270) at the current state
         double xj = __shfl(x, src), vj = __shfl(v, src);
         if (sun) { xj = -0.; vj = -0.; }
-        double ta = 0., tj = 0.;
-        if (row) term_eval(C.cut, sun, pad, x, v, xj, vj, mT, eps2T, r_out_inv, ta, tj);
-        if (row) gather_terms(s_tm, L, q, m, ta, tj, as, js, ad, jd);
+        if (own) {
+            double ta, tj;
+            term_eval(C.cut, sun, pad, x, v, xj, vj, mT, eps2T, r_out_inv, ta, tj);
+            gather_terms<N>(ta, tj, as, js, ad, jd);
+        }
     }
     double tn = own ? time + dt : 1.e10;
     tn = dmin(tn, dpp_f64<DPP_ROW_ROR4>(tn));
@@ -1247,7 +1313,16 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
         const double a0s = as, j0s = js, a0d = ad, j0d = jd;
         double xp = 0., vp = 0.;
         predict_one(Dt, x, v, as, js, ad, jd, xp, vp);
-        double xj = __shfl(xp, src), vj = __shfl(vp, src);
+        double xj, vj;
+        if (N == 2) {
+            // the partner of quad q is quad q ^ 1, which row_ror:4 brings (quads 2, 3 are
+            // copies of 0, 1)
+            xj = dpp_f64<DPP_ROW_ROR4>(xp);
+            vj = dpp_f64<DPP_ROW_ROR4>(vp);
+        } else {
+            xj = __shfl(xp, src);
+            vj = __shfl(vp, src);
+        }
         if (sun) { xj = -0.; vj = -0.; }
         if (P.collision) {
             // collisionDetermination: a pair with R = |dx| / (f_i R_i + f_j R_j) < 1
@@ -1259,7 +1334,7 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
                 if (__ballot(hit)) { handover = true; break; }
             }
         }
-        if (row && time + dt == t_s) {
+        if (own && time + dt == t_s) {
             if (lead) pairs += 3 * (long long)m;
             const double Dt2 = Dt * Dt;
             const double Dt_inv = recip_block(Dt);
@@ -1267,16 +1342,16 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
             const double Dt_inv3 = Dt_inv2 * Dt_inv;
             double ta, tj;
             term_eval(C.cut, sun, pad, xp, vp, xj, vj, mT, eps2T, r_out_inv, ta, tj);
-            gather_terms(s_tm, L, q, m, ta, tj, as, js, ad, jd);
+            gather_terms<N>(ta, tj, as, js, ad, jd);
             for (int ite = 0; ite < 2; ite++) {
                 correct_one(Dt, Dt2, Dt_inv2, Dt_inv3, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s,
                             a2d, a3d);
                 term_eval(C.cut, sun, pad, x, v, xj, vj, mT, eps2T, r_out_inv, ta, tj);
-                gather_terms(s_tm, L, q, m, ta, tj, as, js, ad, jd);
+                gather_terms<N>(ta, tj, as, js, ad, jd);
             }
             time += Dt;
-            dt = dt_block_rows(C, sun, s_tm, L, q, time, dt, etaT, acc0T, sun ? as : ad, sun ? js : jd,
-                               sun ? a2s : a2d, sun ? a3s : a3d);
+            dt = dt_block_rows<N>(C, time, dt, etaT, acc0T, sun ? as : ad, sun ? js : jd, sun ? a2s : a2d,
+                                  sun ? a3s : a3d);
         }
         double tm = own ? time : 1.e10;
         tn = own ? time + dt : 1.e10;
@@ -1290,7 +1365,7 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
     }
     if (handover) {
         // the state at the start of this block step, for integrate_from
-        if (row && sun && !pad) {
+        if (row && sun && !pad && first) {
             H.pos[3 * g + c] = x; H.vel[3 * g + c] = v;
             H.acc_s[3 * g + c] = as; H.jerk_s[3 * g + c] = js;
             H.acc_d[3 * g + c] = ad; H.jerk_d[3 * g + c] = jd;
@@ -1305,7 +1380,7 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
     }
     // final calcGravity with potentials (hermite.h:466-470), one lane per particle
     __syncthreads();
-    if (row && sun && !pad) {
+    if (row && sun && !pad && first) {
         s_x[3 * p + c] = x;
         s_v[3 * p + c] = v;
     }
@@ -1314,14 +1389,14 @@ __device__ bool integrate_terms(const HP &P, HardView &H, const JData &J, double
         s_tm[2 * p + 1] = dt;
     }
     __syncthreads();
-    if (L < n) {
+    if (L < N) {
         const int gi = b + L;
         double xi[3], vi[3], fas[3], fjs[3], fad[3], fjd[3], phs = 0., phd = 0.;
         ld3(s_x, L, xi);
         ld3(s_v, L, vi);
         star_acc_jerk(P, xi, vi, fas, fjs, &phs);
-        gravity_loop_lane<false>(P, J, n, L, s_id[L], xi, vi, s_ri[L], fad, fjd, &phd);
-        pairs += n - 1;
+        gravity_loop_lane<false>(P, J, N, L, s_id[L], xi, vi, s_ri[L], fad, fjd, &phd);
+        pairs += m;
         st3(H.pos, gi, xi); st3(H.vel, gi, vi);
         st3(H.acc_s, gi, fas); st3(H.jerk_s, gi, fjs);
         st3(H.acc_d, gi, fad); st3(H.jerk_d, gi, fjd);
@@ -1391,7 +1466,11 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
     const bool terms = !from_start && n <= P.terms_max;
     bool terms_handover = false;
     if (terms) {
-        if (integrate_terms(P, H, J, s_x, s_v, s_m, s_ri, s_id, s_fr, s_tm, b, n, sys_time, loop, pairs)) return;
+        const bool done =
+            (n == 2) ? integrate_terms<2>(P, H, J, s_x, s_v, s_m, s_ri, s_id, s_fr, s_tm, b, sys_time, loop, pairs)
+            : (n == 3) ? integrate_terms<3>(P, H, J, s_x, s_v, s_m, s_ri, s_id, s_fr, s_tm, b, sys_time, loop, pairs)
+                       : integrate_terms<4>(P, H, J, s_x, s_v, s_m, s_ri, s_id, s_fr, s_tm, b, sys_time, loop, pairs);
+        if (done) return;
         terms_handover = true;
     }
     // side_force: this lane's partner and its constants
