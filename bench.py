@@ -6,8 +6,9 @@ density ~ a^-1.5, Rayleigh e/i (rms e = 2 h, rms i = h), theta = 0.5 with quadru
 super-particles, n_group_limit 256, n_leaf_limit 8, r_out = 3 mutual Hill radii
 (R_cut0 = 3*2^(1/3)), r_in/r_out = 0.1, dt_tree = 2^-4 (the power-of-two soft step
 nearest below 2pi/64), eta = 0.01, perfect-merging collisions on, seed 1.
-A step is one full soft step: kick, clusters, batched Hermite + Kepler drift, tree,
-FP32 EP-EP/EP-SP, FP64 neighbour finalize, kick (energy after the timed region).
+A step is one full soft step (main_p3t.cpp:423-699): kick, clusters, batched Hermite +
+Kepler drift, tree, FP32 EP-EP/EP-SP, FP64 neighbour finalize, kick, energy, merging and
+boundary removal, and the Initial recompute after a collision.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
     python bench.py --workload hard [--clusters 50000]     (batched hard part alone)
@@ -55,12 +56,45 @@ def measured_traffic(name):
     return int(m.group(1)) if m else None
 
 
+def cpu_setup(threads):
+    """The CPU legs run the oracle built on this host with -O3 -march=native (so that it
+    uses the host's own vector ISA), OpenMP threads bound close to the cores.  Returns the
+    host description recorded with the figures."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PROC_BIND"] = "close"
+    os.environ["OMP_PLACES"] = "cores"
+    from oracle import oracle as O
+    native = O.use_native_build()
+    model, isa = "unknown", "unknown"
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("Flags:"):
+                fl = set(line.split(":", 1)[1].split())
+                isa = "avx512f" if "avx512f" in fl else ("avx2" if "avx2" in fl else "sse")
+    except (OSError, ValueError):
+        pass
+    return {"cpu_model": model, "isa": isa, "build": "gcc -O3 -march=native" if native else
+            "gcc -O2 -march=x86-64-v3 (native build failed)", "omp": "OMP_PROC_BIND=close, OMP_PLACES=cores"}
+
+
+def cpu_threads_default():
+    """Host threads for the CPU legs: the cores this process may run on, at most 16 (the
+    host share of one GPU on the measurement machines)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
 def cpu_soft_force(n, seed, threads, pp=None):
     """Oracle FP32 soft force (PIKG-order kernels, OpenMP over groups) on the same disk:
     interactions per second on `threads` host threads."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import oracle as O
-    O.build()
     p = O.params(**params_dict())
     pp = make_disk(n, seed) if pp is None else pp.copy()
     O.set_cutoff_radii(p, pp)
@@ -73,25 +107,25 @@ def cpu_soft_force(n, seed, threads, pp=None):
             "seconds": round(dt, 3), "cores": threads}
 
 
-def cpu_baseline(n, seed, threads):
-    """Oracle full P3T step (CPU restatement, OpenMP) on the same workload: one step
+def cpu_baseline(n, seed, threads, host):
+    """Oracle full P3T step (CPU restatement, OpenMP) on the same workload: five steps
     after initialisation, timed on `threads` host threads."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import oracle as O
-    O.build()
     p = O.params(**params_dict())
     pp = make_disk(n, seed)
     run = O.P3T(p, pp)
-    steps = 2
+    steps = 5
     t0 = time.perf_counter()
     for _ in range(steps):
         run.step()
     dt = time.perf_counter() - t0
     run.close()
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d full soft steps of the same N=%d disk after initialisation, "
-                      "oracle C restatement (PIKG-order FP32 soft kernels, FP64 correction/Hermite/"
-                      "Kepler, OpenMP over groups and clusters), %.2f s" % (steps, n, dt)}
+    out = {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": "%d full soft steps of the same N=%d disk after initialisation, "
+                     "oracle C restatement (PIKG-order FP32 soft kernels, FP64 correction/Hermite/"
+                     "Kepler, OpenMP over groups and clusters), %.2f s" % (steps, n, dt)}
+    out.update(host)
+    return out
 
 
 PEAK_FP64_TFLOPS = 78.6           # MI355X published dense FP64 vector peak
@@ -317,6 +351,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--mode-steps", type=int, default=6, help="steps per forecaster mode compared (0: skip)")
     ap.add_argument("--workload", choices=["step", "hard", "copy", "config4", "config5"], default="step",
                     help="step: full P3T step at N (default); hard: batched hard part alone; "
                          "copy: plain device copy of --copy-mib MiB (calibrates the HBM byte counters); "
@@ -378,21 +413,36 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.n * args.steps / elapsed
 
-    # dominant kernel: FP32 soft force (EP-EP + EP-SP), HIP-event timed on the system stream
+    # FP32 soft force (EP-EP + EP-SP), HIP-event timed on the system stream
     ncalls = max(1, st["n_force_calls"])
     force_ms = st["force_ms"] / ncalls
     pairs_ep = st["n_epep_near"] + st["n_epep_far"]
     pairs_sp = st["n_epsp"]
     flops = FLOP_EPEP * pairs_ep + FLOP_EPSP_QUAD * pairs_sp + FLOP_SPJ_QUAD * st["n_sp_entries"]
     achieved = flops / (force_ms * 1e-3) / 1e12
-    # compute-bound kernel: "mfma" is the compute roof of the schema; the arithmetic is FP32
+    # compute-bound kernel; "mfma" names the schema's compute roof: the arithmetic is FP32
     # VALU (pairwise gravity is not a contraction), priced against the FP32 vector peak
     traffic = measured_traffic("k_soft_f32_n1m_traffic.md") if args.n == 1_000_000 else None
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                "compute": "fp32 VALU, no MFMA",
-                "kernel": "k_soft_f32 (EP-EP near/far + EP-SP quadrupole, one launch per force)",
-                "flops_per_launch": flops, "avg_launch_ms": round(force_ms, 4)}
+    roofline_soft = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "compute": "fp32 VALU, no MFMA",
+                     "kernel": "k_soft_f32 (EP-EP near/far + EP-SP quadrupole, one launch per force)",
+                     "flops_per_launch": flops, "avg_launch_ms": round(force_ms, 4)}
+    # the kernel with the most time in the step: the batched FP64 Hermite (one launch set
+    # per hard part, HIP events on its stream); algorithmic flops = 75 per pair-force
+    # evaluation; it is bound by the block-step chain of the longest cluster (one wave),
+    # not by the chip's FP64 rate
+    n_hl = max(1, st["n_hard_launches"])
+    hk_ms = st["hard_kernel_ms"] / n_hl
+    hk_flops = FLOP_HARD_PAIR * st["hard_pairs"] / n_hl
+    hk_ach = hk_flops / (hk_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(hk_ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(hk_ach / PEAK_FP64_TFLOPS, 6),
+                "traffic": measured_traffic("k_hermite_n1m_traffic.md") if args.n == 1_000_000 else None,
+                "compute": "fp64 VALU, no MFMA; latency-bound on the longest cluster's block-step chain",
+                "kernel": "k_hermite size classes (batched FP64 Hermite, one launch set per step)",
+                "flops_per_launch": round(hk_flops), "avg_launch_ms": round(hk_ms, 4),
+                "us_per_block_step_longest_cluster": round(1e3 * hk_ms / max(1, st["max_block_steps"]), 3)}
 
     # FP64 twin on the same state: interactions/s and the FP32 error
     q32 = s.particles()
@@ -438,6 +488,27 @@ def main():
         sp_.close()
         print("# plain variant done", file=sys.stderr, flush=True)
 
+    # the same steps with the candidate buffers sized by the network (the timed run), by
+    # a uniform per-particle rate, and by the global buffer without any forecast, each
+    # from the warmed-up state (one untimed step, then the timed ones)
+    fc_modes = None
+    if args.mode_steps > 0:
+        fc_modes = {"network": round(ms_per_step, 3)}
+        for mode in ("uniform", "off"):
+            sm = gpu.System(p, warm_state, time_sys=warm_time)
+            if tuning:
+                sm.set_tuning(**tuning)
+            sm.set_forecast(mode)
+            sm.initialize()
+            sm.step(1)
+            tm0 = time.perf_counter()
+            sm.step(args.mode_steps)
+            fc_modes[mode] = round(1e3 * (time.perf_counter() - tm0) / args.mode_steps, 3)
+            sm.close()
+        fc_modes["unit"] = "ms per step"
+        fc_modes["steps"] = args.mode_steps
+        print("# forecaster modes: %s" % fc_modes, file=sys.stderr, flush=True)
+
     de = (e1[0] - e0[0] - (e1[5] - e0[5])) / e0[0]
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
@@ -448,6 +519,7 @@ def main():
                                "theta=0.5 quadrupole, n_group<=256, dt_tree=2^-4, collisions on" % args.n,
                    "n": args.n, "seed": args.seed, "device": name},
         "roofline": roofline,
+        "roofline_soft_f32": roofline_soft,
         "autotune": tuning,
         "soft_force": {"interactions_per_s_fp32": round(ips32, 1), "interactions_per_s_fp64": round(ips64, 1),
                        "fp32_over_fp64": round(ips32 / ips64, 3), "acc_rms_rel_err_fp32_vs_fp64": acc_rms,
@@ -458,18 +530,9 @@ def main():
         "hard": {"clusters": st["n_clusters"], "cluster_particles": st["n_cluster_particles"],
                  "max_cluster": st["max_cluster"], "block_steps": st["block_steps"],
                  "max_block_steps_one_cluster": st["max_block_steps"], "members_of_that_cluster": st["max_block_cluster"], "collisions": st["n_collisions"]},
-        # the kernel with the most time in the step: FP64 Hermite, bound by the chain of
-        # block steps of the longest cluster (one wave), not by the chip's FP64 rate
-        "roofline_hermite": {"bound": "latency (longest cluster's block-step chain)",
-                             "achieved": round(FLOP_HARD_PAIR * st["hard_pairs"] / (st["hard_ms"] * 1e-3) / 1e12, 4),
-                             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                             "frac": round(FLOP_HARD_PAIR * st["hard_pairs"] / (st["hard_ms"] * 1e-3) / 1e12
-                                           / PEAK_FP64_TFLOPS, 6),
-                             "kernel": "k_hermite size classes (hard stage time, HIP events)",
-                             "us_per_block_step_longest_cluster": round(1e3 * st["hard_ms"] / args.steps
-                                                                        / max(1, st["max_block_steps"]), 3)},
         "energy_error_over_timed_steps": de,
         "plain_fp64_per_cluster": plain,
+        "forecast_modes": fc_modes,
         "forecast": {"calls": st["fc_calls"], "slots": st["fc_slots"], "overflowed_groups": st["fc_ovf_groups"],
                      "redone_candidates": st["fc_ovf_candidates"], "loss": st["fc_loss"],
                      "ms_per_step": round(st["fc_ms"] / args.steps, 3)},
@@ -483,9 +546,10 @@ def main():
                          "note": "finalize stage (candidate sort, k_finalize, CSR build) timed as a whole"}
     s.close()
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_threads_default()
+        host = cpu_setup(threads)
         out["soft_force"]["interactions_per_s_cpu_fp32"] = cpu_soft_force(args.n, args.seed, threads, warm_state)
-        out["cpu_baseline"] = cpu_baseline(args.n, args.seed, threads)
+        out["cpu_baseline"] = cpu_baseline(args.n, args.seed, threads, host)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

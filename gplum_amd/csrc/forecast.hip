@@ -79,7 +79,8 @@ __global__ __launch_bounds__(64) void k_fc_capacity(int ng, const int32_t *__res
     if (threadIdx.x == 0) cap[g] = (int32_t)fmin(ceil(s + 3. * sqrt(s) + 8.), 1e9);
 }
 
-// one block: exclusive offsets of the capacities, clipped to the allocated slots
+// one block: exclusive offsets of the capacities (each at most FC_SORT_MAX, the keys a
+// group sorts in LDS), clipped to the allocated slots
 __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *__restrict__ cap,
                                                               int32_t *__restrict__ off, long long slots,
                                                               int64_t *__restrict__ info) {
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *_
     long long carry = 0;
     for (int base = 0; base < ng; base += FC_SCAN_BLOCK) {
         const int g = base + (int)threadIdx.x;
-        const long long v = g < ng ? cap[g] : 0;
+        const long long v = g < ng ? min(cap[g], FC_SORT_MAX) : 0;
         long long tot;
         const long long e = block_excl_scan(v, s_w, &tot);
         if (g < ng) {
@@ -101,42 +102,90 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *_
     if (threadIdx.x == 0) info[0] = carry;
 }
 
-// one block: groups whose exact count fits keep their segment (compacted offsets),
-// the others are listed for the candidate-only pass
+// one block: the final offsets of every group's candidates (exclusive scan of the exact
+// counts: the groups' keys end up in group order, i.e. ascending i); the groups whose
+// count exceeds their segment are listed, with the offsets of their candidates in the
+// overflow area (in list order).  info[1..3] = total, overflowed groups, their total.
+constexpr long long FC_PACK = 1ll << 40;
 __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int32_t *__restrict__ cnt,
                                                               const int32_t *__restrict__ cap,
-                                                              int32_t *__restrict__ comp_off,
+                                                              int32_t *__restrict__ fin_off,
                                                               int32_t *__restrict__ ovf_list,
+                                                              int32_t *__restrict__ ovf_pos,
                                                               int64_t *__restrict__ info) {
     __shared__ long long s_w[FC_SCAN_BLOCK / 64];
-    long long cs = 0, co = 0, ct = 0;
+    long long cs = 0, co = 0;
     for (int base = 0; base < ng; base += FC_SCAN_BLOCK) {
         const int g = base + (int)threadIdx.x;
         const long long c = g < ng ? cnt[g] : 0;
         const bool ovf = g < ng && c > cap[g];
-        long long ts, to, tt;
-        const long long es = block_excl_scan(ovf ? 0 : c, s_w, &ts);
-        const long long eo = block_excl_scan(ovf ? 1 : 0, s_w, &to);
-        (void)block_excl_scan(ovf ? c : 0, s_w, &tt);
+        long long ts, to;
+        const long long es = block_excl_scan(c, s_w, &ts);
+        // overflowed groups and their candidates in one scan (count above bit 40)
+        const long long eo = block_excl_scan(ovf ? FC_PACK + c : 0, s_w, &to);
         if (g < ng) {
-            comp_off[g] = (int32_t)(cs + es);
-            if (ovf) ovf_list[co + eo] = g;
+            fin_off[g] = (int32_t)(cs + es);
+            if (ovf) {
+                const long long q = co + eo;
+                ovf_list[q / FC_PACK] = g;
+                ovf_pos[q / FC_PACK] = (int32_t)(q % FC_PACK);
+            }
         }
-        cs += ts; co += to; ct += tt;
+        cs += ts; co += to;
     }
-    if (threadIdx.x == 0) { info[1] = cs; info[2] = co; info[3] = ct; }
+    if (threadIdx.x == 0) { info[1] = cs; info[2] = co / FC_PACK; info[3] = co % FC_PACK; }
 }
 
-__global__ __launch_bounds__(64) void k_fc_compact(int ng, const int32_t *__restrict__ cnt,
-                                                   const int32_t *__restrict__ cap,
-                                                   const int32_t *__restrict__ seg_off,
-                                                   const int32_t *__restrict__ comp_off,
-                                                   const int2 *__restrict__ seg, int2 *__restrict__ out) {
+// The candidates of a group that fits its segment, as sort keys i 2^kb + j in ascending
+// order (bitonic sort in LDS), at the group's final offset.
+__global__ __launch_bounds__(FC_SORT_BLOCK) void k_fc_sort_keys(int ng, const int32_t *__restrict__ cnt,
+                                                                const int32_t *__restrict__ cap,
+                                                                const int32_t *__restrict__ seg_off,
+                                                                const int32_t *__restrict__ fin_off,
+                                                                const int2 *__restrict__ seg, int kb,
+                                                                uint64_t *__restrict__ keys) {
+    __shared__ uint64_t s_k[FC_SORT_MAX];
     const int g = blockIdx.x;
     if (g >= ng) return;
     const int c = cnt[g];
-    if (c > cap[g]) return;
-    for (int k = (int)threadIdx.x; k < c; k += 64) out[comp_off[g] + k] = seg[seg_off[g] + k];
+    if (c == 0 || c > cap[g]) return;
+    int P = 1;
+    while (P < c) P <<= 1;
+    const int2 *src = seg + seg_off[g];
+    for (int k = (int)threadIdx.x; k < P; k += FC_SORT_BLOCK) {
+        uint64_t v = ~0ull;
+        if (k < c) {
+            const int2 q = src[k];
+            v = ((uint64_t)(uint32_t)q.x << kb) | (uint64_t)(uint32_t)q.y;
+        }
+        s_k[k] = v;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = (int)threadIdx.x; t < P / 2; t += FC_SORT_BLOCK) {
+                const int lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const uint64_t a = s_k[lo], b = s_k[hi];
+                if ((a > b) == up) { s_k[lo] = b; s_k[hi] = a; }
+            }
+            __syncthreads();
+        }
+    uint64_t *dst = keys + fin_off[g];
+    for (int k = (int)threadIdx.x; k < c; k += FC_SORT_BLOCK) dst[k] = s_k[k];
+}
+
+// The overflowed groups' candidates, sorted together (ascending i, so group after group
+// in list order), copied to each group's final offset.
+__global__ void k_fc_scatter_ovf(int n_ovf, const int32_t *__restrict__ ovf_list,
+                                 const int32_t *__restrict__ ovf_pos, const int32_t *__restrict__ cnt,
+                                 const int32_t *__restrict__ fin_off, const uint64_t *__restrict__ sorted,
+                                 uint64_t *__restrict__ keys) {
+    const int k = blockIdx.x;
+    if (k >= n_ovf) return;
+    const int g = ovf_list[k];
+    const int c = cnt[g];
+    for (int t = (int)threadIdx.x; t < c; t += blockDim.x) keys[fin_off[g] + t] = sorted[ovf_pos[k] + t];
 }
 
 __global__ void k_fc_gather_groups(int m, const int32_t *__restrict__ list, const SoftGroup *__restrict__ groups,
@@ -183,7 +232,7 @@ int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hip
     if (s.uniform) {
         hipLaunchKernelGGL(k_fc_fill, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.uniform_rate, s.lam.p);
     } else {
-        int rc = s.net.forward(s.x.p, n, s.lam.p, st);
+        int rc = s.net.predict(s.x.p, n, s.lam.p, st);
         if (rc) return rc;
     }
     if (ng > 0) {
@@ -202,21 +251,30 @@ int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hip
 }
 
 int fc_collect(CandSizer &s, int ng, int64_t out[4], hipStream_t st) {
-    GPLUM_ENSURE(s.comp_off, ng + 1);
+    GPLUM_ENSURE(s.fin_off, ng + 1);
     GPLUM_ENSURE(s.ovf_list, ng + 1);
+    GPLUM_ENSURE(s.ovf_pos, ng + 1);
     if (ng > 0)
         hipLaunchKernelGGL(k_fc_summary, dim3(1), dim3(FC_SCAN_BLOCK), 0, st, ng, s.seg_cnt.p, s.seg_cap.p,
-                           s.comp_off.p, s.ovf_list.p, s.info.p);
+                           s.fin_off.p, s.ovf_list.p, s.ovf_pos.p, s.info.p);
     GPLUM_HIP(hipGetLastError());
     GPLUM_HIP(hipMemcpyAsync(out, s.info.p, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
     return GPLUM_OK;
 }
 
-int fc_compact(CandSizer &s, int ng, int2 *out, hipStream_t st) {
+int fc_sort_keys(CandSizer &s, int ng, int kb, uint64_t *keys, hipStream_t st) {
     if (ng > 0)
-        hipLaunchKernelGGL(k_fc_compact, dim3(ng), dim3(64), 0, st, ng, s.seg_cnt.p, s.seg_cap.p, s.seg_off.p,
-                           s.comp_off.p, s.pairs_seg.p, out);
+        hipLaunchKernelGGL(k_fc_sort_keys, dim3(ng), dim3(FC_SORT_BLOCK), 0, st, ng, s.seg_cnt.p, s.seg_cap.p,
+                           s.seg_off.p, s.fin_off.p, s.pairs_seg.p, kb, keys);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int fc_scatter_overflow(CandSizer &s, int n_ovf, const uint64_t *sorted, uint64_t *keys, hipStream_t st) {
+    if (n_ovf > 0)
+        hipLaunchKernelGGL(k_fc_scatter_ovf, dim3(n_ovf), dim3(256), 0, st, n_ovf, s.ovf_list.p, s.ovf_pos.p,
+                           s.seg_cnt.p, s.fin_off.p, sorted, keys);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }
@@ -230,7 +288,7 @@ int fc_overflow_groups(CandSizer &s, const SoftGroup *groups, int n_ovf, hipStre
     return GPLUM_OK;
 }
 
-int fc_train(CandSizer &s, const int32_t *count_fp32, int n, hipStream_t st) {
+int fc_train(CandSizer &s, const int32_t *count_fp32, int n, hipStream_t st, bool want_loss) {
     if (n <= 0) return GPLUM_OK;
     int rc = s.net.init_params(st, 12345u);
     if (rc) return rc;
@@ -262,12 +320,18 @@ int fc_train(CandSizer &s, const int32_t *count_fp32, int n, hipStream_t st) {
         const long long phase = (long long)((s.phase++ * 2654435761u) % (unsigned)(stride > 0 ? stride : 1));
         hipLaunchKernelGGL(k_fc_sample, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, n, stride, phase, s.x.p,
                            count_fp32, s.xs.p, s.ys.p);
-        double *lp = (k == s.train_steps - 1) ? &s.last_loss : nullptr;
+        double *lp = (want_loss && k == s.train_steps - 1) ? &s.last_loss : nullptr;
         if ((rc = s.net.loss_grad(s.xs.p, s.ys.p, m, lp, st))) return rc;
         if ((rc = s.net.adam_step(st))) return rc;
     }
+    s.last_batch = m;
     s.trained_rounds++;
     return GPLUM_OK;
+}
+
+int fc_read_loss(CandSizer &s, hipStream_t st) {
+    if (s.last_batch <= 0) return GPLUM_OK;
+    return s.net.read_loss(s.last_batch, &s.last_loss, st);
 }
 
 }  // namespace gplum

@@ -14,6 +14,9 @@
 
 namespace gplum {
 
+constexpr int FC_SORT_MAX = 2048;     // a group's segment holds at most this many candidates
+constexpr int FC_SORT_BLOCK = 256;    // (sorted in LDS by one block)
+
 struct FcNorm {
     float mu[MLP_NIN], inv_sd[MLP_NIN];
 };
@@ -30,9 +33,11 @@ struct CandSizer {
     int min_rounds = 1;               // forecast used once this many rounds trained
     unsigned phase = 0;
     double last_loss = 0.;
+    int last_batch = 0;               // samples of the last Adam step (its loss partials stay)
 
     DArray<float> x, lam, xs, ys;     // features [4][n], rates, training batch
-    DArray<int32_t> seg_off, seg_cap, seg_cnt, comp_off, ovf_list;
+    DArray<int32_t> seg_off, seg_cap, seg_cnt, fin_off, ovf_list, ovf_pos;
+    DArray<uint64_t> ovf_keys;        // the overflowed groups' candidates, sorted
     DArray<int64_t> info;             // requested, stored, n_ovf, ovf_total
     DArray<int2> pairs_seg;           // segment storage
     DArray<SoftGroup> ovf_groups;
@@ -53,14 +58,21 @@ int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, 
 // forecast -> per-group capacity -> offsets (clipped to s.seg_capacity); returns the
 // CandBuf for the segment-mode launch
 int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hipStream_t st);
-// after the segment-mode launch: exact counts -> compacted offsets and the list of
-// overflowed groups; out = {requested slots, stored, n_ovf, ovf_total} (synchronises)
+// after the segment-mode launch: exact counts -> final offsets of every group's
+// candidates (group order) and the list of overflowed groups;
+// out = {requested slots, total, n_ovf, ovf_total} (synchronises)
 int fc_collect(CandSizer &s, int ng, int64_t out[4], hipStream_t st);
-// copies the stored segments to out[0..stored)
-int fc_compact(CandSizer &s, int ng, int2 *out, hipStream_t st);
+// the candidates of the groups that fit their segments as sorted keys i 2^kb + j at
+// their final offsets in keys
+int fc_sort_keys(CandSizer &s, int ng, int kb, uint64_t *keys, hipStream_t st);
+// the overflowed groups' candidates (sorted[0..ovf_total), ascending) to their offsets
+int fc_scatter_overflow(CandSizer &s, int n_ovf, const uint64_t *sorted, uint64_t *keys, hipStream_t st);
 // s.ovf_groups[0..n_ovf) = the overflowed groups, for the candidate-only pass
 int fc_overflow_groups(CandSizer &s, const SoftGroup *groups, int n_ovf, hipStream_t st);
-// standardise (first call) and train on this step's exact FP32 candidate counts
-int fc_train(CandSizer &s, const int32_t *count_fp32, int n, hipStream_t st);
+// standardise (first call) and train on this step's exact FP32 candidate counts;
+// want_loss: also evaluate the last step's loss into s.last_loss (synchronises st)
+int fc_train(CandSizer &s, const int32_t *count_fp32, int n, hipStream_t st, bool want_loss);
+// s.last_loss from the loss partials of the last Adam step (synchronises st)
+int fc_read_loss(CandSizer &s, hipStream_t st);
 
 }  // namespace gplum

@@ -162,6 +162,96 @@ __global__ void k_mlp_adam(float *__restrict__ theta, const float *__restrict__ 
     theta[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + 1e-8f);
 }
 
+// ---------------------------------------------- prediction on the matrix cores
+//
+// The rates that size the candidate buffers (fc_plan) need no more than a few digits:
+// the 64x64 hidden layer runs on MFMA (v_mfma_f32_16x16x16bf16_1k: bf16 operands,
+// FP32 accumulation) and tanh comes from the hardware exp and reciprocal.  A wave
+// takes 16 samples per tile: lane l evaluates layer 1 for sample l % 16 and the
+// sixteen hidden units 16 s + 4 (l / 16) + j (s, j = 0..3), which are exactly its A
+// operands (A[i][k] = h1 of sample i, unit k) of the four K-steps; W2 is held as the B
+// operands (B[k][n] = W2[n][k]) for the four 16-column output tiles; the accumulators
+// (D[i][n], i = 4 (l / 16) + j, n = 16 t + l % 16) start at b2, and the output layer
+// sums w3 h2 over the 16 lanes of a row.  Agrees with k_mlp_forward to bf16 precision.
+typedef short mlp_v4s __attribute__((ext_vector_type(4)));
+typedef float mlp_v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline short bf16_rn(float x) {                // round to nearest even
+    const unsigned u = __float_as_uint(x);
+    return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float tanh_fast(float a) {               // 1 - 2 / (1 + e^(2a))
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * a));
+}
+
+constexpr int MLP_PRED_BLOCK = 256;
+
+__global__ __launch_bounds__(MLP_PRED_BLOCK) void k_mlp_predict(const float *__restrict__ s,
+                                                                const float *__restrict__ x, int n,
+                                                                float *__restrict__ lam) {
+    const int l = (int)threadIdx.x & 63;
+    const int r = l >> 4, c = l & 15;
+    // this lane's layer-1 units, W2 as B operands, b2 / w3 of its output columns
+    float w1[16][MLP_NIN], b1[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int k = 16 * (u >> 2) + 4 * r + (u & 3);
+        b1[u] = s[OFF_B1 + k];
+#pragma unroll
+        for (int q = 0; q < MLP_NIN; q++) w1[u][q] = s[OFF_W1 + k * MLP_NIN + q];
+    }
+    mlp_v4s bw[4][4];
+    float b2[4], w3[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int nn = 16 * t + c;
+        b2[t] = s[OFF_B2 + nn];
+        w3[t] = s[OFF_W3 + nn];
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) bw[t][ks][j] = bf16_rn(s[OFF_W2 + nn * MLP_NH + 16 * ks + 4 * r + j]);
+    }
+    const float b3 = s[OFF_B3];
+    const int waves = (int)(gridDim.x * (MLP_PRED_BLOCK / 64));
+    const int tiles = (n + 15) / 16;
+    for (int tile = (int)blockIdx.x * (MLP_PRED_BLOCK / 64) + ((int)threadIdx.x >> 6); tile < tiles;
+         tile += waves) {
+        const int i = 16 * tile + c;
+        float xi[MLP_NIN];
+#pragma unroll
+        for (int q = 0; q < MLP_NIN; q++) xi[q] = (i < n) ? x[(size_t)q * n + i] : 0.f;
+        mlp_v4s a[4];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            float z = b1[u];
+#pragma unroll
+            for (int q = 0; q < MLP_NIN; q++) z = __builtin_fmaf(w1[u][q], xi[q], z);
+            a[u >> 2][u & 3] = bf16_rn(tanh_fast(z));
+        }
+        float zp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            mlp_v4f acc = {b2[t], b2[t], b2[t], b2[t]};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ks], bw[t][ks], acc, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) zp[j] = __builtin_fmaf(w3[t], tanh_fast(acc[j]), zp[j]);
+        }
+        // sum over the 16 output columns of the row (lanes 16 r .. 16 r + 15)
+#pragma unroll
+        for (int d = 8; d > 0; d >>= 1)
+#pragma unroll
+            for (int j = 0; j < 4; j++) zp[j] += __shfl_xor(zp[j], d);
+        // lane c < 4 of row r writes sample 4 r + c of the tile
+        const int o = 16 * tile + 4 * r + c;
+        float zo = zp[0];
+#pragma unroll
+        for (int j = 1; j < 4; j++) zo = (c == j) ? zp[j] : zo;
+        if (c < 4 && o < n) lam[o] = softplus_f(zo + b3);
+    }
+}
+
 // ------------------------------------------------------------------ host
 
 int Forecaster::ensure(int nsamp) {
@@ -210,6 +300,15 @@ int Forecaster::forward(const float *d_x, int n, float *d_lam, hipStream_t st) {
     return GPLUM_OK;
 }
 
+int Forecaster::predict(const float *d_x, int n, float *d_lam, hipStream_t st) {
+    if (n <= 0) return GPLUM_OK;
+    const int tiles = ceil_div(n, 16);
+    const int blocks = std::min(ceil_div(tiles, MLP_PRED_BLOCK / 64), 2048);
+    hipLaunchKernelGGL(k_mlp_predict, dim3(blocks), dim3(MLP_PRED_BLOCK), 0, st, theta.p, d_x, n, d_lam);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
 int Forecaster::loss_grad(const float *d_x, const float *d_y, int n, double *loss, hipStream_t st) {
     int rc = ensure(n);
     if (rc) return rc;
@@ -221,14 +320,18 @@ int Forecaster::loss_grad(const float *d_x, const float *d_y, int n, double *los
                        H2.p, D1.p, D2.p, DZ.p, part.p);
     hipLaunchKernelGGL(k_mlp_grad_sum, dim3(ceil_div(MLP_NPARAM, 256)), dim3(256), 0, st, nchunk, part.p, grad.p);
     GPLUM_HIP(hipGetLastError());
-    if (loss) {
-        std::vector<double> h(nb);
-        GPLUM_HIP(hipMemcpyAsync(h.data(), loss_part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipStreamSynchronize(st));
-        double s = 0.;
-        for (double x : h) s += x;
-        *loss = s / n;
-    }
+    if (loss) return read_loss(n, loss, st);
+    return GPLUM_OK;
+}
+
+int Forecaster::read_loss(int n, double *loss, hipStream_t st) {
+    const int nb = ceil_div(n, 256);
+    std::vector<double> h(nb);
+    GPLUM_HIP(hipMemcpyAsync(h.data(), loss_part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    double s = 0.;
+    for (double x : h) s += x;
+    *loss = s / n;
     return GPLUM_OK;
 }
 
@@ -271,6 +374,22 @@ extern "C" int gplum_mlp_forward(const float *theta, const float *x, int n, floa
     GPLUM_HIP(hipMemcpyAsync(c.f.theta.p, theta, sizeof(float) * gplum::MLP_NPARAM, hipMemcpyHostToDevice, c.st));
     GPLUM_HIP(hipMemcpyAsync(c.x.p, x, sizeof(float) * 4 * n, hipMemcpyHostToDevice, c.st));
     if ((rc = c.f.forward(c.x.p, n, c.lam.p, c.st))) return rc;
+    GPLUM_HIP(hipMemcpyAsync(lam, c.lam.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.st));
+    GPLUM_HIP(hipStreamSynchronize(c.st));
+    return GPLUM_OK;
+}
+
+extern "C" int gplum_mlp_predict(const float *theta, const float *x, int n, float *lam) {
+    if (!theta || (n > 0 && (!x || !lam)) || n < 0) return gplum::fail(GPLUM_E_ARG, "bad arguments");
+    MlpCtx &c = t_mlp;
+    int rc = mlp_ctx_init(c);
+    if (rc) return rc;
+    if ((rc = c.f.ensure(1))) return rc;
+    GPLUM_ENSURE(c.x, 4 * (size_t)n + 4);
+    GPLUM_ENSURE(c.lam, (size_t)n + 1);
+    GPLUM_HIP(hipMemcpyAsync(c.f.theta.p, theta, sizeof(float) * gplum::MLP_NPARAM, hipMemcpyHostToDevice, c.st));
+    GPLUM_HIP(hipMemcpyAsync(c.x.p, x, sizeof(float) * 4 * n, hipMemcpyHostToDevice, c.st));
+    if ((rc = c.f.predict(c.x.p, n, c.lam.p, c.st))) return rc;
     GPLUM_HIP(hipMemcpyAsync(lam, c.lam.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.st));
     GPLUM_HIP(hipStreamSynchronize(c.st));
     return GPLUM_OK;

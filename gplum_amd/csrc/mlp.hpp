@@ -21,8 +21,13 @@ struct Forecaster {
     int init_params(hipStream_t st, unsigned seed);
     // lam[i] = softplus(MLP(x[:, i])); x is feature-major [4][n]
     int forward(const float *d_x, int n, float *d_lam, hipStream_t st);
+    // the same rates with the hidden layer on the matrix cores (bf16 operands, FP32
+    // accumulation) and hardware tanh: for sizing, not for training
+    int predict(const float *d_x, int n, float *d_lam, hipStream_t st);
     // fills grad with d(mean Poisson NLL)/d theta; loss (optional) synchronises the stream
     int loss_grad(const float *d_x, const float *d_y, int n, double *loss, hipStream_t st);
+    // the mean loss of the last loss_grad call over its n samples (synchronises st)
+    int read_loss(int n, double *loss, hipStream_t st);
     int adam_step(hipStream_t st);
 };
 

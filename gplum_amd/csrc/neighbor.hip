@@ -101,14 +101,16 @@ __device__ inline double calc_dt2(double eta, double alpha2, double acc0, const 
     return (Jerk2 > 0.) ? eta * sqrt(Acc2 / Jerk2) : DBL_MAX;
 }
 
-__global__ void k_cand_keys(const int2 *__restrict__ cand, int nc, uint64_t *__restrict__ key) {
+// candidate (i, j) -> sort key i 2^kb + j (kb bits hold any particle index)
+__global__ void k_cand_keys(const int2 *__restrict__ cand, int nc, int kb, uint64_t *__restrict__ key) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int2 q = cand[c];
-    key[c] = ((uint64_t)(uint32_t)q.x << 32) | (uint64_t)(uint32_t)q.y;
+    key[c] = ((uint64_t)(uint32_t)q.x << kb) | (uint64_t)(uint32_t)q.y;
 }
 
-__global__ void k_finalize(NbParams P, PView V, int n, int ncand, int initial, const int32_t *__restrict__ row_off,
+__global__ void k_finalize(NbParams P, PView V, int n, int ncand, int initial, uint64_t jmask,
+                           const int32_t *__restrict__ row_off,
                            const int32_t *__restrict__ number_fp32, const uint64_t *__restrict__ keys,
                            unsigned char *__restrict__ accepted, double *__restrict__ acc,
                            double *__restrict__ phi, double *__restrict__ acc0, int32_t *__restrict__ number,
@@ -129,7 +131,7 @@ __global__ void k_finalize(NbParams P, PView V, int n, int ncand, int initial, c
     // error the host reports after the stage)
     const int b = row_off[i], e = min(b + number_fp32[i], ncand);
     for (int c = b; c < e; c++) {
-        const int j = (int)(uint32_t)(keys[c] & 0xffffffffULL);
+        const int j = (int)(keys[c] & jmask);
         bool acc_flag = false;
         if (V.id[j] != V.id[i]) acc_flag = correct_pair(P, V, i, j, initial != 0, a);
         accepted[c] = acc_flag ? 1 : 0;
@@ -181,7 +183,7 @@ __global__ void k_finalize(NbParams P, PView V, int n, int ncand, int initial, c
     }
 }
 
-__global__ void k_compact_rows(int n, int ncand, const int32_t *__restrict__ row_off,
+__global__ void k_compact_rows(int n, int ncand, uint64_t jmask, const int32_t *__restrict__ row_off,
                                const int32_t *__restrict__ number_fp32, const uint64_t *__restrict__ keys,
                                const unsigned char *__restrict__ accepted, const int32_t *__restrict__ nbr_off,
                                int32_t *__restrict__ nbr_idx) {
@@ -190,7 +192,7 @@ __global__ void k_compact_rows(int n, int ncand, const int32_t *__restrict__ row
     int w = nbr_off[i];
     const int b = row_off[i], e = min(b + number_fp32[i], ncand);
     for (int c = b; c < e; c++)
-        if (accepted[c]) nbr_idx[w++] = (int)(uint32_t)(keys[c] & 0xffffffffULL);
+        if (accepted[c]) nbr_idx[w++] = (int)(keys[c] & jmask);
 }
 
 // ---------------------------------------------------------- union-find
@@ -351,21 +353,36 @@ NbParams make_nb_params(const gplum_params &p) {
     return q;
 }
 
+int cand_key_bits(int n) {
+    int kb = 1;
+    while (kb < 31 && (1ll << kb) < (long long)n) kb++;
+    return kb;
+}
+
+int cand_sort(const int2 *d_cand, int ncand, int kb, DArray<uint64_t> &keys_in, uint64_t *keys_out, Scratch &s,
+              hipStream_t st) {
+    if (ncand <= 0) return GPLUM_OK;
+    GPLUM_ENSURE(keys_in, ncand + 1);
+    hipLaunchKernelGGL(k_cand_keys, dim3(ceil_div(ncand, 256)), dim3(256), 0, st, d_cand, ncand, kb, keys_in.p);
+    size_t bytes = 0;
+    GPLUM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys_in.p, keys_out, ncand, 0, 2 * kb, st));
+    GPLUM_ENSURE(s.tmp, bytes + 16);
+    GPLUM_HIP(hipcub::DeviceRadixSort::SortKeys(s.tmp.p, bytes, keys_in.p, keys_out, ncand, 0, 2 * kb, st));
+    return GPLUM_OK;
+}
+
 int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 *d_cand, int ncand,
-                      bool initial, NbState &nb, Scratch &s, hipStream_t st) {
-    GPLUM_ENSURE(nb.keys_in, ncand + 1);
+                      bool initial, NbState &nb, Scratch &s, hipStream_t st, bool presorted) {
     GPLUM_ENSURE(nb.keys, ncand + 1);
     GPLUM_ENSURE(nb.accepted, ncand + 1);
     GPLUM_ENSURE(nb.row_off, n + 1);
     GPLUM_ENSURE(nb.nbr_off, n + 1);
     GPLUM_ENSURE(nb.acc_d_new, 3 * (size_t)n);
-    if (ncand > 0) {
-        hipLaunchKernelGGL(k_cand_keys, dim3(ceil_div(ncand, 256)), dim3(256), 0, st, d_cand, ncand,
-                           nb.keys_in.p);
-        size_t bytes = 0;
-        GPLUM_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, nb.keys_in.p, nb.keys.p, ncand, 0, 64, st));
-        GPLUM_ENSURE(s.tmp, bytes + 16);
-        GPLUM_HIP(hipcub::DeviceRadixSort::SortKeys(s.tmp.p, bytes, nb.keys_in.p, nb.keys.p, ncand, 0, 64, st));
+    const int kb = cand_key_bits(n);
+    const uint64_t jmask = (1ull << kb) - 1ull;
+    if (!presorted) {
+        int rc0 = cand_sort(d_cand, ncand, kb, nb.keys_in, nb.keys.p, s, st);
+        if (rc0) return rc0;
     }
     int rc = nb_scan(s, ps.number.p, nb.row_off.p, n, st);
     if (rc) return rc;
@@ -375,7 +392,7 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
     NbParams P = make_nb_params(p);
     PView V{ps.pos.p, ps.vel.p, ps.acc_d.p, ps.mass.p, ps.r_out.p, ps.r_out_inv.p, ps.r_search.p, ps.id.p};
     hipLaunchKernelGGL(k_finalize, dim3(ceil_div(n, 128)), dim3(128), 0, st, P, V, n, ncand, initial ? 1 : 0,
-                       nb.row_off.p, nb.number_fp32.p, nb.keys.p, nb.accepted.p, ps.acc.p, ps.phi.p,
+                       jmask, nb.row_off.p, nb.number_fp32.p, nb.keys.p, nb.accepted.p, ps.acc.p, ps.phi.p,
                        ps.acc0.p, ps.number.p, ps.id_cluster.p, nb.acc_d_new.p, ps.jerk_d.p, ps.phi_d.p,
                        ps.acc_s.p, ps.jerk_s.p, ps.phi_s.p, ps.dt.p, ps.time.p);
     if (initial)
@@ -387,7 +404,7 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
     hipLaunchKernelGGL(k_finalize_totals, dim3(1), dim3(64), 0, st, n, ncand, nb.number_fp32.p, nb.row_off.p,
                        ps.number.p, nb.nbr_off.p, nb.info.p);
     GPLUM_ENSURE(nb.nbr_idx, (size_t)ncand + 1);
-    hipLaunchKernelGGL(k_compact_rows, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ncand, nb.row_off.p,
+    hipLaunchKernelGGL(k_compact_rows, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ncand, jmask, nb.row_off.p,
                        nb.number_fp32.p, nb.keys.p, nb.accepted.p, nb.nbr_off.p, nb.nbr_idx.p);
     GPLUM_HIP(hipGetLastError());
     int32_t info[2] = {0, 0};

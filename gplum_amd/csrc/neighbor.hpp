@@ -35,11 +35,18 @@ struct ClusterList {
 
 NbParams make_nb_params(const gplum_params &p);
 
+// Candidate sort keys i 2^kb + j: kb = cand_key_bits(n) bits hold a particle index.
+int cand_key_bits(int n);
+// keys of d_cand[0..ncand) sorted ascending (radix sort of their 2 kb bits) into keys_out
+int cand_sort(const int2 *d_cand, int ncand, int kb, DArray<uint64_t> &keys_in, uint64_t *keys_out, Scratch &s,
+              hipStream_t st);
+
 // correctForceLong[Initial] over the candidate pairs d_cand[0..ncand) emitted by the
 // FP32 EP-EP kernel; ps.number holds the FP32 count on entry and the FP64 count on
-// exit.  Fills nb (accepted-neighbour CSR).
+// exit.  Fills nb (accepted-neighbour CSR).  presorted: nb.keys[0..ncand) already holds
+// the sorted keys (d_cand is not read).
 int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 *d_cand, int ncand,
-                      bool initial, NbState &nb, Scratch &s, hipStream_t st);
+                      bool initial, NbState &nb, Scratch &s, hipStream_t st, bool presorted = false);
 
 // createNeighborCluster + makeList: id_cluster = min id of the component; clusters
 // of the particles with neighbours (members by ascending id), offsets cl.off[0..ncl],

@@ -217,6 +217,10 @@ System::~System() {
     if (log_col) std::fclose(log_col);
     if (st) (void)hipStreamDestroy(st);
     if (st2) (void)hipStreamDestroy(st2);
+    if (st_fc) (void)hipStreamSynchronize(st_fc);
+    if (st_fc) (void)hipStreamDestroy(st_fc);
+    for (hipEvent_t e : {ev_fc_ready, ev_fc_done, ev_fc_t0, ev_fc_t1})
+        if (e) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (auto &e : ev) if (e) (void)hipEventDestroy(e);
@@ -234,6 +238,11 @@ int System::init_device() {
     n_cu = prop.multiProcessorCount;
     GPLUM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     GPLUM_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    GPLUM_HIP(hipStreamCreateWithFlags(&st_fc, hipStreamNonBlocking));
+    GPLUM_HIP(hipEventCreateWithFlags(&ev_fc_ready, hipEventDisableTiming));
+    GPLUM_HIP(hipEventCreateWithFlags(&ev_fc_done, hipEventDisableTiming));
+    GPLUM_HIP(hipEventCreate(&ev_fc_t0));
+    GPLUM_HIP(hipEventCreate(&ev_fc_t1));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     for (auto &e : ev) GPLUM_HIP(hipEventCreate(&e));
@@ -369,9 +378,28 @@ double System::elapsed(int a, int b) {
     return 0.;
 }
 
+int System::fc_join() {
+    if (!fc_pending) return GPLUM_OK;
+    GPLUM_HIP(hipStreamWaitEvent(st, ev_fc_done, 0));
+    fc_pending = false;
+    if (fc_timed) {
+        float ms = 0.f;
+        if (hipEventSynchronize(ev_fc_t1) == hipSuccess && hipEventElapsedTime(&ms, ev_fc_t0, ev_fc_t1) == hipSuccess)
+            stats.fc_train_ms += ms;
+        fc_timed = false;
+    }
+    return GPLUM_OK;
+}
+
+int System::fc_sync() {
+    if (st_fc) GPLUM_HIP(hipStreamSynchronize(st_fc));
+    return fc_join();
+}
+
 // tree + FP32 soft force + FP64 correction (correctForceLong[Initial])
 int System::soft_force(bool initial, bool use_f64) {
     int rc;
+    if ((rc = fc_join())) return rc;
     tick(EV_TREE0);
     if ((rc = reorder())) return rc;
     if ((rc = tree_build_nodes(tree, p, A.pos.p, A.mass.p, A.r_search.p, n, st))) return rc;
@@ -394,6 +422,7 @@ int System::soft_force(bool initial, bool use_f64) {
     SinkEngine out{A.acc.p, A.phi.p, A.number.p, A.rank.p, A.id_max.p, A.id_min.p};
     GPLUM_ENSURE(cand_count, 1);
     unsigned ncand = 0;
+    bool presorted = false;            // nb.keys already holds the sorted candidate keys
     const bool fc_on = fc.enabled && !use_f64;
     const FcGroupView gv{tree.group_start.p, tree.group_count.p, tree.group_node.p, tree.ep_cnt.p,
                          tree.node_bmin.p,   tree.node_bmax.p,    tree.node_rs.p};
@@ -420,15 +449,19 @@ int System::soft_force(bool initial, bool use_f64) {
         tick(EV_FCB0);
         int64_t info[4] = {0, 0, 0, 0};
         if ((rc = fc_collect(fc, ng, info, st))) return rc;
-        const int64_t requested = info[0], stored = info[1], n_ovf = info[2], ovf_total = info[3];
-        if (stored + ovf_total > (int64_t)INT32_MAX) return fail(GPLUM_E_OVERFLOW, "candidate count exceeds 2^31");
-        GPLUM_ENSURE(cand, (size_t)(stored + ovf_total) + 1);
-        if ((rc = fc_compact(fc, ng, cand.p, st))) return rc;
+        const int64_t requested = info[0], total = info[1], n_ovf = info[2], ovf_total = info[3];
+        if (total > (int64_t)INT32_MAX) return fail(GPLUM_E_OVERFLOW, "candidate count exceeds 2^31");
+        // the candidates arrive grouped by i-group (ascending i): each group's segment is
+        // sorted in LDS straight into the finalize's key array, so the global sort of
+        // the other modes is not needed
+        const int kb = cand_key_bits(n);
+        GPLUM_ENSURE(nb.keys, (size_t)total + 1);
+        if ((rc = fc_sort_keys(fc, ng, kb, nb.keys.p, st))) return rc;
         if (n_ovf > 0) {
             if ((rc = fc_overflow_groups(fc, groups.p, (int)n_ovf, st))) return rc;
+            GPLUM_ENSURE(cand, (size_t)ovf_total + 1);
             GPLUM_HIP(hipMemsetAsync(cand_count.p, 0, sizeof(unsigned), st));
             CandBuf cb2{cand.p, cand_count.p, (unsigned)ovf_total};
-            cb2.base = (unsigned)stored;
             launch_soft_f32<SrcRecI, SrcRecJ, SrcNodeSP, SinkDiscard, EP | EMIT | NOFAR>(
                 fc.ovf_groups.p, (int)n_ovf, is, jn, jf, sp, SinkDiscard{}, (float)p.eps2, cb2, st, soft_tile);
             unsigned got = 0;
@@ -437,9 +470,13 @@ int System::soft_force(bool initial, bool use_f64) {
             if ((int64_t)got != ovf_total)
                 return fail(GPLUM_E_STATE, "candidate-only pass found " + std::to_string(got) + " of " +
                                                std::to_string(ovf_total) + " candidates");
+            GPLUM_ENSURE(fc.ovf_keys, (size_t)ovf_total + 1);
+            if ((rc = cand_sort(cand.p, (int)ovf_total, kb, nb.keys_in, fc.ovf_keys.p, scratch, st))) return rc;
+            if ((rc = fc_scatter_overflow(fc, (int)n_ovf, fc.ovf_keys.p, nb.keys.p, st))) return rc;
         }
         tick(EV_FCB1);
-        ncand = (unsigned)(stored + ovf_total);
+        ncand = (unsigned)total;
+        presorted = true;
         stats.fc_calls++;
         stats.fc_slots += std::min<int64_t>(requested, (int64_t)fc.seg_capacity);
         stats.fc_ovf_groups += n_ovf;
@@ -487,17 +524,20 @@ int System::soft_force(bool initial, bool use_f64) {
     if (use_f64) stats.n_force_f64_calls++;
     last_ncand = ncand;
     tick(EV_NB0);
-    if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st))) return rc;
+    if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st, presorted))) return rc;
     tick(EV_NB1);
     stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
     if (fc_on && !fc.uniform && !tuning && !skip_train) {
-        // train on this call's exact FP32 candidate counts
-        tick(EV_FCC0);
-        if ((rc = fc_train(fc, nb.number_fp32.p, n, st))) return rc;
-        tick(EV_FCC1);
-        GPLUM_HIP(hipStreamSynchronize(st));
-        stats.fc_ms += elapsed(EV_FCC0, EV_FCC1);
-        stats.fc_loss = fc.last_loss;
+        // train on this call's exact FP32 candidate counts, on st_fc: the training
+        // overlaps what follows (kick, merging, the next hard part)
+        GPLUM_HIP(hipEventRecord(ev_fc_ready, st));
+        GPLUM_HIP(hipStreamWaitEvent(st_fc, ev_fc_ready, 0));
+        if (timing) GPLUM_HIP(hipEventRecord(ev_fc_t0, st_fc));
+        if ((rc = fc_train(fc, nb.number_fp32.p, n, st_fc, false))) return rc;
+        if (timing) GPLUM_HIP(hipEventRecord(ev_fc_t1, st_fc));
+        GPLUM_HIP(hipEventRecord(ev_fc_done, st_fc));
+        fc_pending = true;
+        fc_timed = timing;
     }
     have_nbr = true;
     // the interaction counts stay on the device (pair_cnt) until the statistics are read
@@ -576,9 +616,11 @@ int System::hard_part(bool persistent) {
         PartPtrs P = ptrs(A);
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
         if (persistent) {
+            tick(EV_HK0);
             if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join,
                                   hard_wg_per_cu, hard_lds_max, hard_giant_min, hard_terms_max)))
                 return rc;
+            tick(EV_HK1);
         } else {
             // the comparator: one launch per cluster
             GPLUM_ENSURE(hb.next, 3);
@@ -623,6 +665,10 @@ int System::hard_part(bool persistent) {
     tick(EV_HARD1);
     GPLUM_HIP(hipStreamSynchronize(st));
     stats.hard_ms += elapsed(EV_HARD0, EV_HARD1);
+    if (ncl > 0 && persistent) {
+        stats.hard_kernel_ms += elapsed(EV_HK0, EV_HK1);
+        stats.n_hard_launches++;
+    }
     return GPLUM_OK;
 }
 
@@ -1092,6 +1138,10 @@ int gplum_system_hard_part(gplum_system *s, int persistent) {
 int gplum_system_stats(gplum_system *s, gplum_stats *out) {
     if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
+    int rc = S->fc_sync();
+    if (rc) return rc;
+    if (S->fc.trained_rounds > 0 && (rc = gplum::fc_read_loss(S->fc, S->st_fc))) return rc;
+    S->stats.fc_loss = S->fc.last_loss;
     *out = S->stats;
     unsigned long long pc[3] = {0ull, 0ull, 0ull};
     if (S->pair_cnt.p) {
@@ -1143,6 +1193,8 @@ int gplum_system_set_event_logs(gplum_system *s, const char *collision_path, con
 int gplum_system_reset_stats(gplum_system *s, int timing) {
     if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
     System *S = reinterpret_cast<System *>(s);
+    int rc = S->fc_sync();
+    if (rc) return rc;
     std::memset(&S->stats, 0, sizeof(S->stats));
     S->timing = timing != 0;
     return GPLUM_OK;
@@ -1195,6 +1247,8 @@ int gplum_system_set_forecast(gplum_system *s, int mode, int train_steps, int ba
     if (!s || mode < 0 || mode > 2 || train_steps < 0 || batch <= 0 || !(lr > 0.f))
         return gplum::fail(GPLUM_E_ARG, "bad arguments");
     System *S = reinterpret_cast<System *>(s);
+    int rc = S->fc_sync();
+    if (rc) return rc;
     S->fc.enabled = mode != 0;
     S->fc.uniform = mode == 2;
     S->fc.train_steps = train_steps;
@@ -1206,6 +1260,8 @@ int gplum_system_set_forecast(gplum_system *s, int mode, int train_steps, int ba
 int gplum_system_get_forecast(gplum_system *s, float *theta, float *mu, float *inv_sd, int *trained) {
     if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
     System *S = reinterpret_cast<System *>(s);
+    int rc = S->fc_sync();
+    if (rc) return rc;
     if (trained) *trained = S->fc.trained_rounds;
     if (theta) {
         if (!S->fc.net.initialized) return gplum::fail(GPLUM_E_STATE, "forecaster has not been trained");
@@ -1221,8 +1277,9 @@ int gplum_system_get_forecast(gplum_system *s, float *theta, float *mu, float *i
 int gplum_system_put_forecast(gplum_system *s, const float *theta, const float *mu, const float *inv_sd) {
     if (!s || !theta || !mu || !inv_sd) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
-    int rc = S->fc.net.ensure(1);
+    int rc = S->fc_sync();
     if (rc) return rc;
+    if ((rc = S->fc.net.ensure(1))) return rc;
     GPLUM_HIP(hipMemcpy(S->fc.net.theta.p, theta, sizeof(float) * gplum::MLP_NPARAM, hipMemcpyHostToDevice));
     GPLUM_HIP(hipMemset(S->fc.net.m.p, 0, sizeof(float) * gplum::MLP_NPARAM));
     GPLUM_HIP(hipMemset(S->fc.net.v.p, 0, sizeof(float) * gplum::MLP_NPARAM));

@@ -22,7 +22,7 @@ struct HardPtrs {
 
 enum {
     EV_TREE0, EV_TREE1, EV_FORCE0, EV_FORCE1, EV_NB0, EV_NB1, EV_HARD0, EV_HARD1,
-    EV_FCA0, EV_FCA1, EV_FCB0, EV_FCB1, EV_FCC0, EV_FCC1, EV_COUNT
+    EV_FCA0, EV_FCA1, EV_FCB0, EV_FCB1, EV_FCC0, EV_FCC1, EV_HK0, EV_HK1, EV_COUNT
 };
 
 struct RadiiScratch {                // setCutoffRadii on the device (radii.hip)
@@ -51,6 +51,11 @@ struct System {
     gplum_params p;
     hipStream_t st = nullptr, st2 = nullptr;   // st2: large clusters of the hard part
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the forecaster's training runs on st_fc, overlapping the next step's hard part;
+    // the next soft force (which rewrites its inputs and reads the network) waits for it
+    hipStream_t st_fc = nullptr;
+    hipEvent_t ev_fc_ready = nullptr, ev_fc_done = nullptr, ev_fc_t0 = nullptr, ev_fc_t1 = nullptr;
+    bool fc_pending = false, fc_timed = false;
     hipEvent_t ev[EV_COUNT] = {};
     int n_cu = 256;
     int n = 0;
@@ -102,6 +107,8 @@ struct System {
     int reorder();
     int gather_all(const int32_t *perm, int m);
     int soft_force(bool initial, bool use_f64);
+    int fc_join();                      // st waits for the pending training (device side)
+    int fc_sync();                      // the host waits for it
     int energy(double out[5]);
     int initialize();
     int kick();

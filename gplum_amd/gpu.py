@@ -79,6 +79,7 @@ def lib():
                                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(f64),
                                            ctypes.POINTER(ctypes.c_int64)]
         L.gplum_mlp_forward.argtypes = [vp, vp, i32, vp]
+        L.gplum_mlp_predict.argtypes = [vp, vp, i32, vp]
         L.gplum_mlp_loss_grad.argtypes = [vp, vp, vp, i32, ctypes.POINTER(f64), vp]
         L.gplum_mlp_train.argtypes = [vp, vp, vp, i32, i32, f32, ctypes.POINTER(f64),
                                       ctypes.POINTER(f64)]
@@ -192,7 +193,8 @@ class Stats(ctypes.Structure):
         "cand_retries", "n_force_calls", "fc_calls", "fc_slots", "fc_ovf_groups",
         "fc_ovf_candidates", "hard_pairs", "max_block_steps", "max_block_cluster", "n_force_f64_calls",
         "n_removed", "n_recomputes")] + [(n, ctypes.c_double) for n in (
-            "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms")]
+            "force_ms", "tree_ms", "finalize_ms", "hard_ms", "time_sys", "fc_loss", "fc_ms",
+            "hard_kernel_ms")] + [("n_hard_launches", ctypes.c_int64), ("fc_train_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -424,6 +426,15 @@ def mlp_forward(theta, x):
     n = xt.shape[1]
     out = np.zeros(n, np.float32)
     check(lib().gplum_mlp_forward(abi.ptr(theta), abi.ptr(xt), n, abi.ptr(out)))
+    return out
+
+
+def mlp_predict(theta, x):
+    """The sizing forecast: mlp_forward's rates with the hidden layer on MFMA (bf16)."""
+    theta, xt = _f32(theta), _f32(np.asarray(x).T)
+    n = xt.shape[1]
+    out = np.zeros(n, np.float32)
+    check(lib().gplum_mlp_predict(abi.ptr(theta), abi.ptr(xt), n, abi.ptr(out)))
     return out
 
 

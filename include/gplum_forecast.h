@@ -26,6 +26,12 @@ extern "C" {
 /* rate[i] = softplus(MLP(x[:, i])) for n samples.  Host pointers; synchronous. */
 int gplum_mlp_forward(const float *theta, const float *x, int n, float *rate);
 
+/* The rates of gplum_mlp_forward with the 64x64 hidden layer on the matrix cores
+ * (MFMA, bf16 operands, FP32 accumulation) and hardware tanh: the forecast the
+ * buffer sizing uses (within about 1e-2 relative of gplum_mlp_forward).  Host
+ * pointers; synchronous. */
+int gplum_mlp_predict(const float *theta, const float *x, int n, float *rate);
+
 /* Mean Poisson NLL over n samples (targets y) and its gradient with respect to theta
  * (grad[4545], float32; summed over samples in FP64, deterministic).  loss may be
  * null.  Host pointers; synchronous. */

@@ -60,7 +60,12 @@ typedef struct gplum_stats {
     double force_ms, tree_ms, finalize_ms, hard_ms;
     double time_sys;
     double fc_loss;                           /* last training loss (Poisson NLL)          */
-    double fc_ms;                             /* features, forecast, fallback and training */
+    double fc_ms;                             /* features, forecast and fallback           */
+    double hard_kernel_ms;                    /* the batched Hermite kernels (one launch set
+                                                 per hard part), HIP events on their stream */
+    int64_t n_hard_launches;                  /* launch sets timed in hard_kernel_ms       */
+    double fc_train_ms;                       /* the forecaster's training (its own stream,
+                                                 overlapping the next hard part)          */
 } gplum_stats;
 
 typedef struct gplum_system gplum_system;

@@ -20,6 +20,21 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_native_build():
+    """Load the oracle built on this host with -O3 -march=native (oracle/_build_native;
+    for the CPU baseline timing) instead of the portable build.  Must be called before
+    the first use; returns False (portable build kept) if the native build fails."""
+    global _LIB_PATH
+    if _lib is not None:
+        return _LIB_PATH.endswith(os.path.join("_build_native", "liboracle.so"))
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, timeout=300)
+    except (OSError, subprocess.SubprocessError):
+        return False
+    _LIB_PATH = os.path.join(_HERE, "_build_native", "liboracle.so")
+    return True
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -38,6 +38,24 @@ def test_forward_and_gradient_match_oracle(n):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("n", [1, 300, 100003])
+def test_predict_matches_forward_to_bf16(n):
+    """The sizing forecast (hidden layer on MFMA with bf16 operands, hardware tanh) against
+    the FP64 oracle's rates, for the initial network and one trained on a rate law:
+    bf16 keeps 8 bits, so rates agree to about 1 % (bars: median 1e-2, max 5e-2)."""
+    x, y, _ = synthetic(n, seed=n + 7)
+    xt, yt, _ = synthetic(20000, seed=9)
+    theta0 = omlp.init_params(seed=3).astype(np.float32)
+    trained, _, _ = gpu.mlp_train(theta0, xt, yt, 150, lr=3e-3)
+    for theta in (theta0, trained):
+        th = np.asarray(theta, np.float32).astype(np.float64)
+        lam_p = gpu.mlp_predict(th, x)
+        lam_o = omlp.forward(th, x)
+        rel = np.abs(lam_p - lam_o) / lam_o
+        print("n %d: predict vs oracle relative error median %.2e max %.2e" % (n, np.median(rel), rel.max()))
+        assert np.median(rel) < 1e-2 and rel.max() < 5e-2
+
+
 def test_gradient_is_deterministic():
     x, y, _ = synthetic(20000, seed=11)
     theta = omlp.init_params(seed=5)
