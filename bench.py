@@ -493,7 +493,9 @@ def main():
     # from the warmed-up state (one untimed step, then the timed ones)
     fc_modes = None
     if args.mode_steps > 0:
-        fc_modes = {"network": round(ms_per_step, 3)}
+        fc_modes = {"network": round(ms_per_step, 3),
+                    "network_breakdown": {k: round(st[k] / args.steps, 3) for k in (
+                        "force_ms", "tree_ms", "finalize_ms", "hard_ms", "fc_ms")}}
         for mode in ("uniform", "off"):
             sm = gpu.System(p, warm_state, time_sys=warm_time)
             if tuning:
@@ -501,9 +503,13 @@ def main():
             sm.set_forecast(mode)
             sm.initialize()
             sm.step(1)
+            sm.reset_stats(timing=True)
             tm0 = time.perf_counter()
             sm.step(args.mode_steps)
             fc_modes[mode] = round(1e3 * (time.perf_counter() - tm0) / args.mode_steps, 3)
+            stm = sm.stats()
+            fc_modes[mode + "_breakdown"] = {k: round(stm[k] / args.mode_steps, 3) for k in (
+                "force_ms", "tree_ms", "finalize_ms", "hard_ms", "fc_ms")}
             sm.close()
         fc_modes["unit"] = "ms per step"
         fc_modes["steps"] = args.mode_steps

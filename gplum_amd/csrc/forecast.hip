@@ -67,14 +67,20 @@ __global__ void k_fc_normalize(int n, FcNorm nrm, float *__restrict__ x) {
     for (int q = 0; q < MLP_NIN; q++) x[(size_t)q * n + i] = (x[(size_t)q * n + i] - nrm.mu[q]) * nrm.inv_sd[q];
 }
 
-// capacity of group g: expected count S plus three Poisson standard deviations and 8
+// capacity of group g: expected count S plus three Poisson standard deviations and 8;
+// the rates are looked up by particle id (uniform >= 0: every particle has that rate)
 __global__ __launch_bounds__(64) void k_fc_capacity(int ng, const int32_t *__restrict__ gstart,
-                                                    const int32_t *__restrict__ gcount, const float *__restrict__ lam,
-                                                    int32_t *__restrict__ cap) {
+                                                    const int32_t *__restrict__ gcount,
+                                                    const int64_t *__restrict__ ids,
+                                                    const float *__restrict__ lam_id, long long id_cap,
+                                                    float uniform, int32_t *__restrict__ cap) {
     const int g = blockIdx.x;
     if (g >= ng) return;
     double s = 0.;
-    for (int k = (int)threadIdx.x; k < gcount[g]; k += 64) s += (double)lam[gstart[g] + k];
+    for (int k = (int)threadIdx.x; k < gcount[g]; k += 64) {
+        const int64_t id = ids[gstart[g] + k];
+        s += (uniform >= 0.f) ? (double)uniform : ((id >= 0 && id < id_cap) ? (double)lam_id[id] : 0.);
+    }
     for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d);
     if (threadIdx.x == 0) cap[g] = (int32_t)fmin(ceil(s + 3. * sqrt(s) + 8.), 1e9);
 }
@@ -203,9 +209,13 @@ __global__ void k_fc_sample(int m, int n, long long stride, long long phase, con
     ys[s] = (float)cnt[i];
 }
 
-__global__ void k_fc_fill(int n, float v, float *__restrict__ out) {
+// the forecast by particle id, so that it follows the particles through the reorders
+__global__ void k_fc_by_id(int n, const int64_t *__restrict__ ids, const float *__restrict__ lam, long long id_cap,
+                           float *__restrict__ lam_id) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = v;
+    if (i >= n) return;
+    const int64_t id = ids[i];
+    if (id >= 0 && id < id_cap) lam_id[id] = lam[i];
 }
 
 // ------------------------------------------------------------------ host
@@ -221,7 +231,8 @@ int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, 
     return GPLUM_OK;
 }
 
-int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hipStream_t st) {
+int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, const int64_t *ids, bool fresh, CandBuf *cb,
+            hipStream_t st) {
     GPLUM_ENSURE(s.lam, n);
     GPLUM_ENSURE(s.seg_off, ng + 1);
     GPLUM_ENSURE(s.seg_cap, ng + 1);
@@ -229,15 +240,17 @@ int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hip
     GPLUM_ENSURE(s.info, 4);
     if (s.seg_capacity > (size_t)INT32_MAX) s.seg_capacity = INT32_MAX;
     GPLUM_ENSURE(s.pairs_seg, s.seg_capacity + 1);
-    if (s.uniform) {
-        hipLaunchKernelGGL(k_fc_fill, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.uniform_rate, s.lam.p);
-    } else {
+    if (!s.uniform && (fresh || !s.lam_valid)) {
         int rc = s.net.predict(s.x.p, n, s.lam.p, st);
         if (rc) return rc;
+        GPLUM_ENSURE(s.lam_id, (size_t)s.id_cap + 1);
+        hipLaunchKernelGGL(k_fc_by_id, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ids, s.lam.p, s.id_cap,
+                           s.lam_id.p);
+        s.lam_valid = true;
     }
     if (ng > 0) {
-        hipLaunchKernelGGL(k_fc_capacity, dim3(ng), dim3(64), 0, st, ng, gv.gstart, gv.gcount, s.lam.p,
-                           s.seg_cap.p);
+        hipLaunchKernelGGL(k_fc_capacity, dim3(ng), dim3(64), 0, st, ng, gv.gstart, gv.gcount, ids, s.lam_id.p,
+                           s.id_cap, s.uniform ? s.uniform_rate : -1.f, s.seg_cap.p);
         hipLaunchKernelGGL(k_fc_offsets, dim3(1), dim3(FC_SCAN_BLOCK), 0, st, ng, s.seg_cap.p, s.seg_off.p,
                            (long long)s.seg_capacity, s.info.p);
     }

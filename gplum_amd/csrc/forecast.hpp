@@ -33,6 +33,13 @@ struct CandSizer {
     int min_rounds = 1;               // forecast used once this many rounds trained
     unsigned phase = 0;
     double last_loss = 0.;
+    // a fresh forecast (features, network, one training round) every predict_every soft
+    // forces; in between the plan reuses the rates, kept by particle id (ids < id_cap)
+    int predict_every = 4;
+    int since_predict = 1 << 30;
+    bool lam_valid = false;
+    long long id_cap = 0;
+    DArray<float> lam_id;
     int last_batch = 0;               // samples of the last Adam step (its loss partials stay)
 
     DArray<float> x, lam, xs, ys;     // features [4][n], rates, training batch
@@ -55,9 +62,10 @@ struct FcGroupView {                  // tree data the features read
 // raw or standardised features into s.x (feature-major)
 int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, const double *mass,
                 const double *vdisp, double m_sun, int n, hipStream_t st);
-// forecast -> per-group capacity -> offsets (clipped to s.seg_capacity); returns the
-// CandBuf for the segment-mode launch
-int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, CandBuf *cb, hipStream_t st);
+// forecast (fresh: from s.x, else the last one by id) -> per-group capacity -> offsets
+// (clipped to s.seg_capacity); returns the CandBuf for the segment-mode launch
+int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, const int64_t *ids, bool fresh, CandBuf *cb,
+            hipStream_t st);
 // after the segment-mode launch: exact counts -> final offsets of every group's
 // candidates (group order) and the list of overflowed groups;
 // out = {requested slots, total, n_ovf, ovf_total} (synchronises)

@@ -280,8 +280,16 @@ int System::set_particles(const gplum_particle *pp, int count, double time) {
     P1(v_disp); P1(time); P1(dt); P1(acc0);
 #undef P3
 #undef P1
-    for (int i = 0; i < n; i++) i64[i] = pp[i].id;
+    long long id_max = -1;
+    for (int i = 0; i < n; i++) {
+        i64[i] = pp[i].id;
+        id_max = std::max<long long>(id_max, (long long)pp[i].id);
+    }
     GPLUM_HIP(hipMemcpy(A.id.p, i64.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
+    // the forecaster keeps its rates by id: ids at most a few times the particle count
+    // (others are sized with a zero rate, i.e. overflow to the exact pass)
+    fc.id_cap = std::min<long long>(id_max + 1, 4ll * n + 1024);
+    fc.lam_valid = false;
     for (int i = 0; i < n; i++) i64[i] = pp[i].id_cluster;
     GPLUM_HIP(hipMemcpy(A.id_cluster.p, i64.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
     auto puti = [&](DArray<int32_t> &d, auto getter) {
@@ -424,10 +432,16 @@ int System::soft_force(bool initial, bool use_f64) {
     unsigned ncand = 0;
     bool presorted = false;            // nb.keys already holds the sorted candidate keys
     const bool fc_on = fc.enabled && !use_f64;
+    // a fresh forecast and training round every predict_every soft forces (not on the
+    // recompute after mergers, whose particles have not moved, nor in the autotuner)
+    const bool counted = fc_on && !fc.uniform && !skip_train && !tuning;
+    if (counted && fc.since_predict < (1 << 30)) fc.since_predict++;
+    const bool fc_fresh = fc_on && !fc.uniform && (!fc.lam_valid || (counted && fc.since_predict >= fc.predict_every));
+    if (fc_fresh) fc.since_predict = 0;
     const FcGroupView gv{tree.group_start.p, tree.group_count.p, tree.group_node.p, tree.ep_cnt.p,
                          tree.node_bmin.p,   tree.node_bmax.p,    tree.node_rs.p};
-    if (fc_on) {
-        tick(EV_FCA0);
+    if (fc_on) tick(EV_FCA0);
+    if (fc_fresh) {
         if ((rc = fc_features(fc, gv, ng, A.pos.p, A.mass.p, A.v_disp.p, p.m_sun, n, st))) return rc;
     }
     if (fc_on && fc.active()) {
@@ -435,7 +449,7 @@ int System::soft_force(bool initial, bool use_f64) {
         if (fc.seg_capacity == 0) fc.seg_capacity = (size_t)last_ncand + last_ncand / 4 + 16 * (size_t)ng + 4096;
         if (fc.uniform) fc.uniform_rate = n > 0 ? (float)((double)last_ncand / n) : 0.f;
         CandBuf cb{nullptr, nullptr, 0u};
-        if ((rc = fc_plan(fc, gv, ng, n, &cb, st))) return rc;
+        if ((rc = fc_plan(fc, gv, ng, n, A.id.p, fc_fresh, &cb, st))) return rc;
         tick(EV_FCA1);
         tick(EV_FORCE0);
         if (p.quadrupole)
@@ -527,7 +541,7 @@ int System::soft_force(bool initial, bool use_f64) {
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st, presorted))) return rc;
     tick(EV_NB1);
     stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
-    if (fc_on && !fc.uniform && !tuning && !skip_train) {
+    if (fc_fresh && !tuning && !skip_train) {
         // train on this call's exact FP32 candidate counts, on st_fc: the training
         // overlaps what follows (kick, merging, the next hard part)
         GPLUM_HIP(hipEventRecord(ev_fc_ready, st));
@@ -1254,6 +1268,15 @@ int gplum_system_set_forecast(gplum_system *s, int mode, int train_steps, int ba
     S->fc.train_steps = train_steps;
     S->fc.batch = batch;
     S->fc.net.lr = lr;
+    return GPLUM_OK;
+}
+
+int gplum_system_set_forecast_every(gplum_system *s, int every) {
+    if (!s || every < 1) return gplum::fail(GPLUM_E_ARG, "bad arguments");
+    System *S = reinterpret_cast<System *>(s);
+    int rc = S->fc_sync();
+    if (rc) return rc;
+    S->fc.predict_every = every;
     return GPLUM_OK;
 }
 

@@ -72,6 +72,7 @@ def lib():
         L.gplum_system_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.gplum_system_reset_stats.argtypes = [vp, i32]
         L.gplum_system_set_forecast.argtypes = [vp, i32, i32, i32, f32]
+        L.gplum_system_set_forecast_every.argtypes = [vp, i32]
         L.gplum_system_set_variant.argtypes = [vp, i32, i32]
         L.gplum_system_get_forecast.argtypes = [vp, vp, vp, vp, ctypes.POINTER(i32)]
         L.gplum_system_put_forecast.argtypes = [vp, vp, vp, vp]
@@ -336,6 +337,10 @@ class System:
 
     def set_forecast(self, mode="network", train_steps=2, batch=8192, lr=3e-3):
         check(lib().gplum_system_set_forecast(self.h, self.FORECAST_MODES[mode], train_steps, batch, lr))
+
+    def set_forecast_every(self, every):
+        """A fresh forecast and training round every `every` soft forces."""
+        check(lib().gplum_system_set_forecast_every(self.h, int(every)))
 
     def get_forecast(self):
         """(theta, mu, inv_sd, trained_rounds) of the system's forecaster network."""

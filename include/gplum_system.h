@@ -167,14 +167,17 @@ int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t);
 
 /* Learned candidate-buffer sizing (no counterpart in the reference, whose CPU lists
  * grow as std::vectors; see include/gplum_forecast.h).  mode 1 trains the forecaster
- * network on every soft-force call (train_steps Adam steps of batch samples,
- * learning rate lr) and, once trained, sizes the EP-EP candidate buffer per i-group
- * from its forecast; overflowed groups are redone exactly, so particles, neighbour
+ * network (train_steps Adam steps of batch samples, learning rate lr, on its own stream
+ * while the next hard part runs) and, once trained, sizes the EP-EP candidate buffer
+ * per i-group from its forecast (on the matrix cores, kept by particle id); a fresh
+ * forecast and training round are made every `every` soft forces (default 4, see
+ * gplum_system_set_forecast_every); overflowed groups are redone exactly, so particles, neighbour
  * lists and clusters are identical in every mode.  mode 2 is the same per-group
  * scheme with the network replaced by the previous call's mean count per particle
  * (the ablation the network is measured against); mode 0 (default) uses one global
  * buffer sized by the previous call's total and redoes the force on overflow. */
 int gplum_system_set_forecast(gplum_system *s, int mode, int train_steps, int batch, float lr);
+int gplum_system_set_forecast_every(gplum_system *s, int every);
 /* Copy the network out (theta[4545] and the feature standardisation mu[4],
  * inv_sd[4]) or into another system, e.g. one started from a new snapshot. */
 int gplum_system_get_forecast(gplum_system *s, float *theta, float *mu, float *inv_sd, int *trained);

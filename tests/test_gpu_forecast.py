@@ -5,8 +5,9 @@
    buffer gives bit-identical particles and neighbour graphs.
 2. What the trained network gains: on the same states, the network's per-group plan
    overflows far fewer i-groups (each overflow costs a candidate-only re-run of that
-   group) than the ablation that gives every particle the previous mean count, at no
-   more planned slots."""
+   group) than the ablation that gives every particle the previous mean count, with
+   about as many planned slots (at most 15 % more; the bf16 forecast renewed every
+   fourth soft force plans about 11 % more)."""
 import numpy as np
 import pytest
 
@@ -66,11 +67,13 @@ def mass_spectrum_disk(n=N, seed=5):
 
 
 def test_network_plan_beats_uniform_rate():
-    """Train online for 6 steps, then compare the two plans over 4 more steps of the
-    same trajectory (the plan does not change the trajectory)."""
+    """Train online for 6 steps (a round every soft force), then compare the two plans
+    over 4 more steps of the same trajectory (the plan does not change the trajectory),
+    the network's forecast renewed every fourth soft force as in production."""
     p = gpu.params(**bench.params_dict())
     s = gpu.System(p, mass_spectrum_disk())
     s.set_forecast("network")
+    s.set_forecast_every(1)
     s.initialize()
     s.step(6)
     theta, mu, isd, rounds = s.get_forecast()
@@ -95,6 +98,6 @@ def test_network_plan_beats_uniform_rate():
               % (name, r["fc_slots"], r["fc_ovf_groups"], r["fc_ovf_candidates"], r["n_groups"],
                  r["n_candidates"], r["fc_loss"], r["fc_ms"]))
     assert un["fc_ovf_groups"] > 0
-    assert nw["fc_slots"] <= 1.1 * un["fc_slots"]
+    assert nw["fc_slots"] <= 1.15 * un["fc_slots"]
     assert nw["fc_ovf_groups"] < 0.5 * un["fc_ovf_groups"]
     assert nw["fc_ovf_candidates"] < 0.5 * un["fc_ovf_candidates"]
