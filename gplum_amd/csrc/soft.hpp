@@ -71,6 +71,16 @@ struct SrcEPJ {                       // reference EPJGrav array (C-ABI path)
         id = q.id_local; rank = q.myrank;
     }
 };
+struct SrcEPJIdx {                    // EPJGrav array by index list (MultiWalk Index)
+    const gplum_epj *p;
+    const int32_t *idx;
+    __device__ void load(int e, double &x, double &y, double &z, double &m, double &ro,
+                         double &rs, int &id, int &rank) const {
+        const gplum_epj &q = p[idx[e]];
+        x = q.pos[0]; y = q.pos[1]; z = q.pos[2]; m = q.mass; ro = q.r_out; rs = q.r_search;
+        id = q.id_local; rank = q.myrank;
+    }
+};
 struct SrcRecI {                      // engine i-particles: contiguous JRec
     const JRec *r;
     __device__ void load(int i, double &x, double &y, double &z, double &ro, double &rs,
@@ -103,6 +113,18 @@ struct SrcSPM {                       // FDPS SPJMonopole array
         m = s.mass; x = s.pos[0]; y = s.pos[1]; z = s.pos[2];
         for (int k = 0; k < 6; k++) q[k] = 0.;
     }
+};
+template <class SP>
+struct SrcSPIdx {                     // FDPS SPJ array by index list (MultiWalk Index)
+    const SP *p;
+    const int32_t *idx;
+    __device__ void load(int e, double &m, double &x, double &y, double &z, double q[6]) const {
+        const SP &s = p[idx[e]];
+        m = s.mass; x = s.pos[0]; y = s.pos[1]; z = s.pos[2];
+        for (int k = 0; k < 6; k++) q[k] = quad_of(s, k);
+    }
+    __device__ static double quad_of(const gplum_spj_quad &s, int k) { return s.quad[k]; }
+    __device__ static double quad_of(const gplum_spj_mono &, int) { return 0.; }
 };
 struct SrcNodeSP {                    // engine: tree-node moments by index list
     const double *mass, *com, *quad;  // com: 3 per node, quad: 6 per node

@@ -219,6 +219,10 @@ System::~System() {
     if (st2) (void)hipStreamDestroy(st2);
     if (st_fc) (void)hipStreamSynchronize(st_fc);
     if (st_fc) (void)hipStreamDestroy(st_fc);
+    if (st_iso) (void)hipStreamSynchronize(st_iso);
+    if (st_iso) (void)hipStreamDestroy(st_iso);
+    for (hipEvent_t e : {ev_iso_fork, ev_iso_join})
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {ev_fc_ready, ev_fc_done, ev_fc_t0, ev_fc_t1})
         if (e) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -239,6 +243,9 @@ int System::init_device() {
     GPLUM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     GPLUM_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
     GPLUM_HIP(hipStreamCreateWithFlags(&st_fc, hipStreamNonBlocking));
+    GPLUM_HIP(hipStreamCreateWithFlags(&st_iso, hipStreamNonBlocking));
+    GPLUM_HIP(hipEventCreateWithFlags(&ev_iso_fork, hipEventDisableTiming));
+    GPLUM_HIP(hipEventCreateWithFlags(&ev_iso_join, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_fc_ready, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_fc_done, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreate(&ev_fc_t0));
@@ -616,6 +623,14 @@ int System::hard_part(bool persistent) {
     if (!have_nbr) return fail(GPLUM_E_STATE, "neighbour graph missing: initialize first");
     tick(EV_HARD0);
     if ((rc = make_clusters(A, n, nb, cl, scratch, p.dt_tree, st))) return rc;
+    // the isolated particles (Kepler drift) on their own stream, alongside the clusters:
+    // the two sets are disjoint (number == 0 vs > 0)
+    IsoArgs ia{A.pos.p, A.vel.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p, A.phi_d.p, A.phi_s.p, A.time.p,
+               A.dt.p, A.acc0.p, A.number.p, A.n_cluster.p};
+    GPLUM_HIP(hipEventRecord(ev_iso_fork, st));
+    GPLUM_HIP(hipStreamWaitEvent(st_iso, ev_iso_fork, 0));
+    if ((rc = isolated_launch(p, ia, n, time_sys, time_sys + p.dt_tree, st_iso))) return rc;
+    GPLUM_HIP(hipEventRecord(ev_iso_join, st_iso));
     const int m = cl.m, ncl = cl.ncl;
     stats.n_clusters = ncl;
     stats.n_cluster_particles = m;
@@ -673,9 +688,7 @@ int System::hard_part(bool persistent) {
             stats.hard_pairs += npr[c];
         }
     }
-    IsoArgs ia{A.pos.p, A.vel.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p, A.phi_d.p, A.phi_s.p, A.time.p,
-               A.dt.p, A.acc0.p, A.number.p, A.n_cluster.p};
-    if ((rc = isolated_launch(p, ia, n, time_sys, time_sys + p.dt_tree, st))) return rc;
+    GPLUM_HIP(hipStreamWaitEvent(st, ev_iso_join, 0));
     tick(EV_HARD1);
     GPLUM_HIP(hipStreamSynchronize(st));
     stats.hard_ms += elapsed(EV_HARD0, EV_HARD1);

@@ -54,6 +54,8 @@ struct System {
     // the forecaster's training runs on st_fc, overlapping the next step's hard part;
     // the next soft force (which rewrites its inputs and reads the network) waits for it
     hipStream_t st_fc = nullptr;
+    hipStream_t st_iso = nullptr;       // the isolated particles' drift, beside the clusters
+    hipEvent_t ev_iso_fork = nullptr, ev_iso_join = nullptr;
     hipEvent_t ev_fc_ready = nullptr, ev_fc_done = nullptr, ev_fc_t0 = nullptr, ev_fc_t1 = nullptr;
     bool fc_pending = false, fc_timed = false;
     hipEvent_t ev[EV_COUNT] = {};

@@ -88,15 +88,51 @@ struct calcForceEPSP {
     }
 };
 
-// Dispatch / retrieve pair for PS::TreeForForce::calcForceAllAndWriteBackMultiWalk.
+// Dispatch / retrieve pair for PS::TreeForForce::calcForceAllAndWriteBackMultiWalk,
+// with quadrupole (USE_QUAD) or monopole super-particles.
 template <class EPI, class EPJ, class SPJ>
 int DispatchKernel(const int tag, const int n_walk, const EPI **epi, const int *n_epi, const EPJ **epj,
                    const int *n_epj, const SPJ **spj, const int *n_spj) {
-    static_assert(sizeof(SPJ) == sizeof(gplum_spj_quad), "multi-walk path uses quadrupole SPJ");
-    check_or_abort(gplum_multiwalk_dispatch(tag, n_walk, reinterpret_cast<const gplum_epi **>(epi), n_epi,
-                                            reinterpret_cast<const gplum_epj **>(epj), n_epj,
-                                            reinterpret_cast<const gplum_spj_quad **>(spj), n_spj),
-                   "DispatchKernel");
+    check_layouts<EPI, EPJ, gplum_force>();
+    static_assert(sizeof(SPJ) == sizeof(gplum_spj_quad) || sizeof(SPJ) == sizeof(gplum_spj_mono),
+                  "SPJ_t must have the SPJQuadrupole or SPJMonopole layout");
+    const gplum_epi **e = reinterpret_cast<const gplum_epi **>(epi);
+    const gplum_epj **j = reinterpret_cast<const gplum_epj **>(epj);
+    if (sizeof(SPJ) == sizeof(gplum_spj_quad))
+        check_or_abort(gplum_multiwalk_dispatch(tag, n_walk, e, n_epi, j, n_epj,
+                                                reinterpret_cast<const gplum_spj_quad **>(spj), n_spj),
+                       "DispatchKernel");
+    else
+        check_or_abort(gplum_multiwalk_dispatch_mono(tag, n_walk, e, n_epi, j, n_epj,
+                                                     reinterpret_cast<const gplum_spj_mono **>(spj), n_spj),
+                       "DispatchKernel");
+    return 0;
+}
+
+// Dispatch functor of PS::TreeForForce::calcForceAllAndWriteBackMultiWalkIndex: index
+// lists into the tree's whole EPJ / SPJ arrays, which are sent when send_flag is set.
+template <class EPI, class EPJ, class SPJ>
+int DispatchKernelIndex(const int tag, const int n_walk, const EPI **epi, const int *n_epi, const int **id_epj,
+                        const int *n_epj, const int **id_spj, const int *n_spj, const EPJ *epj, const int n_epj_tot,
+                        const SPJ *spj, const int n_spj_tot, const bool send_flag) {
+    check_layouts<EPI, EPJ, gplum_force>();
+    static_assert(sizeof(SPJ) == sizeof(gplum_spj_quad) || sizeof(SPJ) == sizeof(gplum_spj_mono),
+                  "SPJ_t must have the SPJQuadrupole or SPJMonopole layout");
+    static_assert(sizeof(int) == sizeof(int32_t), "index lists are S32");
+    const gplum_epi **e = reinterpret_cast<const gplum_epi **>(epi);
+    const int32_t **ij = reinterpret_cast<const int32_t **>(id_epj);
+    const int32_t **is = reinterpret_cast<const int32_t **>(id_spj);
+    const gplum_epj *j = reinterpret_cast<const gplum_epj *>(epj);
+    if (sizeof(SPJ) == sizeof(gplum_spj_quad))
+        check_or_abort(gplum_multiwalk_dispatch_index(tag, n_walk, e, n_epi, ij, n_epj, is, n_spj, j, n_epj_tot,
+                                                      reinterpret_cast<const gplum_spj_quad *>(spj), n_spj_tot,
+                                                      send_flag ? 1 : 0),
+                       "DispatchKernelIndex");
+    else
+        check_or_abort(gplum_multiwalk_dispatch_index_mono(tag, n_walk, e, n_epi, ij, n_epj, is, n_spj, j,
+                                                           n_epj_tot, reinterpret_cast<const gplum_spj_mono *>(spj),
+                                                           n_spj_tot, send_flag ? 1 : 0),
+                       "DispatchKernelIndex");
     return 0;
 }
 template <class FORCE>

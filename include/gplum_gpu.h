@@ -173,6 +173,29 @@ int gplum_multiwalk_dispatch(int tag, int n_walk, const gplum_epi **epi, const i
                              const gplum_epj **epj, const int *n_epj,
                              const gplum_spj_quad **spj, const int *n_spj);
 
+/* The same dispatch with monopole super-particles (USE_QUAD off,
+ * PS::SPJMonopole: mass, pos). */
+int gplum_multiwalk_dispatch_mono(int tag, int n_walk, const gplum_epi **epi, const int *n_epi,
+                                  const gplum_epj **epj, const int *n_epj,
+                                  const gplum_spj_mono **spj, const int *n_spj);
+
+/* Dispatch functor of calcForceAllAndWriteBackMultiWalkIndex: the walks' EP and SP
+ * lists are index lists id_epj[w][0..n_epj[w]) / id_spj[w][0..n_spj[w]) into the tree's
+ * whole arrays epj[0..n_epj_tot) / spj[0..n_spj_tot), which FDPS passes with send_flag
+ * set on the first dispatch of a force call (they are uploaded then and reused by the
+ * later dispatches of that call, of any tag).  Same argument meaning as the FDPS
+ * signature S32 dispatch(S32 tag, S32 n_walk, const EPI**, const S32*, const S32**,
+ * const S32*, const S32**, const S32*, const EPJ*, const S32, const SPJ*, const S32,
+ * const bool); indices outside the whole arrays are an argument error. */
+int gplum_multiwalk_dispatch_index(int tag, int n_walk, const gplum_epi **epi, const int *n_epi,
+                                   const int32_t **id_epj, const int *n_epj, const int32_t **id_spj,
+                                   const int *n_spj, const gplum_epj *epj, int n_epj_tot,
+                                   const gplum_spj_quad *spj, int n_spj_tot, int send_flag);
+int gplum_multiwalk_dispatch_index_mono(int tag, int n_walk, const gplum_epi **epi, const int *n_epi,
+                                        const int32_t **id_epj, const int *n_epj, const int32_t **id_spj,
+                                        const int *n_spj, const gplum_epj *epj, int n_epj_tot,
+                                        const gplum_spj_mono *spj, int n_spj_tot, int send_flag);
+
 /* Retrieve functor: waits for the tag's dispatch and assigns force[w][0..ni[w])
  * (values start from ForceGrav::clear(), src/particle.h:81-85).  FDPS signature
  * S32 retrieve(S32 tag, S32 n_walk, const S32* ni, Force** force). */
