@@ -401,6 +401,7 @@ def main():
         print("# warmup step %d done" % k, file=sys.stderr, flush=True)
     warm_state = s.particles()
     warm_time = s.stats()["time_sys"]
+    warm_fc = s.get_forecast()      # the network as trained over the warm-up steps
     s.reset_stats(timing=True)
     e0 = s.energy()
     t0 = time.perf_counter()
@@ -491,16 +492,17 @@ def main():
     # the same steps with the candidate buffers sized by the network (the timed run), by
     # a uniform per-particle rate, and by the global buffer without any forecast, each
     # from the warmed-up state (one untimed step, then the timed ones)
+    # (results are bit-identical across the modes, so each runs exactly the same work)
     fc_modes = None
     if args.mode_steps > 0:
-        fc_modes = {"network": round(ms_per_step, 3),
-                    "network_breakdown": {k: round(st[k] / args.steps, 3) for k in (
-                        "force_ms", "tree_ms", "finalize_ms", "hard_ms", "fc_ms")}}
-        for mode in ("uniform", "off"):
+        fc_modes = {}
+        for mode in ("network", "uniform", "off"):
             sm = gpu.System(p, warm_state, time_sys=warm_time)
             if tuning:
                 sm.set_tuning(**tuning)
             sm.set_forecast(mode)
+            if mode == "network" and warm_fc[3] > 0:
+                sm.put_forecast(*warm_fc[:3])
             sm.initialize()
             sm.step(1)
             sm.reset_stats(timing=True)

@@ -801,6 +801,184 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
     }
 }
 
+// ------------------------------- eight-wave kernels: lane groups and wave-level reductions
+
+// Per-workgroup scratch of the eight-wave kernels' block steps: one slot per wave for
+// each reduction (separate slots, so that a reduction needs no barrier before its
+// writes), and the list of the block step's active particles.
+struct WaveSlots {
+    double ts[8], sys[8], rm[8];
+    int lo[8], hi[8];
+    int na;
+    int act[HERMITE_LDS];
+};
+
+// Minimum over the workgroup (every lane active, values not NaN): the wave's minimum by
+// DPP, the waves' minima through slot[] and one barrier.  Exact, hence the value of
+// block_min.
+__device__ inline double block_min_waves(double v, double *slot) {
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) slot[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = slot[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = dmin(m, slot[w]);
+    return m;
+}
+
+// (R, lo, hi) lexicographic minimum over the workgroup (every lane active).
+__device__ inline bool rlh_less(double r, int lo, int hi, double r2, int lo2, int hi2) {
+    return r < r2 || (r == r2 && (lo < lo2 || (lo == lo2 && hi < hi2)));
+}
+__device__ inline void block_argmin_waves(double &v, int &lo, int &hi, WaveSlots &W) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const double ov = __shfl_xor(v, d);
+        const int ol = __shfl_xor(lo, d), oh = __shfl_xor(hi, d);
+        if (rlh_less(ov, ol, oh, v, lo, hi)) { v = ov; lo = ol; hi = oh; }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        W.rm[w] = v; W.lo[w] = lo; W.hi[w] = hi;
+    }
+    __syncthreads();
+    v = W.rm[0]; lo = W.lo[0]; hi = W.hi[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+        if (rlh_less(W.rm[w], W.lo[w], W.hi[w], v, lo, hi)) { v = W.rm[w]; lo = W.lo[w]; hi = W.hi[w]; }
+}
+
+// Subtree sums of coop_pair_sum's butterfly.  With all 64 lanes on one particle, lane l
+// holds the partial P_l (pairs j = l, l + 64, ... in ascending j) and the xor butterfly
+// 32, 16, ..., 1 adds them: after the level of distance 2^k every lane holds V_k(l mod
+// 2^k), the sum of the partials m = l (mod 2^k), where V_k(r) = V_(k+1)(r) + V_(k+1)(r +
+// 2^k).  A group of S = 2^s lanes holding one particle computes the same sums: lane r of
+// the group builds V_s(r) from its 64/S partials m = r + S bitrev(q), q = 0, 1, ... (the
+// leaves of that subtree from left to right, so that a binary counter over q adds two
+// sibling subtrees as soon as the second one is complete), then the group runs the
+// butterfly's last s levels.  The same additions of the same values as coop_pair_sum,
+// hence the same force bit for bit.  K = ceil(n / 64) pairs per partial, evaluated side
+// by side (partials with fewer pairs evaluate a clamped j and drop it).  Every lane of
+// the wave calls it (groups of invalid particles compute a stand-in); 3 <= s <= 6.
+template <int K>
+__device__ inline void group_pair_sum(const HP &P, const JData &J, int n, int s, int r, int64_t idi,
+                                      const double xi[3], const double vi[3], double rinv_i, double acc[3],
+                                      double jerk[3]) {
+    const int depth = 6 - s;                      // levels of the lane's own subtree
+    const int nq = 1 << depth;
+    double st[3][6];
+    double c[6];
+    for (int q = 0; q < nq; q++) {
+        const int m = r + ((depth ? (int)(__builtin_bitreverse32((unsigned)q) >> (32 - depth)) : 0) << s);
+        for (int d = 0; d < 6; d++) c[d] = 0.;
+#pragma unroll
+        for (int u0 = 0; u0 < K; u0 += 2) {
+            constexpr int L = 2;
+            double ta[L][3], tj[L][3];
+            int jj[L];
+#pragma unroll
+            for (int u = 0; u < L; u++) {
+                jj[u] = min(m + 64 * (u0 + u), n - 1);
+                if (u0 + u < K) pair_term_j(P, J, jj[u], xi, vi, rinv_i, ta[u], tj[u], nullptr);
+            }
+#pragma unroll
+            for (int u = 0; u < L; u++) {
+                if (u0 + u >= K) continue;
+                const bool use = m + 64 * (u0 + u) < n && J.id[jj[u]] != idi;
+                for (int d = 0; d < 3; d++) {
+                    c[d] = use ? c[d] + ta[u][d] : c[d];
+                    c[3 + d] = use ? c[3 + d] + tj[u][d] : c[3 + d];
+                }
+            }
+        }
+        // binary counter: level k holds the left subtree of 2^k leaves until its sibling
+        // is complete
+        bool stored = false;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (stored || k >= depth) continue;
+            if (q & (1 << k)) {
+                for (int d = 0; d < 6; d++) c[d] = st[k][d] + c[d];
+            } else {
+                for (int d = 0; d < 6; d++) st[k][d] = c[d];
+                stored = true;
+            }
+        }
+    }
+    // the last s levels of the butterfly, inside the group (as in coop_pair_sum)
+    if (s == 6)
+        for (int d = 0; d < 6; d++) c[d] += __shfl_xor(c[d], 32);
+    if (s >= 5)
+        for (int d = 0; d < 6; d++) c[d] += __shfl_xor(c[d], 16);
+    if (s >= 4)
+        for (int d = 0; d < 6; d++) c[d] += dpp_f64<DPP_ROW_ROR8>(c[d]);
+    for (int d = 0; d < 6; d++) c[d] += dpp_f64<DPP_QUAD_3210>(dpp_f64<DPP_ROW_HALF_MIRROR>(c[d]));
+    for (int d = 0; d < 6; d++) c[d] += dpp_f64<DPP_QUAD_2301>(c[d]);
+    for (int d = 0; d < 6; d++) c[d] += dpp_f64<DPP_QUAD_1032>(c[d]);
+    for (int d = 0; d < 3; d++) { acc[d] = c[d]; jerk[d] = c[3 + d]; }
+}
+
+// The active-particle correction of integrate_from (hermite.h:300-375) for clusters
+// with j-data in LDS and no merged particle: the workgroup's lanes are dealt in groups of
+// S = 2^s per active particle (the largest S <= 64 with na S <= 512, at least 8; more
+// than 64 active particles take rounds of 64), each group evaluating its particle's hard
+// force by group_pair_sum.  The steps and their order are those of coop_correct_active,
+// so the results are too.  Every thread of the workgroup calls it (no barrier inside).
+template <int K>
+__device__ void group_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
+                                     int flag_col, long long &pairs, const int *s_act, int na) {
+    const int t = threadIdx.x;
+    const int B = blockDim.x;
+    int s = 6;
+    while (s > 3 && na * (1 << s) > B) s--;
+    const int r = t & ((1 << s) - 1);
+    const int per_round = B >> s;
+    const int wave_first = (t & ~63) >> s;     // the wave's first particle slot in a round
+    for (int a0 = 0; a0 < na; a0 += per_round) {
+        if (a0 + wave_first >= na) continue;   // no particle for this wave (wave-uniform)
+        const int a = a0 + (t >> s);
+        const bool valid = a < na;
+        const int g = b + s_act[valid ? a : a0];
+        const bool lead = valid && r == 0;
+        // Registers are scarce here (the pair sums keep a subtree stack per lane): the
+        // particle's predicted state is reloaded for each correction instead of being
+        // held across the pair sums, and the correction's a2/a3 terms go through H.
+        double as[3], js[3], ad[3], jd[3], x[3], v[3];
+        const double rinv = H.rinv[g];
+        const int64_t id = H.id[g];
+        {
+            double xp[3], vp[3];
+            ld3(H.xp, g, xp); ld3(H.vp, g, vp);
+            group_pair_sum<K>(P, J, n, s, r, id, xp, vp, rinv, ad, jd);
+            star_acc_jerk(P, xp, vp, as, js, nullptr);
+        }
+        const double t_c = t_s - H.time[g];
+        for (int ite = 0; ite < 2; ite++) {
+            {
+                double xp[3], vp[3], a0s[3], j0s[3], a0d[3], j0d[3], a2s[3], a3s[3], a2d[3], a3d[3];
+                ld3(H.xp, g, xp); ld3(H.vp, g, vp);
+                ld3(H.a0_s, g, a0s); ld3(H.j0_s, g, j0s); ld3(H.a0_d, g, a0d); ld3(H.j0_d, g, j0d);
+                correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
+                if (ite == 1 && lead) { st3(H.a2_s, g, a2s); st3(H.a3_s, g, a3s); st3(H.a2_d, g, a2d); st3(H.a3_d, g, a3d); }
+            }
+            star_acc_jerk(P, x, v, as, js, nullptr);
+            group_pair_sum<K>(P, J, n, s, r, id, x, v, rinv, ad, jd);
+        }
+        wave_sync();     // the leader's a2/a3 before the group reads them back
+        const double time = H.time[g] + t_c;
+        double dt = H.dt[g];
+        if (!flag_col) {
+            double a2s[3], a3s[3], a2d[3], a3d[3];
+            ld3(H.a2_s, g, a2s); ld3(H.a3_s, g, a3s); ld3(H.a2_d, g, a2d); ld3(H.a3_d, g, a3d);
+            dt = dt_block(P, time, dt, H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
+        }
+        if (lead) {
+            pairs += 3 * (long long)(n - 1);
+            st3(H.pos, g, x); st3(H.vel, g, v);
+            st3(H.acc_s, g, as); st3(H.jerk_s, g, js); st3(H.acc_d, g, ad); st3(H.jerk_d, g, jd);
+            H.time[g] = time;
+            if (!flag_col) H.dt[g] = dt;
+        }
+    }
+}
+
 // After the collision test found the pair (ic, jc) in a block step and the active
 // particles were corrected (hermite.h:289-296, 449-458): perfect merging, the full hard
 // force of everybody, the merged targets' acc/jerk, the initial block step of the live
@@ -843,7 +1021,7 @@ __device__ void collision_tail(const HP &P, HardView &H, int b, int n, int ic, i
 template <bool LDSJ, bool COOP>
 __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
                                int64_t *s_id, double *s_fr, int b, int n, double &sys_time, long long &loop,
-                               double &ed, int &n_col, long long &pairs) {
+                               double &ed, int &n_col, long long &pairs, WaveSlots *W = nullptr) {
     const int t = threadIdx.x;
     const int B = blockDim.x;
     const double time_end = P.dt_tree;
@@ -852,7 +1030,7 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         double tl = 1.e10;
         for (int i = t; i < n; i += B)
             if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tl = dmin(tl, H.time[b + i] + H.dt[b + i]);
-        const double t_s = block_min(R, tl);
+        const double t_s = COOP ? block_min_waves(tl, W->ts) : block_min(R, tl);
         // predict all (hermite.h:277-283)
         for (int i = t; i < n; i += B) {
             predict(H, b + i, t_s - H.time[b + i]);
@@ -872,7 +1050,39 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         int flag_col = 0, ic = -1, jc = -1;
         if (P.collision) {
             double rmin = 1.;
-            int bi = 0x7fffffff, bj = -1;
+            int bi = 0x7fffffff, bj = COOP ? 0x7fffffff : -1;
+            if (LDSJ && COOP) {
+                // the unordered pairs {a, c} over all threads: c = (a + d) mod n for d = 1 ..
+                // (n - 1) / 2 and every a, and for even n d = n / 2 with a < n / 2.  R is
+                // symmetric bit for bit (squared differences, a commutative sum of f R),
+                // so the (R, lower, higher index) minimum is the (R, i, first j) minimum
+                // of the ordered loop below.
+                const int h = (n - 1) >> 1;
+                const int nh = n * h;
+                const int np = nh + ((n & 1) ? 0 : (n >> 1));
+                for (int k = t; k < np; k += B) {
+                    int a, c;
+                    if (k < nh) {
+                        a = k % n;
+                        c = a + 1 + k / n;
+                        if (c >= n) c -= n;
+                    } else {
+                        a = k - nh;
+                        c = a + (n >> 1);
+                    }
+                    const int lo = min(a, c), hi = max(a, c);
+                    const double frl = s_fr[lo], frh = s_fr[hi];
+                    if (frl < 0. || frh < 0.) continue;
+                    const double dx = s_x[3 * lo] - s_x[3 * hi];
+                    const double dy = s_x[3 * lo + 1] - s_x[3 * hi + 1];
+                    const double dz = s_x[3 * lo + 2] - s_x[3 * hi + 2];
+                    const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+                    const double r2 = frl + frh;
+                    if (!(d2 < COL_FILTER * r2 * r2)) continue;
+                    const double Rr = sqrt(d2) / r2;
+                    if (Rr < 1. && rlh_less(Rr, lo, hi, rmin, bi, bj)) { rmin = Rr; bi = lo; bj = hi; }
+                }
+            } else
             for (int i = t; i < n; i += B) {
                 if (LDSJ) {
                     // predicted positions and f R from LDS
@@ -906,8 +1116,10 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                     }
                 }
             }
-            int gi, gj;
-            const double rm = block_argmin(R, rmin, bi, bj, &gi, &gj);
+            int gi = bi, gj = bj;
+            double rm = rmin;
+            if (COOP) block_argmin_waves(rm, gi, gj, *W);
+            else rm = block_argmin(R, rmin, bi, bj, &gi, &gj);
             if (rm < 1.) {
                 flag_col = 1;
                 if (H.mass[b + gi] < H.mass[b + gj]) { ic = gi; jc = gj; } else { ic = gj; jc = gi; }
@@ -918,7 +1130,36 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
             __syncthreads();
         }
         // correct the active particles (hermite.h:300-375)
-        if (COOP) {
+        if (COOP && LDSJ) {
+            // the active particles, listed in ascending index by the first wave
+            if (t < 64) {
+                int base = 0;
+                for (int i0 = 0; i0 < n; i0 += 64) {
+                    const int i = i0 + t;
+                    int a = 0;
+                    if (i < n) {
+                        a = !(H.flags[b + i] & GPLUM_FLAG_DEAD) && (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
+                        H.act[b + i] = a;
+                    }
+                    const unsigned long long m = __ballot(a);
+                    if (a) W->act[base + __popcll(m & ((1ull << t) - 1ull))] = i;
+                    base += __popcll(m);
+                }
+                if (t == 0) W->na = base;
+            }
+            __threadfence_block();
+            __syncthreads();
+            if (n_col == 0) {
+                // no merged particle: lane groups (the same sums as coop_correct_active)
+                const int na = W->na;
+                const int K = (n + 63) >> 6;
+                if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+                else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+                else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+            } else {
+                coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6);
+            }
+        } else if (COOP) {
             for (int i = t; i < n; i += B)
                 H.act[b + i] = !(H.flags[b + i] & GPLUM_FLAG_DEAD) &&
                                (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
@@ -974,7 +1215,7 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         double tm = 1.e10;
         for (int i = t; i < n; i += B)
             if (!(H.flags[b + i] & GPLUM_FLAG_DEAD)) tm = dmin(tm, H.time[b + i]);
-        sys_time = block_min(R, tm);
+        sys_time = COOP ? block_min_waves(tm, W->sys) : block_min(R, tm);
         loop++;
     }
     for (int i = t; i < n; i += B) { gravity_full(P, H, b, n, i); pairs += n - 1; }
@@ -1640,6 +1881,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
     __shared__ HP s_P;
     // register path, 3 to 8 particles: pair summands (side_force)
     __shared__ double s_tm[REG ? 6 * 64 : 1];
+    // eight-wave kernels: reduction slots and the active list
+    __shared__ __align__(8) unsigned char s_ws[REG ? 8 : sizeof(WaveSlots)];
     const int t = threadIdx.x;
     const int B = blockDim.x;
     if (REG) {
@@ -1689,7 +1932,8 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
                 H.twin_of[b + i] = -1;
             }
             __syncthreads();
-            integrate_from<LDSJ, true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs);
+            integrate_from<LDSJ, true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs,
+                                       reinterpret_cast<WaveSlots *>(s_ws));
         }
         atomicAdd(&s_pairs, (unsigned long long)pairs);
         __syncthreads();
