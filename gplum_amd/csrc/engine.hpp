@@ -38,6 +38,32 @@ struct DArray {
 
 #define GPLUM_ENSURE(arr, cnt) GPLUM_HIP((arr).ensure(cnt))
 
+// Pinned host memory for the small device-to-host reads of a step (grown on demand, kept
+// until destroyed): a copy into it is one DMA transfer, where a copy into pageable memory
+// goes through the runtime's staging.  Every user synchronises before reading it.
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = bytes + bytes / 4 + 256;
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
 // Particle state, structure of arrays.  Vectors are stored interleaved (x,y,z).
 struct ParticleSoA {
     DArray<int64_t> id, id_cluster;
@@ -45,6 +71,7 @@ struct ParticleSoA {
     DArray<double> pos, vel, acc, acc_d, jerk_d, acc_s, jerk_s;   // 3 per particle
     DArray<double> phi, phi_d, phi_s, r_out, r_out_inv, r_search, v_disp, time, dt, acc0;
     DArray<int32_t> number, rank, id_max, id_min, n_cluster, flags;
+    DArray<float> fc_rate;      // forecast neighbour-candidate rate (travels with the particle)
     int ensure(int n);
     void swap(ParticleSoA &o) {
         id.swap(o.id); id_cluster.swap(o.id_cluster);
@@ -56,6 +83,7 @@ struct ParticleSoA {
         time.swap(o.time); dt.swap(o.dt); acc0.swap(o.acc0);
         number.swap(o.number); rank.swap(o.rank); id_max.swap(o.id_max); id_min.swap(o.id_min);
         n_cluster.swap(o.n_cluster); flags.swap(o.flags);
+        fc_rate.swap(o.fc_rate);
     }
 };
 

@@ -68,22 +68,24 @@ __global__ void k_fc_normalize(int n, FcNorm nrm, float *__restrict__ x) {
 }
 
 // capacity of group g: expected count S plus three Poisson standard deviations and 8;
-// the rates are looked up by particle id (uniform >= 0: every particle has that rate)
+// the rates are read in particle order over the group's range (uniform >= 0: every
+// particle has that rate)
 __global__ __launch_bounds__(64) void k_fc_capacity(int ng, const int32_t *__restrict__ gstart,
                                                     const int32_t *__restrict__ gcount,
-                                                    const int64_t *__restrict__ ids,
-                                                    const float *__restrict__ lam_id, long long id_cap,
-                                                    float uniform, int32_t *__restrict__ cap) {
+                                                    const float *__restrict__ rate, float uniform,
+                                                    int32_t *__restrict__ cap) {
     const int g = blockIdx.x;
     if (g >= ng) return;
     double s = 0.;
-    for (int k = (int)threadIdx.x; k < gcount[g]; k += 64) {
-        const int64_t id = ids[gstart[g] + k];
-        s += (uniform >= 0.f) ? (double)uniform : ((id >= 0 && id < id_cap) ? (double)lam_id[id] : 0.);
-    }
+    const float *r = rate + gstart[g];
+    for (int k = (int)threadIdx.x; k < gcount[g]; k += 64) s += (uniform >= 0.f) ? (double)uniform : (double)r[k];
     for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d);
     if (threadIdx.x == 0) cap[g] = (int32_t)fmin(ceil(s + 3. * sqrt(s) + 8.), 1e9);
 }
+
+// Consecutive groups per thread in the one-block scans below: a pass covers
+// FC_SCAN_BLOCK * FC_IPT groups with one block scan of the threads' sums.
+constexpr int FC_IPT = 8;
 
 // one block: exclusive offsets of the capacities (each at most FC_SORT_MAX, the keys a
 // group sorts in LDS), clipped to the allocated slots
@@ -92,16 +94,23 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *_
                                                               int64_t *__restrict__ info) {
     __shared__ long long s_w[FC_SCAN_BLOCK / 64];
     long long carry = 0;
-    for (int base = 0; base < ng; base += FC_SCAN_BLOCK) {
-        const int g = base + (int)threadIdx.x;
-        const long long v = g < ng ? min(cap[g], FC_SORT_MAX) : 0;
+    for (int base = 0; base < ng; base += FC_SCAN_BLOCK * FC_IPT) {
+        const int g0 = base + (int)threadIdx.x * FC_IPT;
+        long long v[FC_IPT], sum = 0;
+        for (int k = 0; k < FC_IPT; k++) {
+            v[k] = g0 + k < ng ? min(cap[g0 + k], FC_SORT_MAX) : 0;
+            sum += v[k];
+        }
         long long tot;
-        const long long e = block_excl_scan(v, s_w, &tot);
-        if (g < ng) {
-            const long long o = carry + e;
-            off[g] = (int32_t)(o < slots ? o : slots);
-            const long long room = slots - o;
-            cap[g] = (int32_t)(room <= 0 ? 0 : (v < room ? v : room));
+        long long o = carry + block_excl_scan(sum, s_w, &tot);
+        for (int k = 0; k < FC_IPT; k++) {
+            const int g = g0 + k;
+            if (g < ng) {
+                off[g] = (int32_t)(o < slots ? o : slots);
+                const long long room = slots - o;
+                cap[g] = (int32_t)(room <= 0 ? 0 : (v[k] < room ? v[k] : room));
+            }
+            o += v[k];
         }
         carry += tot;
     }
@@ -121,21 +130,30 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int3
                                                               int64_t *__restrict__ info) {
     __shared__ long long s_w[FC_SCAN_BLOCK / 64];
     long long cs = 0, co = 0;
-    for (int base = 0; base < ng; base += FC_SCAN_BLOCK) {
-        const int g = base + (int)threadIdx.x;
-        const long long c = g < ng ? cnt[g] : 0;
-        const bool ovf = g < ng && c > cap[g];
+    for (int base = 0; base < ng; base += FC_SCAN_BLOCK * FC_IPT) {
+        const int g0 = base + (int)threadIdx.x * FC_IPT;
+        long long c[FC_IPT], sc = 0, so = 0;
+        for (int k = 0; k < FC_IPT; k++) {
+            const int g = g0 + k;
+            c[k] = g < ng ? cnt[g] : 0;
+            sc += c[k];
+            // overflowed groups and their candidates in one sum (count above bit 40)
+            so += (g < ng && c[k] > cap[g]) ? FC_PACK + c[k] : 0;
+        }
         long long ts, to;
-        const long long es = block_excl_scan(c, s_w, &ts);
-        // overflowed groups and their candidates in one scan (count above bit 40)
-        const long long eo = block_excl_scan(ovf ? FC_PACK + c : 0, s_w, &to);
-        if (g < ng) {
-            fin_off[g] = (int32_t)(cs + es);
-            if (ovf) {
-                const long long q = co + eo;
-                ovf_list[q / FC_PACK] = g;
-                ovf_pos[q / FC_PACK] = (int32_t)(q % FC_PACK);
+        long long es = cs + block_excl_scan(sc, s_w, &ts);
+        long long eo = co + block_excl_scan(so, s_w, &to);
+        for (int k = 0; k < FC_IPT; k++) {
+            const int g = g0 + k;
+            if (g < ng) {
+                fin_off[g] = (int32_t)es;
+                if (c[k] > cap[g]) {
+                    ovf_list[eo / FC_PACK] = g;
+                    ovf_pos[eo / FC_PACK] = (int32_t)(eo % FC_PACK);
+                    eo += FC_PACK + c[k];
+                }
             }
+            es += c[k];
         }
         cs += ts; co += to;
     }
@@ -149,12 +167,14 @@ __global__ __launch_bounds__(FC_SORT_BLOCK) void k_fc_sort_keys(int ng, const in
                                                                 const int32_t *__restrict__ seg_off,
                                                                 const int32_t *__restrict__ fin_off,
                                                                 const int2 *__restrict__ seg, int kb,
-                                                                uint64_t *__restrict__ keys) {
+                                                                uint64_t *__restrict__ keys, long long keys_cap) {
     __shared__ uint64_t s_k[FC_SORT_MAX];
     const int g = blockIdx.x;
     if (g >= ng) return;
     const int c = cnt[g];
-    if (c == 0 || c > cap[g]) return;
+    // (a group whose keys would pass the end of the key array is left out: the caller
+    // launches the sort again on a larger array)
+    if (c == 0 || c > cap[g] || (long long)fin_off[g] + c > keys_cap) return;
     int P = 1;
     while (P < c) P <<= 1;
     const int2 *src = seg + seg_off[g];
@@ -209,15 +229,6 @@ __global__ void k_fc_sample(int m, int n, long long stride, long long phase, con
     ys[s] = (float)cnt[i];
 }
 
-// the forecast by particle id, so that it follows the particles through the reorders
-__global__ void k_fc_by_id(int n, const int64_t *__restrict__ ids, const float *__restrict__ lam, long long id_cap,
-                           float *__restrict__ lam_id) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t id = ids[i];
-    if (id >= 0 && id < id_cap) lam_id[id] = lam[i];
-}
-
 // ------------------------------------------------------------------ host
 
 int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, const double *mass,
@@ -231,9 +242,8 @@ int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, 
     return GPLUM_OK;
 }
 
-int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, const int64_t *ids, bool fresh, CandBuf *cb,
+int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, float *rate, bool fresh, CandBuf *cb,
             hipStream_t st) {
-    GPLUM_ENSURE(s.lam, n);
     GPLUM_ENSURE(s.seg_off, ng + 1);
     GPLUM_ENSURE(s.seg_cap, ng + 1);
     GPLUM_ENSURE(s.seg_cnt, ng + 1);
@@ -241,16 +251,13 @@ int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, const int64_t *i
     if (s.seg_capacity > (size_t)INT32_MAX) s.seg_capacity = INT32_MAX;
     GPLUM_ENSURE(s.pairs_seg, s.seg_capacity + 1);
     if (!s.uniform && (fresh || !s.lam_valid)) {
-        int rc = s.net.predict(s.x.p, n, s.lam.p, st);
+        int rc = s.net.predict(s.x.p, n, rate, st);
         if (rc) return rc;
-        GPLUM_ENSURE(s.lam_id, (size_t)s.id_cap + 1);
-        hipLaunchKernelGGL(k_fc_by_id, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ids, s.lam.p, s.id_cap,
-                           s.lam_id.p);
         s.lam_valid = true;
     }
     if (ng > 0) {
-        hipLaunchKernelGGL(k_fc_capacity, dim3(ng), dim3(64), 0, st, ng, gv.gstart, gv.gcount, ids, s.lam_id.p,
-                           s.id_cap, s.uniform ? s.uniform_rate : -1.f, s.seg_cap.p);
+        hipLaunchKernelGGL(k_fc_capacity, dim3(ng), dim3(64), 0, st, ng, gv.gstart, gv.gcount, rate,
+                           s.uniform ? s.uniform_rate : -1.f, s.seg_cap.p);
         hipLaunchKernelGGL(k_fc_offsets, dim3(1), dim3(FC_SCAN_BLOCK), 0, st, ng, s.seg_cap.p, s.seg_off.p,
                            (long long)s.seg_capacity, s.info.p);
     }
@@ -263,7 +270,7 @@ int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, const int64_t *i
     return GPLUM_OK;
 }
 
-int fc_collect(CandSizer &s, int ng, int64_t out[4], hipStream_t st) {
+int fc_summary(CandSizer &s, int ng, hipStream_t st) {
     GPLUM_ENSURE(s.fin_off, ng + 1);
     GPLUM_ENSURE(s.ovf_list, ng + 1);
     GPLUM_ENSURE(s.ovf_pos, ng + 1);
@@ -271,15 +278,19 @@ int fc_collect(CandSizer &s, int ng, int64_t out[4], hipStream_t st) {
         hipLaunchKernelGGL(k_fc_summary, dim3(1), dim3(FC_SCAN_BLOCK), 0, st, ng, s.seg_cnt.p, s.seg_cap.p,
                            s.fin_off.p, s.ovf_list.p, s.ovf_pos.p, s.info.p);
     GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
+int fc_read_info(CandSizer &s, int64_t out[4], hipStream_t st) {
     GPLUM_HIP(hipMemcpyAsync(out, s.info.p, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
     return GPLUM_OK;
 }
 
-int fc_sort_keys(CandSizer &s, int ng, int kb, uint64_t *keys, hipStream_t st) {
+int fc_sort_keys(CandSizer &s, int ng, int kb, uint64_t *keys, long long keys_cap, hipStream_t st) {
     if (ng > 0)
         hipLaunchKernelGGL(k_fc_sort_keys, dim3(ng), dim3(FC_SORT_BLOCK), 0, st, ng, s.seg_cnt.p, s.seg_cap.p,
-                           s.seg_off.p, s.fin_off.p, s.pairs_seg.p, kb, keys);
+                           s.seg_off.p, s.fin_off.p, s.pairs_seg.p, kb, keys, keys_cap);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

@@ -34,12 +34,11 @@ struct CandSizer {
     unsigned phase = 0;
     double last_loss = 0.;
     // a fresh forecast (features, network, one training round) every predict_every soft
-    // forces; in between the plan reuses the rates, kept by particle id (ids < id_cap)
+    // forces; in between the plan reuses the rates, which travel with the particles
+    // (ParticleSoA::fc_rate: gathered with every reorder and compaction)
     int predict_every = 4;
     int since_predict = 1 << 30;
     bool lam_valid = false;
-    long long id_cap = 0;
-    DArray<float> lam_id;
     int last_batch = 0;               // samples of the last Adam step (its loss partials stay)
 
     DArray<float> x, lam, xs, ys;     // features [4][n], rates, training batch
@@ -62,17 +61,19 @@ struct FcGroupView {                  // tree data the features read
 // raw or standardised features into s.x (feature-major)
 int fc_features(CandSizer &s, const FcGroupView &gv, int ng, const double *pos, const double *mass,
                 const double *vdisp, double m_sun, int n, hipStream_t st);
-// forecast (fresh: from s.x, else the last one by id) -> per-group capacity -> offsets
+// forecast (fresh: from s.x into rate, the particles' rate array; else rate as the
+// reorders left it) -> per-group capacity -> offsets
 // (clipped to s.seg_capacity); returns the CandBuf for the segment-mode launch
-int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, const int64_t *ids, bool fresh, CandBuf *cb,
+int fc_plan(CandSizer &s, const FcGroupView &gv, int ng, int n, float *rate, bool fresh, CandBuf *cb,
             hipStream_t st);
 // after the segment-mode launch: exact counts -> final offsets of every group's
-// candidates (group order) and the list of overflowed groups;
-// out = {requested slots, total, n_ovf, ovf_total} (synchronises)
-int fc_collect(CandSizer &s, int ng, int64_t out[4], hipStream_t st);
+// candidates (group order) and the list of overflowed groups (on the device)
+int fc_summary(CandSizer &s, int ng, hipStream_t st);
+// out = {requested slots, total, n_ovf, ovf_total} of the last summary (synchronises)
+int fc_read_info(CandSizer &s, int64_t out[4], hipStream_t st);
 // the candidates of the groups that fit their segments as sorted keys i 2^kb + j at
-// their final offsets in keys
-int fc_sort_keys(CandSizer &s, int ng, int kb, uint64_t *keys, hipStream_t st);
+// their final offsets in keys; groups whose keys would pass keys_cap are left out
+int fc_sort_keys(CandSizer &s, int ng, int kb, uint64_t *keys, long long keys_cap, hipStream_t st);
 // the overflowed groups' candidates (sorted[0..ovf_total), ascending) to their offsets
 int fc_scatter_overflow(CandSizer &s, int n_ovf, const uint64_t *sorted, uint64_t *keys, hipStream_t st);
 // s.ovf_groups[0..n_ovf) = the overflowed groups, for the candidate-only pass

@@ -203,6 +203,11 @@ struct HardView {          // SoA arrays of the gathered cluster particles
     double *mass, *rinv, *time, *dt, *acc0, *r_planet, *f, *phi_d, *phi_s;
     int64_t *id;
     int32_t *flags, *twin_of, *act;     // act: active in the current block step
+    // collision records (COLREC doubles each, see perfect_merge), when the caller keeps a
+    // collision log: crec null otherwise
+    double *crec = nullptr;
+    unsigned *ccount = nullptr;
+    unsigned ccap = 0;
 };
 
 __device__ inline void ld3(const double *a, int i, double v[3]) {
@@ -557,6 +562,39 @@ __device__ double perfect_merge(const HP &P, HardView &H, int b, int n, int ic, 
         pos_new[k] = (mass_imp * pos_imp[k] + mass_tar * pos_tar[k]) / (mass_imp + mass_tar);
         vel_new[k] = vel_g[k];
     }
+    if (H.crec) {
+        // Collision0::write2File's record (collisionA.h:122-148; inputPair, setParticle
+        // :155-356 with n_frag = 0): the impactor's cluster-local time, ids, masses with
+        // their twins, positions and velocities at the collision, and the relative shifts
+        // of the centre of mass, |g_new - g| / |g| for position and velocity, where g =
+        // pos_g of inputPair (= pos_new here) and g_new = ((m_imp - 0) pos_new + m_tar
+        // pos_new) / (m_imp - 0 + m_tar) of setParticle.  Slot COLREC - 1: the cluster's
+        // base index, for the log's cluster order.
+        const unsigned k = atomicAdd(H.ccount, 1u);
+        if (k < H.ccap) {
+            double *r = H.crec + (size_t)k * COLREC;
+            double dp[3], dv[3];
+            const double mt = mass_imp - 0. + mass_tar;
+            for (int q = 0; q < 3; q++) {
+                dp[q] = ((mass_imp - 0.) * pos_new[q] + mass_tar * pos_new[q]) / mt - pos_new[q];
+                dv[q] = ((mass_imp - 0.) * vel_new[q] + mass_tar * vel_new[q]) / mt - vel_g[q];
+            }
+            r[0] = H.time[I];
+            r[1] = (double)H.id[I];
+            r[2] = (double)H.id[T];
+            r[3] = mass_imp;
+            r[4] = mass_tar;
+            for (int q = 0; q < 3; q++) {
+                r[5 + q] = pos_imp[q]; r[8 + q] = pos_tar[q];
+                r[11 + q] = vel_imp[q]; r[14 + q] = vel_tar[q];
+            }
+            r[17] = sqrt(dot3(dp[0], dp[1], dp[2], dp[0], dp[1], dp[2]) /
+                         dot3(pos_new[0], pos_new[1], pos_new[2], pos_new[0], pos_new[1], pos_new[2]));
+            r[18] = sqrt(dot3(dv[0], dv[1], dv[2], dv[0], dv[1], dv[2]) /
+                         dot3(vel_g[0], vel_g[1], vel_g[2], vel_g[0], vel_g[1], vel_g[2]));
+            r[COLREC - 1] = (double)b;
+        }
+    }
     const double mass_rem = mass_imp + mass_tar;
     H.r_planet[I] = 0.;
     H.r_planet[T] *= cbrt_exact_ops(mass_rem / mass_tar);
@@ -699,9 +737,10 @@ __device__ inline void wave_sync() {
 // force together (coop_pair_sum); every lane carries the same particle state in
 // registers and lane 0 writes it back.  Active particles only read the predicted
 // j-data, so the waves need not wait for each other.  Twins of a merged target are
-// integrated alongside with their own cut-off radius, as in the per-lane path.
+// integrated alongside with their own cut-off radius, as in the per-lane path.  tw: the
+// cluster's twin_of (cluster-local indices; an LDS copy where the kernel keeps one).
 __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
-                                    int flag_col, long long &pairs, int wave, int n_wave) {
+                                    int flag_col, long long &pairs, int wave, int n_wave, const int32_t *tw) {
     const int lane = (int)(threadIdx.x & 63);
     const bool lead = lane == 0;
     int rank = 0;                          // active particles seen so far (wave-uniform)
@@ -728,7 +767,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
             double sa[3], sj[3], mass = H.mass[g];
             for (int k = 0; k < 3; k++) { sa[k] = H.mass[g] * ad[k]; sj[k] = H.mass[g] * jd[k]; }
             for (int j = 0; j < n; j++) {
-                if (H.twin_of[b + j] != i) continue;
+                if (tw[j] != i) continue;
                 double txp[3], tvp[3], tad[3], tjd[3], tas[3], tjs[3];
                 ld3(H.xp, b + j, txp); ld3(H.vp, b + j, tvp);
                 star_acc_jerk(P, txp, tvp, tas, tjs, nullptr);
@@ -745,7 +784,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
             correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
             if (merged && lead)
                 for (int j = 0; j < n; j++)
-                    if (H.twin_of[b + j] == i) {
+                    if (tw[j] == i) {
                         // the twin's correction from its own predicted state and the merged acc/jerk
                         double txp[3], tvp[3], ta0s[3], tj0s[3], ta0d[3], tj0d[3], tas[3], tjs[3];
                         double tx[3], tv[3], t2s[3], t3s[3], t2d[3], t3d[3];
@@ -764,7 +803,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
                 double sa[3], sj[3], mass = H.mass[g];
                 for (int k = 0; k < 3; k++) { sa[k] = H.mass[g] * ad[k]; sj[k] = H.mass[g] * jd[k]; }
                 for (int j = 0; j < n; j++) {
-                    if (H.twin_of[b + j] != i) continue;
+                    if (tw[j] != i) continue;
                     double tx[3], tv[3], tad[3], tjd[3], tas[3], tjs[3];
                     ld3(H.pos, b + j, tx); ld3(H.vel, b + j, tv);
                     star_acc_jerk(P, tx, tv, tas, tjs, nullptr);
@@ -790,7 +829,7 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
             if (!flag_col) H.dt[g] = dt;
             if (merged)
                 for (int j = 0; j < n; j++)
-                    if (H.twin_of[b + j] == i) {
+                    if (tw[j] == i) {
                         st3(H.acc_d, b + j, ad); st3(H.jerk_d, b + j, jd);
                         H.time[b + j] += t_c;
                         if (!flag_col) H.dt[b + j] = dt;
@@ -809,8 +848,9 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
 struct WaveSlots {
     double ts[8], sys[8], rm[8];
     int lo[8], hi[8];
-    int na;
+    int na, nm;                // active particles without twins (listed in act), merged targets
     int act[HERMITE_LDS];
+    int32_t tw[HERMITE_LDS];   // the cluster's twin_of, copied at each block step's prediction
 };
 
 // Minimum over the workgroup (every lane active, values not NaN): the wave's minimum by
@@ -915,8 +955,8 @@ __device__ inline void group_pair_sum(const HP &P, const JData &J, int n, int s,
     for (int d = 0; d < 3; d++) { acc[d] = c[d]; jerk[d] = c[3 + d]; }
 }
 
-// The active-particle correction of integrate_from (hermite.h:300-375) for clusters
-// with j-data in LDS and no merged particle: the workgroup's lanes are dealt in groups of
+// The active-particle correction of integrate_from (hermite.h:300-375) for the active
+// particles without twins of clusters with j-data in LDS: the workgroup's lanes are dealt in groups of
 // S = 2^s per active particle (the largest S <= 64 with na S <= 512, at least 8; more
 // than 64 active particles take rounds of 64), each group evaluating its particle's hard
 // force by group_pair_sum.  The steps and their order are those of coop_correct_active,
@@ -1042,6 +1082,7 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                 s_id[i] = H.id[b + i];
                 // f R of the live particles for the collision test (-1: dead)
                 s_fr[i] = (H.flags[b + i] & GPLUM_FLAG_DEAD) ? -1. : H.f[b + i] * H.r_planet[b + i];
+                if (COOP) W->tw[i] = H.twin_of[b + i];
             }
         }
         __threadfence_block();
@@ -1057,30 +1098,34 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                 // symmetric bit for bit (squared differences, a commutative sum of f R),
                 // so the (R, lower, higher index) minimum is the (R, i, first j) minimum
                 // of the ordered loop below.
-                const int h = (n - 1) >> 1;
-                const int nh = n * h;
-                const int np = nh + ((n & 1) ? 0 : (n >> 1));
-                for (int k = t; k < np; k += B) {
-                    int a, c;
-                    if (k < nh) {
-                        a = k % n;
-                        c = a + 1 + k / n;
-                        if (c >= n) c -= n;
-                    } else {
-                        a = k - nh;
-                        c = a + (n >> 1);
+                // Thread t takes particle a = t mod n and the r-th of R = B / n runs of
+                // the distances d (a difference of signs leaves d2 unchanged).
+                const int R = max(1, B / n);
+                if (t < n * R) {
+                    const int a = t % n, r = t / n;
+                    const int h = (n - 1) >> 1;
+                    const int dlen = (h + R - 1) / R;
+                    const int d0 = 1 + r * dlen, d1 = min(h, d0 + dlen - 1);
+                    const double fra = s_fr[a];
+                    const double xa = s_x[3 * a], ya = s_x[3 * a + 1], za = s_x[3 * a + 2];
+                    auto pair = [&](int c) {
+                        const double frc = s_fr[c];
+                        if (frc < 0.) return;
+                        const double dx = xa - s_x[3 * c];
+                        const double dy = ya - s_x[3 * c + 1];
+                        const double dz = za - s_x[3 * c + 2];
+                        const double d2 = dot3(dx, dy, dz, dx, dy, dz);
+                        const double r2 = fra + frc;
+                        if (!(d2 < COL_FILTER * r2 * r2)) return;
+                        const double Rr = sqrt(d2) / r2;
+                        const int lo = min(a, c), hi = max(a, c);
+                        if (Rr < 1. && rlh_less(Rr, lo, hi, rmin, bi, bj)) { rmin = Rr; bi = lo; bj = hi; }
+                    };
+                    if (fra >= 0.) {
+                        for (int d = d0; d <= d1; d++) pair(a + d < n ? a + d : a + d - n);
+                        // even n: the pairs at distance n / 2, once each (a < n / 2)
+                        if (!(n & 1) && r == R - 1 && a < (n >> 1)) pair(a + (n >> 1));
                     }
-                    const int lo = min(a, c), hi = max(a, c);
-                    const double frl = s_fr[lo], frh = s_fr[hi];
-                    if (frl < 0. || frh < 0.) continue;
-                    const double dx = s_x[3 * lo] - s_x[3 * hi];
-                    const double dy = s_x[3 * lo + 1] - s_x[3 * hi + 1];
-                    const double dz = s_x[3 * lo + 2] - s_x[3 * hi + 2];
-                    const double d2 = dot3(dx, dy, dz, dx, dy, dz);
-                    const double r2 = frl + frh;
-                    if (!(d2 < COL_FILTER * r2 * r2)) continue;
-                    const double Rr = sqrt(d2) / r2;
-                    if (Rr < 1. && rlh_less(Rr, lo, hi, rmin, bi, bj)) { rmin = Rr; bi = lo; bj = hi; }
                 }
             } else
             for (int i = t; i < n; i += B) {
@@ -1131,41 +1176,46 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         }
         // correct the active particles (hermite.h:300-375)
         if (COOP && LDSJ) {
-            // the active particles, listed in ascending index by the first wave
+            // The active particles: those without twins listed in ascending index by the
+            // first wave, for the lane groups; merged targets (which integrate their twins
+            // alongside, never active themselves) flagged in H.act for coop_correct_active.
+            // The two sets write disjoint particles and read only the predicted j-data.
             if (t < 64) {
-                int base = 0;
+                int base = 0, nm = 0;
                 for (int i0 = 0; i0 < n; i0 += 64) {
                     const int i = i0 + t;
-                    int a = 0;
+                    int a = 0, m = 0;
                     if (i < n) {
-                        a = !(H.flags[b + i] & GPLUM_FLAG_DEAD) && (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
-                        H.act[b + i] = a;
+                        const int f = H.flags[b + i];
+                        a = !(f & GPLUM_FLAG_DEAD) && (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
+                        m = a && (f & GPLUM_FLAG_MERGED);
+                        H.act[b + i] = m;
+                        a = a && !m;
                     }
-                    const unsigned long long m = __ballot(a);
-                    if (a) W->act[base + __popcll(m & ((1ull << t) - 1ull))] = i;
-                    base += __popcll(m);
+                    const unsigned long long ma = __ballot(a);
+                    if (a) W->act[base + __popcll(ma & ((1ull << t) - 1ull))] = i;
+                    base += __popcll(ma);
+                    nm += __popcll(__ballot(m));
                 }
-                if (t == 0) W->na = base;
+                if (t == 0) { W->na = base; W->nm = nm; }
             }
             __threadfence_block();
             __syncthreads();
-            if (n_col == 0) {
-                // no merged particle: lane groups (the same sums as coop_correct_active)
-                const int na = W->na;
-                const int K = (n + 63) >> 6;
-                if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
-                else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
-                else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
-            } else {
-                coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6);
-            }
+            const int na = W->na;
+            const int K = (n + 63) >> 6;
+            if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+            else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+            else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+            if (W->nm > 0)
+                coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6, W->tw);
         } else if (COOP) {
             for (int i = t; i < n; i += B)
                 H.act[b + i] = !(H.flags[b + i] & GPLUM_FLAG_DEAD) &&
                                (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
             __threadfence_block();
             __syncthreads();
-            coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6);
+            coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6,
+                                H.twin_of + b);
         } else
         for (int i = t; i < n; i += B) {
             const int g = b + i;
@@ -2102,10 +2152,16 @@ HP make_hp(const gplum_params &p) {
 }
 
 static HardView hard_view(HardBuffers &hb) {
-    return HardView{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
-                    hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
-                    hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
-                    hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p, hb.act.p};
+    HardView H{hb.pos.p, hb.vel.p, hb.acc_s.p, hb.jerk_s.p, hb.acc_d.p, hb.jerk_d.p, hb.xp.p, hb.vp.p,
+               hb.a0_s.p, hb.j0_s.p, hb.a0_d.p, hb.j0_d.p, hb.a2_s.p, hb.a3_s.p, hb.a2_d.p, hb.a3_d.p,
+               hb.mass.p, hb.rinv.p, hb.time.p, hb.dt.p, hb.acc0.p, hb.r_planet.p, hb.f.p, hb.phi_d.p,
+               hb.phi_s.p, hb.id.p, hb.flags.p, hb.twin_of.p, hb.act.p};
+    if (hb.crec_cap > 0) {
+        H.crec = hb.crec.p;
+        H.ccount = hb.ccount.p;
+        H.ccap = hb.crec_cap;
+    }
+    return H;
 }
 
 
@@ -2286,7 +2342,7 @@ __global__ __launch_bounds__(GIANT_T) void k_g_correct(HP P, HardView H, int b, 
     long long pairs = 0;
     const int waves = (int)(blockDim.x >> 6);
     coop_correct_active(P, H, J, b, n, G->t_s, G->flag_col, pairs, (int)blockIdx.x * waves + (int)(threadIdx.x >> 6),
-                        (int)gridDim.x * waves);
+                        (int)gridDim.x * waves, H.twin_of + b);
     if ((threadIdx.x & 63) == 0 && pairs) atomicAdd((unsigned long long *)&G->pairs, (unsigned long long)pairs);
 }
 

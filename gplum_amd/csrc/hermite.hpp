@@ -14,6 +14,8 @@ constexpr int HARD_SEQ_MAX = 64;
 // Clusters of 2 .. TERMS_MAX bodies spread each block step over one lane per particle,
 // term (Sun or partner) and component (4 n^2 <= 64 lanes).
 constexpr int TERMS_MAX = 4;
+// Doubles per collision record (perfect_merge, for the collision log).
+constexpr int COLREC = 20;
 
 // Gathered cluster particles (SoA, vectors interleaved x,y,z).
 struct HardBuffers {
@@ -28,6 +30,11 @@ struct HardBuffers {
     DArray<unsigned char> g_state;      // giant-cluster loop state
     DArray<double> g_r;                 // giant-cluster collision partial minima
     DArray<int> g_i, g_j;
+    // collision records of the last integration (crec_cap > 0: recorded; the caller
+    // sizes the buffers and zeroes ccount before the launch)
+    DArray<double> crec;
+    DArray<unsigned> ccount;
+    unsigned crec_cap = 0;
     int max_size = 0;
     int ensure(int n) {
         GPLUM_ENSURE(pos, 3 * n); GPLUM_ENSURE(vel, 3 * n); GPLUM_ENSURE(acc_s, 3 * n);

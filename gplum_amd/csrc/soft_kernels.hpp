@@ -5,8 +5,10 @@
 // a pair of floats hi = (float)d, lo = (float)(d - hi).  The neighbour predicate
 // uses rx_pikg = xj_hi - xi_hi, which is exactly the PIKG kernel's relative
 // coordinate (gravity_kernel_epep.pikg:53-60), so the FP32 neighbour count stays
-// bit-identical to the oracle; the force uses rx = rx_pikg + (xj_lo - xi_lo), whose
-// error is relative to |r_ij| rather than to |x_j - x_ref|.
+// bit-identical to the oracle; the force of a near pair uses rx = rx_pikg + (xj_lo -
+// xi_lo), whose error is relative to |r_ij| rather than to |x_j - x_ref|.  Far EP-EP
+// pairs (beyond 1.1 max r_search) and EP-SP pairs use rx_pikg itself: at their
+// distances an ulp of |x - x_ref| is small against |r_ij|.
 #pragma once
 #include "soft.hpp"
 
@@ -67,6 +69,14 @@ __device__ inline float rsqrt_refined(float r2) {
     return __builtin_fmaf(0.5f * y, e, y);
 }
 
+// 1/sqrt(r2) by v_rsq_f32 and one textbook Newton step y (3 - r2 y^2) / 2
+// (gravity_kernel_epep.pikg:89-91): for the far pairs, whose summands are small beside the
+// near pairs' and carry no neighbour decision.
+__device__ inline float rsqrt_newton(float r2) {
+    const float y = __builtin_amdgcn_rsqf(r2);
+    return y * __builtin_fmaf(-0.5f * r2, y * y, 1.5f);
+}
+
 // Appends (idi, idj) for the lanes with nb set: one ballot, one atomic per wave.
 // Called by all lanes of a wave together.
 __device__ inline void emit_candidate(bool nb, int idi, int idj, const CandBuf &cand) {
@@ -114,13 +124,15 @@ __device__ inline void epep_near(const IState &I, bool valid, float4 pj, float4 
     a.p -= mr;
 }
 
-// EP-EP pair known to lie beyond 1.1 max(r_search) > r_out: no predicate, no clamp.
-__device__ inline void epep_far(const IState &I, float4 pj, float4 lj, float eps2, Acc32 &a) {
-    const float rx = (pj.x - I.xh) + (lj.x - I.xl);
-    const float ry = (pj.y - I.yh) + (lj.y - I.yl);
-    const float rz = (pj.z - I.zh) + (lj.z - I.zl);
+// EP-EP pair known to lie beyond 1.1 max(r_search) > r_out: no predicate, no clamp.  At
+// this distance the rounded parts alone give the relative coordinate (their error, half
+// an ulp of |x - x_ref| each, is small against |r_ij|), and 1/r takes the textbook step.
+__device__ inline void epep_far(const IState &I, float4 pj, float eps2, Acc32 &a) {
+    const float rx = pj.x - I.xh;
+    const float ry = pj.y - I.yh;
+    const float rz = pj.z - I.zh;
     const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, eps2)));
-    const float rinv = rsqrt_refined(r2);
+    const float rinv = rsqrt_newton(r2);
     const float mr = pj.w * rinv;
     const float mr3 = (rinv * rinv) * mr;
     a.x = __builtin_fmaf(mr3, rx, a.x);
@@ -129,14 +141,15 @@ __device__ inline void epep_far(const IState &I, float4 pj, float4 lj, float eps
     a.p -= mr;
 }
 
+// EP-SP pair: a node opened by neither the opening angle nor the search radius, as far
+// from the group as the far EP-EP pairs or farther (rounded parts, textbook step).
 template <bool QUAD>
-__device__ inline void epsp_pair(const IState &I, float4 pj, float4 lj, float4 qa, float4 qb, float eps2,
-                                 Acc32 &a) {
-    const float rx = (pj.x - I.xh) + (lj.x - I.xl);
-    const float ry = (pj.y - I.yh) + (lj.y - I.yl);
-    const float rz = (pj.z - I.zh) + (lj.z - I.zl);
+__device__ inline void epsp_pair(const IState &I, float4 pj, float4 qa, float4 qb, float eps2, Acc32 &a) {
+    const float rx = pj.x - I.xh;
+    const float ry = pj.y - I.yh;
+    const float rz = pj.z - I.zh;
     const float r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, eps2)));
-    const float rinv = rsqrt_refined(r2);
+    const float rinv = rsqrt_newton(r2);
     const float r2inv = rinv * rinv;
     if (QUAD) {
         const float r3inv = r2inv * rinv;
@@ -215,7 +228,8 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
     double ax = 0., ay = 0., az = 0., pot = 0.;
     int ngb = 0, rankf = 0, idmax = ID_MAX_CLEAR, idmin = 2147483647;
 
-    auto stage_j = [&](const JS &src, int e) {
+    // NEAR: the near list (predicate, clamp, low parts); else the far list (rounded parts)
+    auto stage_j = [&](const JS &src, int e, bool near) {
         double x, y, z, m, ro, rs;
         int id, rk;
         src.load(e, x, y, z, m, ro, rs, id, rk);
@@ -223,10 +237,12 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
         split(x - x0, xh, xl);
         split(y - y0, yh, yl);
         split(z - z0, zh, zl);
-        const float rof = (float)ro, rsf = (float)rs;
         s_a[t] = make_float4(xh, yh, zh, (float)m);
-        s_l[t] = make_float4(xl, yl, zl, 0.f);
-        s_b[t] = make_float4(rof * rof, (rsf * rsf) * 1.0201f, __int_as_float(id), __int_as_float(rk));
+        if (near) {
+            const float rof = (float)ro, rsf = (float)rs;
+            s_l[t] = make_float4(xl, yl, zl, 0.f);
+            s_b[t] = make_float4(rof * rof, (rsf * rsf) * 1.0201f, __int_as_float(id), __int_as_float(rk));
+        }
     };
 
     if (MODE & EP) {
@@ -234,7 +250,7 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
         for (int base = 0; base < G.nj; base += tile) {
             const int cnt = min(tile, G.nj - base);
             __syncthreads();
-            if (t < cnt) stage_j(js, G.j_begin + base + t);
+            if (t < cnt) stage_j(js, G.j_begin + base + t, true);
             __syncthreads();
             if (S.in_split) {
                 for (int k0 = S.sp; k0 < cnt; k0 += SOFT_FOLD * S.nsplit) {
@@ -251,13 +267,13 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
         for (int base = 0; base < ((MODE & NOFAR) ? 0 : G.njf); base += tile) {
             const int cnt = min(tile, G.njf - base);
             __syncthreads();
-            if (t < cnt) stage_j(jf, G.jf_begin + base + t);
+            if (t < cnt) stage_j(jf, G.jf_begin + base + t, false);
             __syncthreads();
             if (S.in_split) {
                 for (int k0 = S.sp; k0 < cnt; k0 += SOFT_FOLD * S.nsplit) {
                     Acc32 a;
                     const int k1 = min(cnt, k0 + SOFT_FOLD * S.nsplit);
-                    for (int k = k0; k < k1; k += S.nsplit) epep_far(I, s_a[k], s_l[k], eps2, a);
+                    for (int k = k0; k < k1; k += S.nsplit) epep_far(I, s_a[k], eps2, a);
                     ax += a.x; ay += a.y; az += a.z; pot += a.p;
                 }
             }
@@ -271,12 +287,7 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
             if (t < cnt) {
                 double m, x, y, z, q[6];
                 ss.load(G.s_begin + base + t, m, x, y, z, q);
-                float xh, xl, yh, yl, zh, zl;
-                split(x - x0, xh, xl);
-                split(y - y0, yh, yl);
-                split(z - z0, zh, zl);
-                s_a[t] = make_float4(xh, yh, zh, (float)m);
-                s_l[t] = make_float4(xl, yl, zl, 0.f);
+                s_a[t] = make_float4((float)(x - x0), (float)(y - y0), (float)(z - z0), (float)m);
                 if (MODE & SPQ) {
                     // per-SPJ hoisted terms of gravity_kernel_epsp.pikg:75-82 (FP32)
                     const float qxx0 = (float)q[0], qyy0 = (float)q[1], qzz0 = (float)q[2];
@@ -292,7 +303,7 @@ __global__ __launch_bounds__(SOFT_BLOCK) void k_soft_f32(const SoftGroup *__rest
                     Acc32 a;
                     const int k1 = min(cnt, k0 + SOFT_FOLD * S.nsplit);
                     for (int k = k0; k < k1; k += S.nsplit)
-                        epsp_pair<(MODE & SPQ) != 0>(I, s_a[k], s_l[k], s_b[k], s_c[k], eps2, a);
+                        epsp_pair<(MODE & SPQ) != 0>(I, s_a[k], s_b[k], s_c[k], eps2, a);
                     ax += a.x; ay += a.y; az += a.z; pot += a.p;
                 }
             }

@@ -38,6 +38,7 @@ int ParticleSoA::ensure(int n) {
     GPLUM_ENSURE(time, n); GPLUM_ENSURE(dt, n); GPLUM_ENSURE(acc0, n);
     GPLUM_ENSURE(number, n); GPLUM_ENSURE(rank, n); GPLUM_ENSURE(id_max, n); GPLUM_ENSURE(id_min, n);
     GPLUM_ENSURE(n_cluster, n); GPLUM_ENSURE(flags, n);
+    GPLUM_ENSURE(fc_rate, n);
     return GPLUM_OK;
 }
 
@@ -49,6 +50,7 @@ struct SoAPtrs {
     double *v1[13];     // mass, r_planet, f, phi, phi_d, phi_s, r_out, r_out_inv, r_search, v_disp, time, dt, acc0
     int64_t *l[2];      // id, id_cluster
     int32_t *w[6];      // number, rank, id_max, id_min, n_cluster, flags
+    float *f1;          // fc_rate
 };
 
 static SoAPtrs soa_ptrs(ParticleSoA &A) {
@@ -56,7 +58,8 @@ static SoAPtrs soa_ptrs(ParticleSoA &A) {
                    {A.mass.p, A.r_planet.p, A.f.p, A.phi.p, A.phi_d.p, A.phi_s.p, A.r_out.p, A.r_out_inv.p,
                     A.r_search.p, A.v_disp.p, A.time.p, A.dt.p, A.acc0.p},
                    {A.id.p, A.id_cluster.p},
-                   {A.number.p, A.rank.p, A.id_max.p, A.id_min.p, A.n_cluster.p, A.flags.p}};
+                   {A.number.p, A.rank.p, A.id_max.p, A.id_min.p, A.n_cluster.p, A.flags.p},
+                   A.fc_rate.p};
 }
 
 __global__ void k_gather_all(int n, const int32_t *__restrict__ perm, SoAPtrs src, SoAPtrs dst) {
@@ -73,6 +76,7 @@ __global__ void k_gather_all(int n, const int32_t *__restrict__ perm, SoAPtrs sr
     for (int k = 0; k < 2; k++) dst.l[k][i] = src.l[k][s];
 #pragma unroll
     for (int k = 0; k < 6; k++) dst.w[k][i] = src.w[k][s];
+    dst.f1[i] = src.f1[s];
 }
 
 __global__ void k_jrec(int n, const double *__restrict__ pos, const double *__restrict__ mass,
@@ -287,15 +291,8 @@ int System::set_particles(const gplum_particle *pp, int count, double time) {
     P1(v_disp); P1(time); P1(dt); P1(acc0);
 #undef P3
 #undef P1
-    long long id_max = -1;
-    for (int i = 0; i < n; i++) {
-        i64[i] = pp[i].id;
-        id_max = std::max<long long>(id_max, (long long)pp[i].id);
-    }
+    for (int i = 0; i < n; i++) i64[i] = pp[i].id;
     GPLUM_HIP(hipMemcpy(A.id.p, i64.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
-    // the forecaster keeps its rates by id: ids at most a few times the particle count
-    // (others are sized with a zero rate, i.e. overflow to the exact pass)
-    fc.id_cap = std::min<long long>(id_max + 1, 4ll * n + 1024);
     fc.lam_valid = false;
     for (int i = 0; i < n; i++) i64[i] = pp[i].id_cluster;
     GPLUM_HIP(hipMemcpy(A.id_cluster.p, i64.data(), 8 * (size_t)n, hipMemcpyHostToDevice));
@@ -456,7 +453,7 @@ int System::soft_force(bool initial, bool use_f64) {
         if (fc.seg_capacity == 0) fc.seg_capacity = (size_t)last_ncand + last_ncand / 4 + 16 * (size_t)ng + 4096;
         if (fc.uniform) fc.uniform_rate = n > 0 ? (float)((double)last_ncand / n) : 0.f;
         CandBuf cb{nullptr, nullptr, 0u};
-        if ((rc = fc_plan(fc, gv, ng, n, A.id.p, fc_fresh, &cb, st))) return rc;
+        if ((rc = fc_plan(fc, gv, ng, n, A.fc_rate.p, fc_fresh, &cb, st))) return rc;
         tick(EV_FCA1);
         tick(EV_FORCE0);
         if (p.quadrupole)
@@ -468,16 +465,24 @@ int System::soft_force(bool initial, bool use_f64) {
         tick(EV_FORCE1);
         GPLUM_HIP(hipGetLastError());
         tick(EV_FCB0);
-        int64_t info[4] = {0, 0, 0, 0};
-        if ((rc = fc_collect(fc, ng, info, st))) return rc;
-        const int64_t requested = info[0], total = info[1], n_ovf = info[2], ovf_total = info[3];
-        if (total > (int64_t)INT32_MAX) return fail(GPLUM_E_OVERFLOW, "candidate count exceeds 2^31");
         // the candidates arrive grouped by i-group (ascending i): each group's segment is
         // sorted in LDS straight into the finalize's key array, so the global sort of
-        // the other modes is not needed
+        // the other modes is not needed.  The sort is queued before the counts are read
+        // back, on a key array sized by the segment storage and the last call's total
+        // (sorted again on a larger one in the rare call whose total passes it).
         const int kb = cand_key_bits(n);
-        GPLUM_ENSURE(nb.keys, (size_t)total + 1);
-        if ((rc = fc_sort_keys(fc, ng, kb, nb.keys.p, st))) return rc;
+        if ((rc = fc_summary(fc, ng, st))) return rc;
+        const size_t keys_cap = std::max(fc.seg_capacity, (size_t)last_ncand + last_ncand / 4) + 1;
+        GPLUM_ENSURE(nb.keys, keys_cap);
+        if ((rc = fc_sort_keys(fc, ng, kb, nb.keys.p, (long long)keys_cap, st))) return rc;
+        int64_t info[4] = {0, 0, 0, 0};
+        if ((rc = fc_read_info(fc, info, st))) return rc;
+        const int64_t requested = info[0], total = info[1], n_ovf = info[2], ovf_total = info[3];
+        if (total > (int64_t)INT32_MAX) return fail(GPLUM_E_OVERFLOW, "candidate count exceeds 2^31");
+        if ((size_t)total + 1 > keys_cap) {
+            GPLUM_ENSURE(nb.keys, (size_t)total + 1);
+            if ((rc = fc_sort_keys(fc, ng, kb, nb.keys.p, (long long)total + 1, st))) return rc;
+        }
         if (n_ovf > 0) {
             if ((rc = fc_overflow_groups(fc, groups.p, (int)n_ovf, st))) return rc;
             GPLUM_ENSURE(cand, (size_t)ovf_total + 1);
@@ -617,6 +622,72 @@ int System::kick() {
     return GPLUM_OK;
 }
 
+// The per-cluster outcome of the hard part reduced on the device (one workgroup), so
+// that the host reads a few words instead of every cluster's entries:
+// out[0] block steps, out[1] the largest count of one cluster, out[2] that cluster's size
+// (the first such cluster), out[3] pair evaluations, out[4] collisions, out[5] 1 if a
+// cluster stopped at the block-step cap; *ed_sum the dissipated energy summed over the
+// clusters with a collision in ascending cluster order (the others contribute +0).
+constexpr int HSTAT_BLOCK = 1024;
+__global__ __launch_bounds__(HSTAT_BLOCK) void k_hard_stats(int ncl, const int *__restrict__ ncol,
+                                                            const double *__restrict__ ed,
+                                                            const long long *__restrict__ nblock,
+                                                            const long long *__restrict__ npair,
+                                                            const int32_t *__restrict__ off, int32_t *__restrict__ col,
+                                                            long long *__restrict__ out, double *__restrict__ ed_sum) {
+    __shared__ unsigned long long s_sum[3];
+    __shared__ unsigned long long s_max;
+    __shared__ int s_cap, s_base, s_w[HSTAT_BLOCK / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) { s_sum[0] = s_sum[1] = s_sum[2] = 0ull; s_max = 0ull; s_cap = 0; s_base = 0; }
+    __syncthreads();
+    unsigned long long bs = 0, pr = 0, nc = 0, mx = 0;
+    for (int c0 = 0; c0 < ncl; c0 += HSTAT_BLOCK) {
+        const int c = c0 + t;
+        int hit = 0;
+        if (c < ncl) {
+            const long long b = nblock[c];
+            if (b < 0) s_cap = 1;
+            else {
+                bs += (unsigned long long)b;
+                // largest count first, then the lowest cluster index
+                const unsigned long long key = ((unsigned long long)b << 32) | (unsigned)(0x7fffffff - c);
+                mx = key > mx ? key : mx;
+            }
+            pr += (unsigned long long)npair[c];
+            nc += (unsigned long long)ncol[c];
+            hit = ncol[c] > 0;
+        }
+        // clusters with a collision, listed in ascending index
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) s_w[w] = __popcll(m);
+        __syncthreads();
+        int before = s_base;
+        for (int k = 0; k < w; k++) before += s_w[k];
+        if (hit) col[before + __popcll(m & ((1ull << lane) - 1ull))] = c;
+        __syncthreads();
+        if (t == 0) for (int k = 0; k < HSTAT_BLOCK / 64; k++) s_base += s_w[k];
+        __syncthreads();
+    }
+    atomicAdd(&s_sum[0], bs);
+    atomicAdd(&s_sum[1], pr);
+    atomicAdd(&s_sum[2], nc);
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    if (t == 0) {
+        const int cm = 0x7fffffff - (int)(unsigned)(s_max & 0xffffffffull);
+        out[0] = (long long)s_sum[0];
+        out[1] = (long long)(s_max >> 32);
+        out[2] = (s_max != 0ull && cm >= 0 && cm < ncl) ? (long long)(off[cm + 1] - off[cm]) : 0;
+        out[3] = (long long)s_sum[1];
+        out[4] = (long long)s_sum[2];
+        out[5] = s_cap;
+        double e = 0.;
+        for (int k = 0; k < s_base; k++) e += ed[col[k]];
+        *ed_sum = e;
+    }
+}
+
 // hard part: clusters + isolated particles (HardSystem::timeIntegrate, hard.h:554-829)
 int System::hard_part(bool persistent) {
     int rc;
@@ -644,6 +715,15 @@ int System::hard_part(bool persistent) {
         HardPtrs H = hard_ptrs();
         PartPtrs P = ptrs(A);
         hipLaunchKernelGGL(k_hard_gather, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, H, P, time_c.p);
+        // collision records for the collision log (at most one collision per particle)
+        hb.crec_cap = 0;
+        if (log_col) {
+            const unsigned cap = (unsigned)std::min<long long>((long long)m / 2 + 16, 1ll << 20);
+            GPLUM_ENSURE(hb.crec, (size_t)cap * COLREC);
+            GPLUM_ENSURE(hb.ccount, 1);
+            GPLUM_HIP(hipMemsetAsync(hb.ccount.p, 0, sizeof(unsigned), st));
+            hb.crec_cap = cap;
+        }
         if (persistent) {
             tick(EV_HK0);
             if ((rc = hard_launch(p, hb, ncl, cl.off.p, cl.order.p, true, n_cu, st, st2, ev_fork, ev_join,
@@ -665,28 +745,26 @@ int System::hard_part(bool persistent) {
         }
         hipLaunchKernelGGL(k_hard_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.index.p, cl.off.p, ncl,
                            H, P, time_c.p);
-        std::vector<int> nc(ncl);
-        std::vector<double> ed(ncl);
-        std::vector<long long> nbk(ncl);
-        GPLUM_HIP(hipMemcpyAsync(nc.data(), hb.ncol.p, sizeof(int) * ncl, hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipMemcpyAsync(ed.data(), hb.edisp.p, sizeof(double) * ncl, hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipMemcpyAsync(nbk.data(), hb.nblock.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
-        std::vector<long long> npr(ncl);
-        GPLUM_HIP(hipMemcpyAsync(npr.data(), hb.npair.p, sizeof(long long) * ncl, hipMemcpyDeviceToHost, st));
-        std::vector<int32_t> off(ncl + 1);
-        GPLUM_HIP(hipMemcpyAsync(off.data(), cl.off.p, sizeof(int32_t) * (ncl + 1), hipMemcpyDeviceToHost, st));
+        GPLUM_ENSURE(hstat, 8);
+        GPLUM_ENSURE(hstat_col, ncl + 1);
+        hipLaunchKernelGGL(k_hard_stats, dim3(1), dim3(HSTAT_BLOCK), 0, st, ncl, hb.ncol.p, hb.edisp.p, hb.nblock.p,
+                           hb.npair.p, cl.off.p, hstat_col.p, hstat.p, reinterpret_cast<double *>(hstat.p + 6));
+        GPLUM_HIP(hst.ensure(sizeof(long long) * 8));
+        long long *hs = hst.as<long long>();
+        GPLUM_HIP(hipMemcpyAsync(hs, hstat.p, sizeof(long long) * 7, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
-        for (int c = 0; c < ncl; c++) {
-            if (nbk[c] < 0) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
-            step_ncol += nc[c];
-            step_edisp += ed[c];
-            stats.block_steps += nbk[c];
-            if (nbk[c] > stats.max_block_steps) {
-                stats.max_block_steps = nbk[c];
-                stats.max_block_cluster = off[c + 1] - off[c];
-            }
-            stats.hard_pairs += npr[c];
+        if (hs[5]) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
+        double ed_sum;
+        std::memcpy(&ed_sum, &hs[6], sizeof(double));
+        stats.block_steps += hs[0];
+        if (hs[1] > stats.max_block_steps) {
+            stats.max_block_steps = hs[1];
+            stats.max_block_cluster = (int)hs[2];
         }
+        stats.hard_pairs += hs[3];
+        step_ncol = (int)hs[4];
+        step_edisp = ed_sum;
+        if (log_col && step_ncol > 0 && (rc = write_collision_log())) return rc;
     }
     GPLUM_HIP(hipStreamWaitEvent(st, ev_iso_join, 0));
     tick(EV_HARD1);
@@ -696,6 +774,39 @@ int System::hard_part(bool persistent) {
         stats.hard_kernel_ms += elapsed(EV_HK0, EV_HK1);
         stats.n_hard_launches++;
     }
+    return GPLUM_OK;
+}
+
+// The collision log (Collision0::write2File, collisionA.h:122-148; written by rank 0 in
+// the order of the collision list, hard.h:978): one line per merger of the last hard
+// part, clusters in processing order and each cluster's mergers in time order.  Perfect
+// merging: no fragments (n_frag 0, id_frag 0, mass_frag 0) and Flag = flag_merge = 1.
+int System::write_collision_log() {
+    GPLUM_HIP(hst.ensure(sizeof(unsigned)));
+    GPLUM_HIP(hipMemcpyAsync(hst.p, hb.ccount.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    const unsigned k = *hst.as<unsigned>();
+    if (k > hb.crec_cap) return fail(GPLUM_E_OVERFLOW, "more collisions than collision-log records");
+    if (k == 0) return GPLUM_OK;
+    GPLUM_HIP(hst.ensure(sizeof(double) * COLREC * (size_t)k));
+    const double *r = hst.as<double>();
+    GPLUM_HIP(hipMemcpyAsync(hst.p, hb.crec.p, sizeof(double) * COLREC * (size_t)k, hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    std::vector<unsigned> ord(k);
+    for (unsigned a = 0; a < k; a++) ord[a] = a;
+    std::sort(ord.begin(), ord.end(), [&](unsigned x, unsigned y) {
+        const double *u = r + (size_t)x * COLREC, *v = r + (size_t)y * COLREC;
+        if (u[COLREC - 1] != v[COLREC - 1]) return u[COLREC - 1] < v[COLREC - 1];
+        return u[0] < v[0];
+    });
+    for (unsigned a : ord) {
+        const double *q = r + (size_t)a * COLREC;
+        std::fprintf(log_col, "%.8f\t%lld\t%lld\t0\t0\t%.15e\t%.15e\t%.15e", time_sys + q[0], (long long)q[1],
+                     (long long)q[2], q[3], q[4], 0.);
+        for (int c = 5; c < 17; c++) std::fprintf(log_col, "\t%.15e", q[c]);
+        std::fprintf(log_col, "\t1\t%.15e\t%.15e\n", q[17], q[18]);
+    }
+    std::fflush(log_col);
     return GPLUM_OK;
 }
 
@@ -724,6 +835,15 @@ __global__ void k_compact_idx(int n, const int32_t *__restrict__ flag, const int
                               int32_t *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && flag[i]) out[scan[i]] = i;
+}
+
+// the flags and ids of the listed particles (the merged bodies found by k_post_flags)
+__global__ void k_sel_info(int k, const int32_t *__restrict__ sel, const int32_t *__restrict__ flags,
+                           const int64_t *__restrict__ id, int32_t *__restrict__ fl_out, int64_t *__restrict__ id_out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= k) return;
+    fl_out[a] = flags[sel[a]];
+    id_out[a] = id[sel[a]];
 }
 
 // one thread per target: its twins in ascending index, the reference's arithmetic
@@ -816,25 +936,32 @@ int System::merge_and_remove(int *n_removed) {
     GPLUM_HIP(hipMemsetAsync(M.count.p, 0, 2 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_post_flags, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.flags.p,
                        r_max * r_max, r_min * r_min, M.flag.p, M.keep.p, M.rem.p, M.count.p);
-    int32_t cnt[2] = {0, 0};
-    GPLUM_HIP(hipMemcpyAsync(cnt, M.count.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hst.ensure(sizeof(int32_t) * 2));
+    GPLUM_HIP(hipMemcpyAsync(hst.p, M.count.p, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
-    const int k = cnt[0], n_rem = cnt[1];
+    const int k = hst.as<int32_t>()[0], n_rem = hst.as<int32_t>()[1];
     if (k == 0 && n_rem == 0) return GPLUM_OK;
     int rc;
     if (k > 0) {
         if ((rc = excl_sum(rad.tmp, M.flag.p, M.scan.p, n, st))) return rc;
         GPLUM_ENSURE(M.sel, k);
         hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.flag.p, M.scan.p, M.sel.p);
-        std::vector<int32_t> sel(k), fl(k);
-        std::vector<int64_t> ids(k);
-        GPLUM_HIP(hipMemcpyAsync(sel.data(), M.sel.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
+        // the listed particles' index, flags and id in one read
+        GPLUM_ENSURE(M.sel_info, 2 * (size_t)k);
+        int32_t *d_fl = reinterpret_cast<int32_t *>(M.sel_info.p);
+        int64_t *d_id = M.sel_info.p + k;
+        hipLaunchKernelGGL(k_sel_info, dim3(ceil_div(k, 64)), dim3(64), 0, st, k, M.sel.p, A.flags.p, A.id.p, d_fl,
+                           d_id);
+        GPLUM_HIP(hst.ensure(16 * (size_t)k));
+        int64_t *h_id = hst.as<int64_t>();
+        int32_t *h_sel = reinterpret_cast<int32_t *>(h_id + k);
+        int32_t *h_fl = h_sel + k;
+        GPLUM_HIP(hipMemcpyAsync(h_id, d_id, sizeof(int64_t) * k, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(h_sel, M.sel.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(h_fl, d_fl, sizeof(int32_t) * k, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
-        for (int a = 0; a < k; a++) {
-            GPLUM_HIP(hipMemcpyAsync(&fl[a], A.flags.p + sel[a], sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            GPLUM_HIP(hipMemcpyAsync(&ids[a], A.id.p + sel[a], sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        }
-        GPLUM_HIP(hipStreamSynchronize(st));
+        const std::vector<int32_t> sel(h_sel, h_sel + k), fl(h_fl, h_fl + k);
+        const std::vector<int64_t> ids(h_id, h_id + k);
         // pair twins with their target (same id), both in ascending index
         std::vector<int32_t> tgt, off{0}, twin;
         for (int a = 0; a < k; a++) {
@@ -920,7 +1047,8 @@ int System::merge_and_remove(int *n_removed) {
     }
     // removeParticle: keep the live particles inside the boundary in their order
     if ((rc = excl_sum(rad.tmp, M.keep.p, M.kscan.p, n, st))) return rc;
-    int32_t tail[2] = {0, 0};
+    GPLUM_HIP(hst.ensure(sizeof(int32_t) * 2));
+    int32_t *tail = hst.as<int32_t>();
     GPLUM_HIP(hipMemcpyAsync(&tail[0], M.keep.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipMemcpyAsync(&tail[1], M.kscan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));

@@ -38,6 +38,7 @@ struct MergeScratch {                 // MergeParticle + boundary removal on the
     DArray<int32_t> tgt, twin_off, twin;
     DArray<double> ed, rdata;
     DArray<int64_t> rid;
+    DArray<int64_t> sel_info;         // flags (as int32) and ids of the listed particles
 };
 
 // A particle that left the system through removeParticlesOutOfBoundary (func.h:209-380),
@@ -87,6 +88,9 @@ struct System {
     ClusterList cl;
     Scratch scratch;
     HardBuffers hb;
+    DArray<long long> hstat;          // k_hard_stats output
+    DArray<int32_t> hstat_col;        // clusters with a collision (k_hard_stats)
+    HostBuf hst;                      // pinned host side of the step's small reads
     DArray<double> time_c, eparts, eparts_step;
     double e_init[5] = {0., 0., 0., 0., 0.};
     // main loop (main_p3t.cpp:423-699): boundary of removeParticlesOutOfBoundary, the
@@ -122,6 +126,7 @@ struct System {
     int autotune(int trials, gplum_tuning *out);
     PartPtrs ptrs(ParticleSoA &s);
     HardPtrs hard_ptrs();
+    int write_collision_log();
     void tick(int k);
     double elapsed(int a, int b);
 };

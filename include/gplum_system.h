@@ -169,7 +169,8 @@ int gplum_system_set_tuning(gplum_system *s, const gplum_tuning *t);
  * grow as std::vectors; see include/gplum_forecast.h).  mode 1 trains the forecaster
  * network (train_steps Adam steps of batch samples, learning rate lr, on its own stream
  * while the next hard part runs) and, once trained, sizes the EP-EP candidate buffer
- * per i-group from its forecast (on the matrix cores, kept by particle id); a fresh
+ * per i-group from its forecast (on the matrix cores; the rates travel with the
+ * particles through every reorder); a fresh
  * forecast and training round are made every `every` soft forces (default 4, see
  * gplum_system_set_forecast_every); overflowed groups are redone exactly, so particles, neighbour
  * lists and clusters are identical in every mode.  mode 2 is the same per-group

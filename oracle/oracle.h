@@ -105,6 +105,10 @@ void orc_calc_dt_initial(const gplum_params *p, gplum_particle *pi);
  * *edisp receives the energy dissipated (src/collisionA.h:358-669). */
 int orc_hermite_cluster(const gplum_params *p, gplum_particle *pp, int n,
                         double *edisp, int64_t *n_block_steps);
+/* The same with the collision log's records (20 doubles per merger, time order; see
+ * collision_record in orc_hard.c); *nrec = mergers, at most cap recorded. */
+int orc_hermite_cluster_log(const gplum_params *p, gplum_particle *pp, int n, double *edisp,
+                            int64_t *n_block_steps, double *rec, int cap, int *nrec);
 /* Test helper: Sun + hard acc/jerk and calcDeltatInitial for one cluster. */
 void orc_cluster_prepare(const gplum_params *p, gplum_particle *pp, int n);
 /* timeIntegrateKepler_isolated (src/hermite.h:787-815), kepler.h:3-93. */
@@ -177,6 +181,9 @@ typedef struct orc_loop {
     double e_now[5];            /* out: calcEnergy at the end of the step (orc_energy)     */
     gplum_particle *removed;    /* out (optional): removed particles, ascending id         */
     int32_t removed_cap;
+    int32_t n_col_rec;          /* out: collision records written (cluster, then time order) */
+    double *col_rec;            /* out (optional): 20 doubles per merger (orc_hermite_cluster_log) */
+    int32_t col_cap;
 } orc_loop;
 /* Defaults of read.h:111-115 and istep = round(time_sys / dt_tree). */
 void orc_loop_default(const gplum_params *p, double time_sys, orc_loop *lp);

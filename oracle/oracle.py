@@ -73,6 +73,8 @@ def lib():
         L.orc_calc_dt_initial.argtypes = [P, vp]
         L.orc_hermite_cluster.argtypes = [P, vp, i32, vp, vp]
         L.orc_hermite_cluster.restype = i32
+        L.orc_hermite_cluster_log.argtypes = [P, vp, i32, vp, vp, vp, i32, vp]
+        L.orc_hermite_cluster_log.restype = i32
         L.orc_kepler_isolated.argtypes = [P, vp, f64, f64]
         L.orc_hermite_isolated.argtypes = [P, vp]
         L.orc_cluster_prepare.argtypes = [P, vp, i32]
@@ -238,6 +240,19 @@ def hermite_cluster(p, pp):
     return ncol, ed.value, nb.value
 
 
+def hermite_cluster_log(p, pp, cap=256):
+    """hermite_cluster with the collision log's records: (n_col, edisp, block_steps,
+    records[n_col, 20]) (fields COLREC_FIELDS)."""
+    rec = np.zeros((cap, 20), dtype=np.float64)
+    ed = ctypes.c_double(0.0)
+    nb = ctypes.c_int64(0)
+    nrec = ctypes.c_int(0)
+    ncol = lib().orc_hermite_cluster_log(ctypes.byref(p), abi.ptr(pp), len(pp), ctypes.byref(ed),
+                                         ctypes.byref(nb), ctypes.c_void_p(rec.ctypes.data), cap,
+                                         ctypes.byref(nrec))
+    return ncol, ed.value, nb.value, rec[: min(nrec.value, cap)].copy()
+
+
 def cluster_prepare(p, pp, offsets):
     """Sun + hard acc/jerk and initial block steps for every cluster (in place)."""
     for c in range(len(offsets) - 1):
@@ -288,7 +303,14 @@ class Loop(ctypes.Structure):
     """orc_loop: main-loop state (boundary, reset interval, step counter, outputs)."""
     _fields_ = [("r_max", ctypes.c_double), ("r_min", ctypes.c_double), ("reset_step", ctypes.c_int32),
                 ("istep", ctypes.c_int32), ("n_remove", ctypes.c_int32), ("recomputed", ctypes.c_int32),
-                ("e_now", ctypes.c_double * 5), ("removed", ctypes.c_void_p), ("removed_cap", ctypes.c_int32)]
+                ("e_now", ctypes.c_double * 5), ("removed", ctypes.c_void_p), ("removed_cap", ctypes.c_int32),
+                ("n_col_rec", ctypes.c_int32), ("col_rec", ctypes.c_void_p), ("col_cap", ctypes.c_int32)]
+
+
+# the collision log's record (orc_hermite_cluster_log, collision_record in orc_hard.c)
+COLREC_FIELDS = ["time", "id_imp", "id_tar", "mass_imp", "mass_tar"] + \
+    ["%s_%s_%s" % (q, w, c) for q, w in (("pos", "imp"), ("pos", "tar"), ("vel", "imp"), ("vel", "tar"))
+     for c in "xyz"] + ["dpos_g", "dvel_g", "pad"]
 
 
 class P3T:
@@ -310,6 +332,10 @@ class P3T:
         self.loop.removed = self.removed_buf.ctypes.data
         self.loop.removed_cap = len(self.removed_buf)
         self.removed = []          # per step: the removed particles (ascending id)
+        self.col_buf = np.zeros((4096, 20), dtype=np.float64)
+        self.loop.col_rec = self.col_buf.ctypes.data
+        self.loop.col_cap = len(self.col_buf)
+        self.collisions = []       # per step: the collision records (cluster, then time order)
         lib().orc_p3t_init(ctypes.byref(p), abi.ptr(self.pp), len(self.pp), ctypes.byref(self.nl),
                            time_sys)
 
@@ -319,6 +345,7 @@ class P3T:
                                   ctypes.byref(self.loop))
         self.time += self.p.dt_tree
         self.removed.append(self.removed_buf[: min(self.loop.n_remove, len(self.removed_buf))].copy())
+        self.collisions.append(self.col_buf[: self.loop.n_col_rec].copy())
         if self.n.value != len(self.pp):
             self.pp = self.pp[: self.n.value].copy()
         return ncol

@@ -324,6 +324,43 @@ static double cutoff_W2(const gplum_params *p, double dr2, double rinv1, double 
 /* Perfect merging: Collision0::inputPair / collisionOutcome / setParticle /
  * calcEnergyDissipation (collisionA.h:91-103, 155-356, 358-669) with n_frag = 0,
  * then the merge_list update of hermite.h:413-423.  Returns edisp. */
+/* Collision0::write2File's record of a merger (collisionA.h:122-148; inputPair and
+ * setParticle, :155-356, with n_frag = 0), taken before perfect_merge changes the pair:
+ * r[0] getTime() = time_c + time of the impactor (particle.h:1181), r[1], r[2] the ids,
+ * r[3], r[4] the masses with their twins, r[5..16] pos_imp, pos_tar, vel_imp, vel_tar,
+ * r[17], r[18] |pos_g_new - pos_g| / |pos_g| and the same for the velocity, where
+ * pos_g_new = masspos / mass_tot with masspos = (m_imp - 0) pos_new + m_tar pos_new and
+ * mass_tot = m_imp - 0 + m_tar (setParticle's sums from zero). */
+enum { ORC_COLREC = 20 };
+static void collision_record(const orc_hp *pp, int n, int ic, int jc, double *r)
+{
+    double mass_imp = pp[ic].b.mass, mass_tar = pp[jc].b.mass;
+    if (is_merged(&pp[ic])) for (int a = 0; a < n; a++) if (pp[a].twin_of == ic) mass_imp += pp[a].b.mass;
+    if (is_merged(&pp[jc])) for (int a = 0; a < n; a++) if (pp[a].twin_of == jc) mass_tar += pp[a].b.mass;
+    double pg[3], vg[3], dp[3], dv[3];
+    for (int k = 0; k < 3; k++) {
+        pg[k] = (mass_imp * pp[ic].b.pos[k] + mass_tar * pp[jc].b.pos[k]) / (mass_imp + mass_tar);
+        vg[k] = (mass_imp * pp[ic].b.vel[k] + mass_tar * pp[jc].b.vel[k]) / (mass_imp + mass_tar);
+    }
+    const double mt = mass_imp - 0. + mass_tar;
+    for (int k = 0; k < 3; k++) {
+        dp[k] = ((mass_imp - 0.) * pg[k] + mass_tar * pg[k]) / mt - pg[k];
+        dv[k] = ((mass_imp - 0.) * vg[k] + mass_tar * vg[k]) / mt - vg[k];
+    }
+    r[0] = pp[ic].time_c + pp[ic].b.time;
+    r[1] = (double)pp[ic].b.id;
+    r[2] = (double)pp[jc].b.id;
+    r[3] = mass_imp;
+    r[4] = mass_tar;
+    for (int k = 0; k < 3; k++) {
+        r[5 + k] = pp[ic].b.pos[k]; r[8 + k] = pp[jc].b.pos[k];
+        r[11 + k] = pp[ic].b.vel[k]; r[14 + k] = pp[jc].b.vel[k];
+    }
+    r[17] = sqrt(dot3(dp, dp) / dot3(pg, pg));
+    r[18] = sqrt(dot3(dv, dv) / dot3(vg, vg));
+    r[19] = 0.;
+}
+
 static double perfect_merge(const gplum_params *p, orc_hp *pp, int n, int ic, int jc)
 {
     /* inputPair */
@@ -409,9 +446,11 @@ static double perfect_merge(const gplum_params *p, orc_hp *pp, int n, int ic, in
     return edisp;
 }
 
-/* timeIntegrate_multi, hermite.h:224-479, over local time [0, dt_tree]. */
-int orc_hermite_cluster(const gplum_params *p, gplum_particle *part, int n,
-                        double *edisp_out, int64_t *n_block_steps)
+/* timeIntegrate_multi, hermite.h:224-479, over local time [0, dt_tree]; each merger's
+ * collision record (collision_record) goes to rec[0..cap) in time order (*nrec: the
+ * number of mergers, rec may be NULL). */
+static int hermite_cluster(const gplum_params *p, gplum_particle *part, int n, double *edisp_out,
+                           int64_t *n_block_steps, double *rec, int cap, int *nrec)
 {
     orc_hp *pp = (orc_hp *)calloc((size_t)n, sizeof(orc_hp));
     for (int i = 0; i < n; i++) {
@@ -484,6 +523,7 @@ int orc_hermite_cluster(const gplum_params *p, gplum_particle *part, int n,
             }
         }
         if (flag_col) {
+            if (rec && n_col < cap) collision_record(pp, n, ic, jc, rec + (size_t)n_col * ORC_COLREC);
             edisp += perfect_merge(p, pp, n, ic, jc);
             n_col++;
             for (int i = 0; i < n; i++) calc_gravity_full(p, pp, n, i);
@@ -514,7 +554,20 @@ int orc_hermite_cluster(const gplum_params *p, gplum_particle *part, int n,
     free(pp);
     if (edisp_out) *edisp_out = edisp;
     if (n_block_steps) *n_block_steps = loop;
+    if (nrec) *nrec = n_col;
     return n_col;
+}
+
+int orc_hermite_cluster(const gplum_params *p, gplum_particle *part, int n,
+                        double *edisp_out, int64_t *n_block_steps)
+{
+    return hermite_cluster(p, part, n, edisp_out, n_block_steps, NULL, 0, NULL);
+}
+
+int orc_hermite_cluster_log(const gplum_params *p, gplum_particle *part, int n, double *edisp_out,
+                            int64_t *n_block_steps, double *rec, int cap, int *nrec)
+{
+    return hermite_cluster(p, part, n, edisp_out, n_block_steps, rec, cap, nrec);
 }
 
 /* Test helper: state a cluster as correctForceLongInitial leaves it -- Sun and hard

@@ -155,16 +155,34 @@ int orc_p3t_step(const gplum_params *p, gplum_particle *pp, int *n_io, orc_nlist
     int nc = orc_make_cluster_list(pp, n, cl_off, cl_idx);
     int n_col = 0;
     double ed = 0.;
+    /* collision records per cluster, joined in cluster order afterwards */
+    double **crec = lp->col_rec ? (double **)calloc((size_t)nc + 1, sizeof(double *)) : NULL;
+    int *ncrec = lp->col_rec ? (int *)calloc((size_t)nc + 1, sizeof(int)) : NULL;
 #pragma omp parallel for schedule(dynamic, 2) reduction(+ : n_col, ed)
     for (int c = 0; c < nc; c++) {
         int m = cl_off[c + 1] - cl_off[c];
         gplum_particle *buf = (gplum_particle *)malloc(sizeof(gplum_particle) * m);
         for (int a = 0; a < m; a++) buf[a] = pp[cl_idx[cl_off[c] + a]];
         double e = 0.;
-        n_col += orc_hermite_cluster(p, buf, m, &e, NULL);
+        if (crec) {
+            crec[c] = (double *)malloc(sizeof(double) * 20 * (size_t)m);
+            n_col += orc_hermite_cluster_log(p, buf, m, &e, NULL, crec[c], m, &ncrec[c]);
+        } else {
+            n_col += orc_hermite_cluster(p, buf, m, &e, NULL);
+        }
         ed += e;
         for (int a = 0; a < m; a++) pp[cl_idx[cl_off[c] + a]] = buf[a];
         free(buf);
+    }
+    lp->n_col_rec = 0;
+    if (crec) {
+        for (int c = 0; c < nc; c++) {
+            for (int a = 0; a < ncrec[c] && lp->n_col_rec < lp->col_cap; a++)
+                memcpy(lp->col_rec + 20 * (size_t)lp->n_col_rec++, crec[c] + 20 * (size_t)a, sizeof(double) * 20);
+            free(crec[c]);
+        }
+        free(crec);
+        free(ncrec);
     }
     free(cl_off);
     free(cl_idx);

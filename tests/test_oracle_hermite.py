@@ -219,6 +219,42 @@ def test_merging_energy_change_equals_edisp(oracle_lib):
     assert abs((e1 - e0) - ed) < 1e-6 * abs(ed)
 
 
+def test_collision_record_of_a_head_on_merger(oracle_lib):
+    """The collision log's record (Collision0::write2File, collisionA.h:122-148) of the
+    head-on pair above: the lighter body is the impactor, the masses are the bodies',
+    the time lies on the block-step grid inside the soft step, the two bodies are closer
+    than f (R_imp + R_tar) (the merging criterion, hermite.h:98-196), and the centre of
+    mass does not move in the merger (the relative shifts, zero in exact arithmetic, are
+    round-off)."""
+    sc = orbit_state(1.0, 0.0, 0.0, 0.0)
+    d = np.array([1e-4, 0.0, 0.0])
+    dv = np.array([-3e-3, 0.0, 0.0])
+    pp = cluster([sc[0] + d, sc[0] - d], [sc[1] + dv, sc[1] - dv], np.array([1e-9, 2e-9]), 1e-2,
+                 r_planet=1e-6, f=10.0)
+    p = oracle_lib.params(eta=0.01, eta_sun=0.01, eta_0=0.002, eta_sun0=0.002, dt_tree=DT_TREE,
+                          dt_min=2.0 ** -40, gamma=0.1, collision=1)
+    out = pp.copy()
+    oracle_lib.cluster_prepare(p, out, np.array([0, 2]))
+    nc, ed, nb, rec = oracle_lib.hermite_cluster_log(p, out)
+    assert nc == 1 and len(rec) == 1
+    r = dict(zip(oracle_lib.COLREC_FIELDS, rec[0]))
+    print("merger at t = %.10e, |dx| %.3e, shifts %.2e %.2e" % (
+        r["time"], np.linalg.norm([r["pos_imp_x"] - r["pos_tar_x"], r["pos_imp_y"] - r["pos_tar_y"],
+                                    r["pos_imp_z"] - r["pos_tar_z"]]), r["dpos_g"], r["dvel_g"]))
+    assert (r["id_imp"], r["id_tar"]) == (0, 1)
+    assert (r["mass_imp"], r["mass_tar"]) == (1e-9, 2e-9)
+    assert 0.0 < r["time"] < DT_TREE
+    assert float(np.ldexp(r["time"], 40)).is_integer()
+    dx = np.array([r["pos_imp_" + c] - r["pos_tar_" + c] for c in "xyz"])
+    assert np.linalg.norm(dx) < 10.0 * (1e-6 + 1e-6)
+    assert r["dpos_g"] < 4e-16 and r["dvel_g"] < 4e-16
+    # the same merger as the plain integration (same state afterwards)
+    out2 = pp.copy()
+    oracle_lib.cluster_prepare(p, out2, np.array([0, 2]))
+    nc2, ed2, nb2 = oracle_lib.hermite_cluster(p, out2)
+    assert (nc2, ed2, nb2) == (nc, ed, nb) and np.array_equal(out2["pos"], out["pos"])
+
+
 def test_config1_run_energy_drift(oracle_lib, init3000):
     """The reference's sample disk (INIT3000, 3000 planetesimals, monopole, merging) over
     64 soft steps of 2^-4: |dE| = |(E - E0 - E_disp) / E0| (energy.h:65-67) stays below
