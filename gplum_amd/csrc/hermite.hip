@@ -494,7 +494,7 @@ __device__ inline double readlane_f64(double v, int lane) {
 __device__ inline void coop_pair_sum(const HP &P, const JData &J, int n, int64_t idi, const double xi[3],
                                      const double vi[3], double rinv_i, double acc[3], double jerk[3]) {
     double a[3] = {0., 0., 0.}, jk[3] = {0., 0., 0.};
-    // two pairs at a time: the eight-wave kernels hold more state per lane
+    // two pairs at a time: the multi-wave kernels hold more state per lane
     pair_sum_strided<2>(P, J, (int)(threadIdx.x & 63), 64, n, idi, xi, vi, rinv_i, a, jk, nullptr);
     // butterfly x += x[lane ^ d] for d = 32, 16, 8, 4, 2, 1 (every lane active): the two
     // levels across rows of 16 lanes by shuffles, the four inside a row by DPP moves
@@ -840,9 +840,9 @@ __device__ void coop_correct_active(const HP &P, HardView &H, const JData &J, in
     }
 }
 
-// ------------------------------- eight-wave kernels: lane groups and wave-level reductions
+// ---------------------------- multi-wave kernels: lane groups and wave-level reductions
 
-// Per-workgroup scratch of the eight-wave kernels' block steps: one slot per wave for
+// Per-workgroup scratch of the multi-wave kernels' block steps: one slot per wave for
 // each reduction (separate slots, so that a reduction needs no barrier before its
 // writes), and the list of the block step's active particles.
 struct WaveSlots {
@@ -851,7 +851,19 @@ struct WaveSlots {
     int na, nm;                // active particles without twins (listed in act), merged targets
     int act[HERMITE_LDS];
     int32_t tw[HERMITE_LDS];   // the cluster's twin_of, copied at each block step's prediction
+    // merged targets of the block step, their twins (concatenated, ascending index per
+    // target) and the lane-group layout of the bodies to integrate: slot k holds body
+    // ent[k] (-1: empty), the first slot of its family efam[k] and the family's size
+    // efn[k] (FAM_MERGED set for a merged family; 1 for a body without twins)
+    int mt[HERMITE_LDS];
+    int twl[HERMITE_LDS];
+    int twn[HERMITE_LDS];
+    int ent[2 * HERMITE_LDS], efam[2 * HERMITE_LDS], efn[2 * HERMITE_LDS];
+    int nslot, gs, nfam;       // slots in use, log2 of the lanes per slot, families placed
 };
+constexpr int FAM_MERGED = 1 << 16;
+constexpr int FAM_MAX = 8;     // largest family integrated in the lane groups (64 / 8 lanes)
+constexpr int FAM_GROUPS = 16; // families per block step in the lane groups (slot capacity)
 
 // Minimum over the workgroup (every lane active, values not NaN): the wave's minimum by
 // DPP, the waves' minima through slot[] and one barrier.  Exact, hence the value of
@@ -955,32 +967,75 @@ __device__ inline void group_pair_sum(const HP &P, const JData &J, int n, int s,
     for (int d = 0; d < 3; d++) { acc[d] = c[d]; jerk[d] = c[3 + d]; }
 }
 
-// The active-particle correction of integrate_from (hermite.h:300-375) for the active
-// particles without twins of clusters with j-data in LDS: the workgroup's lanes are dealt in groups of
-// S = 2^s per active particle (the largest S <= 64 with na S <= 512, at least 8; more
-// than 64 active particles take rounds of 64), each group evaluating its particle's hard
-// force by group_pair_sum.  The steps and their order are those of coop_correct_active,
-// so the results are too.  Every thread of the workgroup calls it (no barrier inside).
+// The active-particle correction of integrate_from (hermite.h:300-375) in lane groups,
+// for clusters with j-data in LDS.  The bodies to integrate are laid out in slots
+// (WaveSlots::ent): the active particles without twins, and merged targets with their
+// twins as families of consecutive slots inside one wave (a merged target integrates its
+// twins alongside, each with its own position and cut-off radius, and mergeAccJerk
+// averages their hard forces by mass).  A slot gets S = 2^s lanes (W.gs); each group
+// evaluates its body's hard force by group_pair_sum, and the members of a family add
+// their forces by shuffles in the order of coop_correct_active (target first, then the
+// twins in ascending index).  The steps and their order are those of
+// coop_correct_active, so the results are too.  Every thread of the workgroup calls it
+// (no barrier inside).
 template <int K>
 __device__ void group_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
-                                     int flag_col, long long &pairs, const int *s_act, int na) {
+                                     int flag_col, long long &pairs, const WaveSlots &W) {
     const int t = threadIdx.x;
     const int B = blockDim.x;
-    int s = 6;
-    while (s > 3 && na * (1 << s) > B) s--;
+    const int s = W.gs, E = 64 >> s;
     const int r = t & ((1 << s) - 1);
+    const int nslot = W.nslot;
+    const bool fams = W.nfam > 0;
     const int per_round = B >> s;
-    const int wave_first = (t & ~63) >> s;     // the wave's first particle slot in a round
-    for (int a0 = 0; a0 < na; a0 += per_round) {
-        if (a0 + wave_first >= na) continue;   // no particle for this wave (wave-uniform)
-        const int a = a0 + (t >> s);
-        const bool valid = a < na;
-        const int g = b + s_act[valid ? a : a0];
+    const int wave_first = (t & ~63) >> s;     // the wave's first slot in a round
+    for (int a0 = 0; a0 < nslot; a0 += per_round) {
+        if (a0 + wave_first >= nslot) continue;   // no body for this wave (wave-uniform)
+        const int slot = a0 + (t >> s);
+        const int e = slot < nslot ? W.ent[slot] : -1;
+        const bool valid = e >= 0;
+        const int g = b + (valid ? e : 0);      // empty slots compute a stand-in
+        int fn = 1, ff = slot;
+        bool mfam = false;
+        if (fams && slot < nslot) {
+            fn = W.efn[slot] & (FAM_MERGED - 1);
+            mfam = (W.efn[slot] & FAM_MERGED) != 0;
+            ff = W.efam[slot];
+        }
+        const bool twin = ff != slot;
+        const int gt = twin ? b + W.ent[ff] : g;  // the family's target
+        const int ffw = ff - (a0 + wave_first);   // its slot within the wave
+        const bool wave_fam = fams && __ballot(mfam) != 0ull;
         const bool lead = valid && r == 0;
-        // Registers are scarce here (the pair sums keep a subtree stack per lane): the
-        // particle's predicted state is reloaded for each correction instead of being
-        // held across the pair sums, and the correction's a2/a3 terms go through H.
+        const double mass_e = H.mass[g];
         double as[3], js[3], ad[3], jd[3], x[3], v[3];
+        // mergeAccJerk over the family (hermite.h:36-74), from the members' lanes
+        auto merge = [&]() __attribute__((always_inline)) {
+            if (!wave_fam) return;
+            double sa[6] = {0., 0., 0., 0., 0., 0.}, msum = 0.;
+            for (int k = 0; k < E; k++) {
+                const int src = ((ffw + k) << s) + r;
+                const double mk = __shfl(mass_e, src);
+                double ak[6];
+                for (int d = 0; d < 3; d++) { ak[d] = __shfl(ad[d], src); ak[3 + d] = __shfl(jd[d], src); }
+                if (k < fn) {
+                    if (k == 0) {
+                        for (int d = 0; d < 6; d++) sa[d] = mk * ak[d];
+                        msum = mk;
+                    } else {
+                        for (int d = 0; d < 6; d++) sa[d] += mk * ak[d];
+                        msum += mk;
+                    }
+                }
+            }
+            if (mfam) {
+                const double minv = 1. / msum;
+                for (int d = 0; d < 3; d++) { ad[d] = sa[d] * minv; jd[d] = sa[3 + d] * minv; }
+            }
+        };
+        // Registers are scarce here (the pair sums keep a subtree stack per lane): the
+        // body's predicted state is reloaded for each correction instead of being held
+        // across the pair sums, and the correction's a2/a3 terms go through H.
         const double rinv = H.rinv[g];
         const int64_t id = H.id[g];
         {
@@ -989,17 +1044,21 @@ __device__ void group_correct_active(const HP &P, HardView &H, const JData &J, i
             group_pair_sum<K>(P, J, n, s, r, id, xp, vp, rinv, ad, jd);
             star_acc_jerk(P, xp, vp, as, js, nullptr);
         }
-        const double t_c = t_s - H.time[g];
+        merge();
+        const double t_c = t_s - H.time[gt];    // twins carry their target's time
         for (int ite = 0; ite < 2; ite++) {
             {
                 double xp[3], vp[3], a0s[3], j0s[3], a0d[3], j0d[3], a2s[3], a3s[3], a2d[3], a3d[3];
                 ld3(H.xp, g, xp); ld3(H.vp, g, vp);
                 ld3(H.a0_s, g, a0s); ld3(H.j0_s, g, j0s); ld3(H.a0_d, g, a0d); ld3(H.j0_d, g, j0d);
                 correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
-                if (ite == 1 && lead) { st3(H.a2_s, g, a2s); st3(H.a3_s, g, a3s); st3(H.a2_d, g, a2d); st3(H.a3_d, g, a3d); }
+                if (ite == 1 && lead && !twin) {
+                    st3(H.a2_s, g, a2s); st3(H.a3_s, g, a3s); st3(H.a2_d, g, a2d); st3(H.a3_d, g, a3d);
+                }
             }
             star_acc_jerk(P, x, v, as, js, nullptr);
             group_pair_sum<K>(P, J, n, s, r, id, x, v, rinv, ad, jd);
+            merge();
         }
         wave_sync();     // the leader's a2/a3 before the group reads them back
         const double time = H.time[g] + t_c;
@@ -1008,6 +1067,10 @@ __device__ void group_correct_active(const HP &P, HardView &H, const JData &J, i
             double a2s[3], a3s[3], a2d[3], a3d[3];
             ld3(H.a2_s, g, a2s); ld3(H.a3_s, g, a3s); ld3(H.a2_d, g, a2d); ld3(H.a3_d, g, a3d);
             dt = dt_block(P, time, dt, H.acc0[g], ad, jd, a2d, a3d, as, js, a2s, a3s);
+        }
+        if (wave_fam) {                           // twins take their target's step
+            const double dtt = __shfl(dt, (ffw << s) + r);
+            if (twin) dt = dtt;
         }
         if (lead) {
             pairs += 3 * (long long)(n - 1);
@@ -1176,11 +1239,15 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
         }
         // correct the active particles (hermite.h:300-375)
         if (COOP && LDSJ) {
-            // The active particles: those without twins listed in ascending index by the
-            // first wave, for the lane groups; merged targets (which integrate their twins
-            // alongside, never active themselves) flagged in H.act for coop_correct_active.
-            // The two sets write disjoint particles and read only the predicted j-data.
+            // The bodies of the block step, laid out by the first wave for the lane groups
+            // (group_correct_active): the merged targets with their twins as families of
+            // consecutive slots that stay inside one wave (at most FAM_MAX members and
+            // FAM_GROUPS families; larger or further ones are flagged in H.act for
+            // coop_correct_active), then the active particles without twins in ascending
+            // index.  The two paths write disjoint particles and read only the predicted
+            // j-data.
             if (t < 64) {
+                const unsigned long long below = (1ull << t) - 1ull;
                 int base = 0, nm = 0;
                 for (int i0 = 0; i0 < n; i0 += 64) {
                     const int i = i0 + t;
@@ -1189,23 +1256,75 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                         const int f = H.flags[b + i];
                         a = !(f & GPLUM_FLAG_DEAD) && (flag_col || H.time[b + i] + H.dt[b + i] == t_s);
                         m = a && (f & GPLUM_FLAG_MERGED);
-                        H.act[b + i] = m;
+                        H.act[b + i] = 0;
                         a = a && !m;
                     }
-                    const unsigned long long ma = __ballot(a);
-                    if (a) W->act[base + __popcll(ma & ((1ull << t) - 1ull))] = i;
+                    const unsigned long long ma = __ballot(a), mm = __ballot(m);
+                    if (a) W->act[base + __popcll(ma & below)] = i;
+                    if (m) W->mt[nm + __popcll(mm & below)] = i;
                     base += __popcll(ma);
-                    nm += __popcll(__ballot(m));
+                    nm += __popcll(mm);
                 }
-                if (t == 0) { W->na = base; W->nm = nm; }
+                // each merged target's twins in ascending index
+                int ntw = 0, nfam = 0, nbig = 0, fmem = 0, fmax = 1;
+                for (int k = 0; k < nm; k++) {
+                    const int tg = W->mt[k];
+                    int c = 0;
+                    for (int i0 = 0; i0 < n; i0 += 64) {
+                        const int i = i0 + t;
+                        const bool q = i < n && W->tw[i] == tg;
+                        const unsigned long long qm = __ballot(q);
+                        if (q) W->twl[ntw + c + __popcll(qm & below)] = i;
+                        c += __popcll(qm);
+                    }
+                    if (1 + c > FAM_MAX || nfam >= FAM_GROUPS) {
+                        if (t == 0) H.act[b + tg] = 1;     // coop_correct_active
+                        c = -1;
+                        nbig++;
+                    } else {
+                        ntw += c;
+                        nfam++;
+                        fmem += 1 + c;
+                        fmax = max(fmax, 1 + c);
+                    }
+                    if (t == 0) W->twn[k] = c;
+                }
+                // lanes per slot: the most that hold every body at once (at least 8, and
+                // few enough that a wave holds the largest family)
+                const int ne = base + fmem;
+                int sg = 6;
+                while (sg > 3 && (ne * (1 << sg) > B || (64 >> sg) < fmax)) sg--;
+                const int E = 64 >> sg;
+                int pos = 0;
+                if (t == 0) {
+                    int tw0 = 0;
+                    for (int k = 0; k < nm; k++) {
+                        const int c = W->twn[k];
+                        if (c < 0) continue;
+                        const int fsz = 1 + c;
+                        if ((pos % E) + fsz > E)
+                            while (pos % E) { W->ent[pos] = -1; W->efam[pos] = pos; W->efn[pos] = 1; pos++; }
+                        W->ent[pos] = W->mt[k];
+                        for (int q = 0; q < c; q++) W->ent[pos + 1 + q] = W->twl[tw0 + q];
+                        for (int q = 0; q < fsz; q++) { W->efam[pos + q] = pos; W->efn[pos + q] = fsz | FAM_MERGED; }
+                        tw0 += c;
+                        pos += fsz;
+                    }
+                }
+                pos = __shfl(pos, 0);
+                for (int k = t; k < base; k += 64) {
+                    W->ent[pos + k] = W->act[k];
+                    W->efam[pos + k] = pos + k;
+                    W->efn[pos + k] = 1;
+                }
+                if (t == 0) { W->nslot = pos + base; W->gs = sg; W->nfam = nfam; W->na = base; W->nm = nbig; }
             }
             __threadfence_block();
             __syncthreads();
-            const int na = W->na;
             const int K = (n + 63) >> 6;
-            if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
-            else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
-            else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, W->act, na);
+            if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, *W);
+            else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, *W);
+            else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, *W);
             if (W->nm > 0)
                 coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6, W->tw);
         } else if (COOP) {
@@ -1912,7 +2031,7 @@ __device__ void integrate_regs(const HP &P, HardView &H, const JData &J, Reduce 
 // Separate kernels keep the register-heavy small-cluster path from lowering the
 // occupancy of the others.
 template <bool LDSJ, bool REG>
-__global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_hermite(HP P_arg, HardView H,
+__device__ __forceinline__ void hermite_clusters(HP P_arg, HardView H,
                                                            const int32_t *__restrict__ cl_off,
                                                            const int32_t *__restrict__ order, int ncl,
                                                            int *__restrict__ next, double *__restrict__ edisp,
@@ -1931,7 +2050,7 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
     __shared__ HP s_P;
     // register path, 3 to 8 particles: pair summands (side_force)
     __shared__ double s_tm[REG ? 6 * 64 : 1];
-    // eight-wave kernels: reduction slots and the active list
+    // multi-wave kernels: reduction slots and the active list
     __shared__ __align__(8) unsigned char s_ws[REG ? 8 : sizeof(WaveSlots)];
     const int t = threadIdx.x;
     const int B = blockDim.x;
@@ -1996,6 +2115,29 @@ __global__ __launch_bounds__(REG ? HERMITE_BLOCK : HERMITE_BIG_BLOCK) void k_her
         }
         __syncthreads();
     }
+}
+
+// The register path: one wave per workgroup, all the registers a lane may have.
+template <bool LDSJ, bool REG>
+__global__ __launch_bounds__(HERMITE_BLOCK) void k_hermite(HP P, HardView H, const int32_t *__restrict__ cl_off,
+                                                           const int32_t *__restrict__ order, int ncl,
+                                                           int *__restrict__ next, double *__restrict__ edisp,
+                                                           int *__restrict__ ncol, long long *__restrict__ nblock,
+                                                           long long *__restrict__ npair, int persistent, int lds_max,
+                                                           int giant_min) {
+    hermite_clusters<LDSJ, REG>(P, H, cl_off, order, ncl, next, edisp, ncol, nblock, npair, persistent, lds_max,
+                                giant_min);
+}
+
+// The larger clusters' kernels: HERMITE_BIG_BLOCK threads, two waves per SIMD (256
+// registers per lane).
+template <bool LDSJ>
+__global__ __launch_bounds__(HERMITE_BIG_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_hermite_multi(
+    HP P, HardView H, const int32_t *__restrict__ cl_off, const int32_t *__restrict__ order, int ncl,
+    int *__restrict__ next, double *__restrict__ edisp, int *__restrict__ ncol, long long *__restrict__ nblock,
+    long long *__restrict__ npair, int persistent, int lds_max, int giant_min) {
+    hermite_clusters<LDSJ, false>(P, H, cl_off, order, ncl, next, edisp, ncol, nblock, npair, persistent, lds_max,
+                                  giant_min);
 }
 
 // ------------------------------------------------------------- isolated
@@ -2171,7 +2313,7 @@ static HardView hard_view(HardBuffers &hb) {
 // with OpenMP inside the cluster, hard.h:583-658) are spread over the whole GPU: each
 // block step is a short sequence of kernels (predict all; collision test over all pairs
 // with per-workgroup minima; decision and active flags; the active particles, one wave
-// each, as in the eight-wave kernels; the collision aftermath phase by phase; next
+// each, as in the multi-wave kernels; the collision aftermath phase by phase; next
 // block time and system time), launched in batches by the host, which checks a done
 // flag between batches.  Same functions, same summation orders: results identical to the size-class
 // kernels.
@@ -2591,11 +2733,11 @@ int hard_launch(const gplum_params &p, HardBuffers &hb, int ncl, const int32_t *
         GPLUM_HIP(hipEventRecord(gpool.fork, st));
     }
     if (hb.max_size > lds_max)
-        hipLaunchKernelGGL((k_hermite<false, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
+        hipLaunchKernelGGL((k_hermite_multi<false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
                            ncl, hb.next.p + 2, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max,
                            giant_min);
     if (big)
-        hipLaunchKernelGGL((k_hermite<true, false>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
+        hipLaunchKernelGGL((k_hermite_multi<true>), dim3(grid), dim3(HERMITE_BIG_BLOCK), 0, sb, P, H, d_off, d_order,
                            ncl, hb.next.p + 1, hb.edisp.p, hb.ncol.p, hb.nblock.p, hb.npair.p, pers, lds_max,
                            giant_min);
     hipLaunchKernelGGL((k_hermite<true, true>), dim3(grid), dim3(HERMITE_BLOCK), 0, st, P, H, d_off, d_order, ncl,
@@ -2661,10 +2803,10 @@ int hard_launch_single(const gplum_params &p, HardBuffers &hb, int c, int n_c, c
         hipLaunchKernelGGL((k_hermite<true, true>), dim3(1), dim3(HERMITE_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
                            hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS, 0x7fffffff);
     else if (n_c <= HERMITE_LDS)
-        hipLaunchKernelGGL((k_hermite<true, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
+        hipLaunchKernelGGL((k_hermite_multi<true>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr, 1,
                            hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS, 0x7fffffff);
     else
-        hipLaunchKernelGGL((k_hermite<false, false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr,
+        hipLaunchKernelGGL((k_hermite_multi<false>), dim3(1), dim3(HERMITE_BIG_BLOCK), 0, st, P, H, d_off + c, nullptr,
                            1, hb.next.p, hb.edisp.p + c, hb.ncol.p + c, hb.nblock.p + c, hb.npair.p + c, 0, HERMITE_LDS, 0x7fffffff);
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;

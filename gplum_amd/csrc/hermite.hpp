@@ -5,7 +5,8 @@
 namespace gplum {
 
 constexpr int HERMITE_BLOCK = 64;      // threads per workgroup, clusters up to HARD_SEQ_MAX
-constexpr int HERMITE_BIG_BLOCK = 512;  // threads per workgroup (8 waves), larger clusters
+// threads per workgroup (8 waves, two per SIMD) of the larger clusters' kernels
+constexpr int HERMITE_BIG_BLOCK = 512;
 constexpr int HERMITE_LDS = 256;     // clusters up to this size keep j-data in LDS
 // Clusters up to HARD_SEQ_MAX particles sum each particle's hard force sequentially in
 // j (one lane per particle); larger ones with 64 lane-strided partial sums and a
