@@ -101,19 +101,23 @@ struct Tree {
     DArray<int32_t> grp_flag, grp_scan;        // per particle
     DArray<int32_t> group_node, group_start, group_count;
     // per group: near EP particles (leaves opened by the search-radius test), far EP
-    // particles (leaves opened only by the opening angle), all EP leaves (walk order),
-    // SP nodes
-    DArray<int32_t> ep_cnt, epf_cnt, sp_cnt, epl_cnt, ep_off, epf_off, sp_off, epl_off;
-    DArray<int32_t> ep_list, epf_list, sp_list, ep_leaf;
+    // particles (leaves opened only by the opening angle), SP nodes; each group's lists
+    // are contiguous, at the per-group offsets (inside its walk workgroup's region)
+    DArray<int32_t> ep_cnt, epf_cnt, sp_cnt, ep_off, epf_off, sp_off;
+    DArray<int32_t> ep_list, epf_list, sp_list;
     DArray<int32_t> walk_queue;                // per-workgroup BFS queues
-    DArray<int32_t> walk_err;
+    DArray<int32_t> walk_use;                  // entries each walk workgroup wrote (3 per workgroup)
+    DArray<int32_t> walk_err;                  // walk error bits
+    DArray<long long> walk_stat;               // largest region use and total, per list
+    HostBuf walk_host;
     DArray<int32_t> lvl;                       // device level ranges + overflow flag
     DArray<int32_t> lvl_part;                  // per-chunk child sums of one level
     DArray<int32_t> lvl_bnd;                   // digit bounds of one level's nodes (8 per node)
     DArray<unsigned char> scan_tmp;            // hipcub temporary storage
-    int64_t n_ep_total = 0, n_epf_total = 0, n_sp_total = 0, n_epl_total = 0;
+    int64_t n_ep_total = 0, n_epf_total = 0, n_sp_total = 0;
     int64_t n_epep_pairs = 0, n_epsp_pairs = 0;
     int queue_cap = 1 << 14;
+    int region_cap[3] = {4096, 4096, 4096};   // list entries per walk workgroup (EP near, EP far, SP)
     int coop_min = 1 << 16;                    // nodes of at least this many keys: workgroup-wide digit search
 };
 

@@ -1185,7 +1185,32 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                         if (Rr < 1. && rlh_less(Rr, lo, hi, rmin, bi, bj)) { rmin = Rr; bi = lo; bj = hi; }
                     };
                     if (fra >= 0.) {
-                        for (int d = d0; d <= d1; d++) pair(a + d < n ? a + d : a + d - n);
+                        // four distances at a time (their LDS reads overlap); the minimum
+                        // does not depend on the order in which pairs are met
+                        int d = d0;
+                        for (; d + 3 <= d1; d += 4) {
+                            double frc[4], ddx[4], ddy[4], ddz[4];
+                            int cc[4];
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                cc[u] = a + d + u < n ? a + d + u : a + d + u - n;
+                                frc[u] = s_fr[cc[u]];
+                                ddx[u] = xa - s_x[3 * cc[u]];
+                                ddy[u] = ya - s_x[3 * cc[u] + 1];
+                                ddz[u] = za - s_x[3 * cc[u] + 2];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                if (frc[u] < 0.) continue;
+                                const double d2 = dot3(ddx[u], ddy[u], ddz[u], ddx[u], ddy[u], ddz[u]);
+                                const double r2 = fra + frc[u];
+                                if (!(d2 < COL_FILTER * r2 * r2)) continue;
+                                const double Rr = sqrt(d2) / r2;
+                                const int lo = min(a, cc[u]), hi = max(a, cc[u]);
+                                if (Rr < 1. && rlh_less(Rr, lo, hi, rmin, bi, bj)) { rmin = Rr; bi = lo; bj = hi; }
+                            }
+                        }
+                        for (; d <= d1; d++) pair(a + d < n ? a + d : a + d - n);
                         // even n: the pairs at distance n / 2, once each (a < n / 2)
                         if (!(n & 1) && r == R - 1 && a < (n >> 1)) pair(a + (n >> 1));
                     }

@@ -4,16 +4,19 @@
 // the ones stated in oracle/orc_tree.c, implemented here level-synchronously:
 //   keys    : 63-bit Morton keys of the particle bbox cube, radix-sorted (stable, so
 //             ties keep the previous order);
-//   nodes   : breadth-first, one kernel triple per level (child count, scan, child
-//             write) with the level ranges kept on the device;
+//   nodes   : breadth-first, two kernels per level (child count, child write) with
+//             the level ranges kept on the device;
 //   moments : bottom-up per level, one thread per node, FP64, FDPS MomentQuadrupole
 //             accumulation order (src/particle.h:1649-1715);
 //   groups  : maximal nodes with <= n_group_limit particles, in particle order;
 //   walk    : one wave64 per i-group, breadth-first queue in global memory, 64 nodes
 //             per step; opened leaves -> EP list, unopened nodes -> SP list, children
 //             of opened internal nodes appended in order (ballot + wave prefix sums),
-//             so the lists equal the sequential FIFO walk.
+//             so the lists equal the sequential FIFO walk; one pass, each workgroup
+//             writing its groups' lists into a region of its own.
 #include <hipcub/hipcub.hpp>
+
+#include <cstring>
 
 #include "tree.hpp"
 
@@ -366,19 +369,6 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_write_children_lv(
     }
 }
 
-// totals of the four interaction-list scans (last count + last offset), one read-back
-__global__ void k_list_totals(int ng, const int32_t *__restrict__ c0, const int32_t *__restrict__ o0,
-                              const int32_t *__restrict__ c1, const int32_t *__restrict__ o1,
-                              const int32_t *__restrict__ c2, const int32_t *__restrict__ o2,
-                              const int32_t *__restrict__ c3, const int32_t *__restrict__ o3,
-                              int32_t *__restrict__ out) {
-    if (threadIdx.x != 0) return;
-    out[0] = ng > 0 ? c0[ng - 1] + o0[ng - 1] : 0;
-    out[1] = ng > 0 ? c1[ng - 1] + o1[ng - 1] : 0;
-    out[2] = ng > 0 ? c2[ng - 1] + o2[ng - 1] : 0;
-    out[3] = ng > 0 ? c3[ng - 1] + o3[ng - 1] : 0;
-}
-
 __global__ void k_lvl_init(int32_t *lvl) {
     if (threadIdx.x < LVL_ERR + 1) lvl[threadIdx.x] = (threadIdx.x == 1) ? 1 : 0;
 }
@@ -489,7 +479,12 @@ __global__ void k_flag01(int n, const int32_t *__restrict__ flag, int32_t *__res
 
 // ------------------------------------------------------------------- walk
 
-template <bool FILL>
+// One pass: workgroup b owns the list regions [b*cap_q, (b+1)*cap_q) of the three
+// lists (q: EP near, EP far, SP) and writes the lists of its groups one after another
+// into them, so a group's lists are contiguous, in the order of the sequential FIFO
+// walk, and placed with no atomics and no second pass; the soft kernel reads them
+// through the per-group offsets and counts only.  use[3b+q] receives the entries the
+// workgroup wrote.  err bits: 1 queue, 4 region too small.
 __global__ __launch_bounds__(64) void k_walk(
     int n_group, const int32_t *__restrict__ gstart, const int32_t *__restrict__ gcount,
     const double *__restrict__ pos, const double *__restrict__ rsearch, const double *__restrict__ os,
@@ -497,15 +492,18 @@ __global__ __launch_bounds__(64) void k_walk(
     const int32_t *__restrict__ nchild_first, const int32_t *__restrict__ nchild,
     const int32_t *__restrict__ nlevel, const double *__restrict__ ncom, const double *__restrict__ bmin,
     const double *__restrict__ bmax, const double *__restrict__ nrs, int32_t *__restrict__ queue_all,
-    int qcap, int32_t *__restrict__ ep_cnt, int32_t *__restrict__ epf_cnt, int32_t *__restrict__ epl_cnt,
-    int32_t *__restrict__ sp_cnt, const int32_t *__restrict__ ep_off, const int32_t *__restrict__ epf_off,
-    const int32_t *__restrict__ epl_off, const int32_t *__restrict__ sp_off, int32_t *__restrict__ ep_list,
-    int32_t *__restrict__ epf_list, int32_t *__restrict__ ep_leaf, int32_t *__restrict__ sp_list,
-    int32_t *__restrict__ err) {
+    int qcap, int32_t *__restrict__ ep_cnt, int32_t *__restrict__ epf_cnt, int32_t *__restrict__ sp_cnt,
+    int32_t *__restrict__ ep_off, int32_t *__restrict__ epf_off, int32_t *__restrict__ sp_off,
+    int32_t *__restrict__ ep_list, int32_t *__restrict__ epf_list, int32_t *__restrict__ sp_list,
+    int cap_ep, int cap_epf, int cap_sp, int32_t *__restrict__ use, int32_t *__restrict__ err) {
     const int lane = __lane_id();
     int32_t *queue = queue_all + (size_t)blockIdx.x * qcap;
+    const int64_t r_ep = (int64_t)blockIdx.x * cap_ep, r_epf = (int64_t)blockIdx.x * cap_epf,
+                  r_sp = (int64_t)blockIdx.x * cap_sp;
+    int u_ep = 0, u_epf = 0, u_sp = 0;      // entries of the region used so far
     const double side = os[3];
-    for (int g = blockIdx.x; g < n_group; g += gridDim.x) {
+    int fail = 0;
+    for (int g = blockIdx.x; g < n_group && !fail; g += gridDim.x) {
         const int s = gstart[g], cnt = gcount[g];
         // group bbox and max r_search (wave reduction; exact)
         double gmn[3] = {1e300, 1e300, 1e300}, gmx[3] = {-1e300, -1e300, -1e300}, grs = 0.;
@@ -528,12 +526,10 @@ __global__ __launch_bounds__(64) void k_walk(
         }
         int head = 0, tail = 1;
         if (lane == 0) queue[0] = 0;
-        int n_ep = 0, n_epf = 0, n_epl = 0, n_sp = 0;
-        const int base_ep = FILL ? ep_off[g] : 0;
-        const int base_epf = FILL ? epf_off[g] : 0;
-        const int base_epl = FILL ? epl_off[g] : 0;
-        const int base_sp = FILL ? sp_off[g] : 0;
-        bool overflow = false;
+        int n_ep = 0, n_epf = 0, n_sp = 0;
+        int32_t *const d_ep = ep_list + r_ep + u_ep;
+        int32_t *const d_epf = epf_list + r_epf + u_epf;
+        int32_t *const d_sp = sp_list + r_sp + u_sp;
         while (head < tail) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             __syncthreads();
@@ -568,50 +564,81 @@ __global__ __launch_bounds__(64) void k_walk(
             const bool f_epf = f_ep && !near;
             const bool f_in = (lane < chunk) && open && !leaf;
             const bool f_sp = (lane < chunk) && !open;
-            int t_ep, t_epf, t_epl, t_sp, t_ch;
+            int t_ep, t_epf, t_sp, t_ch;
             const int o_ep = wave_excl_scan(f_epn ? cc : 0, &t_ep);
             const int o_epf = wave_excl_scan(f_epf ? cc : 0, &t_epf);
-            const int o_epl = wave_excl_scan(f_ep ? 1 : 0, &t_epl);
             const int o_sp = wave_excl_scan(f_sp ? 1 : 0, &t_sp);
             const int o_ch = wave_excl_scan(f_in ? nc : 0, &t_ch);
-            if (tail + t_ch > qcap) {
-                overflow = true;
+            // the totals are wave-uniform, so every lane leaves the loop together
+            if (tail + t_ch > qcap) { fail = 1; break; }
+            if (u_ep + n_ep + t_ep > cap_ep || u_epf + n_epf + t_epf > cap_epf || u_sp + n_sp + t_sp > cap_sp) {
+                fail = 4;
                 break;
             }
             if (f_in) {
                 const int c0 = nchild_first[c];
                 for (int k = 0; k < nc; k++) queue[tail + o_ch + k] = c0 + k;
             }
-            if (FILL) {
-                if (f_ep) ep_leaf[base_epl + n_epl + o_epl] = c;
-                if (f_epn) {
-                    const int st = nstart[c];
-                    for (int k = 0; k < cc; k++) ep_list[base_ep + n_ep + o_ep + k] = st + k;
-                }
-                if (f_epf) {
-                    const int st = nstart[c];
-                    for (int k = 0; k < cc; k++) epf_list[base_epf + n_epf + o_epf + k] = st + k;
-                }
-                if (f_sp) sp_list[base_sp + n_sp + o_sp] = c;
+            if (f_ep) {
+                const int st = nstart[c];
+                int32_t *dst = f_epn ? d_ep + n_ep + o_ep : d_epf + n_epf + o_epf;
+                for (int k = 0; k < cc; k++) dst[k] = st + k;
             }
+            if (f_sp) d_sp[n_sp + o_sp] = c;
             n_ep += t_ep;
             n_epf += t_epf;
-            n_epl += t_epl;
             n_sp += t_sp;
             tail += t_ch;
             head += chunk;
         }
-        if (overflow) {
-            if (lane == 0) atomicExch(err, 1);
-        }
-        if (!FILL && lane == 0) {
+        if (!fail && lane == 0) {
+            ep_off[g] = (int32_t)(r_ep + u_ep);
+            epf_off[g] = (int32_t)(r_epf + u_epf);
+            sp_off[g] = (int32_t)(r_sp + u_sp);
             ep_cnt[g] = n_ep;
             epf_cnt[g] = n_epf;
-            epl_cnt[g] = n_epl;
             sp_cnt[g] = n_sp;
         }
+        u_ep += n_ep;
+        u_epf += n_epf;
+        u_sp += n_sp;
+        __syncthreads();                    // the next group reuses the queue
+    }
+    if (lane == 0) {
+        if (fail) atomicOr(err, fail);
+        use[3 * blockIdx.x] = u_ep;
+        use[3 * blockIdx.x + 1] = u_epf;
+        use[3 * blockIdx.x + 2] = u_sp;
+    }
+}
+
+// one workgroup: per list the largest use of a region (out[0..2]) and the total
+// (out[3..5])
+__global__ __launch_bounds__(1024) void k_walk_use(int nblk, const int32_t *__restrict__ use,
+                                                   long long *__restrict__ out) {
+    __shared__ long long s[6][1024];
+    long long mx[3] = {0, 0, 0}, sm[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += 1024)
+        for (int q = 0; q < 3; q++) {
+            const long long u = use[3 * b + q];
+            mx[q] = u > mx[q] ? u : mx[q];
+            sm[q] += u;
+        }
+    for (int q = 0; q < 3; q++) {
+        s[q][threadIdx.x] = mx[q];
+        s[3 + q][threadIdx.x] = sm[q];
+    }
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 3; q++) {
+                const long long a = s[q][threadIdx.x + w];
+                if (a > s[q][threadIdx.x]) s[q][threadIdx.x] = a;
+                s[3 + q][threadIdx.x] += s[3 + q][threadIdx.x + w];
+            }
         __syncthreads();
     }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 // ------------------------------------------------------------- host side
@@ -739,58 +766,58 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
     GPLUM_ENSURE(t.ep_cnt, ng + 1);
     GPLUM_ENSURE(t.epf_cnt, ng + 1);
     GPLUM_ENSURE(t.sp_cnt, ng + 1);
-    GPLUM_ENSURE(t.epl_cnt, ng + 1);
     GPLUM_ENSURE(t.ep_off, ng + 1);
     GPLUM_ENSURE(t.epf_off, ng + 1);
     GPLUM_ENSURE(t.sp_off, ng + 1);
-    GPLUM_ENSURE(t.epl_off, ng + 1);
-    GPLUM_ENSURE(t.walk_err, 1);
-    const int nblk = std::min(ng, 4096);
+    const int nblk = std::min(std::max(ng, 1), 4096);
+    GPLUM_ENSURE(t.walk_use, 3 * (size_t)nblk);
+    GPLUM_ENSURE(t.walk_err, 2);
+    GPLUM_ENSURE(t.walk_stat, 6);
+    GPLUM_HIP(t.walk_host.ensure(8 * sizeof(long long)));
     const double theta2 = p.theta * p.theta;
-    for (int attempt = 0; attempt < 4; attempt++) {
+    for (int attempt = 0; attempt < 12; attempt++) {
+        for (int q = 0; q < 3; q++)
+            if ((int64_t)nblk * t.region_cap[q] > INT32_MAX)
+                return fail(GPLUM_E_OVERFLOW, "interaction lists exceed 2^31 entries");
+        GPLUM_ENSURE(t.ep_list, (size_t)nblk * t.region_cap[0]);
+        GPLUM_ENSURE(t.epf_list, (size_t)nblk * t.region_cap[1]);
+        GPLUM_ENSURE(t.sp_list, (size_t)nblk * t.region_cap[2]);
         GPLUM_ENSURE(t.walk_queue, (size_t)nblk * t.queue_cap);
         GPLUM_HIP(hipMemsetAsync(t.walk_err.p, 0, sizeof(int32_t), st));
-        hipLaunchKernelGGL((k_walk<false>), dim3(nblk), dim3(64), 0, st, ng, t.group_start.p,
-                           t.group_count.p, d_pos, d_rsearch, t.origin_scale.p, theta2, t.node_start.p,
-                           t.node_count.p, t.node_child.p, t.node_nchild.p, t.node_level.p, t.node_com.p,
-                           t.node_bmin.p, t.node_bmax.p, t.node_rs.p, t.walk_queue.p, t.queue_cap,
-                           t.ep_cnt.p, t.epf_cnt.p, t.epl_cnt.p, t.sp_cnt.p, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, t.walk_err.p);
-        int err = 0;
-        GPLUM_HIP(hipMemcpyAsync(&err, t.walk_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_walk, dim3(nblk), dim3(64), 0, st, ng, t.group_start.p, t.group_count.p, d_pos,
+                           d_rsearch, t.origin_scale.p, theta2, t.node_start.p, t.node_count.p,
+                           t.node_child.p, t.node_nchild.p, t.node_level.p, t.node_com.p, t.node_bmin.p,
+                           t.node_bmax.p, t.node_rs.p, t.walk_queue.p, t.queue_cap, t.ep_cnt.p, t.epf_cnt.p,
+                           t.sp_cnt.p, t.ep_off.p, t.epf_off.p, t.sp_off.p, t.ep_list.p, t.epf_list.p,
+                           t.sp_list.p, t.region_cap[0], t.region_cap[1], t.region_cap[2], t.walk_use.p,
+                           t.walk_err.p);
+        hipLaunchKernelGGL(k_walk_use, dim3(1), dim3(1024), 0, st, nblk, t.walk_use.p, t.walk_stat.p);
+        long long *h = t.walk_host.as<long long>();
+        GPLUM_HIP(hipMemcpyAsync(h, t.walk_stat.p, 6 * sizeof(long long), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(h + 6, t.walk_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
-        if (!err) break;
-        t.queue_cap *= 4;
-        if (attempt == 3) return fail(GPLUM_E_OVERFLOW, "tree walk queue overflow");
+        int32_t err;
+        std::memcpy(&err, h + 6, sizeof(int32_t));
+        if (err & 1) t.queue_cap *= 4;
+        if (err & 4)    // the use of a workgroup that stopped early is a lower bound
+            for (int q = 0; q < 3; q++)
+                t.region_cap[q] = (int)std::min<int64_t>(std::max<int64_t>(2 * (int64_t)t.region_cap[q], h[q] + 64),
+                                                         INT32_MAX);
+        if (!err) {
+            t.n_ep_total = h[3];
+            t.n_epf_total = h[4];
+            t.n_sp_total = h[5];
+            // regions for the next walk: the largest use and an eighth more
+            for (int q = 0; q < 3; q++) {
+                const int64_t want = h[q] + h[q] / 8 + 64;
+                if (want > t.region_cap[q] || want < t.region_cap[q] / 2) t.region_cap[q] = (int)std::min<int64_t>(want, INT32_MAX);
+            }
+            GPLUM_HIP(hipGetLastError());
+            return GPLUM_OK;
+        }
+        if (t.queue_cap > (1 << 26)) break;
     }
-    int rc;
-    if ((rc = excl_scan(t, t.ep_cnt.p, t.ep_off.p, ng, st))) return rc;
-    if ((rc = excl_scan(t, t.epf_cnt.p, t.epf_off.p, ng, st))) return rc;
-    if ((rc = excl_scan(t, t.sp_cnt.p, t.sp_off.p, ng, st))) return rc;
-    if ((rc = excl_scan(t, t.epl_cnt.p, t.epl_off.p, ng, st))) return rc;
-    GPLUM_ENSURE(t.lvl, LVL_ERR + 5);
-    hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, ng, t.ep_cnt.p, t.ep_off.p, t.epf_cnt.p,
-                       t.epf_off.p, t.sp_cnt.p, t.sp_off.p, t.epl_cnt.p, t.epl_off.p, t.lvl.p + LVL_ERR + 1);
-    int32_t tot[4] = {0, 0, 0, 0};
-    GPLUM_HIP(hipMemcpyAsync(tot, t.lvl.p + LVL_ERR + 1, sizeof(tot), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    const int tot_ep = tot[0], tot_epf = tot[1], tot_sp = tot[2], tot_epl = tot[3];
-    t.n_ep_total = tot_ep;
-    t.n_epf_total = tot_epf;
-    t.n_sp_total = tot_sp;
-    t.n_epl_total = tot_epl;
-    GPLUM_ENSURE(t.ep_list, (size_t)tot_ep + 1);
-    GPLUM_ENSURE(t.epf_list, (size_t)tot_epf + 1);
-    GPLUM_ENSURE(t.sp_list, (size_t)tot_sp + 1);
-    GPLUM_ENSURE(t.ep_leaf, (size_t)tot_epl + 1);
-    hipLaunchKernelGGL((k_walk<true>), dim3(nblk), dim3(64), 0, st, ng, t.group_start.p,
-                       t.group_count.p, d_pos, d_rsearch, t.origin_scale.p, theta2, t.node_start.p,
-                       t.node_count.p, t.node_child.p, t.node_nchild.p, t.node_level.p, t.node_com.p,
-                       t.node_bmin.p, t.node_bmax.p, t.node_rs.p, t.walk_queue.p, t.queue_cap, t.ep_cnt.p,
-                       t.epf_cnt.p, t.epl_cnt.p, t.sp_cnt.p, t.ep_off.p, t.epf_off.p, t.epl_off.p, t.sp_off.p,
-                       t.ep_list.p, t.epf_list.p, t.ep_leaf.p, t.sp_list.p, t.walk_err.p);
-    GPLUM_HIP(hipGetLastError());
-    return GPLUM_OK;
+    return fail(GPLUM_E_OVERFLOW, "tree walk queue or list regions overflow");
 }
 
 }  // namespace gplum
