@@ -257,13 +257,28 @@ __global__ void k_segment_flags(int m, const uint64_t *__restrict__ key, int32_t
     flag[k] = (k == 0 || key[k] != key[k - 1]) ? 1 : 0;
 }
 
+// segment starts -> cluster offsets (off holds m + 1 entries, enough for any count);
+// info[0] = the number of clusters
 __global__ void k_segment_offsets(int m, const int32_t *__restrict__ flag, const int32_t *__restrict__ scan,
-                                  int32_t *__restrict__ off, int32_t *__restrict__ size_neg) {
+                                  int32_t *__restrict__ off, int32_t *__restrict__ info) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > m) return;
-    if (k == m) { off[scan[m - 1] + flag[m - 1]] = m; return; }
+    if (k == m) {
+        const int ncl = scan[m - 1] + flag[m - 1];
+        off[ncl] = m;
+        info[0] = ncl;
+        return;
+    }
     if (flag[k]) off[scan[k]] = k;
-    (void)size_neg;
+}
+
+// info[1] = the largest cluster (info[1] cleared before)
+__global__ void k_segment_max(int m, const int32_t *__restrict__ flag, const int32_t *__restrict__ scan,
+                              const int32_t *__restrict__ off, int32_t *__restrict__ info) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m || !flag[k]) return;
+    const int c = scan[k];
+    atomicMax(&info[1], off[c + 1] - off[c]);
 }
 
 // Members of each cluster in ascending id (the reference's particle-array order on one
@@ -289,10 +304,10 @@ __global__ void k_order_by_id(int m, const int32_t *__restrict__ flag, const int
 // Processing order of the clusters: the costliest first, the cost estimated as block
 // steps (dt_tree over the smallest current step of the members) times (4 + size), so that
 // the cluster with the longest chain of block steps starts at once rather than after the
-// bulk of the small ones.  Also the largest size (for the size-class launches).
+// bulk of the small ones.
 __global__ void k_cluster_cost(int ncl, const int32_t *__restrict__ off, const int32_t *__restrict__ index,
                                const double *__restrict__ dt, double dt_tree, int32_t *__restrict__ key,
-                               int32_t *__restrict__ idx, int32_t *__restrict__ max_size) {
+                               int32_t *__restrict__ idx) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncl) return;
     const int b = off[c], e = off[c + 1], sz = e - b;
@@ -304,7 +319,6 @@ __global__ void k_cluster_cost(int ncl, const int32_t *__restrict__ off, const i
     const double cost = (dt_tree / dmin) * (4.0 + sz);
     key[c] = -(int32_t)fmin(2.0e9, 64.0 * log2(cost));   // ascending sort: costliest first
     idx[c] = c;
-    atomicMax(max_size, sz);
 }
 
 // ------------------------------------------------------------------ host
@@ -451,12 +465,21 @@ int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch 
     GPLUM_ENSURE(cl.sscan, m);
     hipLaunchKernelGGL(k_segment_flags, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.key_sorted.p, cl.sflag.p);
     if ((rc = nb_scan(s, cl.sflag.p, cl.sscan.p, m, st))) return rc;
-    int ncl = 0;
-    if ((rc = last_sum(cl.sflag.p, cl.sscan.p, m, &ncl, st))) return rc;
-    cl.ncl = ncl;
-    GPLUM_ENSURE(cl.off, ncl + 1);
+    // offsets, cluster count and largest cluster, read back together
+    GPLUM_ENSURE(cl.off, (size_t)m + 1);
+    GPLUM_ENSURE(cl.max_dev, 2);
+    GPLUM_HIP(hipMemsetAsync(cl.max_dev.p, 0, 2 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_segment_offsets, dim3(ceil_div(m + 1, 256)), dim3(256), 0, st, m, cl.sflag.p,
-                       cl.sscan.p, cl.off.p, nullptr);
+                       cl.sscan.p, cl.off.p, cl.max_dev.p);
+    hipLaunchKernelGGL(k_segment_max, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.sflag.p, cl.sscan.p,
+                       cl.off.p, cl.max_dev.p);
+    GPLUM_HIP(cl.host.ensure(2 * sizeof(int32_t)));
+    int32_t *hinfo = cl.host.as<int32_t>();
+    GPLUM_HIP(hipMemcpyAsync(hinfo, cl.max_dev.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hipStreamSynchronize(st));
+    const int ncl = hinfo[0];
+    cl.ncl = ncl;
+    cl.max_size = hinfo[1];
     GPLUM_ENSURE(cl.index_tmp, m);
     hipLaunchKernelGGL(k_order_by_id, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cl.sflag.p, cl.sscan.p,
                        cl.off.p, cl.index.p, ps.id.p, cl.index_tmp.p);
@@ -466,20 +489,14 @@ int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch 
     GPLUM_ENSURE(cl.size_key_sorted, ncl);
     GPLUM_ENSURE(cl.order_in, ncl);
     GPLUM_ENSURE(cl.order, ncl);
-    GPLUM_ENSURE(cl.max_dev, 1);
-    GPLUM_HIP(hipMemsetAsync(cl.max_dev.p, 0, sizeof(int32_t), st));
     hipLaunchKernelGGL(k_cluster_cost, dim3(ceil_div(ncl, 256)), dim3(256), 0, st, ncl, cl.off.p, cl.index.p,
-                       ps.dt.p, dt_tree, cl.size_key.p, cl.order_in.p, cl.max_dev.p);
+                       ps.dt.p, dt_tree, cl.size_key.p, cl.order_in.p);
     bytes = 0;
     GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, cl.size_key.p, cl.size_key_sorted.p,
                                                   cl.order_in.p, cl.order.p, ncl, 0, 32, st));
     GPLUM_ENSURE(s.tmp, bytes + 16);
     GPLUM_HIP(hipcub::DeviceRadixSort::SortPairs(s.tmp.p, bytes, cl.size_key.p, cl.size_key_sorted.p,
                                                   cl.order_in.p, cl.order.p, ncl, 0, 32, st));
-    int max_size = 0;
-    GPLUM_HIP(hipMemcpyAsync(&max_size, cl.max_dev.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    cl.max_size = max_size;
     GPLUM_HIP(hipGetLastError());
     return GPLUM_OK;
 }

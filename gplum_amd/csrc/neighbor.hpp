@@ -29,7 +29,8 @@ struct NbState {
 struct ClusterList {
     DArray<int32_t> parent, cflag, cscan, idx_sel, index, index_tmp, sflag, sscan, off;
     DArray<uint64_t> ckey, key_sel, key_sorted;
-    DArray<int32_t> size_key, size_key_sorted, order_in, order, max_dev;
+    DArray<int32_t> size_key, size_key_sorted, order_in, order, max_dev;   // max_dev: cluster count, largest
+    HostBuf host;                                                            // read-back of max_dev
     int m = 0, ncl = 0, max_size = 0;
 };
 

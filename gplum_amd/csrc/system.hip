@@ -44,7 +44,8 @@ int ParticleSoA::ensure(int n) {
 
 // ----------------------------------------------------------------- kernels
 
-// Every particle array at once (one launch instead of one per array): dst[i] = src[perm[i]].
+// Every particle array at once (one launch instead of one per array): dst[i] = src[perm[i]];
+// a null source array is not moved.
 struct SoAPtrs {
     double *v3[7];      // pos, vel, acc, acc_d, jerk_d, acc_s, jerk_s
     double *v1[13];     // mass, r_planet, f, phi, phi_d, phi_s, r_out, r_out_inv, r_search, v_disp, time, dt, acc0
@@ -53,12 +54,16 @@ struct SoAPtrs {
     float *f1;          // fc_rate
 };
 
-static SoAPtrs soa_ptrs(ParticleSoA &A) {
-    return SoAPtrs{{A.pos.p, A.vel.p, A.acc.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p},
-                   {A.mass.p, A.r_planet.p, A.f.p, A.phi.p, A.phi_d.p, A.phi_s.p, A.r_out.p, A.r_out_inv.p,
-                    A.r_search.p, A.v_disp.p, A.time.p, A.dt.p, A.acc0.p},
+// force_out false: the soft-force outputs (acc, phi, neighbour count, rank, id_max,
+// id_min) are left out, for a reorder that a soft force follows (it assigns them for
+// every particle).
+static SoAPtrs soa_ptrs(ParticleSoA &A, bool force_out = true) {
+    return SoAPtrs{{A.pos.p, A.vel.p, force_out ? A.acc.p : nullptr, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p},
+                   {A.mass.p, A.r_planet.p, A.f.p, force_out ? A.phi.p : nullptr, A.phi_d.p, A.phi_s.p, A.r_out.p,
+                    A.r_out_inv.p, A.r_search.p, A.v_disp.p, A.time.p, A.dt.p, A.acc0.p},
                    {A.id.p, A.id_cluster.p},
-                   {A.number.p, A.rank.p, A.id_max.p, A.id_min.p, A.n_cluster.p, A.flags.p},
+                   {force_out ? A.number.p : nullptr, force_out ? A.rank.p : nullptr,
+                    force_out ? A.id_max.p : nullptr, force_out ? A.id_min.p : nullptr, A.n_cluster.p, A.flags.p},
                    A.fc_rate.p};
 }
 
@@ -68,14 +73,17 @@ __global__ void k_gather_all(int n, const int32_t *__restrict__ perm, SoAPtrs sr
     const int s = perm[i];
 #pragma unroll
     for (int k = 0; k < 7; k++)
+        if (src.v3[k])
 #pragma unroll
-        for (int c = 0; c < 3; c++) dst.v3[k][3 * i + c] = src.v3[k][3 * s + c];
+            for (int c = 0; c < 3; c++) dst.v3[k][3 * i + c] = src.v3[k][3 * s + c];
 #pragma unroll
-    for (int k = 0; k < 13; k++) dst.v1[k][i] = src.v1[k][s];
+    for (int k = 0; k < 13; k++)
+        if (src.v1[k]) dst.v1[k][i] = src.v1[k][s];
 #pragma unroll
     for (int k = 0; k < 2; k++) dst.l[k][i] = src.l[k][s];
 #pragma unroll
-    for (int k = 0; k < 6; k++) dst.w[k][i] = src.w[k][s];
+    for (int k = 0; k < 6; k++)
+        if (src.w[k]) dst.w[k][i] = src.w[k][s];
     dst.f1[i] = src.f1[s];
 }
 
@@ -358,20 +366,21 @@ int System::get_particles(gplum_particle *pp, int count) {
 
 int System::set_radii() { return device_cutoff_radii(p, A, n, rad, st); }
 
-// Morton sort + physical reorder of every particle array (A <- B gathered by perm).
+// Morton sort + physical reorder of the particle arrays (A <- B gathered by perm); the
+// soft force that follows assigns its outputs, so they are not moved.
 int System::reorder() {
     int rc = tree_sort_keys(tree, A.pos.p, n, st);
     if (rc) return rc;
-    return gather_all(tree.perm.p, n);
+    return gather_all(tree.perm.p, n, false);
 }
 
 // A <- A[perm[0..m)] for every particle array (through B); n becomes m.
-int System::gather_all(const int32_t *perm, int m) {
+int System::gather_all(const int32_t *perm, int m, bool force_out) {
     int rc = B.ensure(std::max(m, 1));
     if (rc) return rc;
     const int gb = ceil_div(m, 256);
     if (m > 0)
-        hipLaunchKernelGGL(k_gather_all, dim3(gb), dim3(256), 0, st, m, perm, soa_ptrs(A), soa_ptrs(B));
+        hipLaunchKernelGGL(k_gather_all, dim3(gb), dim3(256), 0, st, m, perm, soa_ptrs(A, force_out), soa_ptrs(B, force_out));
     GPLUM_HIP(hipGetLastError());
     A.swap(B);
     n = m;
@@ -638,42 +647,66 @@ __global__ __launch_bounds__(HSTAT_BLOCK) void k_hard_stats(int ncl, const int *
     __shared__ unsigned long long s_sum[3];
     __shared__ unsigned long long s_max;
     __shared__ int s_cap, s_base, s_w[HSTAT_BLOCK / 64];
+    __shared__ double s_ed[HSTAT_BLOCK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) { s_sum[0] = s_sum[1] = s_sum[2] = 0ull; s_max = 0ull; s_cap = 0; s_base = 0; }
     __syncthreads();
     unsigned long long bs = 0, pr = 0, nc = 0, mx = 0;
-    for (int c0 = 0; c0 < ncl; c0 += HSTAT_BLOCK) {
-        const int c = c0 + t;
-        int hit = 0;
-        if (c < ncl) {
-            const long long b = nblock[c];
-            if (b < 0) s_cap = 1;
-            else {
-                bs += (unsigned long long)b;
-                // largest count first, then the lowest cluster index
-                const unsigned long long key = ((unsigned long long)b << 32) | (unsigned)(0x7fffffff - c);
-                mx = key > mx ? key : mx;
-            }
-            pr += (unsigned long long)npair[c];
-            nc += (unsigned long long)ncol[c];
-            hit = ncol[c] > 0;
+    constexpr int U = 4;              // chunks per round: their loads in flight together
+    for (int c00 = 0; c00 < ncl; c00 += U * HSTAT_BLOCK) {
+        long long b[U], q[U];
+        int h[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c00 + u * HSTAT_BLOCK + t;
+            b[u] = c < ncl ? nblock[c] : 0;
+            q[u] = c < ncl ? npair[c] : 0;
+            h[u] = c < ncl ? ncol[c] : 0;
         }
-        // clusters with a collision, listed in ascending index
-        const unsigned long long m = __ballot(hit);
-        if (lane == 0) s_w[w] = __popcll(m);
-        __syncthreads();
-        int before = s_base;
-        for (int k = 0; k < w; k++) before += s_w[k];
-        if (hit) col[before + __popcll(m & ((1ull << lane) - 1ull))] = c;
-        __syncthreads();
-        if (t == 0) for (int k = 0; k < HSTAT_BLOCK / 64; k++) s_base += s_w[k];
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c00 + u * HSTAT_BLOCK + t;
+            if (c < ncl) {
+                if (b[u] < 0) s_cap = 1;
+                else {
+                    bs += (unsigned long long)b[u];
+                    // largest count first, then the lowest cluster index
+                    const unsigned long long key = ((unsigned long long)b[u] << 32) | (unsigned)(0x7fffffff - c);
+                    mx = key > mx ? key : mx;
+                }
+                pr += (unsigned long long)q[u];
+                nc += (unsigned long long)h[u];
+            }
+            const int hit = (c < ncl && h[u] > 0) ? 1 : 0;
+            // clusters with a collision, listed in ascending index
+            const unsigned long long m = __ballot(hit);
+            if (lane == 0) s_w[w] = __popcll(m);
+            __syncthreads();
+            int before = s_base;
+            for (int k = 0; k < w; k++) before += s_w[k];
+            if (hit) col[before + __popcll(m & ((1ull << lane) - 1ull))] = c;
+            __syncthreads();
+            if (t == 0) for (int k = 0; k < HSTAT_BLOCK / 64; k++) s_base += s_w[k];
+            __syncthreads();
+        }
     }
     atomicAdd(&s_sum[0], bs);
     atomicAdd(&s_sum[1], pr);
     atomicAdd(&s_sum[2], nc);
     atomicMax(&s_max, mx);
     __syncthreads();
+    // the dissipated energies of the collided clusters, gathered by all threads and
+    // summed by one in ascending cluster order
+    double e = 0.;
+    for (int k0 = 0; k0 < s_base; k0 += HSTAT_BLOCK) {
+        if (k0 + t < s_base) s_ed[t] = ed[col[k0 + t]];
+        __syncthreads();
+        if (t == 0) {
+            const int kn = min(HSTAT_BLOCK, s_base - k0);
+            for (int k = 0; k < kn; k++) e += s_ed[k];
+        }
+        __syncthreads();
+    }
     if (t == 0) {
         const int cm = 0x7fffffff - (int)(unsigned)(s_max & 0xffffffffull);
         out[0] = (long long)s_sum[0];
@@ -682,8 +715,6 @@ __global__ __launch_bounds__(HSTAT_BLOCK) void k_hard_stats(int ncl, const int *
         out[3] = (long long)s_sum[1];
         out[4] = (long long)s_sum[2];
         out[5] = s_cap;
-        double e = 0.;
-        for (int k = 0; k < s_base; k++) e += ed[col[k]];
         *ed_sum = e;
     }
 }
@@ -693,15 +724,19 @@ int System::hard_part(bool persistent) {
     int rc;
     if (!have_nbr) return fail(GPLUM_E_STATE, "neighbour graph missing: initialize first");
     tick(EV_HARD0);
-    if ((rc = make_clusters(A, n, nb, cl, scratch, p.dt_tree, st))) return rc;
-    // the isolated particles (Kepler drift) on their own stream, alongside the clusters:
-    // the two sets are disjoint (number == 0 vs > 0)
+    // the isolated particles (Kepler drift) on their own stream, alongside the cluster
+    // search and the clusters: the two sets are disjoint (number == 0 vs > 0), and the
+    // search reads only the neighbour graph, ids and the members' steps
     IsoArgs ia{A.pos.p, A.vel.p, A.acc_d.p, A.jerk_d.p, A.acc_s.p, A.jerk_s.p, A.phi_d.p, A.phi_s.p, A.time.p,
                A.dt.p, A.acc0.p, A.number.p, A.n_cluster.p};
     GPLUM_HIP(hipEventRecord(ev_iso_fork, st));
     GPLUM_HIP(hipStreamWaitEvent(st_iso, ev_iso_fork, 0));
     if ((rc = isolated_launch(p, ia, n, time_sys, time_sys + p.dt_tree, st_iso))) return rc;
     GPLUM_HIP(hipEventRecord(ev_iso_join, st_iso));
+    if ((rc = make_clusters(A, n, nb, cl, scratch, p.dt_tree, st))) {
+        (void)hipStreamSynchronize(st_iso);
+        return rc;
+    }
     const int m = cl.m, ncl = cl.ncl;
     stats.n_clusters = ncl;
     stats.n_cluster_particles = m;

@@ -111,7 +111,7 @@ struct System {
     int get_particles(gplum_particle *pp, int count);
     int set_radii();
     int reorder();
-    int gather_all(const int32_t *perm, int m);
+    int gather_all(const int32_t *perm, int m, bool force_out = true);
     int soft_force(bool initial, bool use_f64);
     int fc_join();                      // st waits for the pending training (device side)
     int fc_sync();                      // the host waits for it

@@ -196,8 +196,9 @@ __device__ inline int block_sum(int v, int *s_w) {
 // where a workgroup's chunk holds one node) find their digit bounds with the whole
 // workgroup: each round samples 255 cut points of each of the seven search intervals at
 // once and keeps the piece holding the first key of a larger digit, so a node of N keys
-// takes log_256 N rounds of parallel loads instead of log_2 N dependent ones.  Smaller
-// nodes keep one thread each (many nodes per chunk: their searches run side by side).
+// takes log_256 N rounds of parallel loads instead of log_2 N dependent ones.  Nodes of
+// at least coop_min / 256 keys in a chunk of at most 32 nodes take one wave each
+// (log_64 N rounds); the others keep one thread each (their searches run side by side).
 
 // bnd8[d] = first index in [s, e) whose digit at level lv is > d (d = 0..6), bnd8[7] = e:
 // the same bounds as digit_bounds (bnd[d + 1]).  Every thread of the workgroup calls it.
@@ -210,21 +211,26 @@ __device__ void digit_bounds_wg(const uint64_t *__restrict__ key, int s, int e, 
         bool open = false;
         for (int d = 0; d < 7; d++) open = open || (s_hi[d] > s_lo[d]);
         if (!open) break;            // the same in every thread (read after a barrier)
-        if (t < 7) s_first[t] = LEVEL_BLOCK - 1;
-        __syncthreads();
-        if (t < LEVEL_BLOCK - 1) {
-            for (int d = 0; d < 7; d++) {
-                const int lo = s_lo[d], hi = s_hi[d];
-                if (hi <= lo) continue;
-                const int q = lo + (int)(((long long)(hi - lo) * (t + 1)) >> 8);   // cut point t, in [lo, hi)
-                if (digit_at(key[q], lv) > d) atomicMin(&s_first[d], t);
-            }
+        int dg[7];
+#pragma unroll
+        for (int d = 0; d < 7; d++) {          // the seven loads first, in flight together
+            const int lo = s_lo[d], hi = s_hi[d];
+            const int q = lo + (int)(((long long)(hi - lo) * (t + 1)) >> 8);   // cut point t, in [lo, hi)
+            dg[d] = (t < LEVEL_BLOCK - 1 && hi > lo) ? digit_at(key[q], lv) : -1;
+        }
+        // the first cut point with a larger digit: a ballot per wave, then the smallest
+        // over the waves (no atomics: the predicate rises with t)
+#pragma unroll
+        for (int d = 0; d < 7; d++) {
+            const unsigned long long m = __ballot(dg[d] > d);
+            if (__lane_id() == 0) s_first[d * (LEVEL_BLOCK / 64) + (t >> 6)] = m ? (t & ~63) + __ffsll((long long)m) - 1 : LEVEL_BLOCK - 1;
         }
         __syncthreads();
         if (t < 7) {
             const int lo = s_lo[t], hi = s_hi[t];
             if (hi > lo) {
-                const int f = s_first[t];
+                int f = LEVEL_BLOCK - 1;
+                for (int w = 0; w < LEVEL_BLOCK / 64; w++) f = min(f, s_first[t * (LEVEL_BLOCK / 64) + w]);
                 const long long len = hi - lo;
                 // cut points q_k = lo + (len (k + 1)) >> 8; the first index with a larger
                 // digit lies in (q_{f-1}, q_f] (f = 255: none among the cut points)
@@ -241,18 +247,60 @@ __device__ void digit_bounds_wg(const uint64_t *__restrict__ key, int s, int e, 
     __syncthreads();
 }
 
+// The same bounds found by one wave: each round samples 63 cut points of each open
+// interval (lane t: cut point t) and keeps the piece holding the first key of a larger
+// digit (a ballot), so a node of N keys takes log_64 N rounds.  Every lane calls it
+// with the same node and receives the same bounds.
+__device__ void digit_bounds_wave(const uint64_t *__restrict__ key, int s, int e, int lv, int bnd8[8]) {
+    const int t = __lane_id();
+    int lo[7], hi[7];
+#pragma unroll
+    for (int d = 0; d < 7; d++) { lo[d] = s; hi[d] = e; }
+    for (;;) {
+        bool open = false;
+#pragma unroll
+        for (int d = 0; d < 7; d++) open = open || (hi[d] > lo[d]);
+        if (!open) break;                  // wave-uniform
+        int dg[7];
+#pragma unroll
+        for (int d = 0; d < 7; d++) {
+            const long long len = hi[d] - lo[d];
+            const int q = lo[d] + (int)((len * (t + 1)) >> 6);    // in [lo, hi) for t < 63
+            dg[d] = (t < 63 && len > 0) ? digit_at(key[q], lv) : -1;
+        }
+#pragma unroll
+        for (int d = 0; d < 7; d++) {
+            if (hi[d] <= lo[d]) continue;
+            const unsigned long long m = __ballot(dg[d] > d);
+            const int f = m ? __ffsll((long long)m) - 1 : 63;     // 63: none of the cut points
+            const long long len = hi[d] - lo[d];
+            // cut points q_k = lo + (len (k + 1)) >> 6; the first index with a larger
+            // digit lies in (q_{f-1}, q_f]
+            const int nlo = (f == 0) ? lo[d] : lo[d] + (int)((len * f) >> 6) + 1;
+            const int nhi = (f == 63) ? hi[d] : lo[d] + (int)((len * (f + 1)) >> 6);
+            lo[d] = nlo;
+            hi[d] = nhi;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 7; d++) bnd8[d] = lo[d];
+    bnd8[7] = e;
+}
+
 __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
     const uint64_t *__restrict__ key, const int32_t *__restrict__ lvl, int L, int n_leaf,
     const int32_t *__restrict__ start, const int32_t *__restrict__ count, const int32_t *__restrict__ level,
-    int32_t *__restrict__ nchild, int32_t *__restrict__ part, int32_t *__restrict__ bnd_out, int coop_min) {
+    int32_t *__restrict__ nchild, int32_t *__restrict__ part, int32_t *__restrict__ bnd_out, int coop_min,
+    int wave_min) {
     __shared__ int s_w[LEVEL_BLOCK / 64];
-    __shared__ int s_big[LEVEL_BLOCK], s_nbig, s_lo[7], s_hi[7], s_first[7];
+    __shared__ int s_big[LEVEL_BLOCK], s_nbig, s_lo[7], s_hi[7], s_first[7 * (LEVEL_BLOCK / 64)];
+    __shared__ int s_mid[LEVEL_BLOCK], s_nmid;
     int lb, m, c0, c1;
     level_chunk(lvl, L, lb, m, c0, c1);
     if (m <= 0) return;
     int sum = 0;
     for (int r0 = c0; r0 < c1; r0 += LEVEL_BLOCK) {
-        if (threadIdx.x == 0) s_nbig = 0;
+        if (threadIdx.x == 0) { s_nbig = 0; s_nmid = 0; }
         __syncthreads();
         const int r = r0 + (int)threadIdx.x;
         if (r < c1) {
@@ -262,6 +310,11 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
             if (cnt > n_leaf && lv < MAXLEVEL) {
                 if (cnt >= coop_min) {
                     s_big[atomicAdd(&s_nbig, 1)] = r;   // the workgroup searches it below
+                    nc = -1;
+                } else if (cnt >= wave_min && c1 - c0 <= 32) {
+                    // a wave searches it below (few nodes per workgroup: otherwise one
+                    // thread per node, all side by side, takes fewer rounds)
+                    s_mid[atomicAdd(&s_nmid, 1)] = r;
                     nc = -1;
                 } else {
                     int bnd[9];
@@ -279,6 +332,25 @@ __global__ __launch_bounds__(LEVEL_BLOCK) void k_count_children_lv(
             }
         }
         __syncthreads();
+        const int nmid = s_nmid;
+        for (int k = (int)(threadIdx.x >> 6); k < nmid; k += LEVEL_BLOCK / 64) {
+            const int rb = s_mid[k];
+            const int c = lb + rb;
+            const int s = start[c];
+            int bnd8[8];
+            digit_bounds_wave(key, s, s + count[c], level[c], bnd8);
+            if (__lane_id() == 0) {
+                int nc = 0, prev = s;
+#pragma unroll
+                for (int d = 0; d < 8; d++) {
+                    nc += bnd8[d] > prev ? 1 : 0;
+                    prev = bnd8[d];
+                    bnd_out[8 * (size_t)rb + d] = bnd8[d];
+                }
+                nchild[rb] = nc;
+                sum += nc;
+            }
+        }
         const int nbig = s_nbig;
         for (int k = 0; k < nbig; k++) {
             const int rb = s_big[k];
@@ -708,7 +780,7 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
     for (int L = 0; L <= MAXLEVEL; L++) {
         hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
-                           t.lvl_part.p, t.lvl_bnd.p, t.coop_min);
+                           t.lvl_part.p, t.lvl_bnd.p, t.coop_min, std::max(1, t.coop_min >> 8));
         hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
                            t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
                            t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
