@@ -45,14 +45,16 @@ def make_disk(n, seed=1):
     return ic.disk(n, 1e-12, 0.5, 2.0, seed=seed)
 
 
-def measured_traffic(name):
+def measured_traffic(name, key=None):
     """HBM bytes per launch of a kernel from its committed counter summary under
-    profiles/ (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy); None if absent."""
+    profiles/ (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy), for the launch
+    shape `key` (e.g. "n_group_limit=128") when the summary lists several; None if absent."""
     import re
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(path).read(), re.M)
+    tag = r"\[" + re.escape(key) + r"\]" if key else ""
+    m = re.search(r"^traffic_bytes_per_launch" + tag + r":\s*(\d+)", open(path).read(), re.M)
     return int(m.group(1)) if m else None
 
 
@@ -423,7 +425,10 @@ def main():
     achieved = flops / (force_ms * 1e-3) / 1e12
     # compute-bound kernel; "mfma" names the schema's compute roof: the arithmetic is FP32
     # VALU (pairwise gravity is not a contraction), priced against the FP32 vector peak
-    traffic = measured_traffic("k_soft_f32_n1m_traffic.md") if args.n == 1_000_000 else None
+    # counters of the launch shape in effect (the autotuner's i-group size, else the default)
+    ng_lim = tuning["n_group_limit"] if tuning else params_dict()["n_group_limit"]
+    traffic = (measured_traffic("k_soft_f32_n1m_traffic.md", "n_group_limit=%d" % ng_lim)
+               if args.n == 1_000_000 else None)
     roofline_soft = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "compute": "fp32 VALU, no MFMA",
