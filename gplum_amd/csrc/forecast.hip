@@ -84,8 +84,9 @@ __global__ __launch_bounds__(64) void k_fc_capacity(int ng, const int32_t *__res
 }
 
 // Consecutive groups per thread in the one-block scans below: a pass covers
-// FC_SCAN_BLOCK * FC_IPT groups with one block scan of the threads' sums.
-constexpr int FC_IPT = 8;
+// FC_SCAN_BLOCK * FC_IPT groups with one block scan of the threads' sums (24: the
+// i-groups of a million particles in one pass, their loads in flight together).
+constexpr int FC_IPT = 24;
 
 // one block: exclusive offsets of the capacities (each at most FC_SORT_MAX, the keys a
 // group sorts in LDS), clipped to the allocated slots
@@ -96,7 +97,8 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *_
     long long carry = 0;
     for (int base = 0; base < ng; base += FC_SCAN_BLOCK * FC_IPT) {
         const int g0 = base + (int)threadIdx.x * FC_IPT;
-        long long v[FC_IPT], sum = 0;
+        int v[FC_IPT];
+        long long sum = 0;
         for (int k = 0; k < FC_IPT; k++) {
             v[k] = g0 + k < ng ? min(cap[g0 + k], FC_SORT_MAX) : 0;
             sum += v[k];
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *_
             if (g < ng) {
                 off[g] = (int32_t)(o < slots ? o : slots);
                 const long long room = slots - o;
-                cap[g] = (int32_t)(room <= 0 ? 0 : (v[k] < room ? v[k] : room));
+                cap[g] = (int32_t)(room <= 0 ? 0 : ((long long)v[k] < room ? (long long)v[k] : room));
             }
             o += v[k];
         }
@@ -132,13 +134,16 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int3
     long long cs = 0, co = 0;
     for (int base = 0; base < ng; base += FC_SCAN_BLOCK * FC_IPT) {
         const int g0 = base + (int)threadIdx.x * FC_IPT;
-        long long c[FC_IPT], sc = 0, so = 0;
+        int c[FC_IPT];
+        bool over[FC_IPT];
+        long long sc = 0, so = 0;
         for (int k = 0; k < FC_IPT; k++) {
             const int g = g0 + k;
             c[k] = g < ng ? cnt[g] : 0;
+            over[k] = g < ng && c[k] > cap[g];
             sc += c[k];
             // overflowed groups and their candidates in one sum (count above bit 40)
-            so += (g < ng && c[k] > cap[g]) ? FC_PACK + c[k] : 0;
+            so += over[k] ? FC_PACK + c[k] : 0;
         }
         long long ts, to;
         long long es = cs + block_excl_scan(sc, s_w, &ts);
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int3
             const int g = g0 + k;
             if (g < ng) {
                 fin_off[g] = (int32_t)es;
-                if (c[k] > cap[g]) {
+                if (over[k]) {
                     ovf_list[eo / FC_PACK] = g;
                     ovf_pos[eo / FC_PACK] = (int32_t)(eo % FC_PACK);
                     eo += FC_PACK + c[k];

@@ -32,7 +32,9 @@ struct SplitInfo {
         if (ni_pad == 0) ni_pad = 64;
         nsplit = SOFT_BLOCK / ni_pad;
         ii = (int)threadIdx.x % ni_pad;
-        sp = (int)threadIdx.x / ni_pad;
+        // ni_pad is a multiple of 64: a wave's lanes share their split, so the j loops
+        // below run on scalar counters and broadcast LDS reads
+        sp = __builtin_amdgcn_readfirstlane((int)threadIdx.x / ni_pad);
         in_split = sp < nsplit;
         valid = in_split && ii < ni;
     }

@@ -1090,7 +1090,9 @@ int System::merge_and_remove(int *n_removed) {
     const int m = tail[0] + tail[1];
     GPLUM_ENSURE(M.perm, m + 1);
     hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.keep.p, M.kscan.p, M.perm.p);
-    return gather_all(M.perm.p, m);
+    // a merge or a removal is always followed by the recomputed soft force, which assigns
+    // the force outputs of every particle: they are not moved
+    return gather_all(M.perm.p, m, false);
 }
 
 // Energy::calcEnergy of the step (main_p3t.cpp:629, 692): the partial sums stay on the
