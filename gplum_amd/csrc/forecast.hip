@@ -8,33 +8,6 @@ namespace gplum {
 
 constexpr int FC_SCAN_BLOCK = 1024;
 
-// Exclusive scan over one block of FC_SCAN_BLOCK threads (16 waves); *tot receives
-// the block total.  s_w holds 16 wave totals.
-__device__ inline long long block_excl_scan(long long v, long long *s_w, long long *tot) {
-    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
-    constexpr int NW = FC_SCAN_BLOCK / 64;
-    long long x = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const long long y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) s_w[w] = x;
-    __syncthreads();
-    if (w == 0) {
-        long long t = lane < NW ? s_w[lane] : 0;
-        for (int d = 1; d < NW; d <<= 1) {
-            const long long y = __shfl_up(t, d);
-            if (lane >= d) t += y;
-        }
-        if (lane < NW) s_w[lane] = t;
-    }
-    __syncthreads();
-    const long long excl = x - v + (w > 0 ? s_w[w - 1] : 0);
-    *tot = s_w[NW - 1];
-    __syncthreads();
-    return excl;
-}
-
 // Features of the particles of group g (one 64-thread block per group):
 //  0  log density of the group's near list: (n_near + 1) / V, V = the group's
 //     particle box grown by 1.1 max r_search on every side
@@ -83,36 +56,76 @@ __global__ __launch_bounds__(64) void k_fc_capacity(int ng, const int32_t *__res
     if (threadIdx.x == 0) cap[g] = (int32_t)fmin(ceil(s + 3. * sqrt(s) + 8.), 1e9);
 }
 
-// Consecutive groups per thread in the one-block scans below: a pass covers
-// FC_SCAN_BLOCK * FC_IPT groups with one block scan of the threads' sums (24: the
-// i-groups of a million particles in one pass, their loads in flight together).
-constexpr int FC_IPT = 24;
+// The one-block scans below: wave w of a pass covers FC_IPT * 64 consecutive groups, lane
+// l the groups w * 64 * FC_IPT + k * 64 + l (k < FC_IPT), so that every load and store is
+// coalesced; the exclusive offsets follow the group order (a wave scan per k with a
+// running carry, then the waves' totals); a lane's FC_IPT loads are in flight together.
+constexpr int FC_IPT = 8;
+
+__device__ inline long long wave_incl_scan_ll(long long x) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// the waves' totals of a pass: *before = the waves before this one, returns the pass total
+__device__ inline long long block_wave_totals(long long wave_total, long long *s_w, long long *before) {
+    const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
+    constexpr int NW = FC_SCAN_BLOCK / 64;
+    if (lane == 0) s_w[w] = wave_total;
+    __syncthreads();
+    long long b = 0, tot = 0;
+    for (int j = 0; j < NW; j++) {
+        b += j < w ? s_w[j] : 0;
+        tot += s_w[j];
+    }
+    __syncthreads();
+    *before = b;
+    return tot;
+}
+
+__device__ inline long long wave_sum_ll(long long x) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d);
+    return x;
+}
 
 // one block: exclusive offsets of the capacities (each at most FC_SORT_MAX, the keys a
-// group sorts in LDS), clipped to the allocated slots
+// group sorts in LDS), clipped to the allocated slots.  Per pass: the waves' totals
+// first, then each wave's scan in group order, written as it goes.
 __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_offsets(int ng, int32_t *__restrict__ cap,
                                                               int32_t *__restrict__ off, long long slots,
                                                               int64_t *__restrict__ info) {
     __shared__ long long s_w[FC_SCAN_BLOCK / 64];
+    const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
     long long carry = 0;
     for (int base = 0; base < ng; base += FC_SCAN_BLOCK * FC_IPT) {
-        const int g0 = base + (int)threadIdx.x * FC_IPT;
+        const int g0 = base + w * 64 * FC_IPT + lane;
         int v[FC_IPT];
         long long sum = 0;
+#pragma unroll
         for (int k = 0; k < FC_IPT; k++) {
-            v[k] = g0 + k < ng ? min(cap[g0 + k], FC_SORT_MAX) : 0;
+            const int g = g0 + 64 * k;
+            v[k] = g < ng ? min(cap[g], FC_SORT_MAX) : 0;
             sum += v[k];
         }
-        long long tot;
-        long long o = carry + block_excl_scan(sum, s_w, &tot);
+        long long before;
+        const long long tot = block_wave_totals(wave_sum_ll(sum), s_w, &before);
+        long long run = carry + before;
         for (int k = 0; k < FC_IPT; k++) {
-            const int g = g0 + k;
+            const long long x = wave_incl_scan_ll(v[k]);
+            const long long o = run + x - v[k];
+            run += __shfl(x, 63);
+            const int g = g0 + 64 * k;
             if (g < ng) {
                 off[g] = (int32_t)(o < slots ? o : slots);
                 const long long room = slots - o;
                 cap[g] = (int32_t)(room <= 0 ? 0 : ((long long)v[k] < room ? (long long)v[k] : room));
             }
-            o += v[k];
         }
         carry += tot;
     }
@@ -131,34 +144,40 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int3
                                                               int32_t *__restrict__ ovf_pos,
                                                               int64_t *__restrict__ info) {
     __shared__ long long s_w[FC_SCAN_BLOCK / 64];
+    const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
     long long cs = 0, co = 0;
     for (int base = 0; base < ng; base += FC_SCAN_BLOCK * FC_IPT) {
-        const int g0 = base + (int)threadIdx.x * FC_IPT;
+        const int g0 = base + w * 64 * FC_IPT + lane;
         int c[FC_IPT];
         bool over[FC_IPT];
         long long sc = 0, so = 0;
+#pragma unroll
         for (int k = 0; k < FC_IPT; k++) {
-            const int g = g0 + k;
+            const int g = g0 + 64 * k;
             c[k] = g < ng ? cnt[g] : 0;
             over[k] = g < ng && c[k] > cap[g];
             sc += c[k];
-            // overflowed groups and their candidates in one sum (count above bit 40)
+            // overflowed groups and their candidates in one sum (the group count above bit 40)
             so += over[k] ? FC_PACK + c[k] : 0;
         }
-        long long ts, to;
-        long long es = cs + block_excl_scan(sc, s_w, &ts);
-        long long eo = co + block_excl_scan(so, s_w, &to);
+        long long bs, bo;
+        const long long ts = block_wave_totals(wave_sum_ll(sc), s_w, &bs);
+        const long long to = block_wave_totals(wave_sum_ll(so), s_w, &bo);
+        long long rs = cs + bs, ro = co + bo;
         for (int k = 0; k < FC_IPT; k++) {
-            const int g = g0 + k;
+            const long long vo = over[k] ? FC_PACK + c[k] : 0;
+            const long long xs = wave_incl_scan_ll(c[k]), xo = wave_incl_scan_ll(vo);
+            const long long es = rs + xs - c[k], eo = ro + xo - vo;
+            rs += __shfl(xs, 63);
+            ro += __shfl(xo, 63);
+            const int g = g0 + 64 * k;
             if (g < ng) {
                 fin_off[g] = (int32_t)es;
                 if (over[k]) {
                     ovf_list[eo / FC_PACK] = g;
                     ovf_pos[eo / FC_PACK] = (int32_t)(eo % FC_PACK);
-                    eo += FC_PACK + c[k];
                 }
             }
-            es += c[k];
         }
         cs += ts; co += to;
     }

@@ -771,24 +771,36 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
     GPLUM_ENSURE(t.grp_flag, cap);   // reused as per-level child counts
     hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, n, t.node_start.p, t.node_count.p,
                        t.node_level.p, t.node_parent.p);
-    // levels 0..MAXLEVEL back to back, ranges kept on the device (a level past the last
-    // one is empty and its kernels return at once); one read-back at the end
+    // levels back to back, ranges kept on the device (a level past the last one is empty
+    // and its kernels return at once); one read-back at the end
     GPLUM_ENSURE(t.lvl, LVL_ERR + 5);
     hipLaunchKernelGGL(k_lvl_init, dim3(1), dim3(64), 0, st, t.lvl.p);
     GPLUM_ENSURE(t.lvl_part, LEVEL_GRID);
     GPLUM_ENSURE(t.lvl_bnd, 8 * ((size_t)n + 64));   // a level holds at most n nodes
-    for (int L = 0; L <= MAXLEVEL; L++) {
-        hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
-                           p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
-                           t.lvl_part.p, t.lvl_bnd.p, t.coop_min, std::max(1, t.coop_min >> 8));
-        hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
-                           t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
-                           t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
-    }
+    // levels up to one past the previous tree's depth (their kernels cost launches even
+    // when empty); if the level after them holds nodes, the rest follow after the read-back
+    auto levels = [&](int L0, int L1) {
+        for (int L = L0; L < L1; L++) {
+            hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
+                               p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
+                               t.lvl_part.p, t.lvl_bnd.p, t.coop_min, std::max(1, t.coop_min >> 8));
+            hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
+                               t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
+                               t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
+        }
+    };
+    const int L_end = t.n_level > 0 ? std::min(MAXLEVEL + 1, t.n_level + 2) : MAXLEVEL + 1;
+    levels(0, L_end);
     std::vector<int32_t> lv(LVL_ERR + 1);
     GPLUM_HIP(hipMemcpyAsync(lv.data(), t.lvl.p, sizeof(int32_t) * (LVL_ERR + 1), hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
     GPLUM_HIP(hipGetLastError());
+    if (L_end <= MAXLEVEL && !lv[LVL_ERR] && lv[L_end + 1] > lv[L_end]) {
+        levels(L_end, MAXLEVEL + 1);     // level L_end holds nodes: a deeper tree than before
+        GPLUM_HIP(hipMemcpyAsync(lv.data(), t.lvl.p, sizeof(int32_t) * (LVL_ERR + 1), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipStreamSynchronize(st));
+        GPLUM_HIP(hipGetLastError());
+    }
     if (lv[LVL_ERR]) return fail(GPLUM_E_OVERFLOW, "tree node capacity exceeded");
     t.level_begin.assign(1, 0);
     for (int L = 1; L <= MAXLEVEL + 2 && lv[L] > lv[L - 1]; L++) t.level_begin.push_back(lv[L]);
