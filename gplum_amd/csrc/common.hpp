@@ -98,8 +98,12 @@ __host__ __device__ inline double exp_exact_ops(double x) {
     const double INV_LN2 = 1.44269504088896338700e+00;
     const double k = rint(x * INV_LN2);
     const double r = (x - k * LN2_HI) - k * LN2_LO;
-    double s = 1. + r * (1. / 13.);
-    for (int m = 12; m >= 1; m--) s = 1. + (r / m) * s;
+    // Taylor series of e^r to degree 13, Horner form with the coefficients 1/m! rounded
+    // to double (one multiply and one add per degree, no division)
+    constexpr double C[14] = {1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333, 0.001388888888888889, 0.0001984126984126984, 2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10};
+    double s = C[13];
+#pragma unroll
+    for (int m = 12; m >= 0; m--) s = C[m] + r * s;
     return ldexp(s, (int)k);
 }
 

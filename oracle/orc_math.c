@@ -24,7 +24,9 @@ double orc_cbrt_exact_ops(double x)
 }
 
 /* e^x: k = rint(x / ln 2), r = x - k ln 2 (Cody-Waite, exact high part), Taylor
- * series of e^r to degree 13 in Horner form, times 2^k. */
+ * series of e^r to degree 13 in Horner form (coefficients 1/m! rounded to double, one
+ * multiply and one add per degree), times 2^k. */
+static const double ORC_EXP_C[14] = {1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333, 0.001388888888888889, 0.0001984126984126984, 2.48015873015873e-05, 2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10};
 double orc_exp_exact_ops(double x)
 {
     if (x < -745.2) return 0.;
@@ -33,7 +35,7 @@ double orc_exp_exact_ops(double x)
     const double INV_LN2 = 1.44269504088896338700e+00;
     const double k = rint(x * INV_LN2);
     const double r = (x - k * LN2_HI) - k * LN2_LO;
-    double s = 1. + r * (1. / 13.);
-    for (int m = 12; m >= 1; m--) s = 1. + (r / m) * s;
+    double s = ORC_EXP_C[13];
+    for (int m = 12; m >= 0; m--) s = ORC_EXP_C[m] + r * s;
     return ldexp(s, (int)k);
 }
