@@ -980,7 +980,8 @@ __device__ inline void group_pair_sum(const HP &P, const JData &J, int n, int s,
 // (no barrier inside).
 template <int K>
 __device__ void group_correct_active(const HP &P, HardView &H, const JData &J, int b, int n, double t_s,
-                                     int flag_col, long long &pairs, const WaveSlots &W) {
+                                     int flag_col, long long &pairs, const WaveSlots &W,
+                                     const double *__restrict__ a0_lds) {
     const int t = threadIdx.x;
     const int B = blockDim.x;
     const int s = W.gs, E = 64 >> s;
@@ -1034,13 +1035,15 @@ __device__ void group_correct_active(const HP &P, HardView &H, const JData &J, i
             }
         };
         // Registers are scarce here (the pair sums keep a subtree stack per lane): the
-        // body's predicted state is reloaded for each correction instead of being held
+        // body's predicted state and its acc/jerk at the start of the block step are
+        // reread from LDS (the j-data and a0) for each correction instead of being held
         // across the pair sums, and the correction's a2/a3 terms go through H.
+        const int el = valid ? e : 0;
         const double rinv = H.rinv[g];
         const int64_t id = H.id[g];
         {
             double xp[3], vp[3];
-            ld3(H.xp, g, xp); ld3(H.vp, g, vp);
+            ld3(J.x, el, xp); ld3(J.v, el, vp);
             group_pair_sum<K>(P, J, n, s, r, id, xp, vp, rinv, ad, jd);
             star_acc_jerk(P, xp, vp, as, js, nullptr);
         }
@@ -1049,8 +1052,11 @@ __device__ void group_correct_active(const HP &P, HardView &H, const JData &J, i
         for (int ite = 0; ite < 2; ite++) {
             {
                 double xp[3], vp[3], a0s[3], j0s[3], a0d[3], j0d[3], a2s[3], a3s[3], a2d[3], a3d[3];
-                ld3(H.xp, g, xp); ld3(H.vp, g, vp);
-                ld3(H.a0_s, g, a0s); ld3(H.j0_s, g, j0s); ld3(H.a0_d, g, a0d); ld3(H.j0_d, g, j0d);
+                ld3(J.x, el, xp); ld3(J.v, el, vp);
+                const double *q = a0_lds + 12 * el;
+                for (int k = 0; k < 3; k++) {
+                    a0s[k] = q[k]; j0s[k] = q[3 + k]; a0d[k] = q[6 + k]; j0d[k] = q[9 + k];
+                }
                 correct_regs(t_c, xp, vp, as, a0s, j0s, js, ad, a0d, j0d, jd, x, v, a2s, a3s, a2d, a3d);
                 if (ite == 1 && lead && !twin) {
                     st3(H.a2_s, g, a2s); st3(H.a3_s, g, a3s); st3(H.a2_d, g, a2d); st3(H.a3_d, g, a3d);
@@ -1124,7 +1130,8 @@ __device__ void collision_tail(const HP &P, HardView &H, int b, int n, int ic, i
 template <bool LDSJ, bool COOP>
 __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce &R, double *s_x, double *s_v,
                                int64_t *s_id, double *s_fr, int b, int n, double &sys_time, long long &loop,
-                               double &ed, int &n_col, long long &pairs, WaveSlots *W = nullptr) {
+                               double &ed, int &n_col, long long &pairs, WaveSlots *W = nullptr,
+                               double *s_a0 = nullptr) {
     const int t = threadIdx.x;
     const int B = blockDim.x;
     const double time_end = P.dt_tree;
@@ -1145,7 +1152,16 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
                 s_id[i] = H.id[b + i];
                 // f R of the live particles for the collision test (-1: dead)
                 s_fr[i] = (H.flags[b + i] & GPLUM_FLAG_DEAD) ? -1. : H.f[b + i] * H.r_planet[b + i];
-                if (COOP) W->tw[i] = H.twin_of[b + i];
+                if (COOP) {
+                    W->tw[i] = H.twin_of[b + i];
+                    // acc/jerk at the start of the block step (predict's a0), for the lane groups
+                    for (int k = 0; k < 3; k++) {
+                        s_a0[12 * i + k] = H.a0_s[3 * (b + i) + k];
+                        s_a0[12 * i + 3 + k] = H.j0_s[3 * (b + i) + k];
+                        s_a0[12 * i + 6 + k] = H.a0_d[3 * (b + i) + k];
+                        s_a0[12 * i + 9 + k] = H.j0_d[3 * (b + i) + k];
+                    }
+                }
             }
         }
         __threadfence_block();
@@ -1347,9 +1363,9 @@ __device__ void integrate_from(const HP &P, HardView &H, const JData &J, Reduce 
             __threadfence_block();
             __syncthreads();
             const int K = (n + 63) >> 6;
-            if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, *W);
-            else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, *W);
-            else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, *W);
+            if (K <= 2) group_correct_active<2>(P, H, J, b, n, t_s, flag_col, pairs, *W, s_a0);
+            else if (K == 3) group_correct_active<3>(P, H, J, b, n, t_s, flag_col, pairs, *W, s_a0);
+            else group_correct_active<4>(P, H, J, b, n, t_s, flag_col, pairs, *W, s_a0);
             if (W->nm > 0)
                 coop_correct_active(P, H, J, b, n, t_s, flag_col, pairs, (int)(threadIdx.x >> 6), B >> 6, W->tw);
         } else if (COOP) {
@@ -2075,8 +2091,10 @@ __device__ __forceinline__ void hermite_clusters(HP P_arg, HardView H,
     __shared__ HP s_P;
     // register path, 3 to 8 particles: pair summands (side_force)
     __shared__ double s_tm[REG ? 6 * 64 : 1];
-    // multi-wave kernels: reduction slots and the active list
+    // multi-wave kernels: reduction slots and the active list, and each particle's
+    // acc/jerk at the start of the block step (12 per particle) for the lane groups
     __shared__ __align__(8) unsigned char s_ws[REG ? 8 : sizeof(WaveSlots)];
+    __shared__ double s_a0[(REG || !LDSJ) ? 1 : 12 * HERMITE_LDS];
     const int t = threadIdx.x;
     const int B = blockDim.x;
     if (REG) {
@@ -2127,7 +2145,7 @@ __device__ __forceinline__ void hermite_clusters(HP P_arg, HardView H,
             }
             __syncthreads();
             integrate_from<LDSJ, true>(P, H, J, R, s_x, s_v, s_id, s_fr, b, n, sys_time, loop, ed, n_col, pairs,
-                                       reinterpret_cast<WaveSlots *>(s_ws));
+                                       reinterpret_cast<WaveSlots *>(s_ws), s_a0);
         }
         atomicAdd(&s_pairs, (unsigned long long)pairs);
         __syncthreads();
