@@ -242,10 +242,20 @@ int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScrat
     hipLaunchKernelGGL(k_radii_apply, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, P, R.range.p, R.vd.p, R.np.p,
                        A.pos.p, A.vel.p, A.mass.p, A.v_disp.p, A.r_out.p, A.r_out_inv.p, A.r_search.p);
     GPLUM_HIP(hipGetLastError());
-    int err = 0;
-    GPLUM_HIP(hipMemcpyAsync(&err, R.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    if (err) return fail(GPLUM_E_ARG, "setCutoffRadii: particle ids must lie in [0, 2^58)");
+    // the flag (an id outside [0, 2^58), which set_particles rejects, or a bin index out of
+    // range from a non-finite position) is read back without a wait and checked by
+    // radii_check at the caller's next synchronisation
+    GPLUM_HIP(R.err_host.ensure(sizeof(int)));
+    GPLUM_HIP(hipMemcpyAsync(R.err_host.p, R.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    R.err_pending = true;
+    return GPLUM_OK;
+}
+
+int radii_check(RadiiScratch &R) {
+    if (!R.err_pending) return GPLUM_OK;
+    R.err_pending = false;
+    if (*R.err_host.as<int>())
+        return fail(GPLUM_E_ARG, "setCutoffRadii: particle ids must lie in [0, 2^58) and positions be finite");
     return GPLUM_OK;
 }
 

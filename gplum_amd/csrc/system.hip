@@ -235,11 +235,17 @@ System::~System() {
     if (st_iso) (void)hipStreamDestroy(st_iso);
     for (hipEvent_t e : {ev_iso_fork, ev_iso_join})
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {ev_fc_ready, ev_fc_done, ev_fc_t0, ev_fc_t1})
+    for (hipEvent_t e : {ev_fc_ready, ev_fc_done})
         if (e) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : evd) if (e) (void)hipEventDestroy(e);
+    for (const TimedPair &q : tpend) {
+        (void)hipEventDestroy(q.a);
+        (void)hipEventDestroy(q.b);
+    }
+    for (hipEvent_t e : ev_free) (void)hipEventDestroy(e);
 }
 
 int System::init_device() {
@@ -260,8 +266,6 @@ int System::init_device() {
     GPLUM_HIP(hipEventCreateWithFlags(&ev_iso_join, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_fc_ready, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_fc_done, hipEventDisableTiming));
-    GPLUM_HIP(hipEventCreate(&ev_fc_t0));
-    GPLUM_HIP(hipEventCreate(&ev_fc_t1));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     GPLUM_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     for (auto &e : ev) GPLUM_HIP(hipEventCreate(&e));
@@ -277,6 +281,9 @@ PartPtrs System::ptrs(ParticleSoA &s) {
 int System::set_particles(const gplum_particle *pp, int count, double time) {
     int rc = init_device();
     if (rc) return rc;
+    for (int i = 0; i < count; i++)
+        if (pp[i].id < 0 || pp[i].id >= (int64_t(1) << 58))
+            return fail(GPLUM_E_ARG, "particle ids must lie in [0, 2^58)");
     n = count;
     time_sys = time;
     if ((rc = A.ensure(n)) || (rc = B.ensure(n))) return rc;
@@ -389,7 +396,52 @@ int System::gather_all(const int32_t *perm, int m, bool force_out) {
 }
 
 void System::tick(int k) {
-    if (timing) (void)hipEventRecord(ev[k], st);
+    if (!timing) return;
+    if (tuning) {                  // the autotuner reads its intervals at once (elapsed)
+        (void)hipEventRecord(ev[k], st);
+        return;
+    }
+    hipEvent_t e = timing_event();
+    if (!e) return;
+    if (evd[k]) ev_free.push_back(evd[k]);   // an earlier tick that no interval took
+    evd[k] = e;
+    (void)hipEventRecord(e, st);
+}
+
+hipEvent_t System::timing_event() {
+    hipEvent_t e = nullptr;
+    if (!ev_free.empty()) {
+        e = ev_free.back();
+        ev_free.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+// *dst += the time between ticks a and b, once both have completed
+void System::accum(double *dst, int a, int b) {
+    if (!timing) return;
+    if (tuning) {
+        *dst += elapsed(a, b);
+        return;
+    }
+    if (!evd[a] || !evd[b]) return;
+    tpend.push_back(TimedPair{evd[a], evd[b], dst});
+    evd[a] = evd[b] = nullptr;
+    if (tpend.size() > 1024) resolve_timing();
+}
+
+// The pending intervals (their end events are recorded on streams the caller has just
+// synchronised, or are waited for here).
+void System::resolve_timing() {
+    for (const TimedPair &q : tpend) {
+        float ms = 0.f;
+        if (hipEventSynchronize(q.b) == hipSuccess && hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) *q.dst += ms;
+        ev_free.push_back(q.a);
+        ev_free.push_back(q.b);
+    }
+    tpend.clear();
 }
 
 double System::elapsed(int a, int b) {
@@ -403,12 +455,6 @@ int System::fc_join() {
     if (!fc_pending) return GPLUM_OK;
     GPLUM_HIP(hipStreamWaitEvent(st, ev_fc_done, 0));
     fc_pending = false;
-    if (fc_timed) {
-        float ms = 0.f;
-        if (hipEventSynchronize(ev_fc_t1) == hipSuccess && hipEventElapsedTime(&ms, ev_fc_t0, ev_fc_t1) == hipSuccess)
-            stats.fc_train_ms += ms;
-        fc_timed = false;
-    }
     return GPLUM_OK;
 }
 
@@ -424,6 +470,7 @@ int System::soft_force(bool initial, bool use_f64) {
     tick(EV_TREE0);
     if ((rc = reorder())) return rc;
     if ((rc = tree_build_nodes(tree, p, A.pos.p, A.mass.p, A.r_search.p, n, st))) return rc;
+    if ((rc = radii_check(rad))) return rc;     // the tree build has synchronised the stream
     if ((rc = tree_walk(tree, p, A.pos.p, A.r_search.p, st))) return rc;
     tick(EV_TREE1);
     GPLUM_ENSURE(jrec, n);
@@ -516,7 +563,8 @@ int System::soft_force(bool initial, bool use_f64) {
         stats.fc_slots += std::min<int64_t>(requested, (int64_t)fc.seg_capacity);
         stats.fc_ovf_groups += n_ovf;
         stats.fc_ovf_candidates += ovf_total;
-        stats.fc_ms += elapsed(EV_FCA0, EV_FCA1) + elapsed(EV_FCB0, EV_FCB1);
+        accum(&stats.fc_ms, EV_FCA0, EV_FCA1);
+        accum(&stats.fc_ms, EV_FCB0, EV_FCB1);
         // the plan did not fit the segment storage: grow it for the next call
         if ((size_t)requested > fc.seg_capacity) fc.seg_capacity = (size_t)requested + (size_t)requested / 8;
     } else {
@@ -551,28 +599,34 @@ int System::soft_force(bool initial, bool use_f64) {
             cand_cap = ncand + ncand / 4 + 1024;   // too small: grow and redo the step
             stats.cand_retries++;
         }
-        if (fc_on) stats.fc_ms += elapsed(EV_FCA0, EV_FCA1);
+        if (fc_on) accum(&stats.fc_ms, EV_FCA0, EV_FCA1);
     }
-    stats.force_ms += elapsed(EV_FORCE0, EV_FORCE1);
-    stats.tree_ms += elapsed(EV_TREE0, EV_TREE1);
+    accum(&stats.force_ms, EV_FORCE0, EV_FORCE1);
+    accum(&stats.tree_ms, EV_TREE0, EV_TREE1);
     stats.n_force_calls++;
     if (use_f64) stats.n_force_f64_calls++;
     last_ncand = ncand;
     tick(EV_NB0);
     if ((rc = neighbor_finalize(p, A, n, cand.p, (int)ncand, initial, nb, scratch, st, presorted))) return rc;
     tick(EV_NB1);
-    stats.finalize_ms += elapsed(EV_NB0, EV_NB1);
+    accum(&stats.finalize_ms, EV_NB0, EV_NB1);
     if (fc_fresh && !tuning && !skip_train) {
         // train on this call's exact FP32 candidate counts, on st_fc: the training
         // overlaps what follows (kick, merging, the next hard part)
         GPLUM_HIP(hipEventRecord(ev_fc_ready, st));
         GPLUM_HIP(hipStreamWaitEvent(st_fc, ev_fc_ready, 0));
-        if (timing) GPLUM_HIP(hipEventRecord(ev_fc_t0, st_fc));
+        hipEvent_t t0 = timing ? timing_event() : nullptr;
+        if (t0) GPLUM_HIP(hipEventRecord(t0, st_fc));
         if ((rc = fc_train(fc, nb.number_fp32.p, n, st_fc, false))) return rc;
-        if (timing) GPLUM_HIP(hipEventRecord(ev_fc_t1, st_fc));
+        hipEvent_t t1 = t0 ? timing_event() : nullptr;
+        if (t1) {
+            GPLUM_HIP(hipEventRecord(t1, st_fc));
+            tpend.push_back(TimedPair{t0, t1, &stats.fc_train_ms});
+        } else if (t0) {
+            ev_free.push_back(t0);
+        }
         GPLUM_HIP(hipEventRecord(ev_fc_done, st_fc));
         fc_pending = true;
-        fc_timed = timing;
     }
     have_nbr = true;
     // the interaction counts stay on the device (pair_cnt) until the statistics are read
@@ -788,6 +842,7 @@ int System::hard_part(bool persistent) {
         long long *hs = hst.as<long long>();
         GPLUM_HIP(hipMemcpyAsync(hs, hstat.p, sizeof(long long) * 7, hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
+        resolve_timing();          // the earlier intervals have completed with the stream
         if (hs[5]) return fail(GPLUM_E_STATE, "hard cluster exceeded the block-step cap");
         double ed_sum;
         std::memcpy(&ed_sum, &hs[6], sizeof(double));
@@ -803,10 +858,9 @@ int System::hard_part(bool persistent) {
     }
     GPLUM_HIP(hipStreamWaitEvent(st, ev_iso_join, 0));
     tick(EV_HARD1);
-    GPLUM_HIP(hipStreamSynchronize(st));
-    stats.hard_ms += elapsed(EV_HARD0, EV_HARD1);
+    accum(&stats.hard_ms, EV_HARD0, EV_HARD1);
     if (ncl > 0 && persistent) {
-        stats.hard_kernel_ms += elapsed(EV_HK0, EV_HK1);
+        accum(&stats.hard_kernel_ms, EV_HK0, EV_HK1);
         stats.n_hard_launches++;
     }
     return GPLUM_OK;
@@ -934,6 +988,7 @@ __global__ void k_post_flags(int n, const double *__restrict__ pos, const int32_
     keep[i] = (live && !out) ? 1 : 0;
     if (s) atomicAdd(&cnt[0], 1);
     if (out) atomicAdd(&cnt[1], 1);
+    if (!(live && !out)) atomicAdd(&cnt[2], 1);     // dropped by the compaction
 }
 
 // the removed particles' fields for the energy bookkeeping and the remove log
@@ -967,14 +1022,14 @@ int System::merge_and_remove(int *n_removed) {
     GPLUM_ENSURE(M.kscan, n + 1);
     GPLUM_ENSURE(M.rem, n + 1);
     GPLUM_ENSURE(M.rscan, n + 1);
-    GPLUM_ENSURE(M.count, 2);
-    GPLUM_HIP(hipMemsetAsync(M.count.p, 0, 2 * sizeof(int32_t), st));
+    GPLUM_ENSURE(M.count, 3);
+    GPLUM_HIP(hipMemsetAsync(M.count.p, 0, 3 * sizeof(int32_t), st));
     hipLaunchKernelGGL(k_post_flags, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.flags.p,
                        r_max * r_max, r_min * r_min, M.flag.p, M.keep.p, M.rem.p, M.count.p);
-    GPLUM_HIP(hst.ensure(sizeof(int32_t) * 2));
-    GPLUM_HIP(hipMemcpyAsync(hst.p, M.count.p, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st));
+    GPLUM_HIP(hst.ensure(sizeof(int32_t) * 3));
+    GPLUM_HIP(hipMemcpyAsync(hst.p, M.count.p, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, st));
     GPLUM_HIP(hipStreamSynchronize(st));
-    const int k = hst.as<int32_t>()[0], n_rem = hst.as<int32_t>()[1];
+    const int k = hst.as<int32_t>()[0], n_rem = hst.as<int32_t>()[1], n_drop = hst.as<int32_t>()[2];
     if (k == 0 && n_rem == 0) return GPLUM_OK;
     int rc;
     if (k > 0) {
@@ -1080,14 +1135,10 @@ int System::merge_and_remove(int *n_removed) {
         n_remove_tot += n_rem;
         *n_removed = n_rem;
     }
-    // removeParticle: keep the live particles inside the boundary in their order
+    // removeParticle: keep the live particles inside the boundary in their order (their
+    // count known from the flags' read-back)
     if ((rc = excl_sum(rad.tmp, M.keep.p, M.kscan.p, n, st))) return rc;
-    GPLUM_HIP(hst.ensure(sizeof(int32_t) * 2));
-    int32_t *tail = hst.as<int32_t>();
-    GPLUM_HIP(hipMemcpyAsync(&tail[0], M.keep.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipMemcpyAsync(&tail[1], M.kscan.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    const int m = tail[0] + tail[1];
+    const int m = n - n_drop;
     GPLUM_ENSURE(M.perm, m + 1);
     hipLaunchKernelGGL(k_compact_idx, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, M.keep.p, M.kscan.p, M.perm.p);
     // a merge or a removal is always followed by the recomputed soft force, which assigns
@@ -1285,7 +1336,11 @@ int gplum_system_soft_force(gplum_system *s, int initial, int use_f64) {
 
 int gplum_system_set_radii(gplum_system *s) {
     if (!s) return gplum::fail(GPLUM_E_ARG, "null system");
-    return reinterpret_cast<System *>(s)->set_radii();
+    System *S = reinterpret_cast<System *>(s);
+    int rc = S->set_radii();
+    if (rc) return rc;
+    GPLUM_HIP(hipStreamSynchronize(S->st));
+    return gplum::radii_check(S->rad);
 }
 
 int gplum_system_step(gplum_system *s, int nsteps) {
@@ -1332,6 +1387,8 @@ int gplum_system_stats(gplum_system *s, gplum_stats *out) {
     System *S = reinterpret_cast<System *>(s);
     int rc = S->fc_sync();
     if (rc) return rc;
+    GPLUM_HIP(hipStreamSynchronize(S->st));
+    S->resolve_timing();
     if (S->fc.trained_rounds > 0 && (rc = gplum::fc_read_loss(S->fc, S->st_fc))) return rc;
     S->stats.fc_loss = S->fc.last_loss;
     *out = S->stats;
@@ -1387,6 +1444,8 @@ int gplum_system_reset_stats(gplum_system *s, int timing) {
     System *S = reinterpret_cast<System *>(s);
     int rc = S->fc_sync();
     if (rc) return rc;
+    GPLUM_HIP(hipStreamSynchronize(S->st));
+    S->resolve_timing();
     std::memset(&S->stats, 0, sizeof(S->stats));
     S->timing = timing != 0;
     return GPLUM_OK;

@@ -30,8 +30,12 @@ struct RadiiScratch {                // setCutoffRadii on the device (radii.hip)
     DArray<uint64_t> key, key_sorted;
     DArray<int> np, err, brange, cbase;
     DArray<unsigned char> tmp;
+    HostBuf err_host;                 // the last call's error flag, read without a wait
+    bool err_pending = false;
 };
 int device_cutoff_radii(const gplum_params &p, ParticleSoA &A, int n, RadiiScratch &R, hipStream_t st);
+// the error flag of the last device_cutoff_radii, once the stream has passed it
+int radii_check(RadiiScratch &R);
 
 struct MergeScratch {                 // MergeParticle + boundary removal on the device (system.hip)
     DArray<int32_t> flag, scan, sel, keep, kscan, perm, rem, rscan, ridx, count;
@@ -57,9 +61,19 @@ struct System {
     hipStream_t st_fc = nullptr;
     hipStream_t st_iso = nullptr;       // the isolated particles' drift, beside the clusters
     hipEvent_t ev_iso_fork = nullptr, ev_iso_join = nullptr;
-    hipEvent_t ev_fc_ready = nullptr, ev_fc_done = nullptr, ev_fc_t0 = nullptr, ev_fc_t1 = nullptr;
-    bool fc_pending = false, fc_timed = false;
+    hipEvent_t ev_fc_ready = nullptr, ev_fc_done = nullptr;
+    bool fc_pending = false;
     hipEvent_t ev[EV_COUNT] = {};
+    // Timed intervals outside the autotuner are not waited for: each tick records a fresh
+    // event (evd), each interval waits in tpend until a synchronisation the step makes
+    // anyway (or the statistics are read), so timing adds no host round trips.
+    struct TimedPair {
+        hipEvent_t a, b;
+        double *dst;
+    };
+    hipEvent_t evd[EV_COUNT] = {};
+    std::vector<hipEvent_t> ev_free;
+    std::vector<TimedPair> tpend;
     int n_cu = 256;
     int n = 0;
     double time_sys = 0., edisp = 0., step_edisp = 0.;
@@ -129,6 +143,9 @@ struct System {
     int write_collision_log();
     void tick(int k);
     double elapsed(int a, int b);
+    hipEvent_t timing_event();
+    void accum(double *dst, int a, int b);
+    void resolve_timing();
 };
 
 }  // namespace gplum
