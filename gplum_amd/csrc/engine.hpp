@@ -117,6 +117,8 @@ struct Tree {
     int64_t n_ep_total = 0, n_epf_total = 0, n_sp_total = 0;
     int64_t n_epep_pairs = 0, n_epsp_pairs = 0;
     int queue_cap = 1 << 14;
+    int n_part = 0;                            // particles of the last build
+    DArray<int32_t> ng_dev;                    // the group count on the device (for the walk)
     int region_cap[3] = {4096, 4096, 4096};   // list entries per walk workgroup (EP near, EP far, SP)
     int coop_min = 1 << 16;                    // nodes of at least this many keys: workgroup-wide digit search
 };

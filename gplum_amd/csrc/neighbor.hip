@@ -421,12 +421,21 @@ int neighbor_finalize(const gplum_params &p, ParticleSoA &ps, int n, const int2 
     hipLaunchKernelGGL(k_compact_rows, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ncand, jmask, nb.row_off.p,
                        nb.number_fp32.p, nb.keys.p, nb.accepted.p, nb.nbr_off.p, nb.nbr_idx.p);
     GPLUM_HIP(hipGetLastError());
-    int32_t info[2] = {0, 0};
-    GPLUM_HIP(hipMemcpyAsync(info, nb.info.p, sizeof(info), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
+    // the check and the accepted total are read back without a wait: nb_resolve takes
+    // them at the caller's next synchronisation
+    GPLUM_HIP(nb.info_host.ensure(2 * sizeof(int32_t)));
+    GPLUM_HIP(hipMemcpyAsync(nb.info_host.p, nb.info.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    nb.info_pending = true;
+    nb.n = n;
+    return GPLUM_OK;
+}
+
+int nb_resolve(NbState &nb) {
+    if (!nb.info_pending) return GPLUM_OK;
+    nb.info_pending = false;
+    const int32_t *info = nb.info_host.as<int32_t>();
     if (info[0]) return fail(GPLUM_E_STATE, "candidate count mismatch (FP32 counts vs candidates found)");
     nb.n_nbr = info[1];
-    nb.n = n;
     return GPLUM_OK;
 }
 
@@ -437,8 +446,7 @@ int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch 
     GPLUM_ENSURE(cl.cflag, n);
     GPLUM_ENSURE(cl.cscan, n);
     hipLaunchKernelGGL(k_uf_init, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, cl.parent.p);
-    if (nb.n_nbr > 0)
-        hipLaunchKernelGGL(k_uf_hook, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, nb.nbr_off.p, nb.nbr_idx.p,
+    hipLaunchKernelGGL(k_uf_hook, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, nb.nbr_off.p, nb.nbr_idx.p,
                            ps.id.p, cl.parent.p);
     hipLaunchKernelGGL(k_uf_label, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ps.number.p, ps.id.p,
                        cl.parent.p, ps.id_cluster.p, cl.ckey.p, cl.cflag.p);
@@ -446,6 +454,7 @@ int make_clusters(ParticleSoA &ps, int n, NbState &nb, ClusterList &cl, Scratch 
     if (rc) return rc;
     int m = 0;
     if ((rc = last_sum(cl.cflag.p, cl.cscan.p, n, &m, st))) return rc;
+    if ((rc = nb_resolve(nb))) return rc;       // the stream has passed the finalize
     cl.m = m;
     cl.ncl = 0;
     if (m == 0) return GPLUM_OK;

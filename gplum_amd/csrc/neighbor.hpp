@@ -21,8 +21,12 @@ struct NbState {
     DArray<int32_t> row_off, number_fp32, nbr_off, nbr_idx;
     DArray<int32_t> info;               // device: candidate-total check, accepted total
     DArray<double> acc_d_new;
+    HostBuf info_host;                  // info read back without a wait (nb_resolve)
+    bool info_pending = false;
     int n = 0, n_nbr = 0;
 };
+// the last finalize's check and accepted total, once the stream has passed it
+int nb_resolve(NbState &nb);
 
 // Hard clusters: particles with neighbours sorted by (id_cluster, index)
 // (HardSystem::makeList, src/hard.h:321-438).

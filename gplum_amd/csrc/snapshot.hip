@@ -174,6 +174,7 @@ int gplum_system_snapshot(gplum_system *s, const char *path, int32_t id_next) {
     if (rc) return rc;
     double e[5];
     if ((rc = S->energy(e))) return rc;
+    if ((rc = S->merge_resolve())) return rc;   // energy() has synchronised the stream
     gplum_snapshot_header hd;
     std::memset(&hd, 0, sizeof(hd));
     hd.time = S->time_sys;
@@ -242,6 +243,7 @@ int gplum_system_restart_write(gplum_system *s, const char *path, int32_t id_nex
     if (rc) return rc;
     double e[5];
     if ((rc = S->energy(e))) return rc;
+    if ((rc = S->merge_resolve())) return rc;   // energy() has synchronised the stream
     gplum_file_header hd;
     std::memset(&hd, 0, sizeof(hd));
     hd.n_body = S->n;
@@ -257,6 +259,7 @@ int gplum_system_energy_line(gplum_system *s, char *buf, int cap) {
     System *S = reinterpret_cast<System *>(s);
     double e[5];
     int rc = S->energy(e);
+    if (!rc) rc = S->merge_resolve();   // energy() has synchronised the stream
     if (rc) return rc;
     // outputStep (src/func.h:87-130): time, n_tot, etot, dE = (etot - etot0 - edisp)/etot0,
     // largest cluster, number of clusters, isolated particles

@@ -470,7 +470,9 @@ int System::soft_force(bool initial, bool use_f64) {
     tick(EV_TREE0);
     if ((rc = reorder())) return rc;
     if ((rc = tree_build_nodes(tree, p, A.pos.p, A.mass.p, A.r_search.p, n, st))) return rc;
-    if ((rc = radii_check(rad))) return rc;     // the tree build has synchronised the stream
+    // the tree build has synchronised the stream: the read-backs queued before it are in
+    if ((rc = radii_check(rad))) return rc;
+    if ((rc = merge_resolve())) return rc;
     if ((rc = tree_walk(tree, p, A.pos.p, A.r_search.p, st))) return rc;
     tick(EV_TREE1);
     GPLUM_ENSURE(jrec, n);
@@ -1013,6 +1015,18 @@ __global__ void k_gather_removed(int k, const int32_t *__restrict__ idx, PartPtr
 // on the host to pair them up.  The removed particles (normally none, a handful at
 // most) are copied to the host, which books their energy as the reference's rank 0 does
 // (func.h:305-333, taken in ascending id) and writes the remove log (func.h:347-352).
+// The merged targets' energies of the last merge_and_remove, added to edisp in target
+// order (the caller has synchronised the stream since).
+int System::merge_resolve() {
+    MergeScratch &M = mrg;
+    if (M.ed_pending > 0) {
+        const double *ed = M.ed_host.as<double>();
+        for (int a = 0; a < M.ed_pending; a++) edisp += ed[a];
+        M.ed_pending = 0;
+    }
+    return GPLUM_OK;
+}
+
 int System::merge_and_remove(int *n_removed) {
     *n_removed = 0;
     MergeScratch &M = mrg;
@@ -1077,13 +1091,16 @@ int System::merge_and_remove(int *n_removed) {
                                          st));
             hipLaunchKernelGGL(k_merge_apply, dim3(ceil_div(ntg, 64)), dim3(64), 0, st, ntg, M.tgt.p, M.twin_off.p,
                                M.twin.p, A.mass.p, A.vel.p, A.phi.p, A.phi_d.p, A.flags.p, M.ed.p);
-            std::vector<double> ed(ntg);
-            GPLUM_HIP(hipMemcpyAsync(ed.data(), M.ed.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
-            GPLUM_HIP(hipStreamSynchronize(st));
-            for (int a = 0; a < ntg; a++) edisp += ed[a];
+            // the merged targets' energies are read back without a wait and added to
+            // edisp, in target order, at the next synchronisation (merge_resolve)
+            GPLUM_HIP(M.ed_host.ensure(sizeof(double) * ntg));
+            GPLUM_HIP(hipMemcpyAsync(M.ed_host.p, M.ed.p, sizeof(double) * ntg, hipMemcpyDeviceToHost, st));
+            M.ed_pending = ntg;
         }
     }
     if (n_rem > 0) {
+        GPLUM_HIP(hipStreamSynchronize(st));
+        if ((rc = merge_resolve())) return rc;    // the mergers' energies before the removals'
         // the removed particles after the merge (a target's mass, velocity and potentials
         // are its merged ones), ascending index, then ascending id
         if ((rc = excl_sum(rad.tmp, M.rem.p, M.rscan.p, n, st))) return rc;
@@ -1359,14 +1376,20 @@ int gplum_system_energy(gplum_system *s, double out[6]) {
     if (!s || !out) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
     int rc = S->energy(out);
+    if (rc) return rc;
+    GPLUM_HIP(hipStreamSynchronize(S->st));
+    if ((rc = S->merge_resolve())) return rc;
     out[5] = S->edisp;
-    return rc;
+    return GPLUM_OK;
 }
 
 int gplum_system_neighbors(gplum_system *s, int32_t *offset, int32_t *index, int cap, int *total) {
     if (!s || !offset || !total) return gplum::fail(GPLUM_E_ARG, "null argument");
     System *S = reinterpret_cast<System *>(s);
     if (!S->have_nbr) return gplum::fail(GPLUM_E_STATE, "no neighbour graph for the current order");
+    GPLUM_HIP(hipStreamSynchronize(S->st));
+    int rc = gplum::nb_resolve(S->nb);
+    if (rc) return rc;
     *total = S->nb.n_nbr;
     GPLUM_HIP(hipMemcpy(offset, S->nb.nbr_off.p, sizeof(int32_t) * (S->n + 1), hipMemcpyDeviceToHost));
     if (index && cap >= S->nb.n_nbr && S->nb.n_nbr > 0)
@@ -1389,6 +1412,8 @@ int gplum_system_stats(gplum_system *s, gplum_stats *out) {
     if (rc) return rc;
     GPLUM_HIP(hipStreamSynchronize(S->st));
     S->resolve_timing();
+    if ((rc = gplum::nb_resolve(S->nb))) return rc;
+    if ((rc = S->merge_resolve())) return rc;
     if (S->fc.trained_rounds > 0 && (rc = gplum::fc_read_loss(S->fc, S->st_fc))) return rc;
     S->stats.fc_loss = S->fc.last_loss;
     *out = S->stats;

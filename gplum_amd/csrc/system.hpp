@@ -43,6 +43,8 @@ struct MergeScratch {                 // MergeParticle + boundary removal on the
     DArray<double> ed, rdata;
     DArray<int64_t> rid;
     DArray<int64_t> sel_info;         // flags (as int32) and ids of the listed particles
+    HostBuf ed_host;                  // the merged targets' energies, read back without a wait
+    int ed_pending = 0;
 };
 
 // A particle that left the system through removeParticlesOutOfBoundary (func.h:209-380),
@@ -143,6 +145,7 @@ struct System {
     int write_collision_log();
     void tick(int k);
     double elapsed(int a, int b);
+    int merge_resolve();
     hipEvent_t timing_event();
     void accum(double *dst, int a, int b);
     void resolve_timing();

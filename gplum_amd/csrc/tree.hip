@@ -544,6 +544,12 @@ __global__ void k_group_write(int n, int G, const int32_t *__restrict__ flag,
     gcount[g] = rem < G ? rem : G;
 }
 
+// the number of groups (last exclusive offset + last flag), left on the device for the walk
+__global__ void k_group_total(int n, const int32_t *__restrict__ flag01, const int32_t *__restrict__ scan,
+                              int32_t *__restrict__ ng) {
+    if (threadIdx.x == 0) *ng = n > 0 ? scan[n - 1] + flag01[n - 1] : 0;
+}
+
 __global__ void k_flag01(int n, const int32_t *__restrict__ flag, int32_t *__restrict__ out) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a < n) out[a] = flag[a] ? 1 : 0;
@@ -558,7 +564,7 @@ __global__ void k_flag01(int n, const int32_t *__restrict__ flag, int32_t *__res
 // through the per-group offsets and counts only.  use[3b+q] receives the entries the
 // workgroup wrote.  err bits: 1 queue, 4 region too small.
 __global__ __launch_bounds__(64) void k_walk(
-    int n_group, const int32_t *__restrict__ gstart, const int32_t *__restrict__ gcount,
+    const int32_t *__restrict__ ng_dev, const int32_t *__restrict__ gstart, const int32_t *__restrict__ gcount,
     const double *__restrict__ pos, const double *__restrict__ rsearch, const double *__restrict__ os,
     double theta2, const int32_t *__restrict__ nstart, const int32_t *__restrict__ ncount,
     const int32_t *__restrict__ nchild_first, const int32_t *__restrict__ nchild,
@@ -569,6 +575,7 @@ __global__ __launch_bounds__(64) void k_walk(
     int32_t *__restrict__ ep_list, int32_t *__restrict__ epf_list, int32_t *__restrict__ sp_list,
     int cap_ep, int cap_epf, int cap_sp, int32_t *__restrict__ use, int32_t *__restrict__ err) {
     const int lane = __lane_id();
+    const int n_group = *ng_dev;
     int32_t *queue = queue_all + (size_t)blockIdx.x * qcap;
     const int64_t r_ep = (int64_t)blockIdx.x * cap_ep, r_epf = (int64_t)blockIdx.x * cap_epf,
                   r_sp = (int64_t)blockIdx.x * cap_sp;
@@ -724,18 +731,6 @@ static int excl_scan(Tree &t, const T *in, T *out, int n, hipStream_t st) {
     return GPLUM_OK;
 }
 
-// Sum of the last element of an exclusive scan + the last input (total).
-template <class T>
-static int scan_total(const T *in, const T *scan, int n, T *total, hipStream_t st) {
-    T a = 0, b = 0;
-    if (n <= 0) { *total = 0; return GPLUM_OK; }
-    GPLUM_HIP(hipMemcpyAsync(&a, in + n - 1, sizeof(T), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipMemcpyAsync(&b, scan + n - 1, sizeof(T), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    *total = a + b;
-    return GPLUM_OK;
-}
-
 int tree_sort_keys(Tree &t, const double *d_pos, int n, hipStream_t st) {
     const int nb = std::min(1024, std::max(1, ceil_div(n, 256)));
     GPLUM_ENSURE(t.bbox, 6 * nb + 8);
@@ -831,12 +826,15 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
                        t.nchild_scan.p);
     int rc = excl_scan(t, t.nchild_scan.p, t.grp_scan.p, n, st);
     if (rc) return rc;
-    int ng = 0;
-    if ((rc = scan_total(t.nchild_scan.p, t.grp_scan.p, n, &ng, st))) return rc;
-    t.n_group = ng;
-    GPLUM_ENSURE(t.group_node, ng);
-    GPLUM_ENSURE(t.group_start, ng);
-    GPLUM_ENSURE(t.group_count, ng);
+    // the group count stays on the device: the walk reads it there and reads it back with
+    // its own results (at most one group per particle until then)
+    GPLUM_ENSURE(t.ng_dev, 1);
+    hipLaunchKernelGGL(k_group_total, dim3(1), dim3(64), 0, st, n, t.nchild_scan.p, t.grp_scan.p, t.ng_dev.p);
+    t.n_part = n;
+    t.n_group = -1;
+    GPLUM_ENSURE(t.group_node, std::max(n, 1));
+    GPLUM_ENSURE(t.group_start, std::max(n, 1));
+    GPLUM_ENSURE(t.group_count, std::max(n, 1));
     hipLaunchKernelGGL(k_group_write, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, p.n_group_limit,
                        t.grp_flag.p, t.grp_scan.p, t.node_start.p, t.node_count.p, t.group_node.p,
                        t.group_start.p, t.group_count.p);
@@ -846,14 +844,15 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
 
 int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double *d_rsearch,
               hipStream_t st) {
-    const int ng = t.n_group;
-    GPLUM_ENSURE(t.ep_cnt, ng + 1);
-    GPLUM_ENSURE(t.epf_cnt, ng + 1);
-    GPLUM_ENSURE(t.sp_cnt, ng + 1);
-    GPLUM_ENSURE(t.ep_off, ng + 1);
-    GPLUM_ENSURE(t.epf_off, ng + 1);
-    GPLUM_ENSURE(t.sp_off, ng + 1);
-    const int nblk = std::min(std::max(ng, 1), 4096);
+    // the group count is on the device (at most one group per particle)
+    const int ng_max = std::max(t.n_part, 1);
+    GPLUM_ENSURE(t.ep_cnt, ng_max + 1);
+    GPLUM_ENSURE(t.epf_cnt, ng_max + 1);
+    GPLUM_ENSURE(t.sp_cnt, ng_max + 1);
+    GPLUM_ENSURE(t.ep_off, ng_max + 1);
+    GPLUM_ENSURE(t.epf_off, ng_max + 1);
+    GPLUM_ENSURE(t.sp_off, ng_max + 1);
+    const int nblk = std::min(ng_max, 4096);
     GPLUM_ENSURE(t.walk_use, 3 * (size_t)nblk);
     GPLUM_ENSURE(t.walk_err, 2);
     GPLUM_ENSURE(t.walk_stat, 6);
@@ -868,7 +867,7 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
         GPLUM_ENSURE(t.sp_list, (size_t)nblk * t.region_cap[2]);
         GPLUM_ENSURE(t.walk_queue, (size_t)nblk * t.queue_cap);
         GPLUM_HIP(hipMemsetAsync(t.walk_err.p, 0, sizeof(int32_t), st));
-        hipLaunchKernelGGL(k_walk, dim3(nblk), dim3(64), 0, st, ng, t.group_start.p, t.group_count.p, d_pos,
+        hipLaunchKernelGGL(k_walk, dim3(nblk), dim3(64), 0, st, t.ng_dev.p, t.group_start.p, t.group_count.p, d_pos,
                            d_rsearch, t.origin_scale.p, theta2, t.node_start.p, t.node_count.p,
                            t.node_child.p, t.node_nchild.p, t.node_level.p, t.node_com.p, t.node_bmin.p,
                            t.node_bmax.p, t.node_rs.p, t.walk_queue.p, t.queue_cap, t.ep_cnt.p, t.epf_cnt.p,
@@ -879,9 +878,12 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
         long long *h = t.walk_host.as<long long>();
         GPLUM_HIP(hipMemcpyAsync(h, t.walk_stat.p, 6 * sizeof(long long), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipMemcpyAsync(h + 6, t.walk_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(h + 7, t.ng_dev.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
-        int32_t err;
+        int32_t err, ng;
         std::memcpy(&err, h + 6, sizeof(int32_t));
+        std::memcpy(&ng, h + 7, sizeof(int32_t));
+        t.n_group = ng;
         if (err & 1) t.queue_cap *= 4;
         if (err & 4)    // the use of a workgroup that stopped early is a lower bound
             for (int q = 0; q < 3; q++)
