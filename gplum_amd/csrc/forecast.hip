@@ -165,11 +165,17 @@ __global__ __launch_bounds__(FC_SCAN_BLOCK) void k_fc_summary(int ng, const int3
         const long long to = block_wave_totals(wave_sum_ll(so), s_w, &bo);
         long long rs = cs + bs, ro = co + bo;
         for (int k = 0; k < FC_IPT; k++) {
-            const long long vo = over[k] ? FC_PACK + c[k] : 0;
-            const long long xs = wave_incl_scan_ll(c[k]), xo = wave_incl_scan_ll(vo);
-            const long long es = rs + xs - c[k], eo = ro + xo - vo;
+            const long long xs = wave_incl_scan_ll(c[k]);
+            const long long es = rs + xs - c[k];
             rs += __shfl(xs, 63);
-            ro += __shfl(xo, 63);
+            // the overflow list's scan only where the wave has an overflowed group (rare)
+            long long eo = ro;
+            if (__ballot(over[k])) {
+                const long long vo = over[k] ? FC_PACK + c[k] : 0;
+                const long long xo = wave_incl_scan_ll(vo);
+                eo = ro + xo - vo;
+                ro += __shfl(xo, 63);
+            }
             const int g = g0 + 64 * k;
             if (g < ng) {
                 fin_off[g] = (int32_t)es;
