@@ -121,6 +121,10 @@ struct Tree {
     DArray<int32_t> ng_dev;                    // the group count on the device (for the walk)
     int region_cap[3] = {4096, 4096, 4096};   // list entries per walk workgroup (EP near, EP far, SP)
     int coop_min = 1 << 16;                    // nodes of at least this many keys: workgroup-wide digit search
+    // the build in progress (finished by the walk's read-back): its particle arrays, node
+    // capacity and the levels launched so far
+    const double *src_pos = nullptr, *src_mass = nullptr, *src_rs = nullptr;
+    int src_n = 0, cap = 0, L_end = 0;
 };
 
 }  // namespace gplum

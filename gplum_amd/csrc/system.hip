@@ -470,10 +470,10 @@ int System::soft_force(bool initial, bool use_f64) {
     tick(EV_TREE0);
     if ((rc = reorder())) return rc;
     if ((rc = tree_build_nodes(tree, p, A.pos.p, A.mass.p, A.r_search.p, n, st))) return rc;
-    // the tree build has synchronised the stream: the read-backs queued before it are in
+    if ((rc = tree_walk(tree, p, A.pos.p, A.r_search.p, st))) return rc;
+    // the walk has synchronised the stream: the read-backs queued before it are in
     if ((rc = radii_check(rad))) return rc;
     if ((rc = merge_resolve())) return rc;
-    if ((rc = tree_walk(tree, p, A.pos.p, A.r_search.p, st))) return rc;
     tick(EV_TREE1);
     GPLUM_ENSURE(jrec, n);
     hipLaunchKernelGGL(k_jrec, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, A.pos.p, A.mass.p, A.r_out.p,

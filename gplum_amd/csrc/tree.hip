@@ -445,23 +445,25 @@ __global__ void k_lvl_init(int32_t *lvl) {
     if (threadIdx.x < LVL_ERR + 1) lvl[threadIdx.x] = (threadIdx.x == 1) ? 1 : 0;
 }
 
-__global__ void k_leaf_tail(int lb, int le, int32_t *nchild_out, int32_t *child) {
-    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= le) return;
-    nchild_out[c] = 0;
-    child[c] = -1;
-}
-
 // ---------------------------------------------------------------- moments
 
-__global__ void k_moments(int lb, int le, const double *__restrict__ pos, const double *__restrict__ mass,
-                          const double *__restrict__ rsearch, const int32_t *__restrict__ start,
-                          const int32_t *__restrict__ count, const int32_t *__restrict__ child,
-                          const int32_t *__restrict__ nchild, double *__restrict__ nm, double *__restrict__ ncom,
-                          double *__restrict__ nq, double *__restrict__ bmin, double *__restrict__ bmax,
-                          double *__restrict__ nrs) {
-    const int c = lb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= le) return;
+// The levels up to L_end have been built (launched before the host knows how many levels
+// the tree has) and the tree is whole: no node overflow, and no nodes at level L_end,
+// whose children would not have been built.  Kernels after the level build check this
+// on the device and do nothing otherwise; the host reads the ranges back with the walk's
+// results and finishes the tree then.
+__device__ inline bool tree_whole(const int32_t *__restrict__ lvl, int L_end) {
+    return !lvl[LVL_ERR] && !(L_end <= MAXLEVEL && lvl[L_end + 1] > lvl[L_end]);
+}
+
+// Moments of node c: mass, centre of mass, quadrupole about it, bounding box and largest
+// search radius, from its particles (a leaf) or its children (in child order).
+__device__ inline void node_moments(int c, const double *__restrict__ pos, const double *__restrict__ mass,
+                                    const double *__restrict__ rsearch, const int32_t *__restrict__ start,
+                                    const int32_t *__restrict__ count, const int32_t *__restrict__ child,
+                                    const int32_t *__restrict__ nchild, double *__restrict__ nm,
+                                    double *__restrict__ ncom, double *__restrict__ nq, double *__restrict__ bmin,
+                                    double *__restrict__ bmax, double *__restrict__ nrs) {
     double m = 0., mp[3] = {0., 0., 0.}, q[6] = {0., 0., 0., 0., 0., 0.};
     double bmn[3], bmx[3], rs = 0., com[3];
     const int s = start[c], cnt = count[c];
@@ -514,18 +516,35 @@ __global__ void k_moments(int lb, int le, const double *__restrict__ pos, const 
     nrs[c] = rs;
 }
 
+// one level's moments (its range read from lvl), the levels launched deepest first
+__global__ void k_moments(int L, int L_end, const int32_t *__restrict__ lvl, const double *__restrict__ pos,
+                          const double *__restrict__ mass, const double *__restrict__ rsearch,
+                          const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                          const int32_t *__restrict__ child, const int32_t *__restrict__ nchild,
+                          double *__restrict__ nm, double *__restrict__ ncom, double *__restrict__ nq,
+                          double *__restrict__ bmin, double *__restrict__ bmax, double *__restrict__ nrs) {
+    if (!tree_whole(lvl, L_end)) return;
+    const int le = lvl[L + 1];
+    for (int c = lvl[L] + blockIdx.x * blockDim.x + threadIdx.x; c < le; c += gridDim.x * blockDim.x)
+        node_moments(c, pos, mass, rsearch, start, count, child, nchild, nm, ncom, nq, bmin, bmax, nrs);
+}
+
 // ----------------------------------------------------------------- groups
 
-__global__ void k_group_flags(int n_node, int G, const int32_t *__restrict__ start,
+// the node count lvl[L_end] is read on the device; no flags (so no groups) unless the
+// tree is whole
+__global__ void k_group_flags(int L_end, const int32_t *__restrict__ lvl, int G, const int32_t *__restrict__ start,
                               const int32_t *__restrict__ count, const int32_t *__restrict__ nchild,
                               const int32_t *__restrict__ parent, int32_t *__restrict__ flag) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_node) return;
-    const int cnt = count[c];
-    const bool cand = (cnt <= G) || (nchild[c] == 0);
-    const bool top = (parent[c] < 0) || (count[parent[c]] > G);
-    if (!(cand && top)) return;
-    for (int off = 0; off < cnt; off += G) flag[start[c] + off] = c + 1;
+    if (!tree_whole(lvl, L_end)) return;
+    const int n_node = lvl[L_end];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_node; c += gridDim.x * blockDim.x) {
+        const int cnt = count[c];
+        const bool cand = (cnt <= G) || (nchild[c] == 0);
+        const bool top = (parent[c] < 0) || (count[parent[c]] > G);
+        if (!(cand && top)) continue;
+        for (int off = 0; off < cnt; off += G) flag[start[c] + off] = c + 1;
+    }
 }
 
 __global__ void k_group_write(int n, int G, const int32_t *__restrict__ flag,
@@ -753,6 +772,52 @@ int tree_sort_keys(Tree &t, const double *d_pos, int n, hipStream_t st) {
     return GPLUM_OK;
 }
 
+// Level kernels for levels [L0, L1) (ranges kept on the device; a level past the last one
+// is empty and its kernels return at once).
+static void launch_levels(Tree &t, const gplum_params &p, int L0, int L1, int cap, hipStream_t st) {
+    for (int L = L0; L < L1; L++) {
+        hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
+                           p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
+                           t.lvl_part.p, t.lvl_bnd.p, t.coop_min, std::max(1, t.coop_min >> 8));
+        hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
+                           t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
+                           t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
+    }
+}
+
+// Moments (levels L_end-1 .. 0) and i-groups of a tree built up to level L_end, all
+// guarded on the device by tree_whole.  Launch widths follow the previous tree's level
+// sizes (the kernels loop over whatever the level holds).
+static int launch_moments_groups(Tree &t, const gplum_params &p, int L_end, hipStream_t st) {
+    const int n = t.src_n;
+    for (int L = L_end - 1; L >= 0; L--) {
+        int hint = ceil_div(n, 4);
+        if (L + 1 < (int)t.level_begin.size()) {
+            const int sz = t.level_begin[L + 1] - t.level_begin[L];
+            hint = sz + sz / 4 + 64;
+        }
+        hipLaunchKernelGGL(k_moments, dim3(std::max(1, ceil_div(hint, 128))), dim3(128), 0, st, L, L_end, t.lvl.p,
+                           t.src_pos, t.src_mass, t.src_rs, t.node_start.p, t.node_count.p, t.node_child.p,
+                           t.node_nchild.p, t.node_mass.p, t.node_com.p, t.node_quad.p, t.node_bmin.p,
+                           t.node_bmax.p, t.node_rs.p);
+    }
+    GPLUM_HIP(hipMemsetAsync(t.grp_flag.p, 0, sizeof(int32_t) * n, st));
+    hipLaunchKernelGGL(k_group_flags, dim3(std::max(1, ceil_div(n, 256))), dim3(256), 0, st, L_end, t.lvl.p,
+                       p.n_group_limit, t.node_start.p, t.node_count.p, t.node_nchild.p, t.node_parent.p,
+                       t.grp_flag.p);
+    hipLaunchKernelGGL(k_flag01, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, t.grp_flag.p, t.nchild_scan.p);
+    int rc = excl_scan(t, t.nchild_scan.p, t.grp_scan.p, n, st);
+    if (rc) return rc;
+    // the group count stays on the device: the walk reads it there and reads it back with
+    // its own results (at most one group per particle until then)
+    hipLaunchKernelGGL(k_group_total, dim3(1), dim3(64), 0, st, n, t.nchild_scan.p, t.grp_scan.p, t.ng_dev.p);
+    hipLaunchKernelGGL(k_group_write, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, p.n_group_limit,
+                       t.grp_flag.p, t.grp_scan.p, t.node_start.p, t.node_count.p, t.group_node.p,
+                       t.group_start.p, t.group_count.p);
+    GPLUM_HIP(hipGetLastError());
+    return GPLUM_OK;
+}
+
 int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const double *d_mass,
                      const double *d_rsearch, int n, hipStream_t st) {
     const int cap = 2 * n + 64;
@@ -762,84 +827,39 @@ int tree_build_nodes(Tree &t, const gplum_params &p, const double *d_pos, const 
     GPLUM_ENSURE(t.node_nchild, cap);
     GPLUM_ENSURE(t.node_level, cap);
     GPLUM_ENSURE(t.node_parent, cap);
-    GPLUM_ENSURE(t.nchild_scan, cap);
-    GPLUM_ENSURE(t.grp_flag, cap);   // reused as per-level child counts
+    GPLUM_ENSURE(t.nchild_scan, std::max(n, cap));
+    GPLUM_ENSURE(t.grp_flag, cap);   // per-level child counts, then per-particle group flags
+    GPLUM_ENSURE(t.grp_scan, n);
+    // the node count is known only on the device until the walk's read-back: node arrays
+    // at the node capacity
+    GPLUM_ENSURE(t.node_mass, cap);
+    GPLUM_ENSURE(t.node_com, 3 * (size_t)cap);
+    GPLUM_ENSURE(t.node_quad, 6 * (size_t)cap);
+    GPLUM_ENSURE(t.node_bmin, 3 * (size_t)cap);
+    GPLUM_ENSURE(t.node_bmax, 3 * (size_t)cap);
+    GPLUM_ENSURE(t.node_rs, cap);
+    GPLUM_ENSURE(t.ng_dev, 1);
+    GPLUM_ENSURE(t.group_node, std::max(n, 1));
+    GPLUM_ENSURE(t.group_start, std::max(n, 1));
+    GPLUM_ENSURE(t.group_count, std::max(n, 1));
     hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, st, n, t.node_start.p, t.node_count.p,
                        t.node_level.p, t.node_parent.p);
-    // levels back to back, ranges kept on the device (a level past the last one is empty
-    // and its kernels return at once); one read-back at the end
     GPLUM_ENSURE(t.lvl, LVL_ERR + 5);
     hipLaunchKernelGGL(k_lvl_init, dim3(1), dim3(64), 0, st, t.lvl.p);
     GPLUM_ENSURE(t.lvl_part, LEVEL_GRID);
     GPLUM_ENSURE(t.lvl_bnd, 8 * ((size_t)n + 64));   // a level holds at most n nodes
     // levels up to one past the previous tree's depth (their kernels cost launches even
-    // when empty); if the level after them holds nodes, the rest follow after the read-back
-    auto levels = [&](int L0, int L1) {
-        for (int L = L0; L < L1; L++) {
-            hipLaunchKernelGGL(k_count_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
-                               p.n_leaf_limit, t.node_start.p, t.node_count.p, t.node_level.p, t.grp_flag.p,
-                               t.lvl_part.p, t.lvl_bnd.p, t.coop_min, std::max(1, t.coop_min >> 8));
-            hipLaunchKernelGGL(k_write_children_lv, dim3(LEVEL_GRID), dim3(LEVEL_BLOCK), 0, st, t.key.p, t.lvl.p, L,
-                               t.lvl_part.p, cap, t.node_start.p, t.node_count.p, t.node_level.p, t.node_parent.p,
-                               t.node_child.p, t.node_nchild.p, t.grp_flag.p, t.lvl_bnd.p);
-        }
-    };
-    const int L_end = t.n_level > 0 ? std::min(MAXLEVEL + 1, t.n_level + 2) : MAXLEVEL + 1;
-    levels(0, L_end);
-    std::vector<int32_t> lv(LVL_ERR + 1);
-    GPLUM_HIP(hipMemcpyAsync(lv.data(), t.lvl.p, sizeof(int32_t) * (LVL_ERR + 1), hipMemcpyDeviceToHost, st));
-    GPLUM_HIP(hipStreamSynchronize(st));
-    GPLUM_HIP(hipGetLastError());
-    if (L_end <= MAXLEVEL && !lv[LVL_ERR] && lv[L_end + 1] > lv[L_end]) {
-        levels(L_end, MAXLEVEL + 1);     // level L_end holds nodes: a deeper tree than before
-        GPLUM_HIP(hipMemcpyAsync(lv.data(), t.lvl.p, sizeof(int32_t) * (LVL_ERR + 1), hipMemcpyDeviceToHost, st));
-        GPLUM_HIP(hipStreamSynchronize(st));
-        GPLUM_HIP(hipGetLastError());
-    }
-    if (lv[LVL_ERR]) return fail(GPLUM_E_OVERFLOW, "tree node capacity exceeded");
-    t.level_begin.assign(1, 0);
-    for (int L = 1; L <= MAXLEVEL + 2 && lv[L] > lv[L - 1]; L++) t.level_begin.push_back(lv[L]);
-    t.n_level = (int)t.level_begin.size() - 1;
-    t.n_node = t.level_begin.back();
-    GPLUM_ENSURE(t.node_mass, t.n_node);
-    GPLUM_ENSURE(t.node_com, 3 * t.n_node);
-    GPLUM_ENSURE(t.node_quad, 6 * t.n_node);
-    GPLUM_ENSURE(t.node_bmin, 3 * t.n_node);
-    GPLUM_ENSURE(t.node_bmax, 3 * t.n_node);
-    GPLUM_ENSURE(t.node_rs, t.n_node);
-    for (int l = t.n_level - 1; l >= 0; l--) {
-        const int b = t.level_begin[l], e = t.level_begin[l + 1];
-        hipLaunchKernelGGL(k_moments, dim3(ceil_div(e - b, 128)), dim3(128), 0, st, b, e, d_pos, d_mass,
-                           d_rsearch, t.node_start.p, t.node_count.p, t.node_child.p, t.node_nchild.p,
-                           t.node_mass.p, t.node_com.p, t.node_quad.p, t.node_bmin.p, t.node_bmax.p,
-                           t.node_rs.p);
-    }
-    // groups
-    GPLUM_ENSURE(t.grp_flag, n);
-    GPLUM_ENSURE(t.grp_scan, n);
-    GPLUM_ENSURE(t.nchild_scan, std::max(n, cap));
-    GPLUM_HIP(hipMemsetAsync(t.grp_flag.p, 0, sizeof(int32_t) * n, st));
-    hipLaunchKernelGGL(k_group_flags, dim3(ceil_div(t.n_node, 256)), dim3(256), 0, st, t.n_node,
-                       p.n_group_limit, t.node_start.p, t.node_count.p, t.node_nchild.p,
-                       t.node_parent.p, t.grp_flag.p);
-    hipLaunchKernelGGL(k_flag01, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, t.grp_flag.p,
-                       t.nchild_scan.p);
-    int rc = excl_scan(t, t.nchild_scan.p, t.grp_scan.p, n, st);
-    if (rc) return rc;
-    // the group count stays on the device: the walk reads it there and reads it back with
-    // its own results (at most one group per particle until then)
-    GPLUM_ENSURE(t.ng_dev, 1);
-    hipLaunchKernelGGL(k_group_total, dim3(1), dim3(64), 0, st, n, t.nchild_scan.p, t.grp_scan.p, t.ng_dev.p);
+    // when empty); a deeper tree is finished after the walk's read-back (tree_walk)
+    t.src_pos = d_pos;
+    t.src_mass = d_mass;
+    t.src_rs = d_rsearch;
+    t.src_n = n;
+    t.cap = cap;
+    t.L_end = t.n_level > 0 ? std::min(MAXLEVEL + 1, t.n_level + 2) : MAXLEVEL + 1;
+    launch_levels(t, p, 0, t.L_end, cap, st);
     t.n_part = n;
     t.n_group = -1;
-    GPLUM_ENSURE(t.group_node, std::max(n, 1));
-    GPLUM_ENSURE(t.group_start, std::max(n, 1));
-    GPLUM_ENSURE(t.group_count, std::max(n, 1));
-    hipLaunchKernelGGL(k_group_write, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, p.n_group_limit,
-                       t.grp_flag.p, t.grp_scan.p, t.node_start.p, t.node_count.p, t.group_node.p,
-                       t.group_start.p, t.group_count.p);
-    GPLUM_HIP(hipGetLastError());
-    return GPLUM_OK;
+    return launch_moments_groups(t, p, t.L_end, st);
 }
 
 int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double *d_rsearch,
@@ -856,7 +876,7 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
     GPLUM_ENSURE(t.walk_use, 3 * (size_t)nblk);
     GPLUM_ENSURE(t.walk_err, 2);
     GPLUM_ENSURE(t.walk_stat, 6);
-    GPLUM_HIP(t.walk_host.ensure(8 * sizeof(long long)));
+    GPLUM_HIP(t.walk_host.ensure(8 * sizeof(long long) + (LVL_ERR + 1) * sizeof(int32_t)));
     const double theta2 = p.theta * p.theta;
     for (int attempt = 0; attempt < 12; attempt++) {
         for (int q = 0; q < 3; q++)
@@ -876,10 +896,30 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
                            t.walk_err.p);
         hipLaunchKernelGGL(k_walk_use, dim3(1), dim3(1024), 0, st, nblk, t.walk_use.p, t.walk_stat.p);
         long long *h = t.walk_host.as<long long>();
+        int32_t *lv = reinterpret_cast<int32_t *>(h + 8);
         GPLUM_HIP(hipMemcpyAsync(h, t.walk_stat.p, 6 * sizeof(long long), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipMemcpyAsync(h + 6, t.walk_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipMemcpyAsync(h + 7, t.ng_dev.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        GPLUM_HIP(hipMemcpyAsync(lv, t.lvl.p, (LVL_ERR + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         GPLUM_HIP(hipStreamSynchronize(st));
+        GPLUM_HIP(hipGetLastError());
+        // the tree first: the kernels after the level build did nothing unless it is whole
+        if (lv[LVL_ERR]) return fail(GPLUM_E_OVERFLOW, "tree node capacity exceeded");
+        if (t.L_end <= MAXLEVEL && lv[t.L_end + 1] > lv[t.L_end]) {
+            // level L_end holds nodes: a deeper tree than before; its remaining levels,
+            // then moments, groups and the walk again
+            launch_levels(t, p, t.L_end, MAXLEVEL + 1, t.cap, st);
+            t.L_end = MAXLEVEL + 1;
+            int rc = launch_moments_groups(t, p, t.L_end, st);
+            if (rc) return rc;
+            continue;
+        }
+        if (t.n_group < 0) {
+            t.level_begin.assign(1, 0);
+            for (int L = 1; L <= MAXLEVEL + 2 && lv[L] > lv[L - 1]; L++) t.level_begin.push_back(lv[L]);
+            t.n_level = (int)t.level_begin.size() - 1;
+            t.n_node = t.level_begin.back();
+        }
         int32_t err, ng;
         std::memcpy(&err, h + 6, sizeof(int32_t));
         std::memcpy(&ng, h + 7, sizeof(int32_t));
@@ -898,7 +938,6 @@ int tree_walk(Tree &t, const gplum_params &p, const double *d_pos, const double 
                 const int64_t want = h[q] + h[q] / 8 + 64;
                 if (want > t.region_cap[q] || want < t.region_cap[q] / 2) t.region_cap[q] = (int)std::min<int64_t>(want, INT32_MAX);
             }
-            GPLUM_HIP(hipGetLastError());
             return GPLUM_OK;
         }
         if (t.queue_cap > (1 << 26)) break;

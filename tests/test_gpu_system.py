@@ -94,6 +94,38 @@ def test_tree_force_and_neighbours_match_oracle(oracle_lib, quad, coop_min):
     s.close()
 
 
+def test_deeper_tree_than_the_previous_build(oracle_lib):
+    """The level build launches one level past the previous tree's depth and finishes a
+    deeper tree after the walk's read-back: a 64-particle ring (a few levels), then on
+    the same system a ring with a clump of 200 particles within 1e-6 (down to the
+    deepest level); forces and neighbour sets against the oracle's tree."""
+    po, pg = both_params(oracle_lib, quadrupole=1, **CFG1)
+    s = gpu.System(pg, ring(64, seed=5))
+    s.set_radii()
+    s.soft_force(initial=True, use_f64=True)
+    pp = ring(6000, seed=22)
+    rng = np.random.default_rng(23)
+    pp["pos"][:200] = pp["pos"][200] + rng.uniform(-1e-6, 1e-6, size=(200, 3))
+    pp["vel"][:200] = pp["vel"][200]
+    ref = pp.copy()
+    oracle_lib.set_cutoff_radii(po, ref)
+    tree = oracle_lib.tree_build(po, ref)
+    oracle_lib.soft_force(po, ref, tree, use_f64=True)
+    tree.free()
+    off_o, idx_o = oracle_lib.correct_force_long(po, ref, initial=True)
+    s.set_particles(pp)
+    s.set_radii()
+    s.soft_force(initial=True, use_f64=True)
+    g = s.particles()
+    off_g, idx_g = s.neighbors()
+    assert gpu.neighbor_sets_by_id(g, off_g, idx_g) == gpu.neighbor_sets_by_id(ref, off_o, idx_o)
+    go, gg = by_id(ref), by_id(g)
+    assert np.array_equal(go["neighbor"]["number"], gg["neighbor"]["number"])
+    assert helpers.rel_err_rms(gg["acc"], go["acc"]) < 1e-10
+    assert helpers.rel_err_rms(gg["phi"], go["phi"]) < 1e-10
+    s.close()
+
+
 def parse_collision_log(path):
     """Collision0::write2File lines (collisionA.h:122-148)."""
     rows = []
